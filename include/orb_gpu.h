@@ -1,0 +1,126 @@
+/*
+ * orb_gpu.h -- C ABI of the MI355X ORB front end.
+ *
+ * Drop-in boundary for ORB-SLAM2's data-parallel hot path.  Each entry point
+ * names the reference interface it replaces (ORB-SLAM2 file:line).  Plain
+ * pointers and sizes only; every call returns an int status (ORB_OK or a
+ * negative ORB_E* code) and never throws.
+ *
+ * Ownership: the caller owns host buffers; device buffers belong to the
+ * context or extractor handle.  A handle is not thread-safe; distinct handles
+ * may be used concurrently from different threads.
+ */
+#ifndef ORB_GPU_H
+#define ORB_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORB_OK 0
+#define ORB_E_ARG (-1)       /* bad argument (null, size, parameter range) */
+#define ORB_E_CAPACITY (-2)  /* caller's output buffer too small */
+#define ORB_E_DEVICE (-3)    /* HIP runtime error */
+#define ORB_E_OOM (-4)       /* device allocation failed */
+#define ORB_E_OVERFLOW (-5)  /* an internal bound was exceeded (see orb_extractor_capacity) */
+#define ORB_E_ADMISSION (-6) /* serving: request refused by the memory budget */
+
+/* Same layout as cv::KeyPoint: pt.x, pt.y, size, angle, response, octave,
+ * class_id (28 bytes). */
+typedef struct orb_keypoint {
+    float x, y, size, angle, response;
+    int octave, class_id;
+} orb_keypoint;
+
+typedef struct orb_context orb_context;
+typedef struct orb_extractor orb_extractor;
+
+/* Device context: binds a HIP device and owns the stream all work runs on.
+ * No reference counterpart (the reference is CPU-only). */
+int orb_context_create(int device, orb_context** out);
+void orb_context_destroy(orb_context* ctx);
+int orb_context_synchronize(orb_context* ctx);
+
+/* ORBextractor::ORBextractor(nfeatures, scaleFactor, nlevels, iniThFAST,
+ * minThFAST)  (include/ORBextractor.h:93, src/ORBextractor.cc:469-571).
+ * nlevels <= 16, scaleFactor > 1. */
+int orb_extractor_create(orb_context* ctx, int nfeatures, float scaleFactor, int nlevels, int iniThFAST,
+                         int minThFAST, orb_extractor** out);
+void orb_extractor_destroy(orb_extractor* ex);
+
+/* GetLevels / GetScaleFactor (ORBextractor.h:119-124). */
+int orb_extractor_levels(const orb_extractor* ex);
+float orb_extractor_scale_factor(const orb_extractor* ex);
+/* GetScaleFactors / GetInverseScaleFactors / GetScaleSigmaSquares /
+ * GetInverseScaleSigmaSquares (ORBextractor.h:127-159); any pointer may be
+ * NULL; each non-NULL one receives nlevels floats. */
+int orb_extractor_scales(const orb_extractor* ex, float* scale, float* inv_scale, float* sigma2,
+                         float* inv_sigma2);
+/* mnFeaturesPerLevel (ORBextractor.cc:509-526): nlevels ints. */
+int orb_extractor_quotas(const orb_extractor* ex, int* quota);
+/* Upper bound on keypoints per call: sum over levels of max(quota+3, 4*roots),
+ * roots from the aspect ratio of width x height. */
+int orb_extractor_capacity(const orb_extractor* ex, int width, int height);
+
+/* ORBextractor::operator()(image, mask, keypoints, descriptors)
+ * (include/ORBextractor.h:111, src/ORBextractor.cc:1533-1649).
+ * image: width x height CV_8UC1 host pixels with row stride `stride` bytes.
+ * On return *n keypoints (level-0 coordinates, octave order then quadtree
+ * order) and n x 32 descriptor bytes.  An empty image (width or height 0)
+ * returns ORB_OK with *n = 0, as the reference returns silently.  The mask
+ * argument of the reference is ignored by the reference and is not taken. */
+int orb_extract(orb_extractor* ex, const uint8_t* image, int width, int height, size_t stride,
+                orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n);
+
+/* Same, with the image already in device memory (row pitch in bytes).  The
+ * results stay on the device until read with orb_extractor_results; the call
+ * returns once the work is enqueued. */
+int orb_extract_device(orb_extractor* ex, const uint8_t* d_image, int width, int height, size_t pitch);
+int orb_extractor_results(orb_extractor* ex, orb_keypoint* keypoints, int capacity, uint8_t* descriptors,
+                          int* n);
+
+/* mvImagePyramid[level] (ORBextractor.h:162) of the last call, copied to a
+ * host buffer (dst may be NULL to query the size). */
+int orb_extractor_pyramid_level(orb_extractor* ex, int level, uint8_t* dst, size_t dst_stride, int* width,
+                                int* height);
+
+/* ORBmatcher::DescriptorDistance(a, b) (include/ORBmatcher.h:65,
+ * src/ORBmatcher.cc:2083-2103): Hamming distance of two 32-byte rows. */
+int orb_descriptor_distance(const uint8_t* a, const uint8_t* b);
+
+/* Brute-force matching of nq query rows against nt train rows with the
+ * best/second-best rule of ORBmatcher::SearchForInitialization
+ * (src/ORBmatcher.cc:614-647) and no spatial window: idx[i] = best train index
+ * if best <= th_low and best < (float)second * ratio, else -1; dist[i] = best. */
+int orb_match_bf(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* train, int nt, int th_low,
+                 float ratio, int32_t* idx, int32_t* dist);
+
+/* Frame::ComputeStereoMatches() (include/Frame.h:249, src/Frame.cc:831-1082)
+ * on keypoints/descriptors of a left and a right image, reading the two
+ * extractors' pyramids from their last calls.  bf = Camera.bf, b = bf/fx.
+ * uRight/depth receive nl floats each (-1 where unmatched). */
+int orb_stereo_match(orb_extractor* left, orb_extractor* right, const orb_keypoint* kl, const uint8_t* dl,
+                     int nl, const orb_keypoint* kr, const uint8_t* dr, int nr, float bf, float b, float* uRight,
+                     float* depth);
+
+/* The stereo Frame constructor's hot path (src/Frame.cc:146-169): extract
+ * both images in the same launches, then ComputeStereoMatches on the device
+ * with no host round trip. */
+int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* im_left, const uint8_t* im_right,
+                     int width, int height, size_t stride, float bf, float b, orb_keypoint* kl, uint8_t* dl,
+                     int* nl, orb_keypoint* kr, uint8_t* dr, int* nr, int capacity, float* uRight, float* depth);
+
+/* Debug accessors used by the parity tests.
+ * kind 0: pyramid level, 1: its 7x7 blur, 2: FAST arc strength map. */
+int orb_extractor_debug_image(orb_extractor* ex, int level, int kind, uint8_t* dst, size_t dst_stride);
+/* FAST candidates of a level as (x, y, response) triples relative to the
+ * level's 16-px border, sorted in the reference's vToDistributeKeys order. */
+int orb_extractor_debug_candidates(orb_extractor* ex, int level, int* xyr, int capacity, int* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

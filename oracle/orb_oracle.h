@@ -1,0 +1,115 @@
+/*
+ * orb_oracle.h -- CPU oracle for the ORB front-end hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the GPU product
+ * is compared against, and the CPU baseline that bench.py times.  Nothing in
+ * the product (orb_slam2_detailed_comments_amd/, include/) may link, load or
+ * call it.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg use it.
+ *
+ * It restates, function by function, ORB-SLAM2's
+ *   src/ORBextractor.cc  (pyramid, FAST per cell, DistributeOctTree, IC_Angle,
+ *                          rotated BRIEF, operator())
+ *   src/ORBmatcher.cc:2083-2103 (DescriptorDistance) and the best/second-best
+ *                          update of SearchForInitialization (:614-647)
+ *   src/Frame.cc:831-1082 (ComputeStereoMatches)
+ * together with a written restatement of the OpenCV 2.4/3.x primitives that
+ * path calls (resize INTER_LINEAR 8U, GaussianBlur 7x7 sigma 2 8U, FAST-9 with
+ * non-max suppression, fastAtan2, cvRound).  OpenCV itself is not available,
+ * so for those primitives the oracle follows OpenCV's generic (scalar) code
+ * path; the header comment of each function says which.
+ */
+#ifndef ORB_ORACLE_H
+#define ORB_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same layout as cv::KeyPoint (28 bytes). */
+typedef struct oracle_keypoint {
+    float x, y, size, angle, response;
+    int octave, class_id;
+} oracle_keypoint;
+
+typedef struct oracle_extractor oracle_extractor;
+
+/* ORBextractor::ORBextractor (src/ORBextractor.cc:469-571). */
+oracle_extractor* oracle_extractor_create(int nfeatures, float scaleFactor, int nlevels,
+                                          int iniThFAST, int minThFAST);
+void oracle_extractor_destroy(oracle_extractor* ex);
+
+/* Scale tables and per-level feature quotas computed by the constructor. */
+int oracle_extractor_tables(const oracle_extractor* ex, float* scale, float* inv_scale,
+                            float* sigma2, float* inv_sigma2, int* quota, int* umax16);
+
+/* Descriptor sampling variant, for counting rounding-boundary straddles:
+ *   0  canonical: cos/sin rounded from a double evaluation, x*b+y*a unfused
+ *   1  libm cosf/sinf instead of the canonical cos/sin
+ *   2  fused multiply-add fma(x,b,y*a) / fma(x,a,-(y*b))
+ *   3  fused multiply-add fma(y,a,x*b) / fma(-y,b,x*a)                      */
+int oracle_set_descriptor_variant(oracle_extractor* ex, int variant);
+
+/* ORBextractor::operator() (src/ORBextractor.cc:1533-1649).
+ * Returns 0 on success; *n receives the keypoint count (<= cap). */
+int oracle_extract(oracle_extractor* ex, const uint8_t* img, int w, int h, int stride,
+                   oracle_keypoint* kps, int cap, uint8_t* desc, int* n);
+
+/* Intermediate results of the last oracle_extract call. */
+int oracle_level_size(const oracle_extractor* ex, int level, int* w, int* h);
+/* kind 0: pyramid level; kind 1: its 7x7 blur (computed for every level). */
+int oracle_level_image(oracle_extractor* ex, int level, int kind, uint8_t* dst);
+/* FAST candidates handed to DistributeOctTree for a level, in order:
+ * triples (x, y, response) in the border-relative frame of :1135-1136. */
+int oracle_level_candidates(const oracle_extractor* ex, int level, int* xyr, int cap, int* n);
+
+/* Stand-alone primitives. */
+int oracle_resize_linear(const uint8_t* src, int sw, int sh, int sstride,
+                         uint8_t* dst, int dw, int dh);
+int oracle_gaussian7(const uint8_t* src, int w, int h, int stride, uint8_t* dst);
+int oracle_gaussian7_kernel(int* k7);
+/* cv::FAST(window, kps, threshold, nonmax=true) on a w x h window: triples
+ * (x, y, score) in OpenCV's output order. */
+int oracle_fast(const uint8_t* img, int w, int h, int stride, int threshold,
+                int* xyr, int cap, int* n);
+float oracle_fast_atan2(float y, float x);
+float oracle_ic_angle(const uint8_t* img, int stride, int cx, int cy);
+void oracle_cos_sin(float rad, float* c, float* s);
+
+/* ORBmatcher::DescriptorDistance (src/ORBmatcher.cc:2083-2103). */
+int oracle_descriptor_distance(const uint8_t* a, const uint8_t* b);
+
+/* Brute-force best/second-best matching with the update rule of
+ * ORBmatcher::SearchForInitialization (src/ORBmatcher.cc:614-647): accept iff
+ * best <= th_low and best < (float)second * ratio.  idx = -1 when rejected. */
+int oracle_match_bf(const uint8_t* q, int nq, const uint8_t* t, int nt, int th_low,
+                    float ratio, int32_t* idx, int32_t* dist, int nthreads);
+
+/* k=2 nearest neighbours by (distance, index); best/second distances and
+ * indices, plus accept = best < ratio * second. */
+int oracle_knn2(const uint8_t* q, int nq, const uint8_t* m, int64_t nm, float ratio,
+                int32_t* idx2, int32_t* dist2, uint8_t* accept, int nthreads);
+
+/* Frame::ComputeStereoMatches (src/Frame.cc:831-1082) using the pyramids of
+ * the two extractors' last calls.  uRight/depth are -1 where unmatched. */
+int oracle_stereo_match(oracle_extractor* left, oracle_extractor* right,
+                        const oracle_keypoint* kl, const uint8_t* dl, int nl,
+                        const oracle_keypoint* kr, const uint8_t* dr, int nr,
+                        float bf, float b, float* uRight, float* depth);
+
+/* One stereo frame the way Frame's stereo constructor runs it
+ * (src/Frame.cc:146-169): left and right extraction on two threads, join,
+ * then ComputeStereoMatches on the calling thread. */
+int oracle_stereo_frame(oracle_extractor* left, oracle_extractor* right,
+                        const uint8_t* imL, const uint8_t* imR, int w, int h,
+                        float bf, float b, oracle_keypoint* kl, uint8_t* dl, int* nl,
+                        oracle_keypoint* kr, uint8_t* dr, int* nr, int cap,
+                        float* uRight, float* depth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,229 @@
+"""ctypes front end of the CPU oracle (liborb_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the timed CPU baseline.  The
+product package never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborb_oracle.so")
+_lib = None
+
+KEYPOINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"),
+                           ("response", "<f4"), ("octave", "<i4"), ("class_id", "<i4")])
+
+
+def build():
+    """Compile liborb_oracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P, I, F, V = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, None
+        sig = {
+            "oracle_extractor_create": (P, [I, F, I, I, I]),
+            "oracle_extractor_destroy": (V, [P]),
+            "oracle_extractor_tables": (I, [P, P, P, P, P, P, P]),
+            "oracle_set_descriptor_variant": (I, [P, I]),
+            "oracle_extract": (I, [P, P, I, I, I, P, I, P, P]),
+            "oracle_level_size": (I, [P, I, P, P]),
+            "oracle_level_image": (I, [P, I, I, P]),
+            "oracle_level_candidates": (I, [P, I, P, I, P]),
+            "oracle_resize_linear": (I, [P, I, I, I, P, I, I]),
+            "oracle_gaussian7": (I, [P, I, I, I, P]),
+            "oracle_gaussian7_kernel": (I, [P]),
+            "oracle_fast": (I, [P, I, I, I, I, P, I, P]),
+            "oracle_fast_atan2": (F, [F, F]),
+            "oracle_ic_angle": (F, [P, I, I, I]),
+            "oracle_cos_sin": (V, [F, P, P]),
+            "oracle_descriptor_distance": (I, [P, P]),
+            "oracle_match_bf": (I, [P, I, P, I, I, F, P, P, I]),
+            "oracle_knn2": (I, [P, I, P, ctypes.c_int64, F, P, P, P, I]),
+            "oracle_stereo_match": (I, [P, P, P, P, I, P, P, I, F, F, P, P]),
+            "oracle_stereo_frame": (I, [P, P, P, P, I, I, F, F, P, P, P, P, P, P, I, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class OracleExtractor:
+    """ORBextractor(nfeatures, scaleFactor, nlevels, iniThFAST, minThFAST) on the CPU."""
+
+    def __init__(self, nfeatures=1000, scaleFactor=1.2, nlevels=8, iniThFAST=20, minThFAST=7):
+        self.nfeatures, self.nlevels = nfeatures, nlevels
+        self.h = lib().oracle_extractor_create(nfeatures, scaleFactor, nlevels, iniThFAST, minThFAST)
+        if not self.h:
+            raise ValueError("invalid extractor parameters")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.oracle_extractor_destroy(self.h)
+            self.h = None
+
+    def tables(self):
+        n = self.nlevels
+        s, inv, s2, inv2 = (np.zeros(n, np.float32) for _ in range(4))
+        quota = np.zeros(n, np.int32)
+        umax = np.zeros(16, np.int32)
+        lib().oracle_extractor_tables(self.h, _p(s), _p(inv), _p(s2), _p(inv2), _p(quota), _p(umax))
+        return dict(scale=s, inv_scale=inv, sigma2=s2, inv_sigma2=inv2, quota=quota, umax=umax)
+
+    def set_variant(self, v):
+        if lib().oracle_set_descriptor_variant(self.h, v) != 0:
+            raise ValueError(v)
+
+    def capacity(self):
+        return self.nfeatures + 3 * self.nlevels + 64
+
+    def __call__(self, img):
+        img = np.ascontiguousarray(img, np.uint8)
+        cap = self.capacity()
+        kps = np.zeros(cap, KEYPOINT_DTYPE)
+        desc = np.zeros((cap, 32), np.uint8)
+        n = ctypes.c_int(0)
+        rc = lib().oracle_extract(self.h, _p(img), img.shape[1], img.shape[0], img.shape[1], _p(kps), cap,
+                                  _p(desc), ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"oracle_extract failed: {rc}")
+        return kps[:n.value].copy(), desc[:n.value].copy()
+
+    def level_size(self, level):
+        w, h = ctypes.c_int(), ctypes.c_int()
+        lib().oracle_level_size(self.h, level, ctypes.byref(w), ctypes.byref(h))
+        return w.value, h.value
+
+    def level_image(self, level, blurred=False):
+        w, h = self.level_size(level)
+        out = np.zeros((h, w), np.uint8)
+        lib().oracle_level_image(self.h, level, 1 if blurred else 0, _p(out))
+        return out
+
+    def candidates(self, level):
+        cap = 1 << 20
+        buf = np.zeros((cap, 3), np.int32)
+        n = ctypes.c_int()
+        rc = lib().oracle_level_candidates(self.h, level, _p(buf), cap, ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError(rc)
+        return buf[:n.value].copy()
+
+
+def resize_linear(src, dw, dh):
+    src = np.ascontiguousarray(src, np.uint8)
+    out = np.zeros((dh, dw), np.uint8)
+    lib().oracle_resize_linear(_p(src), src.shape[1], src.shape[0], src.shape[1], _p(out), dw, dh)
+    return out
+
+
+def gaussian7(src):
+    src = np.ascontiguousarray(src, np.uint8)
+    out = np.zeros_like(src)
+    if lib().oracle_gaussian7(_p(src), src.shape[1], src.shape[0], src.shape[1], _p(out)) != 0:
+        raise ValueError("image too small")
+    return out
+
+
+def gaussian7_kernel():
+    k = np.zeros(7, np.int32)
+    lib().oracle_gaussian7_kernel(_p(k))
+    return k
+
+
+def fast(img, threshold):
+    img = np.ascontiguousarray(img, np.uint8)
+    cap = img.size
+    buf = np.zeros((cap, 3), np.int32)
+    n = ctypes.c_int()
+    lib().oracle_fast(_p(img), img.shape[1], img.shape[0], img.shape[1], threshold, _p(buf), cap, ctypes.byref(n))
+    return buf[:n.value].copy()
+
+
+def fast_atan2(y, x):
+    return lib().oracle_fast_atan2(y, x)
+
+
+def ic_angle(img, cx, cy):
+    img = np.ascontiguousarray(img, np.uint8)
+    return lib().oracle_ic_angle(_p(img), img.shape[1], cx, cy)
+
+
+def cos_sin(rad):
+    c, s = ctypes.c_float(), ctypes.c_float()
+    lib().oracle_cos_sin(rad, ctypes.byref(c), ctypes.byref(s))
+    return c.value, s.value
+
+
+def descriptor_distance(a, b):
+    a = np.ascontiguousarray(a, np.uint8)
+    b = np.ascontiguousarray(b, np.uint8)
+    return lib().oracle_descriptor_distance(_p(a), _p(b))
+
+
+def match_bf(q, t, th_low=50, ratio=0.6, nthreads=1):
+    q = np.ascontiguousarray(q, np.uint8)
+    t = np.ascontiguousarray(t, np.uint8)
+    idx = np.zeros(len(q), np.int32)
+    dist = np.zeros(len(q), np.int32)
+    lib().oracle_match_bf(_p(q), len(q), _p(t), len(t), th_low, ratio, _p(idx), _p(dist), nthreads)
+    return idx, dist
+
+
+def knn2(q, m, ratio=0.7, nthreads=1):
+    q = np.ascontiguousarray(q, np.uint8)
+    m = np.ascontiguousarray(m, np.uint8)
+    idx = np.zeros((len(q), 2), np.int32)
+    dist = np.zeros((len(q), 2), np.int32)
+    acc = np.zeros(len(q), np.uint8)
+    lib().oracle_knn2(_p(q), len(q), _p(m), len(m), ratio, _p(idx), _p(dist), _p(acc), nthreads)
+    return idx, dist, acc.astype(bool)
+
+
+def stereo_match(left, right, kl, dl, kr, dr, bf, b):
+    kl = np.ascontiguousarray(kl)
+    kr = np.ascontiguousarray(kr)
+    dl = np.ascontiguousarray(dl, np.uint8)
+    dr = np.ascontiguousarray(dr, np.uint8)
+    uR = np.zeros(len(kl), np.float32)
+    dep = np.zeros(len(kl), np.float32)
+    lib().oracle_stereo_match(left.h, right.h, _p(kl), _p(dl), len(kl), _p(kr), _p(dr), len(kr), bf, b,
+                              _p(uR), _p(dep))
+    return uR, dep
+
+
+def stereo_frame(left, right, imL, imR, bf, b):
+    """Frame-constructor order: L/R extraction on two threads, then stereo."""
+    imL = np.ascontiguousarray(imL, np.uint8)
+    imR = np.ascontiguousarray(imR, np.uint8)
+    cap = left.capacity()
+    kl = np.zeros(cap, KEYPOINT_DTYPE)
+    kr = np.zeros(cap, KEYPOINT_DTYPE)
+    dl = np.zeros((cap, 32), np.uint8)
+    dr = np.zeros((cap, 32), np.uint8)
+    nl, nr = ctypes.c_int(), ctypes.c_int()
+    uR = np.zeros(cap, np.float32)
+    dep = np.zeros(cap, np.float32)
+    rc = lib().oracle_stereo_frame(left.h, right.h, _p(imL), _p(imR), imL.shape[1], imL.shape[0], bf, b,
+                                   _p(kl), _p(dl), ctypes.byref(nl), _p(kr), _p(dr), ctypes.byref(nr), cap,
+                                   _p(uR), _p(dep))
+    if rc != 0:
+        raise RuntimeError(rc)
+    n = nl.value
+    return kl[:n], dl[:n], kr[:nr.value], dr[:nr.value], uR[:n], dep[:n]

@@ -1,0 +1,298 @@
+"""MI355X-native ORB front end for ORB-SLAM2's data-parallel hot path.
+
+Python mirror of the reference interface over the C ABI in include/orb_gpu.h
+(liborb_gpu.so, built in-tree by build.py):
+
+* ``ORBextractor(nfeatures, scaleFactor, nlevels, iniThFAST, minThFAST)``
+  with ``__call__(image, mask=None) -> (keypoints, descriptors)``, the
+  ``Get*`` accessors and ``mvImagePyramid``  (include/ORBextractor.h:93-162)
+* ``ORBmatcher.DescriptorDistance(a, b)``     (include/ORBmatcher.h:65)
+* ``match_bf``                                (best/second rule of
+  ORBmatcher::SearchForInitialization, src/ORBmatcher.cc:614-647)
+* ``compute_stereo_matches`` / ``stereo_frame`` (Frame::ComputeStereoMatches,
+  src/Frame.cc:831-1082, and the stereo Frame constructor)
+
+Keypoints are numpy structured arrays with cv::KeyPoint's fields.  There is no
+CPU fallback: if the native library is missing every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "match_bf", "compute_stereo_matches",
+           "stereo_frame", "lib", "library_path", "OrbError"]
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_PKG, "liborb_gpu.so")
+
+KEYPOINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"),
+                           ("response", "<f4"), ("octave", "<i4"), ("class_id", "<i4")])
+
+ORB_OK, ORB_E_ARG, ORB_E_CAPACITY, ORB_E_DEVICE, ORB_E_OOM, ORB_E_OVERFLOW, ORB_E_ADMISSION = 0, -1, -2, -3, -4, -5, -6
+_ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "HIP device error",
+           -4: "device allocation failed", -5: "internal bound exceeded", -6: "refused by memory budget"}
+
+# Every symbol the C ABI exports (include/orb_gpu.h), with its ctypes signature.
+_P, _I, _F, _S, _V = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t, None
+SIGNATURES = {
+    "orb_context_create": (_I, [_I, _P]),
+    "orb_context_destroy": (_V, [_P]),
+    "orb_context_synchronize": (_I, [_P]),
+    "orb_extractor_create": (_I, [_P, _I, _F, _I, _I, _I, _P]),
+    "orb_extractor_destroy": (_V, [_P]),
+    "orb_extractor_levels": (_I, [_P]),
+    "orb_extractor_scale_factor": (_F, [_P]),
+    "orb_extractor_scales": (_I, [_P, _P, _P, _P, _P]),
+    "orb_extractor_quotas": (_I, [_P, _P]),
+    "orb_extractor_capacity": (_I, [_P, _I, _I]),
+    "orb_extract": (_I, [_P, _P, _I, _I, _S, _P, _I, _P, _P]),
+    "orb_extract_device": (_I, [_P, _P, _I, _I, _S]),
+    "orb_extractor_results": (_I, [_P, _P, _I, _P, _P]),
+    "orb_extractor_pyramid_level": (_I, [_P, _I, _P, _S, _P, _P]),
+    "orb_descriptor_distance": (_I, [_P, _P]),
+    "orb_match_bf": (_I, [_P, _P, _I, _P, _I, _I, _F, _P, _P]),
+    "orb_stereo_match": (_I, [_P, _P, _P, _P, _I, _P, _P, _I, _F, _F, _P, _P]),
+    "orb_stereo_frame": (_I, [_P, _P, _P, _P, _I, _I, _S, _F, _F, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
+    "orb_extractor_debug_image": (_I, [_P, _I, _I, _P, _S]),
+    "orb_extractor_debug_candidates": (_I, [_P, _I, _P, _I, _P]),
+}
+
+_lib = None
+
+
+class OrbError(RuntimeError):
+    def __init__(self, fn, rc):
+        super().__init__(f"{fn} failed: {_ERRORS.get(rc, rc)} ({rc})")
+        self.rc = rc
+
+
+def lib():
+    """Load liborb_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise OSError(f"{library_path} is missing: run orb_slam2_detailed_comments_amd/build.py "
+                          "(or __graft_entry__.build())")
+        L = ctypes.CDLL(library_path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(fn, rc):
+    if rc != ORB_OK:
+        raise OrbError(fn, rc)
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """A HIP device and the stream the front end runs on."""
+
+    _default = None
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check("orb_context_create", lib().orb_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    @classmethod
+    def default(cls):
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    def synchronize(self):
+        _check("orb_context_synchronize", lib().orb_context_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orb_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class ORBextractor:
+    """ORBextractor(nfeatures, scaleFactor, nlevels, iniThFAST, minThFAST).
+
+    Mirrors include/ORBextractor.h:93-162; the pixel work runs on the GPU.
+    """
+
+    def __init__(self, nfeatures, scaleFactor, nlevels, iniThFAST, minThFAST, context=None):
+        self.ctx = context or Context.default()
+        h = ctypes.c_void_p()
+        _check("orb_extractor_create", lib().orb_extractor_create(self.ctx.h, int(nfeatures), float(scaleFactor),
+                                                                  int(nlevels), int(iniThFAST), int(minThFAST),
+                                                                  ctypes.byref(h)))
+        self.h = h
+        self.nfeatures, self.nlevels = int(nfeatures), int(nlevels)
+        self._size = None
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orb_extractor_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- getters (ORBextractor.h:119-159)
+    def GetLevels(self):
+        return lib().orb_extractor_levels(self.h)
+
+    def GetScaleFactor(self):
+        return lib().orb_extractor_scale_factor(self.h)
+
+    def _scales(self):
+        n = self.nlevels
+        out = [np.zeros(n, np.float32) for _ in range(4)]
+        _check("orb_extractor_scales", lib().orb_extractor_scales(self.h, *[_ptr(a) for a in out]))
+        return out
+
+    def GetScaleFactors(self):
+        return self._scales()[0]
+
+    def GetInverseScaleFactors(self):
+        return self._scales()[1]
+
+    def GetScaleSigmaSquares(self):
+        return self._scales()[2]
+
+    def GetInverseScaleSigmaSquares(self):
+        return self._scales()[3]
+
+    def quotas(self):
+        q = np.zeros(self.nlevels, np.int32)
+        _check("orb_extractor_quotas", lib().orb_extractor_quotas(self.h, _ptr(q)))
+        return q
+
+    def capacity(self, width, height):
+        rc = lib().orb_extractor_capacity(self.h, int(width), int(height))
+        if rc < 0:
+            raise OrbError("orb_extractor_capacity", rc)
+        return rc
+
+    # -- operator() (ORBextractor.h:111)
+    def __call__(self, image, mask=None):
+        """Returns (keypoints[N] KEYPOINT_DTYPE, descriptors[N, 32] uint8)."""
+        img = np.asarray(image)
+        if img.dtype != np.uint8 or img.ndim != 2:
+            raise ValueError("image must be a 2-D uint8 (CV_8UC1) array")
+        if img.size == 0:
+            return np.zeros(0, KEYPOINT_DTYPE), np.zeros((0, 32), np.uint8)
+        if img.strides[1] != 1:
+            img = np.ascontiguousarray(img)
+        h, w = img.shape
+        cap = self.capacity(w, h)
+        kps = np.zeros(cap, KEYPOINT_DTYPE)
+        desc = np.zeros((cap, 32), np.uint8)
+        n = ctypes.c_int()
+        _check("orb_extract", lib().orb_extract(self.h, _ptr(img), w, h, img.strides[0], _ptr(kps), cap,
+                                                _ptr(desc), ctypes.byref(n)))
+        self._size = (w, h)
+        return kps[:n.value].copy(), desc[:n.value].copy()
+
+    def pyramid_level(self, level):
+        w, h = ctypes.c_int(), ctypes.c_int()
+        _check("orb_extractor_pyramid_level",
+               lib().orb_extractor_pyramid_level(self.h, level, None, 0, ctypes.byref(w), ctypes.byref(h)))
+        out = np.zeros((h.value, w.value), np.uint8)
+        _check("orb_extractor_pyramid_level",
+               lib().orb_extractor_pyramid_level(self.h, level, _ptr(out), w.value, None, None))
+        return out
+
+    @property
+    def mvImagePyramid(self):
+        return [self.pyramid_level(l) for l in range(self.nlevels)]
+
+    def debug_image(self, level, kind):
+        w, h = ctypes.c_int(), ctypes.c_int()
+        _check("orb_extractor_pyramid_level",
+               lib().orb_extractor_pyramid_level(self.h, level, None, 0, ctypes.byref(w), ctypes.byref(h)))
+        out = np.zeros((h.value, w.value), np.uint8)
+        _check("orb_extractor_debug_image", lib().orb_extractor_debug_image(self.h, level, kind, _ptr(out), w.value))
+        return out
+
+    def debug_candidates(self, level):
+        n = ctypes.c_int()
+        rc = lib().orb_extractor_debug_candidates(self.h, level, None, 0, ctypes.byref(n))
+        if rc not in (ORB_OK, ORB_E_CAPACITY):
+            raise OrbError("orb_extractor_debug_candidates", rc)
+        buf = np.zeros((max(n.value, 1), 3), np.int32)
+        _check("orb_extractor_debug_candidates",
+               lib().orb_extractor_debug_candidates(self.h, level, _ptr(buf), n.value, ctypes.byref(n)))
+        return buf[:n.value].copy()
+
+
+class ORBmatcher:
+    """ORBmatcher's thresholds (src/ORBmatcher.cc:49-51) and distance."""
+    TH_HIGH = 100
+    TH_LOW = 50
+    HISTO_LENGTH = 30
+
+    @staticmethod
+    def DescriptorDistance(a, b):
+        a = np.ascontiguousarray(a, np.uint8).reshape(-1)
+        b = np.ascontiguousarray(b, np.uint8).reshape(-1)
+        if a.size < 32 or b.size < 32:
+            raise ValueError("descriptors are 32-byte rows")
+        return lib().orb_descriptor_distance(_ptr(a), _ptr(b))
+
+
+def match_bf(query, train, th_low=ORBmatcher.TH_LOW, ratio=0.6, context=None):
+    """Brute-force best/second-best matching on the GPU: (idx[-1 if none], dist)."""
+    ctx = context or Context.default()
+    q = np.ascontiguousarray(query, np.uint8)
+    t = np.ascontiguousarray(train, np.uint8)
+    idx = np.zeros(len(q), np.int32)
+    dist = np.zeros(len(q), np.int32)
+    _check("orb_match_bf", lib().orb_match_bf(ctx.h, _ptr(q), len(q), _ptr(t), len(t), int(th_low), float(ratio),
+                                             _ptr(idx), _ptr(dist)))
+    return idx, dist
+
+
+def compute_stereo_matches(left, right, kl, dl, kr, dr, bf, b):
+    """Frame::ComputeStereoMatches on the GPU -> (mvuRight, mvDepth)."""
+    kl = np.ascontiguousarray(kl, KEYPOINT_DTYPE)
+    kr = np.ascontiguousarray(kr, KEYPOINT_DTYPE)
+    dl = np.ascontiguousarray(dl, np.uint8)
+    dr = np.ascontiguousarray(dr, np.uint8)
+    uR = np.zeros(len(kl), np.float32)
+    dep = np.zeros(len(kl), np.float32)
+    _check("orb_stereo_match", lib().orb_stereo_match(left.h, right.h, _ptr(kl), _ptr(dl), len(kl), _ptr(kr),
+                                                     _ptr(dr), len(kr), float(bf), float(b), _ptr(uR),
+                                                     _ptr(dep)))
+    return uR, dep
+
+
+def stereo_frame(left, right, im_left, im_right, bf, b):
+    """Both extractions plus stereo matching in one device pass.
+
+    Returns (kl, dl, kr, dr, mvuRight, mvDepth)."""
+    imL = np.ascontiguousarray(im_left, np.uint8)
+    imR = np.ascontiguousarray(im_right, np.uint8)
+    if imL.shape != imR.shape:
+        raise ValueError("left and right images must have the same size")
+    h, w = imL.shape
+    cap = max(left.capacity(w, h), right.capacity(w, h))
+    kl = np.zeros(cap, KEYPOINT_DTYPE)
+    kr = np.zeros(cap, KEYPOINT_DTYPE)
+    dl = np.zeros((cap, 32), np.uint8)
+    dr = np.zeros((cap, 32), np.uint8)
+    uR = np.zeros(cap, np.float32)
+    dep = np.zeros(cap, np.float32)
+    nl, nr = ctypes.c_int(), ctypes.c_int()
+    _check("orb_stereo_frame", lib().orb_stereo_frame(left.h, right.h, _ptr(imL), _ptr(imR), w, h, imL.strides[0],
+                                                     float(bf), float(b), _ptr(kl), _ptr(dl), ctypes.byref(nl),
+                                                     _ptr(kr), _ptr(dr), ctypes.byref(nr), cap, _ptr(uR),
+                                                     _ptr(dep)))
+    n = nl.value
+    return kl[:n].copy(), dl[:n].copy(), kr[:nr.value].copy(), dr[:nr.value].copy(), uR[:n].copy(), dep[:n].copy()

@@ -1,0 +1,1107 @@
+// orb_kernels.hip -- gfx950 kernels of the ORB front end.
+//
+// Pipeline per image (one launch per stage covers every job of a run):
+//   level_kernel   (x nlevels) pyramid level l from level l-1 (8-bit fixed
+//                  point bilinear, ORBextractor.cc:1677), its 7x7 sigma-2
+//                  blur (ORBextractor.cc:1607-1615) and the FAST arc
+//                  strength of every pixel, from one LDS tile with a 3-px halo
+//   cells_kernel   FAST-9 per 30-px cell with the iniThFAST/minThFAST retry
+//                  and 3x3 non-max suppression (ORBextractor.cc:1075-1142)
+//   octree_kernel  DistributeOctTree as a per-level workgroup quadtree
+//                  (ORBextractor.cc:688-1033)
+//   describe_kernel IC_Angle + rotated BRIEF + output assembly
+//                  (ORBextractor.cc:94-204, 1590-1648)
+// plus stereo matching (Frame.cc:831-1082) and brute-force Hamming matching.
+//
+// All float code is compiled with -ffp-contract=off: every float statement
+// that the reference evaluates is evaluated here with the same operations in
+// the same order, so results are bit-identical to the CPU statement.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <limits.h>
+
+#include "orb_types.h"
+
+__constant__ signed char c_pattern[1024] = {
+#include "orb_pattern.inc"
+};
+__constant__ int c_umax[16];   // orientation disc half-widths (ctor, :542-570)
+__constant__ int c_gauss[7];   // 7x7 sigma-2 integer taps (sum need not be 256)
+
+extern "C" hipError_t orb_upload_constants(const int* umax16, const int* gauss7)
+{
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_umax), umax16, 16 * sizeof(int));
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_gauss), gauss7, 7 * sizeof(int));
+}
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int reflect101(int i, int n)
+{
+    if (i < 0) i = -i;
+    if (i >= n) i = 2 * n - 2 - i;
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+__device__ __forceinline__ int find_segment(const int* starts, int n, int x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int lanes_below(unsigned long long mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// ================================================================ level
+// Extended tile: output tile plus 3 px on each side (blur radius and FAST
+// ring radius), with coordinates reflected (REFLECT_101) at the level edges.
+#define EXT_W (ORB_TILE_W + 6)
+#define EXT_H (ORB_TILE_H + 6)
+#define EXT_P 72
+
+__device__ __forceinline__ int fast_strength(const uint8_t (*t)[EXT_P], int cx, int cy)
+{
+    // Bresenham radius-3 circle in circular order (OpenCV makeOffsets(16)).
+    const int v = t[cy][cx];
+    int d[16];
+    d[0] = t[cy + 3][cx + 0] - v;  d[1] = t[cy + 3][cx + 1] - v;
+    d[2] = t[cy + 2][cx + 2] - v;  d[3] = t[cy + 1][cx + 3] - v;
+    d[4] = t[cy + 0][cx + 3] - v;  d[5] = t[cy - 1][cx + 3] - v;
+    d[6] = t[cy - 2][cx + 2] - v;  d[7] = t[cy - 3][cx + 1] - v;
+    d[8] = t[cy - 3][cx + 0] - v;  d[9] = t[cy - 3][cx - 1] - v;
+    d[10] = t[cy - 2][cx - 2] - v; d[11] = t[cy - 1][cx - 3] - v;
+    d[12] = t[cy + 0][cx - 3] - v; d[13] = t[cy + 1][cx - 3] - v;
+    d[14] = t[cy + 2][cx - 2] - v; d[15] = t[cy + 3][cx - 1] - v;
+    // Minimum and maximum over every run of 9 contiguous ring pixels.
+    int mn[16], mx[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        mn[k] = min(d[k], d[(k + 1) & 15]);
+        mx[k] = max(d[k], d[(k + 1) & 15]);
+    }
+    int mn4[16], mx4[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        mn4[k] = min(mn[k], mn[(k + 2) & 15]);
+        mx4[k] = max(mx[k], mx[(k + 2) & 15]);
+    }
+    int bright = -256, dark = 256;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int m9 = min(min(mn4[k], mn4[(k + 4) & 15]), d[(k + 8) & 15]);
+        const int x9 = max(max(mx4[k], mx4[(k + 4) & 15]), d[(k + 8) & 15]);
+        bright = max(bright, m9);
+        dark = min(dark, x9);
+    }
+    // A pixel is a FAST-9 corner at threshold t iff strength > t, and then
+    // OpenCV's cornerScore equals strength - 1.
+    const int s = max(bright, -dark);
+    return s < 0 ? 0 : s;
+}
+
+__global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__ jobs,
+                                                    const RunTable* __restrict__ rt, int level)
+{
+    __shared__ uint8_t tile[EXT_H][EXT_P];
+    __shared__ int hs[EXT_H][ORB_TILE_W];
+    __shared__ int s_job;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_job = find_segment(rt->tile_start[level], rt->njobs, blockIdx.x);
+    __syncthreads();
+    const ImageJob& J = jobs[s_job];
+    const LevelJob& L = J.lv[level];
+    const int t = blockIdx.x - rt->tile_start[level][s_job];
+    const int tx0 = (t % L.tiles_x) * ORB_TILE_W;
+    const int ty0 = (t / L.tiles_x) * ORB_TILE_H;
+    const int w = L.w, h = L.h, pitch = L.pitch;
+
+    if (level == 0) {
+        for (int e = tid; e < EXT_W * EXT_H; e += 256) {
+            const int ex = e % EXT_W, ey = e / EXT_W;
+            const int x = reflect101(tx0 - 3 + ex, w), y = reflect101(ty0 - 3 + ey, h);
+            tile[ey][ex] = L.lvl[(size_t)y * pitch + x];
+        }
+    } else {
+        // cv::resize INTER_LINEAR, 8U fixed point, from level-1 (see oracle
+        // spec): double source coordinate, float fraction, 11-bit taps.
+        const LevelJob& P = J.lv[level - 1];
+        const int sw = P.w, sh = P.h, sp = P.pitch;
+        const double scale_x = 1. / ((double)w / sw);
+        const double scale_y = 1. / ((double)h / sh);
+        for (int e = tid; e < EXT_W * EXT_H; e += 256) {
+            const int ex = e % EXT_W, ey = e / EXT_W;
+            const int x = reflect101(tx0 - 3 + ex, w), y = reflect101(ty0 - 3 + ey, h);
+            float fx = (float)((x + 0.5) * scale_x - 0.5);
+            int sx = (int)floorf(fx);
+            fx -= sx;
+            if (sx < 0) { fx = 0.f; sx = 0; }
+            if (sx >= sw - 1) { fx = 0.f; sx = sw - 1; }
+            const int a0 = (int)rintf((1.f - fx) * 2048.f), a1 = (int)rintf(fx * 2048.f);
+            float fy = (float)((y + 0.5) * scale_y - 0.5);
+            const int sy = (int)floorf(fy);
+            fy -= sy;
+            const int b0 = (int)rintf((1.f - fy) * 2048.f), b1 = (int)rintf(fy * 2048.f);
+            const int r0 = min(max(sy, 0), sh - 1), r1 = min(max(sy + 1, 0), sh - 1);
+            const int nx = sx + 1 < sw ? sx + 1 : sx;
+            const uint8_t* p0 = P.lvl + (size_t)r0 * sp;
+            const uint8_t* p1 = P.lvl + (size_t)r1 * sp;
+            const int S0 = p0[sx] * a0 + p0[nx] * a1;
+            const int S1 = p1[sx] * a0 + p1[nx] * a1;
+            const int v = (S0 * b0 + S1 * b1 + (1 << 21)) >> 22;
+            tile[ey][ex] = (uint8_t)(v > 255 ? 255 : (v < 0 ? 0 : v));
+        }
+    }
+    __syncthreads();
+
+    // Each thread owns 4 horizontally adjacent output pixels.
+    const int ly = tid >> 4, lx = (tid & 15) * 4;
+    const int y = ty0 + ly, x0 = tx0 + lx;
+    const bool row_in = y < h;
+
+    if (level > 0 && row_in) {
+        uint8_t* dst = L.lvl + (size_t)y * pitch + x0;
+        if (x0 + 3 < w) {
+            uchar4 v = make_uchar4(tile[ly + 3][lx + 3], tile[ly + 3][lx + 4], tile[ly + 3][lx + 5],
+                                   tile[ly + 3][lx + 6]);
+            *(uchar4*)dst = v;
+        } else {
+            for (int k = 0; k < 4; k++)
+                if (x0 + k < w) dst[k] = tile[ly + 3][lx + 3 + k];
+        }
+    }
+
+    // Horizontal blur pass over all extended rows.
+    for (int e = tid; e < EXT_H * ORB_TILE_W; e += 256) {
+        const int ey = e / ORB_TILE_W, ex = e % ORB_TILE_W;
+        int acc = 0;
+#pragma unroll
+        for (int k = 0; k < 7; k++) acc += c_gauss[k] * tile[ey][ex + k];
+        hs[ey][ex] = acc;
+    }
+    __syncthreads();
+
+    if (row_in) {
+        uint8_t bl[4], fm[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int acc = 0;
+#pragma unroll
+            for (int r = 0; r < 7; r++) acc += c_gauss[r] * hs[ly + r][lx + k];
+            const int v = (acc + (1 << 15)) >> 16;
+            bl[k] = (uint8_t)(v > 255 ? 255 : v);
+            const int x = x0 + k;
+            fm[k] = (x >= 3 && x < w - 3 && y >= 3 && y < h - 3)
+                        ? (uint8_t)fast_strength(tile, lx + 3 + k, ly + 3) : (uint8_t)0;
+        }
+        uint8_t* db = L.blur + (size_t)y * pitch + x0;
+        uint8_t* dm = L.fastm + (size_t)y * pitch + x0;
+        if (x0 + 3 < w) {
+            *(uchar4*)db = make_uchar4(bl[0], bl[1], bl[2], bl[3]);
+            *(uchar4*)dm = make_uchar4(fm[0], fm[1], fm[2], fm[3]);
+        } else {
+            for (int k = 0; k < 4; k++)
+                if (x0 + k < w) {
+                    db[k] = bl[k];
+                    dm[k] = fm[k];
+                }
+        }
+    }
+}
+
+// ================================================================ cells
+// One wavefront per 30-px cell.  The cell's detection area (window minus the
+// 3-px FAST margin) is staged in LDS with a zero ring, so that neighbours
+// outside the window count as score 0, exactly like cv::FAST on the window.
+#define CELL_P 64
+
+__device__ __forceinline__ bool cell_keep(const uint8_t* s, int cx, int cy, int t)
+{
+    const uint8_t* c = s + (cy + 1) * CELL_P + (cx + 1);
+    const int mc = c[0];
+    if (mc <= t) return false;
+    const int sc = mc - 1;
+    const int off[8] = {-CELL_P - 1, -CELL_P, -CELL_P + 1, -1, 1, CELL_P - 1, CELL_P, CELL_P + 1};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int mq = c[off[k]];
+        const int sq = mq > t ? mq - 1 : 0;
+        if (sq >= sc) return false;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void cells_kernel(const ImageJob* __restrict__ jobs,
+                                                    const RunTable* __restrict__ rt)
+{
+    __shared__ uint8_t sm[4][CELL_P * CELL_P];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nseg = rt->njobs * rt->max_levels;
+    const int cell = blockIdx.x * 4 + wv;
+    const bool valid = cell < rt->cell_start[nseg];
+    int f = 0;
+    if (valid) f = find_segment(rt->cell_start, nseg, cell);
+    const int jb = f / rt->max_levels, level = f % rt->max_levels;
+    const ImageJob& J = jobs[jb];
+    const LevelJob& L = J.lv[level];
+    int aw = 0, ah = 0, iniX = 0, iniY = 0, ci = 0, cj = 0;
+    if (valid) {
+        const int local = cell - rt->cell_start[f];
+        ci = local / L.nCols;
+        cj = local % L.nCols;
+        iniY = ORB_MIN_BORDER + ci * L.hCell;
+        iniX = ORB_MIN_BORDER + cj * L.wCell;
+        const int maxY = min(iniY + L.hCell + 6, L.maxBorderY);
+        const int maxX = min(iniX + L.wCell + 6, L.maxBorderX);
+        // Cells skipped by ComputeKeyPointsOctTree (:1083, :1101).
+        if (iniY < L.maxBorderY - 3 && iniX < L.maxBorderX - 6) {
+            aw = max(maxX - iniX - 6, 0);
+            ah = max(maxY - iniY - 6, 0);
+        }
+    }
+    uint8_t* s = sm[wv];
+    // Column `lane` of the staged area for every staged row.
+    for (int ry = 0; ry < ah + 2; ry++) {
+        uint8_t v = 0;
+        if (lane >= 1 && lane <= aw && ry >= 1 && ry <= ah)
+            v = L.fastm[(size_t)(iniY + 2 + ry) * L.pitch + iniX + 2 + lane];
+        s[ry * CELL_P + lane] = v;
+    }
+    __syncthreads();
+
+    int t = J.iniTh < 0 ? 0 : (J.iniTh > 255 ? 255 : J.iniTh);
+    int n = 0;
+    if (lane < aw)
+        for (int r = 0; r < ah; r++) n += cell_keep(s, lane, r, t);
+    n = wave_sum(n);
+    if (n == 0) {
+        // No corner survived at iniThFAST: redo the cell at minThFAST.
+        t = J.minTh < 0 ? 0 : (J.minTh > 255 ? 255 : J.minTh);
+        if (lane < aw)
+            for (int r = 0; r < ah; r++) n += cell_keep(s, lane, r, t);
+        n = wave_sum(n);
+    }
+    if (n == 0) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&J.counters[ORB_CNT_KEYS + level], n);
+    base = __shfl(base, 0, 64);
+    if (base + n > L.key_cap && lane == 0) atomicOr(&J.counters[ORB_CNT_ERR], ORB_ERR_KEYS);
+    int off = 0;
+    for (int r = 0; r < ah; r++) {
+        const bool k = lane < aw && cell_keep(s, lane, r, t);
+        const unsigned long long m = __ballot(k);
+        if (k) {
+            const int pos = base + off + lanes_below(m);
+            const int xl = lane + 3, yl = r + 3;   // FAST window coordinates
+            const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | ((unsigned)yl << 6) | (unsigned)xl;
+            const unsigned resp = (unsigned)s[(r + 1) * CELL_P + lane + 1] - 1u;
+            if (pos < L.key_cap) L.keys[pos] = ((unsigned long long)resp << 32) | ok;
+        }
+        off += __popcll(m);
+    }
+}
+
+// ================================================================ octree
+// DistributeOctTree (ORBextractor.cc:688-1033) on one workgroup per level.
+//
+// The reference keeps a std::list of nodes.  Phase 1 repeats full passes,
+// splitting every node with more than one key and pushing its non-empty
+// children to the list front (n1..n4, so they read n4,n3,n2,n1); phase 2
+// sorts the expandable nodes by (size, address) and splits from the largest
+// until the quota is reached.  Here the list is an array in LDS that is
+// rebuilt each pass with prefix sums; each candidate carries the index of the
+// list node that holds it.  Node order, membership and retained keys are
+// exactly the list's.  Ties in the phase-2 sort are broken by creation order
+// (newer = larger), the oracle's fixed stand-in for the heap address.
+#define OCT_T 1024
+
+struct OctShared {
+    int rx[2][ORB_LIST_CAP];     // x0 | x1 << 16
+    int ry[2][ORB_LIST_CAP];     // y0 | y1 << 16
+    int cnt[2][ORB_LIST_CAP];
+    int seq[2][ORB_LIST_CAP];
+    int cc[ORB_LIST_CAP][4];
+    short map[ORB_LIST_CAP][4];
+    short keep[ORB_LIST_CAP];
+    unsigned long long sortk[ORB_LIST_CAP];
+    unsigned long long best[ORB_LIST_CAP];
+    int scan[OCT_T / 64 + 1];
+    int vars[8];
+};
+
+__device__ __forceinline__ int block_scan(int v, int* sh, int* total)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        const int wsum = lane < OCT_T / 64 ? sh[lane] : 0;
+        int s = wsum;
+#pragma unroll
+        for (int o = 1; o < OCT_T / 64; o <<= 1) {
+            const int y = __shfl_up(s, o, 64);
+            if (lane >= o) s += y;
+        }
+        if (lane < OCT_T / 64) sh[lane] = s - wsum;
+        if (lane == OCT_T / 64 - 1) sh[OCT_T / 64] = s;
+    }
+    __syncthreads();
+    const int r = x - v + sh[wid];
+    *total = sh[OCT_T / 64];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ void key_xy(unsigned long long key, int wCell, int hCell, int* x, int* y)
+{
+    const unsigned ok = (unsigned)key;
+    const int xl = ok & 63, yl = (ok >> 6) & 63, j = (ok >> 12) & 255, i = (ok >> 20) & 255;
+    *x = j * wCell + xl;
+    *y = i * hCell + yl;
+}
+
+__device__ __forceinline__ int quad_of(int rx, int ry, int x, int y)
+{
+    const int x0 = rx & 0xFFFF, x1 = rx >> 16, y0 = ry & 0xFFFF, y1 = ry >> 16;
+    const int mx = x0 + ((x1 - x0 + 1) >> 1), my = y0 + ((y1 - y0 + 1) >> 1);
+    return (x < mx ? 0 : 1) + (y < my ? 0 : 2);
+}
+
+__device__ __forceinline__ void child_rect(int rx, int ry, int q, int* crx, int* cry)
+{
+    const int x0 = rx & 0xFFFF, x1 = rx >> 16, y0 = ry & 0xFFFF, y1 = ry >> 16;
+    const int mx = x0 + ((x1 - x0 + 1) >> 1), my = y0 + ((y1 - y0 + 1) >> 1);
+    const int cx0 = (q & 1) ? mx : x0, cx1 = (q & 1) ? x1 : mx;
+    const int cy0 = (q & 2) ? my : y0, cy1 = (q & 2) ? y1 : my;
+    *crx = cx0 | (cx1 << 16);
+    *cry = cy0 | (cy1 << 16);
+}
+
+__global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restrict__ jobs, int max_levels)
+{
+    __shared__ OctShared S;
+    const int tid = threadIdx.x;
+    const int jb = blockIdx.x / max_levels, level = blockIdx.x % max_levels;
+    const ImageJob& J = jobs[jb];
+    if (level >= J.nlevels) return;
+    const LevelJob& L = J.lv[level];
+    int* ctr = J.counters;
+    const int K = min(ctr[ORB_CNT_KEYS + level], L.key_cap);
+    const int N = L.quota;
+    const int wC = L.wCell, hC = L.hCell;
+    const unsigned long long* keys = L.keys;
+    int* node = L.node;
+    int total;
+
+    // ---- roots (ORBextractor.cc:695-763)
+    if (tid < ORB_MAX_ROOTS) S.cc[tid][0] = 0;
+    __syncthreads();
+    for (int k = tid; k < K; k += OCT_T) {
+        int x, y;
+        key_xy(keys[k], wC, hC, &x, &y);
+        int r = (int)__fdiv_rn((float)x, L.hX);
+        r = min(r, L.nIni - 1);
+        node[k] = r;
+        atomicAdd(&S.cc[r][0], 1);
+    }
+    __syncthreads();
+    {
+        const int c = tid < L.nIni ? S.cc[tid][0] : 0;
+        const int pos = block_scan(c > 0 ? 1 : 0, S.scan, &total);
+        if (tid < L.nIni && c > 0) {
+            S.rx[0][pos] = L.root_x[tid] | (L.root_x[tid + 1] << 16);
+            S.ry[0][pos] = 0 | ((L.maxBorderY - ORB_MIN_BORDER) << 16);
+            S.cnt[0][pos] = c;
+            S.seq[0][pos] = tid;
+        }
+        if (tid < L.nIni) S.keep[tid] = (short)pos;
+    }
+    __syncthreads();
+    for (int k = tid; k < K; k += OCT_T) node[k] = S.keep[node[k]];
+    int cur = 0;
+    int size = total;
+    bool phase2 = false;
+    __syncthreads();
+
+    // ---- phase 1: full passes (ORBextractor.cc:779-893)
+    while (true) {
+        const int prev = size;
+        for (int i = tid; i < size; i += OCT_T) S.cc[i][0] = S.cc[i][1] = S.cc[i][2] = S.cc[i][3] = 0;
+        __syncthreads();
+        for (int k = tid; k < K; k += OCT_T) {
+            const int i = node[k];
+            if (S.cnt[cur][i] >= 2) {
+                int x, y;
+                key_xy(keys[k], wC, hC, &x, &y);
+                atomicAdd(&S.cc[i][quad_of(S.rx[cur][i], S.ry[cur][i], x, y)], 1);
+            }
+        }
+        __syncthreads();
+        const int i = tid;
+        const bool split = i < size && S.cnt[cur][i] >= 2;
+        int nc = 0;
+        if (split) nc = (S.cc[i][0] > 0) + (S.cc[i][1] > 0) + (S.cc[i][2] > 0) + (S.cc[i][3] > 0);
+        const int kp = (i < size && !split) ? 1 : 0;
+        int C, nkeep;
+        const int ex_nc = block_scan(nc, S.scan, &C);
+        const int ex_kp = block_scan(kp, S.scan, &nkeep);
+        const int nsize = C + nkeep;
+        int nexp = 0;
+        if (nsize <= ORB_LIST_CAP) {
+            const int nb = cur ^ 1;
+            if (split) {
+                const int base = C - ex_nc - nc;
+                int r = 0;
+                for (int q = 3; q >= 0; q--) {
+                    const int c = S.cc[i][q];
+                    if (c > 0) {
+                        const int pos = base + r++;
+                        int crx, cry;
+                        child_rect(S.rx[cur][i], S.ry[cur][i], q, &crx, &cry);
+                        S.rx[nb][pos] = crx;
+                        S.ry[nb][pos] = cry;
+                        S.cnt[nb][pos] = c;
+                        S.seq[nb][pos] = ex_nc + (nc - r);   // creation order n1..n4
+                        S.map[i][q] = (short)pos;
+                        nexp += c >= 2;
+                    } else {
+                        S.map[i][q] = -1;
+                    }
+                }
+            } else if (kp) {
+                const int pos = C + ex_kp;
+                S.rx[nb][pos] = S.rx[cur][i];
+                S.ry[nb][pos] = S.ry[cur][i];
+                S.cnt[nb][pos] = S.cnt[cur][i];
+                S.seq[nb][pos] = S.seq[cur][i];
+                S.keep[i] = (short)pos;
+            }
+        }
+        int nToExpand;
+        block_scan(nexp, S.scan, &nToExpand);
+        if (nsize > ORB_LIST_CAP) {
+            if (tid == 0) atomicOr(&ctr[ORB_CNT_ERR], ORB_ERR_LIST);
+            size = 0;
+            break;
+        }
+        for (int k = tid; k < K; k += OCT_T) {
+            const int n0 = node[k];
+            if (S.cnt[cur][n0] >= 2) {
+                int x, y;
+                key_xy(keys[k], wC, hC, &x, &y);
+                node[k] = S.map[n0][quad_of(S.rx[cur][n0], S.ry[cur][n0], x, y)];
+            } else {
+                node[k] = S.keep[n0];
+            }
+        }
+        cur ^= 1;
+        size = nsize;
+        __syncthreads();
+        if (size >= N || size == prev) break;
+        if (size + nToExpand * 3 > N) {
+            phase2 = true;
+            break;
+        }
+    }
+
+    // ---- phase 2: split the largest nodes until the quota (:908-991)
+    while (phase2) {
+        const int prev = size;
+        for (int i = tid; i < size; i += OCT_T) S.cc[i][0] = S.cc[i][1] = S.cc[i][2] = S.cc[i][3] = 0;
+        __syncthreads();
+        for (int k = tid; k < K; k += OCT_T) {
+            const int i = node[k];
+            if (S.cnt[cur][i] >= 2) {
+                int x, y;
+                key_xy(keys[k], wC, hC, &x, &y);
+                atomicAdd(&S.cc[i][quad_of(S.rx[cur][i], S.ry[cur][i], x, y)], 1);
+            }
+        }
+        // Expandable nodes, sorted by (size, creation order), largest first.
+        const bool e = tid < size && S.cnt[cur][tid] >= 2;
+        int nE;
+        const int ex_e = block_scan(e ? 1 : 0, S.scan, &nE);
+        int npow = 1;
+        while (npow < nE) npow <<= 1;
+        if (e)
+            S.sortk[ex_e] = ((unsigned long long)S.cnt[cur][tid] << 32) |
+                            ((unsigned long long)(unsigned)S.seq[cur][tid] << 16) | (unsigned)tid;
+        for (int j = nE + tid; j < npow; j += OCT_T) S.sortk[j] = 0ull;
+        __syncthreads();
+        for (int k2 = 2; k2 <= npow; k2 <<= 1) {
+            for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
+                for (int a = tid; a < npow; a += OCT_T) {
+                    const int b = a ^ j2;
+                    if (b > a) {
+                        const unsigned long long va = S.sortk[a], vb = S.sortk[b];
+                        const bool desc = (a & k2) == 0;
+                        if (desc ? (va < vb) : (va > vb)) {
+                            S.sortk[a] = vb;
+                            S.sortk[b] = va;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // Gains in processing order and the first split reaching the quota.
+        const int j = tid;
+        int node_j = -1, nc = 0;
+        if (j < nE) {
+            node_j = (int)(S.sortk[j] & 0xFFFF);
+            nc = (S.cc[node_j][0] > 0) + (S.cc[node_j][1] > 0) + (S.cc[node_j][2] > 0) + (S.cc[node_j][3] > 0);
+        }
+        int gsum;
+        const int ex_g = block_scan(j < nE ? nc - 1 : 0, S.scan, &gsum);
+        if (tid == 0) S.vars[0] = nE - 1;
+        __syncthreads();
+        if (j < nE && size + ex_g + nc - 1 >= N) atomicMin(&S.vars[0], j);
+        __syncthreads();
+        const int jstar = S.vars[0];
+        const bool proc = j <= jstar && j < nE;
+        int Cp;
+        const int ex_nc = block_scan(proc ? nc : 0, S.scan, &Cp);
+        // Mark processed list nodes.
+        if (tid < size) S.keep[tid] = 0;
+        __syncthreads();
+        if (proc) S.keep[node_j] = -1;
+        __syncthreads();
+        const bool kp = tid < size && S.keep[tid] == 0;
+        int nkeep;
+        const int ex_kp = block_scan(kp ? 1 : 0, S.scan, &nkeep);
+        const int nsize = Cp + nkeep;
+        if (nsize > ORB_LIST_CAP) {
+            if (tid == 0) atomicOr(&ctr[ORB_CNT_ERR], ORB_ERR_LIST);
+            size = 0;
+            break;
+        }
+        const int nb = cur ^ 1;
+        if (proc) {
+            const int i = node_j;
+            const int base = Cp - ex_nc - nc;
+            int r = 0;
+            for (int q = 3; q >= 0; q--) {
+                const int c = S.cc[i][q];
+                if (c > 0) {
+                    const int pos = base + r++;
+                    int crx, cry;
+                    child_rect(S.rx[cur][i], S.ry[cur][i], q, &crx, &cry);
+                    S.rx[nb][pos] = crx;
+                    S.ry[nb][pos] = cry;
+                    S.cnt[nb][pos] = c;
+                    S.seq[nb][pos] = ex_nc + (nc - r);
+                    S.map[i][q] = (short)pos;
+                } else {
+                    S.map[i][q] = -1;
+                }
+            }
+        }
+        int keeppos = -1;
+        if (kp) {
+            const int i = tid;
+            keeppos = Cp + ex_kp;
+            S.rx[nb][keeppos] = S.rx[cur][i];
+            S.ry[nb][keeppos] = S.ry[cur][i];
+            S.cnt[nb][keeppos] = S.cnt[cur][i];
+            S.seq[nb][keeppos] = S.seq[cur][i];
+        }
+        __syncthreads();
+        if (kp) S.keep[tid] = (short)keeppos;
+        __syncthreads();
+        for (int k = tid; k < K; k += OCT_T) {
+            const int n0 = node[k];
+            const int kv = S.keep[n0];
+            if (kv < 0) {
+                int x, y;
+                key_xy(keys[k], wC, hC, &x, &y);
+                node[k] = S.map[n0][quad_of(S.rx[cur][n0], S.ry[cur][n0], x, y)];
+            } else {
+                node[k] = kv;
+            }
+        }
+        cur ^= 1;
+        size = nsize;
+        __syncthreads();
+        if (size >= N || size == prev) break;
+    }
+
+    // ---- retain the best response per node; first in key order on ties
+    for (int i = tid; i < size; i += OCT_T) S.best[i] = 0ull;
+    __syncthreads();
+    for (int k = tid; k < K; k += OCT_T) {
+        const unsigned long long key = keys[k];
+        const unsigned long long packed = (key & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFull - (key & 0xFFFFFFFFull));
+        atomicMax(&S.best[node[k]], packed);
+    }
+    __syncthreads();
+    const int cap = min(L.list_cap, ORB_LIST_CAP);
+    if (size > cap) {
+        if (tid == 0) atomicOr(&ctr[ORB_CNT_ERR], ORB_ERR_LIST);
+        size = 0;
+    }
+    for (int i = tid; i < size; i += OCT_T) {
+        const unsigned long long p = S.best[i];
+        L.ret[i] = (p & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFull - (p & 0xFFFFFFFFull));
+    }
+    if (tid == 0) ctr[ORB_CNT_RET + level] = size;
+}
+
+// ============================================================== describe
+// fastAtan2 (OpenCV 2.4/3.x): degree-7 odd polynomial, degrees in [0,360).
+__device__ __forceinline__ float fast_atan2_dev(float y, float x)
+{
+    const float P1 = 0.9997878412794807f * (float)(180 / 3.1415926535897932384626433832795);
+    const float P3 = -0.3258083974640975f * (float)(180 / 3.1415926535897932384626433832795);
+    const float P5 = 0.1555786518463281f * (float)(180 / 3.1415926535897932384626433832795);
+    const float P7 = -0.04432655554792128f * (float)(180 / 3.1415926535897932384626433832795);
+    const float ax = fabsf(x), ay = fabsf(y);
+    float a, c, c2;
+    if (ax >= ay) {
+        c = __fdiv_rn(ay, ax + (float)DBL_EPSILON);
+        c2 = c * c;
+        a = (((P7 * c2 + P5) * c2 + P3) * c2 + P1) * c;
+    } else {
+        c = __fdiv_rn(ax, ay + (float)DBL_EPSILON);
+        c2 = c * c;
+        a = 90.f - (((P7 * c2 + P5) * c2 + P3) * c2 + P1) * c;
+    }
+    if (x < 0) a = 180.f - a;
+    if (y < 0) a = 360.f - a;
+    return a;
+}
+
+// cos/sin of a float angle, evaluated in double by Cody-Waite reduction and
+// Taylor series, then rounded to float: a fixed method so that the CPU
+// statement and this kernel agree bit for bit.
+__device__ __forceinline__ void sincos_canonical_dev(double x, double* s, double* c)
+{
+    const double kPio2Hi = 1.57079632673412561417e+00;
+    const double kPio2Lo = 6.07710050650619224932e-11;
+    const double kTwoOverPi = 6.36619772367581382433e-01;
+    const double kd = rint(x * kTwoOverPi);
+    const int k = (int)kd;
+    const double r = (x - kd * kPio2Hi) - kd * kPio2Lo;
+    const double r2 = r * r;
+    double ps = 1.0 / 355687428096000.0;
+    ps = ps * r2 - 1.0 / 1307674368000.0;
+    ps = ps * r2 + 1.0 / 6227020800.0;
+    ps = ps * r2 - 1.0 / 39916800.0;
+    ps = ps * r2 + 1.0 / 362880.0;
+    ps = ps * r2 - 1.0 / 5040.0;
+    ps = ps * r2 + 1.0 / 120.0;
+    ps = ps * r2 - 1.0 / 6.0;
+    const double sr = r + r * r2 * ps;
+    double pc = 1.0 / 6402373705728000.0;
+    pc = pc * r2 - 1.0 / 20922789888000.0;
+    pc = pc * r2 + 1.0 / 87178291200.0;
+    pc = pc * r2 - 1.0 / 479001600.0;
+    pc = pc * r2 + 1.0 / 3628800.0;
+    pc = pc * r2 - 1.0 / 40320.0;
+    pc = pc * r2 + 1.0 / 720.0;
+    pc = pc * r2 - 1.0 / 24.0;
+    pc = pc * r2 + 0.5;
+    const double cr = 1.0 - r2 * pc;
+    switch (k & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+    }
+}
+
+// One wavefront per output keypoint.
+__global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restrict__ jobs,
+                                                       const RunTable* __restrict__ rt)
+{
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (slot >= rt->slot_start[rt->njobs]) return;
+    const int jb = find_segment(rt->slot_start, rt->njobs, slot);
+    const ImageJob& J = jobs[jb];
+    const int s = slot - rt->slot_start[jb];
+    const int* ctr = J.counters;
+    int level = -1, idx = 0, total = 0;
+    for (int l = 0; l < J.nlevels; l++) {
+        const int c = ctr[ORB_CNT_RET + l];
+        if (level < 0 && s < total + c) {
+            level = l;
+            idx = s - total;
+        }
+        total += c;
+    }
+    if (s == 0 && lane == 0) {
+        J.counters[ORB_CNT_NOUT] = total <= J.out_cap ? total : 0;
+        if (total > J.out_cap) atomicOr(&J.counters[ORB_CNT_ERR], ORB_ERR_OUT);
+    }
+    if (level < 0 || total > J.out_cap) return;
+    const LevelJob& L = J.lv[level];
+    const unsigned long long key = L.ret[idx];
+    int xr, yr;
+    key_xy(key, L.wCell, L.hCell, &xr, &yr);
+    const int cx = xr + ORB_MIN_BORDER, cy = yr + ORB_MIN_BORDER;
+    const int pitch = L.pitch;
+
+    // IC_Angle (ORBextractor.cc:94-141): m10 = sum u*I, m01 = sum v*I over
+    // the disc |u| <= umax[|v|].  Lane = column u, the two half-waves split
+    // the rows.
+    const uint8_t* img = L.lvl + (size_t)cy * pitch + cx;
+    const int u = (lane & 31) - 15;
+    int m10 = 0, m01 = 0;
+    if ((lane & 31) < 31) {
+        const int au = u < 0 ? -u : u;
+        const int v0 = lane < 32 ? 0 : -15, v1 = lane < 32 ? 15 : -1;
+        for (int v = v0; v <= v1; v++) {
+            const int av = v < 0 ? -v : v;
+            if (au <= c_umax[av]) {
+                const int val = img[v * pitch + u];
+                m10 += u * val;
+                m01 += v * val;
+            }
+        }
+    }
+    m10 = wave_sum(m10);
+    m01 = wave_sum(m01);
+    const float angle_deg = fast_atan2_dev((float)m01, (float)m10);
+
+    // Rotated BRIEF (ORBextractor.cc:153-204) on the blurred level.
+    const float factorPI = (float)(3.1415926535897932384626433832795 / 180.f);
+    const float angle = angle_deg * factorPI;
+    double sd, cd;
+    sincos_canonical_dev((double)angle, &sd, &cd);
+    const float a = (float)cd, b = (float)sd;
+    const uint8_t* center = L.blur + (size_t)cy * pitch + cx;
+    int nib = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; bit++) {
+        const int p = lane * 4 + bit;   // pair index: byte p/8, bit p%8
+        const float x0 = (float)c_pattern[4 * p], y0 = (float)c_pattern[4 * p + 1];
+        const float x1 = (float)c_pattern[4 * p + 2], y1 = (float)c_pattern[4 * p + 3];
+        const int i0 = center[(int)rintf(x0 * b + y0 * a) * pitch + (int)rintf(x0 * a - y0 * b)];
+        const int i1 = center[(int)rintf(x1 * b + y1 * a) * pitch + (int)rintf(x1 * a - y1 * b)];
+        nib |= (i0 < i1) << bit;
+    }
+    const int hi = __shfl_down(nib, 1, 64);
+    const int out = s;
+    if ((lane & 1) == 0) J.desc[(size_t)out * 32 + (lane >> 1)] = (uint8_t)(nib | (hi << 4));
+    if (lane == 0) {
+        const float sc = L.scale;
+        OrbKeyPoint kp;
+        float fx = (float)cx, fy = (float)cy;
+        if (level != 0) {
+            fx = fx * sc;
+            fy = fy * sc;
+        }
+        kp.x = fx;
+        kp.y = fy;
+        kp.size = (float)(int)(31 * sc);
+        kp.angle = angle_deg;
+        kp.response = (float)(unsigned)(key >> 32);
+        kp.octave = level;
+        kp.class_id = -1;
+        J.kps[out] = kp;
+    }
+}
+
+// ============================================================ zero fill
+__global__ void zero_counters_kernel(const ImageJob* __restrict__ jobs, int njobs)
+{
+    const int j = blockIdx.x;
+    if (j < njobs && threadIdx.x < ORB_CNT_SLOTS) jobs[j].counters[threadIdx.x] = 0;
+}
+
+__global__ void zero_ints_kernel(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ========================================================= Hamming helpers
+__device__ __forceinline__ int hamming32(const uint4 a0, const uint4 a1, const uint8_t* b)
+{
+    const uint4 b0 = *(const uint4*)b, b1 = *(const uint4*)(b + 16);
+    return __popc(a0.x ^ b0.x) + __popc(a0.y ^ b0.y) + __popc(a0.z ^ b0.z) + __popc(a0.w ^ b0.w) +
+           __popc(a1.x ^ b1.x) + __popc(a1.y ^ b1.y) + __popc(a1.z ^ b1.z) + __popc(a1.w ^ b1.w);
+}
+
+// ================================================================ stereo
+// Frame::ComputeStereoMatches (Frame.cc:831-1082), one wavefront per left
+// keypoint.  Search: right keypoints whose row band [floor(y-2s), ceil(y+2s)]
+// holds row (size_t)vL, octave within +-1, uR in [uL-maxD, uL]; best is the
+// smallest (distance, index) below TH_HIGH.  Refinement: 11x11 SAD of
+// centre-subtracted patches over 11 shifts, parabola fit, depth = bf/disp.
+
+__global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs A)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int iL = blockIdx.x * 4 + wv;
+    const int nl = *A.nl, nr = *A.nr;
+    if (iL >= nl || iL >= A.cap_l) return;
+    float outU = -1.0f, outD = -1.0f;
+    const OrbKeyPoint kpL = A.kl[iL];
+    const int levelL = kpL.octave;
+    const float vL = kpL.y, uL = kpL.x;
+    const int row = (int)vL;
+    const float minZ = A.mb;
+    const float minD = 0;
+    const float maxD = A.bf / minZ;
+    const float minU = uL - maxD;
+    const float maxU = uL - minD;
+    bool go = row >= 0 && row < A.nrows && !(maxU < 0);
+    int bestDist = 100, bestIdx = INT_MAX;
+    if (go) {
+        const uint4 a0 = *(const uint4*)(A.dl + (size_t)iL * 32);
+        const uint4 a1 = *(const uint4*)(A.dl + (size_t)iL * 32 + 16);
+        for (int iR = lane; iR < nr; iR += 64) {
+            const OrbKeyPoint kpR = A.kr[iR];
+            const float r = 2.0f * A.scale[kpR.octave];
+            const int maxr = (int)ceilf(kpR.y + r);
+            const int minr = (int)floorf(kpR.y - r);
+            if (row < minr || row > maxr) continue;
+            if (kpR.octave < levelL - 1 || kpR.octave > levelL + 1) continue;
+            const float uR = kpR.x;
+            if (uR >= minU && uR <= maxU) {
+                const int dist = hamming32(a0, a1, A.dr + (size_t)iR * 32);
+                if (dist < bestDist) {
+                    bestDist = dist;
+                    bestIdx = iR;
+                }
+            }
+        }
+        // Smallest (distance, index) across the wavefront.
+        for (int o = 32; o > 0; o >>= 1) {
+            const int od = __shfl_xor(bestDist, o, 64), oi = __shfl_xor(bestIdx, o, 64);
+            if (od < bestDist || (od == bestDist && oi < bestIdx)) {
+                bestDist = od;
+                bestIdx = oi;
+            }
+        }
+        go = bestDist < 75 && bestIdx != INT_MAX;
+    }
+    int sx0 = 0, sy0 = 0, sxR = 0;
+    float scaleduR0 = 0.f;
+    const uint8_t *pl = nullptr, *pr = nullptr;
+    int pitchL = 0, pitchR = 0;
+    if (go) {
+        const float uR0 = A.kr[bestIdx].x;
+        const float scaleFactor = A.inv_scale[levelL];
+        const float scaleduL = roundf(kpL.x * scaleFactor);
+        const float scaledvL = roundf(kpL.y * scaleFactor);
+        scaleduR0 = roundf(uR0 * scaleFactor);
+        const LevelJob& LL = A.left->lv[levelL];
+        const LevelJob& LR = A.right->lv[levelL];
+        const float iniu = scaleduR0 - 5 - 5;
+        const float endu = scaleduR0 + 5 + 5 + 1;
+        if (iniu < 0 || endu >= LR.w) go = false;
+        sx0 = (int)scaleduL;
+        sy0 = (int)scaledvL;
+        sxR = (int)scaleduR0;
+        // Keypoints of this extractor always leave room for the 11x11
+        // patches; caller-supplied ones are checked instead of read past the
+        // level (the reference would index outside the cv::Mat).
+        if (sy0 - 5 < 0 || sy0 + 5 >= LL.h || sy0 + 5 >= LR.h || sx0 - 5 < 0 || sx0 + 5 >= LL.w) go = false;
+        pl = LL.lvl;
+        pr = LR.lvl;
+        pitchL = LL.pitch;
+        pitchR = LR.pitch;
+    }
+    if (go) {
+        // 121 (shift, row) tasks over 64 lanes; per task 11 absolute
+        // differences of centre-subtracted pixels, then one wave sum per shift.
+        const int cL = pl[(size_t)sy0 * pitchL + sx0];
+        int acc[2] = {0, 0};
+#pragma unroll
+        for (int h2 = 0; h2 < 2; h2++) {
+            const int task = lane + 64 * h2;
+            if (task < 121) {
+                const int inc = task / 11 - 5, dy = task % 11 - 5;
+                const int xc = sxR + inc;
+                const int cR = pr[(size_t)sy0 * pitchR + xc];
+                const uint8_t* rl = pl + (size_t)(sy0 + dy) * pitchL + sx0 - 5;
+                const uint8_t* rr = pr + (size_t)(sy0 + dy) * pitchR + xc - 5;
+                int a = 0;
+#pragma unroll
+                for (int dx = 0; dx < 11; dx++) a += abs((rl[dx] - cL) - (rr[dx] - cR));
+                acc[h2] = a;
+            }
+        }
+        int sad[11];
+#pragma unroll
+        for (int sh = 0; sh < 11; sh++) {
+            int v = 0;
+            if (lane / 11 == sh) v += acc[0];
+            if (lane + 64 < 121 && (lane + 64) / 11 == sh) v += acc[1];
+            sad[sh] = wave_sum(v);
+        }
+        if (lane == 0) {
+            int best = INT_MAX, bestinc = 0;
+            float vD[11];
+            for (int inc = -5; inc <= 5; inc++) {
+                const float dist = (float)sad[inc + 5];
+                if (dist < best) {
+                    best = (int)dist;
+                    bestinc = inc;
+                }
+                vD[inc + 5] = dist;
+            }
+            if (bestinc != -5 && bestinc != 5) {
+                const float dist1 = vD[5 + bestinc - 1];
+                const float dist2 = vD[5 + bestinc];
+                const float dist3 = vD[5 + bestinc + 1];
+                const float deltaR = __fdiv_rn(dist1 - dist3, 2.0f * (dist1 + dist3 - 2.0f * dist2));
+                if (!(deltaR < -1 || deltaR > 1)) {
+                    float bestuR = A.scale[levelL] * ((float)scaleduR0 + (float)bestinc + deltaR);
+                    float disparity = (uL - bestuR);
+                    if (disparity >= minD && disparity < maxD) {
+                        if (disparity <= 0) {
+                            disparity = 0.01f;
+                            bestuR = (float)((double)uL - 0.01);
+                        }
+                        outD = __fdiv_rn(A.bf, disparity);
+                        outU = bestuR;
+                        const int m = atomicAdd(A.match_count, 1);
+                        A.match_sad[m] = best;
+                        A.match_idx[m] = iL;
+                    }
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        A.uRight[iL] = outU;
+        A.depth[iL] = outD;
+    }
+}
+
+// Median-based outlier cut (Frame.cc:1066-1081): sort the matched SAD values,
+// th = 1.5f*1.4f*median, drop matches with SAD >= th.
+#define MEDIAN_CAP 4096
+__global__ __launch_bounds__(1024) void stereo_median_kernel(const StereoArgs A)
+{
+    __shared__ int v[MEDIAN_CAP];
+    const int M = min(*A.match_count, MEDIAN_CAP);
+    if (M == 0) return;
+    int npow = 1;
+    while (npow < M) npow <<= 1;
+    for (int i = threadIdx.x; i < npow; i += 1024) v[i] = i < M ? A.match_sad[i] : INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= npow; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int a = threadIdx.x; a < npow; a += 1024) {
+                const int b = a ^ j;
+                if (b > a) {
+                    const int va = v[a], vb = v[b];
+                    const bool up = (a & k) == 0;
+                    if (up ? (va > vb) : (va < vb)) {
+                        v[a] = vb;
+                        v[b] = va;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    const float median = (float)v[M / 2];
+    const float k = 1.5f * 1.4f;
+    const float thDist = k * median;
+    for (int i = threadIdx.x; i < M; i += 1024) {
+        if (!((float)A.match_sad[i] < thDist)) {
+            const int iL = A.match_idx[i];
+            A.uRight[iL] = -1;
+            A.depth[iL] = -1;
+        }
+    }
+}
+
+// ============================================================ brute force
+// Best/second-best Hamming match per query with the update rule of
+// ORBmatcher::SearchForInitialization (ORBmatcher.cc:614-647): the best is
+// the first minimum in index order, the second is the next-smallest distance
+// in the multiset; accept iff best <= th_low and best < (float)second*ratio.
+__global__ __launch_bounds__(256) void match_bf_kernel(const uint8_t* __restrict__ q, const int* __restrict__ nq_ptr,
+                                                       const uint8_t* __restrict__ t, const int* __restrict__ nt_ptr,
+                                                       int th_low, float ratio, int* __restrict__ idx,
+                                                       int* __restrict__ dist, int cap_q)
+{
+    const int lane = threadIdx.x & 63;
+    const int iq = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nq = *nq_ptr, nt = *nt_ptr;
+    if (iq >= nq || iq >= cap_q) return;
+    const uint4 a0 = *(const uint4*)(q + (size_t)iq * 32);
+    const uint4 a1 = *(const uint4*)(q + (size_t)iq * 32 + 16);
+    int d0 = INT_MAX, i0 = INT_MAX, d1 = INT_MAX;
+    for (int j = lane; j < nt; j += 64) {
+        const int d = hamming32(a0, a1, t + (size_t)j * 32);
+        if (d < d0) {
+            d1 = d0;
+            d0 = d;
+            i0 = j;
+        } else if (d < d1) {
+            d1 = d;
+        }
+    }
+    int bd = d0, bi = i0;
+    for (int o = 32; o > 0; o >>= 1) {
+        const int od = __shfl_xor(bd, o, 64), oi = __shfl_xor(bi, o, 64);
+        if (od < bd || (od == bd && oi < bi)) {
+            bd = od;
+            bi = oi;
+        }
+    }
+    int sec = (i0 == bi) ? d1 : d0;
+    for (int o = 32; o > 0; o >>= 1) sec = min(sec, __shfl_xor(sec, o, 64));
+    if (lane == 0) {
+        const bool ok = bi != INT_MAX && bd <= th_low && (float)bd < (float)sec * ratio;
+        idx[iq] = ok ? bi : -1;
+        dist[iq] = bd;
+    }
+}
+
+// ============================================================= launchers
+extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable* d_rt, const RunTable* h_rt,
+                                         hipStream_t st)
+{
+    zero_counters_kernel<<<h_rt->njobs, 64, 0, st>>>(d_jobs, h_rt->njobs);
+    for (int l = 0; l < h_rt->max_levels; l++) {
+        const int nb = h_rt->tile_start[l][h_rt->njobs];
+        if (nb > 0) level_kernel<<<nb, 256, 0, st>>>(d_jobs, d_rt, l);
+    }
+    const int ncells = h_rt->cell_start[h_rt->njobs * h_rt->max_levels];
+    if (ncells > 0) cells_kernel<<<(ncells + 3) / 4, 256, 0, st>>>(d_jobs, d_rt);
+    octree_kernel<<<h_rt->njobs * h_rt->max_levels, OCT_T, 0, st>>>(d_jobs, h_rt->max_levels);
+    const int nslots = h_rt->slot_start[h_rt->njobs];
+    if (nslots > 0) describe_kernel<<<(nslots + 3) / 4, 256, 0, st>>>(d_jobs, d_rt);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t orb_launch_stereo(const StereoArgs* a, hipStream_t st)
+{
+    zero_ints_kernel<<<1, 64, 0, st>>>(a->match_count, 1);
+    if (a->cap_l > 0) stereo_kernel<<<(a->cap_l + 3) / 4, 256, 0, st>>>(*a);
+    stereo_median_kernel<<<1, 1024, 0, st>>>(*a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const uint8_t* t, const int* nt,
+                                          int th_low, float ratio, int* idx, int* dist, int cap_q,
+                                          hipStream_t st)
+{
+    if (cap_q > 0) match_bf_kernel<<<(cap_q + 3) / 4, 256, 0, st>>>(q, nq, t, nt, th_low, ratio, idx, dist, cap_q);
+    return hipGetLastError();
+}

@@ -1,0 +1,765 @@
+// orb_runtime.cpp -- host side of the C ABI declared in include/orb_gpu.h.
+//
+// The host work is the scalar bookkeeping that ORBextractor does on the CPU
+// before any pixel is touched: scale tables and per-level quotas (the
+// constructor), level sizes, the FAST cell grid and the quadtree roots of each
+// level.  Those values are computed here with the same float statements as the
+// reference and packed into ImageJob descriptors; the pixel and keypoint work
+// runs in the gfx950 kernels of orb_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "orb_gpu.h"
+#include "orb_types.h"
+
+extern "C" hipError_t orb_upload_constants(const int* umax16, const int* gauss7);
+extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable* d_rt, const RunTable* h_rt,
+                                         hipStream_t st);
+extern "C" hipError_t orb_launch_stereo(const StereoArgs* a, hipStream_t st);
+extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const uint8_t* t, const int* nt,
+                                          int th_low, float ratio, int* idx, int* dist, int cap_q,
+                                          hipStream_t st);
+
+namespace {
+
+
+inline int cv_round(float v) { return (int)std::lrintf(v); }
+
+#define HIP_TRY(x)                               \
+    do {                                         \
+        if ((x) != hipSuccess) return ORB_E_DEVICE; \
+    } while (0)
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Integer taps of getGaussianKernel(7, 2) scaled by 256 (OpenCV's 8U
+// separable path): float kernel normalised in double, rounded half-even.
+void gaussian_taps(int k[7])
+{
+    float cf[7];
+    double sum = 0;
+    for (int i = 0; i < 7; i++) {
+        const double x = i - 3.0;
+        cf[i] = (float)std::exp((-0.5 / (2.0 * 2.0)) * x * x);
+        sum += cf[i];
+    }
+    sum = 1. / sum;
+    for (int i = 0; i < 7; i++) cf[i] = (float)(cf[i] * sum);
+    for (int i = 0; i < 7; i++) k[i] = cv_round(cf[i] * 256.f);
+}
+
+// umax: half-widths of the radius-15 orientation disc (ORBextractor.cc:542-570).
+void disc_umax(int umax[16])
+{
+    const int half = 15;
+    const int vmax = (int)std::floor(half * std::sqrt(2.f) / 2 + 1);
+    const int vmin = (int)std::ceil(half * std::sqrt(2.f) / 2);
+    const double hp2 = half * half;
+    for (int v = 0; v <= vmax; ++v) umax[v] = (int)std::lrint(std::sqrt(hp2 - v * v));
+    for (int v = half, v0 = 0; v >= vmin; --v) {
+        while (umax[v0] == umax[v0 + 1]) ++v0;
+        umax[v] = v0;
+        ++v0;
+    }
+}
+
+struct LevelPlan {
+    int w, h, pitch;
+    size_t img_off;
+    int tiles_x, tiles_y, ncells;
+    int nCols, nRows, wCell, hCell, maxBorderX, maxBorderY;
+    int nIni;
+    float hX;
+    int root_x[ORB_MAX_ROOTS + 1];
+    int key_cap, list_cap;
+    size_t key_off, ret_off;
+};
+
+struct Plan {
+    int W = 0, H = 0;
+    LevelPlan lv[ORB_MAX_LEVELS];
+    size_t img_bytes = 0, nkeys = 0, nret = 0;
+    int out_cap = 0;
+};
+
+}  // namespace
+
+struct orb_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // brute-force matcher scratch
+    uint8_t* d_q = nullptr;
+    uint8_t* d_t = nullptr;
+    int* d_out = nullptr;
+    size_t q_cap = 0, t_cap = 0;
+};
+
+struct orb_extractor {
+    orb_context* ctx = nullptr;
+    int nfeatures = 0, nlevels = 0, iniTh = 0, minTh = 0;
+    float scaleFactorF = 1.2f;
+    double scaleFactor = 1.2;
+    std::vector<float> scale, invScale, sigma2, invSigma2;
+    std::vector<int> quota;
+
+    Plan plan;
+    unsigned gen = 0;             // bumped whenever buffers or plan change
+    char* d_mem = nullptr;
+    size_t mem_bytes = 0;
+    ImageJob hjob{};
+    ImageJob* d_job = nullptr;
+    RunTable* h_rt = nullptr;     // pinned
+    RunTable* d_rt = nullptr;
+    // stereo state (this extractor as the left view)
+    float* d_uR = nullptr;
+    float* d_depth = nullptr;
+    int* d_msad = nullptr;
+    int* d_midx = nullptr;
+    int* d_misc = nullptr;        // [0] match count, [1] nl, [2] nr
+    OrbKeyPoint* d_kin = nullptr; // uploaded keypoints: left then right
+    uint8_t* d_din = nullptr;
+    int in_cap = 0;
+    // two-view run (this = left, pair_right = right)
+    const orb_extractor* pair_right = nullptr;
+    unsigned pair_gen_l = 0, pair_gen_r = 0;
+    ImageJob* d_pair_jobs = nullptr;
+    RunTable* h_pair_rt = nullptr;
+    RunTable* d_pair_rt = nullptr;
+
+    // ORBextractor::ORBextractor (ORBextractor.cc:469-526): scale tables in
+    // float (mvScaleFactor[i-1]*scaleFactor with the double member), inverse
+    // tables, and the geometric per-level quota with the remainder on top.
+    void init_tables()
+    {
+        scale.assign(nlevels, 1.f);
+        sigma2.assign(nlevels, 1.f);
+        for (int i = 1; i < nlevels; i++) {
+            scale[i] = (float)(scale[i - 1] * scaleFactor);
+            sigma2[i] = scale[i] * scale[i];
+        }
+        invScale.resize(nlevels);
+        invSigma2.resize(nlevels);
+        for (int i = 0; i < nlevels; i++) {
+            invScale[i] = 1.0f / scale[i];
+            invSigma2[i] = 1.0f / sigma2[i];
+        }
+        quota.assign(nlevels, 0);
+        const float factor = (float)(1.0f / scaleFactor);
+        float nDesired = nfeatures * (1 - factor) / (1 - (float)std::pow((double)factor, (double)nlevels));
+        int sum = 0;
+        for (int l = 0; l < nlevels - 1; l++) {
+            quota[l] = cv_round(nDesired);
+            sum += quota[l];
+            nDesired *= factor;
+        }
+        quota[nlevels - 1] = std::max(nfeatures - sum, 0);
+    }
+
+    // Level sizes (ComputePyramid, :1663), FAST cell grids (:1047-1071) and
+    // quadtree roots (DistributeOctTree, :695-720) for a W x H image.
+    int make_plan(int W, int H, Plan& p) const
+    {
+        p = Plan();
+        p.W = W;
+        p.H = H;
+        size_t img = 0, nk = 0, nr = 0;
+        int out = 0;
+        for (int l = 0; l < nlevels; l++) {
+            LevelPlan& L = p.lv[l];
+            L.w = cv_round((float)W * invScale[l]);
+            L.h = cv_round((float)H * invScale[l]);
+            if (L.w < 40 || L.h < 40) return ORB_E_ARG;
+            L.pitch = (int)align_up((size_t)L.w, 64);
+            L.img_off = img;
+            img += align_up((size_t)L.pitch * L.h, 256);
+            L.tiles_x = (L.w + ORB_TILE_W - 1) / ORB_TILE_W;
+            L.tiles_y = (L.h + ORB_TILE_H - 1) / ORB_TILE_H;
+            const int minBorderX = ORB_MIN_BORDER, minBorderY = ORB_MIN_BORDER;
+            L.maxBorderX = L.w - ORB_EDGE + 3;
+            L.maxBorderY = L.h - ORB_EDGE + 3;
+            const float width = (float)(L.maxBorderX - minBorderX);
+            const float height = (float)(L.maxBorderY - minBorderY);
+            L.nCols = (int)(width / 30.f);
+            L.nRows = (int)(height / 30.f);
+            if (L.nCols <= 0 || L.nRows <= 0 || L.nCols > 255 || L.nRows > 255) return ORB_E_ARG;
+            L.wCell = (int)std::ceil(width / L.nCols);
+            L.hCell = (int)std::ceil(height / L.nRows);
+            if (L.wCell > 58 || L.hCell > 58) return ORB_E_ARG;
+            L.ncells = L.nCols * L.nRows;
+            L.nIni = (int)std::round((float)(L.maxBorderX - minBorderX) / (L.maxBorderY - minBorderY));
+            if (L.nIni <= 0 || L.nIni > ORB_MAX_ROOTS) return ORB_E_ARG;
+            L.hX = (float)(L.maxBorderX - minBorderX) / L.nIni;
+            for (int i = 0; i <= L.nIni; i++) L.root_x[i] = (int)(L.hX * (float)i);
+            L.key_cap = L.ncells * ((L.wCell + 1) / 2) * ((L.hCell + 1) / 2);
+            L.list_cap = std::max(quota[l] + 3, 4 * L.nIni);
+            if (L.list_cap > ORB_LIST_CAP) return ORB_E_ARG;
+            L.key_off = nk;
+            nk += (size_t)L.key_cap;
+            L.ret_off = nr;
+            nr += (size_t)L.list_cap;
+            out += L.list_cap;
+        }
+        p.img_bytes = img;
+        p.nkeys = nk;
+        p.nret = nr;
+        p.out_cap = out;
+        return ORB_OK;
+    }
+
+    // Device buffers and the job descriptor for the current image size.
+    int ensure(int W, int H)
+    {
+        if (plan.W == W && plan.H == H && d_mem) return ORB_OK;
+        Plan p;
+        int rc = make_plan(W, H, p);
+        if (rc) return rc;
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t o = off;
+            off = align_up(off + bytes, 256);
+            return o;
+        };
+        const size_t o_lvl = take(p.img_bytes), o_blur = take(p.img_bytes), o_fm = take(p.img_bytes);
+        const size_t o_keys = take(p.nkeys * 8), o_node = take(p.nkeys * 4), o_ret = take(p.nret * 8);
+        const size_t o_ctr = take(ORB_CNT_SLOTS * 4);
+        const size_t o_kps = take((size_t)p.out_cap * sizeof(OrbKeyPoint));
+        const size_t o_desc = take((size_t)p.out_cap * 32);
+        const size_t o_uR = take((size_t)p.out_cap * 4), o_dep = take((size_t)p.out_cap * 4);
+        const size_t o_msad = take((size_t)p.out_cap * 4), o_midx = take((size_t)p.out_cap * 4);
+        const size_t o_misc = take(64);
+        const size_t total = off;
+        if (total > mem_bytes) {
+            if (d_mem) (void)hipFree(d_mem);
+            d_mem = nullptr;
+            mem_bytes = 0;
+            if (hipMalloc(&d_mem, total) != hipSuccess) {
+                d_mem = nullptr;
+                return ORB_E_OOM;
+            }
+            mem_bytes = total;
+        }
+        plan = p;
+        gen++;
+        ImageJob& J = hjob;
+        std::memset(&J, 0, sizeof(J));
+        J.nlevels = nlevels;
+        J.iniTh = iniTh;
+        J.minTh = minTh;
+        J.out_cap = p.out_cap;
+        J.counters = (int*)(d_mem + o_ctr);
+        J.kps = (OrbKeyPoint*)(d_mem + o_kps);
+        J.desc = (uint8_t*)(d_mem + o_desc);
+        for (int l = 0; l < nlevels; l++) {
+            const LevelPlan& P = p.lv[l];
+            LevelJob& L = J.lv[l];
+            L.w = P.w;
+            L.h = P.h;
+            L.pitch = P.pitch;
+            L.tiles_x = P.tiles_x;
+            L.tiles_y = P.tiles_y;
+            L.nCols = P.nCols;
+            L.nRows = P.nRows;
+            L.wCell = P.wCell;
+            L.hCell = P.hCell;
+            L.maxBorderX = P.maxBorderX;
+            L.maxBorderY = P.maxBorderY;
+            L.quota = quota[l];
+            L.nIni = P.nIni;
+            L.hX = P.hX;
+            L.scale = scale[l];
+            for (int i = 0; i <= P.nIni; i++) L.root_x[i] = P.root_x[i];
+            L.key_cap = P.key_cap;
+            L.list_cap = P.list_cap;
+            L.lvl = (uint8_t*)(d_mem + o_lvl + P.img_off);
+            L.blur = (uint8_t*)(d_mem + o_blur + P.img_off);
+            L.fastm = (uint8_t*)(d_mem + o_fm + P.img_off);
+            L.keys = (unsigned long long*)(d_mem + o_keys) + P.key_off;
+            L.node = (int*)(d_mem + o_node) + P.key_off;
+            L.ret = (unsigned long long*)(d_mem + o_ret) + P.ret_off;
+        }
+        d_uR = (float*)(d_mem + o_uR);
+        d_depth = (float*)(d_mem + o_dep);
+        d_msad = (int*)(d_mem + o_msad);
+        d_midx = (int*)(d_mem + o_midx);
+        d_misc = (int*)(d_mem + o_misc);
+        if (!d_job && hipMalloc(&d_job, sizeof(ImageJob)) != hipSuccess) return ORB_E_OOM;
+        if (!d_rt && hipMalloc(&d_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
+        if (!h_rt && hipHostMalloc(&h_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
+        const orb_extractor* one[1] = {this};
+        build_table(one, 1, *h_rt);
+        HIP_TRY(hipMemcpy(d_job, &hjob, sizeof(ImageJob), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_rt, h_rt, sizeof(RunTable), hipMemcpyHostToDevice));
+        return ORB_OK;
+    }
+
+    static void build_table(const orb_extractor* const* exs, int n, RunTable& rt)
+    {
+        std::memset(&rt, 0, sizeof(rt));
+        rt.njobs = n;
+        int maxl = 0;
+        for (int j = 0; j < n; j++) maxl = std::max(maxl, exs[j]->nlevels);
+        rt.max_levels = maxl;
+        for (int l = 0; l < maxl; l++) {
+            int acc = 0;
+            for (int j = 0; j < n; j++) {
+                rt.tile_start[l][j] = acc;
+                if (l < exs[j]->nlevels) acc += exs[j]->plan.lv[l].tiles_x * exs[j]->plan.lv[l].tiles_y;
+            }
+            rt.tile_start[l][n] = acc;
+        }
+        int acc = 0;
+        for (int j = 0; j < n; j++)
+            for (int l = 0; l < maxl; l++) {
+                rt.cell_start[j * maxl + l] = acc;
+                if (l < exs[j]->nlevels) acc += exs[j]->plan.lv[l].ncells;
+            }
+        rt.cell_start[n * maxl] = acc;
+        acc = 0;
+        for (int j = 0; j < n; j++) {
+            rt.slot_start[j] = acc;
+            acc += exs[j]->plan.out_cap;
+        }
+        rt.slot_start[n] = acc;
+    }
+
+    int upload_image(const uint8_t* img, size_t stride, hipMemcpyKind kind)
+    {
+        HIP_TRY(hipMemcpy2DAsync(hjob.lv[0].lvl, hjob.lv[0].pitch, img, stride, plan.W, plan.H, kind,
+                                 ctx->stream));
+        return ORB_OK;
+    }
+
+    int check_errors()
+    {
+        int c[ORB_CNT_SLOTS];
+        HIP_TRY(hipMemcpy(c, hjob.counters, sizeof(c), hipMemcpyDeviceToHost));
+        return c[ORB_CNT_ERR] ? ORB_E_OVERFLOW : ORB_OK;
+    }
+
+    int read_results(orb_keypoint* kps, int cap, uint8_t* desc, int* n)
+    {
+        int c[ORB_CNT_SLOTS];
+        HIP_TRY(hipMemcpyAsync(c, hjob.counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (c[ORB_CNT_ERR]) return ORB_E_OVERFLOW;
+        const int cnt = c[ORB_CNT_NOUT];
+        *n = cnt;
+        if (cnt > cap) return ORB_E_CAPACITY;
+        if (cnt > 0) {
+            if (kps)
+                HIP_TRY(hipMemcpyAsync(kps, hjob.kps, (size_t)cnt * sizeof(orb_keypoint), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            if (desc)
+                HIP_TRY(hipMemcpyAsync(desc, hjob.desc, (size_t)cnt * 32, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+        return ORB_OK;
+    }
+
+    // Two-view run table (left = this, right = r) for the stereo frame.
+    int ensure_pair(const orb_extractor* r)
+    {
+        if (pair_right == r && pair_gen_l == gen && pair_gen_r == r->gen) return ORB_OK;
+        if (!d_pair_jobs && hipMalloc(&d_pair_jobs, 2 * sizeof(ImageJob)) != hipSuccess) return ORB_E_OOM;
+        if (!d_pair_rt && hipMalloc(&d_pair_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
+        if (!h_pair_rt && hipHostMalloc(&h_pair_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
+        const orb_extractor* two[2] = {this, r};
+        build_table(two, 2, *h_pair_rt);
+        ImageJob jobs[2] = {hjob, r->hjob};
+        HIP_TRY(hipMemcpy(d_pair_jobs, jobs, sizeof(jobs), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_pair_rt, h_pair_rt, sizeof(RunTable), hipMemcpyHostToDevice));
+        pair_right = r;
+        pair_gen_l = gen;
+        pair_gen_r = r->gen;
+        return ORB_OK;
+    }
+
+    int ensure_inputs(int cap)
+    {
+        if (cap <= in_cap) return ORB_OK;
+        if (d_kin) (void)hipFree(d_kin);
+        if (d_din) (void)hipFree(d_din);
+        d_kin = nullptr;
+        d_din = nullptr;
+        in_cap = 0;
+        if (hipMalloc(&d_kin, 2 * (size_t)cap * sizeof(OrbKeyPoint)) != hipSuccess) return ORB_E_OOM;
+        if (hipMalloc(&d_din, 2 * (size_t)cap * 32) != hipSuccess) return ORB_E_OOM;
+        in_cap = cap;
+        return ORB_OK;
+    }
+
+    void stereo_args(StereoArgs& a, const orb_extractor* r, const ImageJob* dl_job, const ImageJob* dr_job,
+                     float bf, float b) const
+    {
+        std::memset(&a, 0, sizeof(a));
+        a.left = dl_job;
+        a.right = dr_job;
+        a.nrows = plan.lv[0].h;
+        a.bf = bf;
+        a.mb = b;
+        for (int l = 0; l < nlevels; l++) {
+            a.scale[l] = scale[l];
+            a.inv_scale[l] = invScale[l];
+        }
+        a.uRight = d_uR;
+        a.depth = d_depth;
+        a.match_sad = d_msad;
+        a.match_idx = d_midx;
+        a.match_count = d_misc;
+        (void)r;
+    }
+
+    ~orb_extractor()
+    {
+        if (d_mem) (void)hipFree(d_mem);
+        if (d_job) (void)hipFree(d_job);
+        if (d_rt) (void)hipFree(d_rt);
+        if (h_rt) (void)hipHostFree(h_rt);
+        if (d_kin) (void)hipFree(d_kin);
+        if (d_din) (void)hipFree(d_din);
+        if (d_pair_jobs) (void)hipFree(d_pair_jobs);
+        if (d_pair_rt) (void)hipFree(d_pair_rt);
+        if (h_pair_rt) (void)hipHostFree(h_pair_rt);
+    }
+};
+
+// ====================================================================== C ABI
+extern "C" {
+
+int orb_context_create(int device, orb_context** out)
+{
+    if (!out) return ORB_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return ORB_E_DEVICE;
+    HIP_TRY(hipSetDevice(device));
+    orb_context* c = new orb_context();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ORB_E_DEVICE;
+    }
+    int umax[16], gauss[7];
+    disc_umax(umax);
+    gaussian_taps(gauss);
+    if (orb_upload_constants(umax, gauss) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return ORB_E_DEVICE;
+    }
+    *out = c;
+    return ORB_OK;
+}
+
+void orb_context_destroy(orb_context* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_q) (void)hipFree(ctx->d_q);
+    if (ctx->d_t) (void)hipFree(ctx->d_t);
+    if (ctx->d_out) (void)hipFree(ctx->d_out);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int orb_context_synchronize(orb_context* ctx)
+{
+    if (!ctx) return ORB_E_ARG;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ORB_OK;
+}
+
+int orb_extractor_create(orb_context* ctx, int nfeatures, float scaleFactor, int nlevels, int iniThFAST,
+                         int minThFAST, orb_extractor** out)
+{
+    if (!ctx || !out || nfeatures < 0 || nlevels < 1 || nlevels > ORB_MAX_LEVELS || !(scaleFactor > 1.0f))
+        return ORB_E_ARG;
+    orb_extractor* e = new orb_extractor();
+    e->ctx = ctx;
+    e->nfeatures = nfeatures;
+    e->nlevels = nlevels;
+    e->iniTh = iniThFAST;
+    e->minTh = minThFAST;
+    e->scaleFactorF = scaleFactor;
+    e->scaleFactor = scaleFactor;
+    e->init_tables();
+    *out = e;
+    return ORB_OK;
+}
+
+void orb_extractor_destroy(orb_extractor* ex)
+{
+    if (!ex) return;
+    (void)hipSetDevice(ex->ctx->device);
+    (void)hipStreamSynchronize(ex->ctx->stream);
+    delete ex;
+}
+
+int orb_extractor_levels(const orb_extractor* ex) { return ex ? ex->nlevels : ORB_E_ARG; }
+
+float orb_extractor_scale_factor(const orb_extractor* ex) { return ex ? (float)ex->scaleFactor : 0.f; }
+
+int orb_extractor_scales(const orb_extractor* ex, float* scale, float* inv_scale, float* sigma2, float* inv_sigma2)
+{
+    if (!ex) return ORB_E_ARG;
+    for (int i = 0; i < ex->nlevels; i++) {
+        if (scale) scale[i] = ex->scale[i];
+        if (inv_scale) inv_scale[i] = ex->invScale[i];
+        if (sigma2) sigma2[i] = ex->sigma2[i];
+        if (inv_sigma2) inv_sigma2[i] = ex->invSigma2[i];
+    }
+    return ORB_OK;
+}
+
+int orb_extractor_quotas(const orb_extractor* ex, int* quota)
+{
+    if (!ex || !quota) return ORB_E_ARG;
+    for (int i = 0; i < ex->nlevels; i++) quota[i] = ex->quota[i];
+    return ORB_OK;
+}
+
+int orb_extractor_capacity(const orb_extractor* ex, int width, int height)
+{
+    if (!ex) return ORB_E_ARG;
+    Plan p;
+    const int rc = ex->make_plan(width, height, p);
+    return rc ? rc : p.out_cap;
+}
+
+int orb_extract_device(orb_extractor* ex, const uint8_t* d_image, int width, int height, size_t pitch)
+{
+    if (!ex || width < 0 || height < 0) return ORB_E_ARG;
+    if (width == 0 || height == 0) return ORB_OK;
+    if (!d_image || pitch < (size_t)width) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ex->ctx->device));
+    int rc = ex->ensure(width, height);
+    if (rc) return rc;
+    rc = ex->upload_image(d_image, pitch, hipMemcpyDeviceToDevice);
+    if (rc) return rc;
+    HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, ex->ctx->stream));
+    return ORB_OK;
+}
+
+int orb_extractor_results(orb_extractor* ex, orb_keypoint* kps, int cap, uint8_t* desc, int* n)
+{
+    if (!ex || !n) return ORB_E_ARG;
+    *n = 0;
+    if (!ex->d_mem) return ORB_OK;
+    HIP_TRY(hipSetDevice(ex->ctx->device));
+    return ex->read_results(kps, cap, desc, n);
+}
+
+int orb_extract(orb_extractor* ex, const uint8_t* image, int width, int height, size_t stride,
+                orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n)
+{
+    if (!ex || !n || width < 0 || height < 0) return ORB_E_ARG;
+    *n = 0;
+    if (width == 0 || height == 0) return ORB_OK;  // empty image: silent return (:1537)
+    if (!image || stride < (size_t)width) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ex->ctx->device));
+    int rc = ex->ensure(width, height);
+    if (rc) return rc;
+    rc = ex->upload_image(image, stride, hipMemcpyHostToDevice);
+    if (rc) return rc;
+    HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, ex->ctx->stream));
+    return ex->read_results(keypoints, capacity, descriptors, n);
+}
+
+int orb_extractor_pyramid_level(orb_extractor* ex, int level, uint8_t* dst, size_t dst_stride, int* width,
+                                int* height)
+{
+    if (!ex || !ex->d_mem || level < 0 || level >= ex->nlevels) return ORB_E_ARG;
+    if (width) *width = ex->plan.lv[level].w;
+    if (height) *height = ex->plan.lv[level].h;
+    return orb_extractor_debug_image(ex, level, 0, dst, dst_stride);
+}
+
+int orb_descriptor_distance(const uint8_t* a, const uint8_t* b)
+{
+    if (!a || !b) return ORB_E_ARG;
+    int d = 0;
+    for (int i = 0; i < 4; i++) {
+        uint64_t x, y;
+        std::memcpy(&x, a + 8 * i, 8);
+        std::memcpy(&y, b + 8 * i, 8);
+        d += __builtin_popcountll(x ^ y);
+    }
+    return d;
+}
+
+int orb_match_bf(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* train, int nt, int th_low,
+                 float ratio, int32_t* idx, int32_t* dist)
+{
+    if (!ctx || nq < 0 || nt < 0 || (nq && (!query || !idx || !dist)) || (nt && !train)) return ORB_E_ARG;
+    if (nq == 0) return ORB_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if ((size_t)nq > ctx->q_cap) {
+        if (ctx->d_q) (void)hipFree(ctx->d_q);
+        if (ctx->d_out) (void)hipFree(ctx->d_out);
+        ctx->d_q = nullptr;
+        ctx->d_out = nullptr;
+        ctx->q_cap = 0;
+        if (hipMalloc(&ctx->d_q, (size_t)nq * 32) != hipSuccess) return ORB_E_OOM;
+        if (hipMalloc(&ctx->d_out, (size_t)nq * 8 + 64) != hipSuccess) return ORB_E_OOM;
+        ctx->q_cap = nq;
+    }
+    if ((size_t)std::max(nt, 1) > ctx->t_cap) {
+        if (ctx->d_t) (void)hipFree(ctx->d_t);
+        ctx->d_t = nullptr;
+        ctx->t_cap = 0;
+        if (hipMalloc(&ctx->d_t, (size_t)std::max(nt, 1) * 32) != hipSuccess) return ORB_E_OOM;
+        ctx->t_cap = std::max(nt, 1);
+    }
+    int* d_idx = ctx->d_out;
+    int* d_dist = ctx->d_out + nq;
+    int* d_n = ctx->d_out + 2 * (size_t)nq;
+    const int nn[2] = {nq, nt};
+    HIP_TRY(hipMemcpyAsync(ctx->d_q, query, (size_t)nq * 32, hipMemcpyHostToDevice, ctx->stream));
+    if (nt) HIP_TRY(hipMemcpyAsync(ctx->d_t, train, (size_t)nt * 32, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d_n, nn, sizeof(nn), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(orb_launch_match_bf(ctx->d_q, d_n, ctx->d_t, d_n + 1, th_low, ratio, d_idx, d_dist, nq, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(idx, d_idx, (size_t)nq * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dist, d_dist, (size_t)nq * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ORB_OK;
+}
+
+int orb_stereo_match(orb_extractor* left, orb_extractor* right, const orb_keypoint* kl, const uint8_t* dl, int nl,
+                     const orb_keypoint* kr, const uint8_t* dr, int nr, float bf, float b, float* uRight,
+                     float* depth)
+{
+    if (!left || !right || nl < 0 || nr < 0 || (nl && (!kl || !dl || !uRight || !depth)) || (nr && (!kr || !dr)))
+        return ORB_E_ARG;
+    if (nl == 0) return ORB_OK;
+    if (!left->d_mem || !right->d_mem || left->plan.W != right->plan.W || left->plan.H != right->plan.H ||
+        left->nlevels != right->nlevels)
+        return ORB_E_ARG;
+    for (int i = 0; i < nl; i++)
+        if (kl[i].octave < 0 || kl[i].octave >= left->nlevels) return ORB_E_ARG;
+    for (int i = 0; i < nr; i++)
+        if (kr[i].octave < 0 || kr[i].octave >= left->nlevels) return ORB_E_ARG;
+    if (nl > left->plan.out_cap) return ORB_E_CAPACITY;
+    HIP_TRY(hipSetDevice(left->ctx->device));
+    hipStream_t st = left->ctx->stream;
+    int rc = left->ensure_pair(right);
+    if (rc) return rc;
+    rc = left->ensure_inputs(std::max(std::max(nl, nr), 1));
+    if (rc) return rc;
+    OrbKeyPoint* d_kl = left->d_kin;
+    OrbKeyPoint* d_kr = left->d_kin + left->in_cap;
+    uint8_t* d_dl = left->d_din;
+    uint8_t* d_dr = left->d_din + (size_t)left->in_cap * 32;
+    static_assert(sizeof(orb_keypoint) == sizeof(OrbKeyPoint), "keypoint layout");
+    HIP_TRY(hipMemcpyAsync(d_kl, kl, (size_t)nl * sizeof(OrbKeyPoint), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_dl, dl, (size_t)nl * 32, hipMemcpyHostToDevice, st));
+    if (nr) {
+        HIP_TRY(hipMemcpyAsync(d_kr, kr, (size_t)nr * sizeof(OrbKeyPoint), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_dr, dr, (size_t)nr * 32, hipMemcpyHostToDevice, st));
+    }
+    const int counts[2] = {nl, nr};
+    HIP_TRY(hipMemcpyAsync(left->d_misc + 1, counts, sizeof(counts), hipMemcpyHostToDevice, st));
+    StereoArgs a;
+    left->stereo_args(a, right, left->d_pair_jobs, left->d_pair_jobs + 1, bf, b);
+    a.kl = d_kl;
+    a.dl = d_dl;
+    a.nl = left->d_misc + 1;
+    a.kr = d_kr;
+    a.dr = d_dr;
+    a.nr = left->d_misc + 2;
+    a.cap_l = nl;
+    HIP_TRY(orb_launch_stereo(&a, st));
+    HIP_TRY(hipMemcpyAsync(uRight, left->d_uR, (size_t)nl * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(depth, left->d_depth, (size_t)nl * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return ORB_OK;
+}
+
+int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* im_left, const uint8_t* im_right,
+                     int width, int height, size_t stride, float bf, float b, orb_keypoint* kl, uint8_t* dl,
+                     int* nl, orb_keypoint* kr, uint8_t* dr, int* nr, int capacity, float* uRight, float* depth)
+{
+    if (!left || !right || left == right || !nl || !nr || width < 0 || height < 0) return ORB_E_ARG;
+    *nl = *nr = 0;
+    if (width == 0 || height == 0) return ORB_OK;
+    if (!im_left || !im_right || stride < (size_t)width || left->nlevels != right->nlevels ||
+        left->ctx != right->ctx)
+        return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(left->ctx->device));
+    hipStream_t st = left->ctx->stream;
+    int rc = left->ensure(width, height);
+    if (!rc) rc = right->ensure(width, height);
+    if (!rc) rc = left->ensure_pair(right);
+    if (rc) return rc;
+    rc = left->upload_image(im_left, stride, hipMemcpyHostToDevice);
+    if (!rc) rc = right->upload_image(im_right, stride, hipMemcpyHostToDevice);
+    if (rc) return rc;
+    HIP_TRY(orb_launch_extract(left->d_pair_jobs, left->d_pair_rt, left->h_pair_rt, st));
+    StereoArgs a;
+    left->stereo_args(a, right, left->d_pair_jobs, left->d_pair_jobs + 1, bf, b);
+    a.kl = left->hjob.kps;
+    a.dl = left->hjob.desc;
+    a.nl = left->hjob.counters + ORB_CNT_NOUT;
+    a.kr = right->hjob.kps;
+    a.dr = right->hjob.desc;
+    a.nr = right->hjob.counters + ORB_CNT_NOUT;
+    a.cap_l = left->plan.out_cap;
+    HIP_TRY(orb_launch_stereo(&a, st));
+    rc = left->read_results(kl, capacity, dl, nl);
+    if (rc) return rc;
+    rc = right->read_results(kr, capacity, dr, nr);
+    if (rc) return rc;
+    if (*nl > 0) {
+        if (uRight) HIP_TRY(hipMemcpyAsync(uRight, left->d_uR, (size_t)*nl * 4, hipMemcpyDeviceToHost, st));
+        if (depth) HIP_TRY(hipMemcpyAsync(depth, left->d_depth, (size_t)*nl * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return ORB_OK;
+}
+
+int orb_extractor_debug_image(orb_extractor* ex, int level, int kind, uint8_t* dst, size_t dst_stride)
+{
+    if (!ex || !ex->d_mem || level < 0 || level >= ex->nlevels || kind < 0 || kind > 2) return ORB_E_ARG;
+    if (!dst) return ORB_OK;
+    const LevelJob& L = ex->hjob.lv[level];
+    const uint8_t* src = kind == 0 ? L.lvl : (kind == 1 ? L.blur : L.fastm);
+    if (dst_stride < (size_t)L.w) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ex->ctx->device));
+    HIP_TRY(hipStreamSynchronize(ex->ctx->stream));
+    HIP_TRY(hipMemcpy2D(dst, dst_stride, src, L.pitch, L.w, L.h, hipMemcpyDeviceToHost));
+    return ORB_OK;
+}
+
+int orb_extractor_debug_candidates(orb_extractor* ex, int level, int* xyr, int capacity, int* n)
+{
+    if (!ex || !ex->d_mem || !n || level < 0 || level >= ex->nlevels) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ex->ctx->device));
+    HIP_TRY(hipStreamSynchronize(ex->ctx->stream));
+    int c[ORB_CNT_SLOTS];
+    HIP_TRY(hipMemcpy(c, ex->hjob.counters, sizeof(c), hipMemcpyDeviceToHost));
+    const LevelJob& L = ex->hjob.lv[level];
+    const int k = std::min(c[ORB_CNT_KEYS + level], L.key_cap);
+    *n = k;
+    if (k > capacity) return ORB_E_CAPACITY;
+    std::vector<unsigned long long> keys(k);
+    if (k) HIP_TRY(hipMemcpy(keys.data(), L.keys, (size_t)k * 8, hipMemcpyDeviceToHost));
+    std::sort(keys.begin(), keys.end(), [](unsigned long long a, unsigned long long b) {
+        return (unsigned)a < (unsigned)b;
+    });
+    for (int i = 0; i < k; i++) {
+        const unsigned ok = (unsigned)keys[i];
+        const int xl = ok & 63, yl = (ok >> 6) & 63, j = (ok >> 12) & 255, r = (ok >> 20) & 255;
+        xyr[3 * i] = j * L.wCell + xl;
+        xyr[3 * i + 1] = r * L.hCell + yl;
+        xyr[3 * i + 2] = (int)(keys[i] >> 32);
+    }
+    return ORB_OK;
+}
+
+}  // extern "C"

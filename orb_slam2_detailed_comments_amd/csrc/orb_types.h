@@ -1,0 +1,96 @@
+// orb_types.h -- job descriptors shared by the host runtime and the gfx950
+// kernels of the ORB front end.
+//
+// One ImageJob describes one image going through the extraction pipeline
+// (pyramid -> FAST cells -> quadtree distribution -> orientation + rotated
+// BRIEF).  Every job carries device pointers to its own buffers, so a single
+// launch of each stage can cover any set of images: the two views of a stereo
+// frame, a mixed-size batch, or the frames admitted by the serving loop.
+#pragma once
+#include <stdint.h>
+
+#define ORB_MAX_LEVELS 16
+#define ORB_MAX_ROOTS 16
+#define ORB_LIST_CAP 512       // quadtree list capacity per level (LDS)
+#define ORB_TILE_W 64          // level kernel output tile
+#define ORB_TILE_H 16
+#define ORB_EDGE 19            // EDGE_THRESHOLD, ORBextractor.cc:81
+#define ORB_MIN_BORDER 16      // EDGE_THRESHOLD - 3, ORBextractor.cc:1049
+
+// Counter slots per job (int32, zeroed at the start of each run).
+#define ORB_CNT_KEYS 0                       // [16] FAST candidates per level
+#define ORB_CNT_RET (ORB_CNT_KEYS + 16)      // [16] keypoints kept per level
+#define ORB_CNT_NOUT (ORB_CNT_RET + 16)      // total keypoints
+#define ORB_CNT_ERR (ORB_CNT_NOUT + 1)       // error bits
+#define ORB_CNT_MATCH (ORB_CNT_ERR + 1)      // stereo matches recorded
+#define ORB_CNT_SLOTS 64
+
+#define ORB_ERR_KEYS 1        // candidate buffer overflow
+#define ORB_ERR_LIST 2        // quadtree list overflow
+#define ORB_ERR_OUT 4         // output capacity overflow
+
+// cv::KeyPoint layout (28 bytes).
+struct OrbKeyPoint {
+    float x, y, size, angle, response;
+    int octave, class_id;
+};
+
+struct LevelJob {
+    int w, h, pitch;                 // level size and row pitch (bytes)
+    int tiles_x, tiles_y;            // level kernel tiles
+    int nCols, nRows, wCell, hCell;  // FAST cell grid (ORBextractor.cc:1065-1071)
+    int maxBorderX, maxBorderY;
+    int quota;                       // mnFeaturesPerLevel[level]
+    int nIni;                        // quadtree roots
+    float hX;                        // root width (float, ORBextractor.cc:701)
+    float scale;                     // mvScaleFactor[level]
+    int root_x[ORB_MAX_ROOTS + 1];   // (int)(hX*i), i = 0..nIni
+    int key_cap;                     // candidate capacity
+    int list_cap;                    // kept-keypoint capacity (<= ORB_LIST_CAP)
+    uint8_t* lvl;                    // level image
+    uint8_t* blur;                   // 7x7 sigma-2 blur of the level
+    uint8_t* fastm;                  // FAST arc strength per pixel
+    unsigned long long* keys;        // candidates: response<<32 | order key
+    int* node;                       // per-candidate quadtree node (scratch)
+    unsigned long long* ret;         // kept keypoints in output order
+};
+
+struct ImageJob {
+    int nlevels, iniTh, minTh;
+    int out_cap;                     // keypoint output capacity
+    int* counters;                   // ORB_CNT_SLOTS ints
+    OrbKeyPoint* kps;                // out_cap keypoints
+    uint8_t* desc;                   // out_cap x 32 bytes
+    LevelJob lv[ORB_MAX_LEVELS];
+};
+
+// Launch table for one run over a set of jobs (prefix sums in block units).
+struct RunTable {
+    int njobs;
+    int max_levels;
+    int tile_start[ORB_MAX_LEVELS][65];   // per level: first tile block of job j
+    int cell_start[65 * ORB_MAX_LEVELS];  // per (job, level): first cell wave
+    int slot_start[65];                   // describe kernel: first slot of job j
+};
+#define ORB_MAX_JOBS 64
+
+// Arguments of one stereo match (Frame::ComputeStereoMatches).
+struct StereoArgs {
+    const ImageJob* left;
+    const ImageJob* right;
+    const OrbKeyPoint* kl;
+    const uint8_t* dl;
+    const int* nl;
+    const OrbKeyPoint* kr;
+    const uint8_t* dr;
+    const int* nr;
+    int cap_l;
+    int nrows;
+    float bf, mb;
+    float scale[ORB_MAX_LEVELS], inv_scale[ORB_MAX_LEVELS];
+    float* uRight;
+    float* depth;
+    int* match_sad;
+    int* match_idx;
+    int* match_count;
+};

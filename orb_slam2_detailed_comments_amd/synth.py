@@ -1,0 +1,160 @@
+"""Seeded synthetic inputs for the ORB front end.
+
+Every workload used by the tests and the benchmark is generated here from an
+integer seed with numpy's PCG64 generator, so the GPU path and the CPU oracle
+see byte-identical images.
+
+* ``shapes_frame``   -- random axis-aligned rectangles and ellipses on a
+  random background, plus uniform noise, Gaussian-blurred (sigma 1) and
+  clipped to uint8.
+* ``warp_affine``    -- rotation about the image centre plus a translation,
+  bilinear, border clamped (the second frame of the small plain workload and
+  the camera-stream sequences).
+* ``stereo_pair``    -- a left frame and the right view obtained through a
+  piecewise-planar disparity map (Voronoi regions of random seeds, one plane
+  per region, disparity clipped to [0, max_disp]).
+* ``descriptor_sets``-- uniform random 256-bit queries and a map set with
+  planted near-duplicates (Binomial(256, p) flipped bits).
+* ``stream_frames``  -- per-camera sequences: each frame is the previous one
+  rotated U[0,2] deg, shifted U[0,8] px, plus fresh uniform noise.
+"""
+import numpy as np
+
+
+def _gauss_kernel(sigma):
+    r = int(np.ceil(3 * sigma))
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    k = np.exp(-0.5 * (x / sigma) ** 2)
+    return k / k.sum(), r
+
+
+def _blur(img, sigma):
+    k, r = _gauss_kernel(sigma)
+    p = np.pad(img, r, mode="reflect")
+    h, w = img.shape
+    tmp = np.zeros((h + 2 * r, w), np.float64)
+    for i, c in enumerate(k):
+        tmp += c * p[:, i:i + w]
+    out = np.zeros((h, w), np.float64)
+    for i, c in enumerate(k):
+        out += c * tmp[i:i + h, :]
+    return out
+
+
+def shapes_frame(width, height, seed, n_shapes=200, noise=20.0, sigma=1.0):
+    """One uint8 frame of random rectangles/ellipses plus blurred noise."""
+    rng = np.random.default_rng(seed)
+    img = np.full((height, width), rng.uniform(0, 255), np.float64)
+    for _ in range(n_shapes):
+        ellipse = bool(rng.integers(0, 2))
+        cx, cy = rng.uniform(0, width), rng.uniform(0, height)
+        rx = rng.uniform(4, max(8.0, width / 8))
+        ry = rng.uniform(4, max(8.0, height / 8))
+        val = rng.uniform(0, 255)
+        x0, x1 = max(0, int(cx - rx)), min(width, int(cx + rx) + 1)
+        y0, y1 = max(0, int(cy - ry)), min(height, int(cy + ry) + 1)
+        if x0 >= x1 or y0 >= y1:
+            continue
+        if ellipse:
+            yy, xx = np.mgrid[y0:y1, x0:x1]
+            m = ((xx - cx) / rx) ** 2 + ((yy - cy) / ry) ** 2 <= 1.0
+            img[y0:y1, x0:x1][m] = val
+        else:
+            img[y0:y1, x0:x1] = val
+    img += rng.uniform(-noise, noise, size=(height, width))
+    img = _blur(img, sigma)
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8)
+
+
+def _bilinear(img, xs, ys):
+    h, w = img.shape
+    xs = np.clip(xs, 0, w - 1)
+    ys = np.clip(ys, 0, h - 1)
+    x0 = np.floor(xs).astype(np.int64)
+    y0 = np.floor(ys).astype(np.int64)
+    x1 = np.minimum(x0 + 1, w - 1)
+    y1 = np.minimum(y0 + 1, h - 1)
+    fx = xs - x0
+    fy = ys - y0
+    f = img.astype(np.float64)
+    top = f[y0, x0] * (1 - fx) + f[y0, x1] * fx
+    bot = f[y1, x0] * (1 - fx) + f[y1, x1] * fx
+    return top * (1 - fy) + bot * fy
+
+
+def warp_affine(img, angle_deg, tx, ty):
+    """Rotate by angle_deg about the centre and translate by (tx, ty)."""
+    h, w = img.shape
+    cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+    t = np.deg2rad(angle_deg)
+    c, s = np.cos(t), np.sin(t)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    u, v = xx - cx - tx, yy - cy - ty
+    xs = c * u + s * v + cx
+    ys = -s * u + c * v + cy
+    out = _bilinear(img, xs, ys)
+    return np.clip(np.rint(out), 0, 255).astype(np.uint8)
+
+
+def disparity_map(width, height, seed, n_planes=12, max_disp=64.0):
+    """Piecewise-planar disparity: one plane per Voronoi region of n seeds."""
+    rng = np.random.default_rng(seed)
+    sx = rng.uniform(0, width, n_planes)
+    sy = rng.uniform(0, height, n_planes)
+    d0 = rng.uniform(0, max_disp, n_planes)
+    gx = rng.uniform(-0.05, 0.05, n_planes)
+    gy = rng.uniform(-0.05, 0.05, n_planes)
+    yy, xx = np.mgrid[0:height, 0:width].astype(np.float64)
+    dist = (xx[None] - sx[:, None, None]) ** 2 + (yy[None] - sy[:, None, None]) ** 2
+    region = np.argmin(dist, axis=0)
+    disp = d0[region] + gx[region] * (xx - sx[region]) + gy[region] * (yy - sy[region])
+    return np.clip(disp, 0.0, max_disp)
+
+
+def stereo_pair(width, height, seed, n_planes=12, max_disp=64.0):
+    """(left, right, disparity): right(x, y) = left(x + d(x, y), y)."""
+    left = shapes_frame(width, height, seed)
+    disp = disparity_map(width, height, seed + 7919, n_planes, max_disp)
+    yy, xx = np.mgrid[0:height, 0:width].astype(np.float64)
+    right = np.clip(np.rint(_bilinear(left, xx + disp, yy)), 0, 255).astype(np.uint8)
+    return left, right, disp
+
+
+def frame_pair(width, height, seed, angle_deg=3.0, shift=(6.0, 4.0)):
+    """A frame and its copy rotated by angle_deg and shifted by `shift`."""
+    a = shapes_frame(width, height, seed)
+    return a, warp_affine(a, angle_deg, shift[0], shift[1])
+
+
+def descriptor_sets(n_query, n_map, seed, planted_frac=0.1, flip_p=0.08):
+    """Random 256-bit queries and a map set with planted near-duplicates.
+
+    Returns (queries[n_query, 32], map[n_map, 32], planted_query[n_map]) where
+    planted_query[j] is the query a planted map row was derived from, or -1.
+    """
+    rng = np.random.default_rng(seed)
+    q = rng.integers(0, 256, size=(n_query, 32), dtype=np.uint8)
+    m = rng.integers(0, 256, size=(n_map, 32), dtype=np.uint8)
+    n_plant = int(round(planted_frac * n_map))
+    rows = rng.choice(n_map, size=n_plant, replace=False)
+    src = rng.integers(0, n_query, size=n_plant)
+    flips = rng.random((n_plant, 256)) < flip_p
+    mask = np.packbits(flips, axis=1, bitorder="little")
+    m[rows] = q[src] ^ mask
+    planted = np.full(n_map, -1, np.int64)
+    planted[rows] = src
+    return q, m, planted
+
+
+def stream_frames(width, height, seed, n_frames, max_rot=2.0, max_shift=8.0, noise=10.0):
+    """A camera sequence: each frame is the previous one moved plus noise."""
+    rng = np.random.default_rng(seed + 104729)
+    cur = shapes_frame(width, height, seed)
+    frames = [cur]
+    for _ in range(n_frames - 1):
+        moved = warp_affine(cur, rng.uniform(0, max_rot), rng.uniform(0, max_shift),
+                            rng.uniform(0, max_shift)).astype(np.float64)
+        moved += rng.uniform(-noise, noise, size=moved.shape)
+        cur = np.clip(np.rint(moved), 0, 255).astype(np.uint8)
+        frames.append(cur)
+    return frames
