@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: ORB extract + stereo match frames/s at 1241x376, 2000 keypoints.
+
+A step is one pass of the hot path over one batch of synthetic stereo frames
+resident in HBM: extraction of both views (pyramid, FAST cells, quadtree,
+orientation, rotated BRIEF) and Frame::ComputeStereoMatches for every frame
+of the batch, replayed from one captured hipGraph.  value = frames/s where a
+frame is one stereo pair.
+
+Also reported:
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average duration (HIP events on the library's stream), against
+                MI355X HBM peak 8 TB/s
+  cpu_baseline  the CPU oracle (a restatement of the reference path) timed on
+                this host's cores on a bounded sample of the same frames:
+                floor(cores/2) independent Frame pipelines, each extracting
+                L/R on two threads then matching (src/Frame.cc:146-169)
+One GPU only: --gpus N>1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s ORB extract+match @1241x376/2000kp; p99 ms at 16 streams"
+W, H, NF, NLEV = 1241, 376, 2000, 8
+BF, FX = 386.1, 718.856
+HBM_PEAK_GBS = 8000.0
+
+
+def make_frames(n, seed0=1000):
+    from orb_slam2_detailed_comments_amd import synth
+    return [synth.stereo_pair(W, H, seed0 + i)[:2] for i in range(n)]
+
+
+def level_sizes(w, h, nlevels=NLEV, scale=1.2):
+    inv = [1.0]
+    s = np.float32(1.0)
+    for _ in range(1, nlevels):
+        s = np.float32(np.float64(s) * np.float64(np.float32(scale)))
+        inv.append(np.float32(1.0) / s)
+    return [(int(np.rint(np.float32(w) * np.float32(i))), int(np.rint(np.float32(h) * np.float32(i)))) for i in inv]
+
+
+def stage_bytes(stage, n_images, kps_per_image, cands_per_image):
+    """Algorithmic HBM bytes of one launch of a stage over n_images images."""
+    sizes = level_sizes(W, H)
+    if stage.startswith("pyramid_l"):
+        l = int(stage[len("pyramid_l"):])
+        w, h = sizes[l]
+        src = W * H if l == 0 else sizes[l - 1][0] * sizes[l - 1][1]
+        # read the source level; write level (l>0), its blur and FAST strength
+        return n_images * (src + (3 if l > 0 else 2) * w * h)
+    if stage == "fast_cells":
+        return n_images * (sum(w * h for w, h in sizes) + 8 * cands_per_image)
+    if stage == "octree":
+        return n_images * (8 * cands_per_image + 8 * kps_per_image)
+    if stage == "describe":
+        # 721-px orientation disc + 512 BRIEF samples per keypoint, 60 B out
+        return n_images * kps_per_image * (721 + 512 + 60)
+    if stage == "stereo_match":
+        return (n_images // 2) * kps_per_image * (28 + 32 + 8 + 2 * 121 * 2)
+    return None
+
+
+def cpu_baseline(frames, seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = len(os.sched_getaffinity(0))
+    b = np.float32(BF) / np.float32(FX)
+
+    def pipeline(stop, counter, idx0):
+        L = pyoracle.OracleExtractor(NF, 1.2, NLEV, 20, 7)
+        R = pyoracle.OracleExtractor(NF, 1.2, NLEV, 20, 7)
+        i = idx0
+        while not stop.is_set():
+            imL, imR = frames[i % len(frames)]
+            pyoracle.stereo_frame(L, R, imL, imR, BF, float(b))
+            counter.append(1)
+            i += 1
+
+    def run(npipe, secs):
+        stop, counter = threading.Event(), []
+        th = [threading.Thread(target=pipeline, args=(stop, counter, k)) for k in range(npipe)]
+        for t in th:
+            t.start()
+        time.sleep(0.5)  # warm-up
+        c0, t0 = len(counter), time.perf_counter()
+        time.sleep(secs)
+        c1, t1 = len(counter), time.perf_counter()
+        stop.set()
+        for t in th:
+            t.join()
+        return (c1 - c0) / (t1 - t0)
+
+    single = run(1, seconds * 0.3)
+    npipe = max(1, cores // 2)
+    allc = run(npipe, seconds * 0.7)
+    return {"value": round(allc, 2), "unit": "frames/s", "cores": 2 * npipe, "kind": "port",
+            "sample": f"{len(frames)} synthetic 1241x376 stereo frames cycled for ~{seconds:.0f}s: {npipe} "
+                      f"independent Frame pipelines (L/R extraction on 2 threads + stereo match) on "
+                      f"{cores} host cores",
+            "reference_structure_frames_per_s": round(single, 2), "host_cores": cores}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=32, help="stereo frames per step")
+    ap.add_argument("--mode", choices=["graph", "stages", "per-image"], default="graph")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box in this build has one MI355X, and the ORB front end "
+              "shards by frame/stream with no collective, so run it with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import orb_slam2_detailed_comments_amd as orb
+    mode = {"graph": orb.Batch.GRAPH, "stages": orb.Batch.STAGE_LAUNCHES, "per-image": orb.Batch.PER_IMAGE}[args.mode]
+    B = args.batch
+    frames = make_frames(B)
+    print(f"[bench] {B} stereo frames {W}x{H} generated", file=sys.stderr, flush=True)
+    ctx = orb.Context(0)
+    batch = orb.Batch([(W, H)] * (2 * B), NF, 1.2, NLEV, 20, 7, context=ctx)
+    for i, (l, r) in enumerate(frames):
+        batch.upload(2 * i, l)
+        batch.upload(2 * i + 1, r)
+    b = float(np.float32(BF) / np.float32(FX))
+    batch.set_stereo(list(range(0, 2 * B, 2)), list(range(1, 2 * B, 2)), BF, b)
+
+    for _ in range(args.warmup):
+        batch.run(mode)
+    batch.synchronize()
+    print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(mode)
+    batch.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = B * args.steps / (t1 - t0)
+    print(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # Result sizes of the last step (keypoints and FAST candidates per image).
+    kps = [len(batch.results(s)[0]) for s in range(2 * B)]
+    matched = [int((batch.stereo_results(p)[0] >= 0).sum()) for p in range(B)]
+
+    stages = batch.stage_times(args.steps)
+    dom = max((k for k in stages if k != "stereo_median"), key=lambda k: stages[k])
+    cands = None
+    bytes_dom = stage_bytes(dom, 2 * B, float(np.mean(kps)), 10 * NF if cands is None else cands)
+    achieved = bytes_dom / (stages[dom] * 1e-3) / 1e9 if bytes_dom else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2) if achieved else None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
+                "traffic": traffic, "avg_launch_ms": round(stages[dom], 4),
+                "algorithmic_bytes_per_launch": bytes_dom,
+                "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
+
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u8",
+           "data": "synthetic (seeded shapes+noise frames; right view = left through a 12-plane disparity map)",
+           "config": {"workload": f"stereo extract+match, {W}x{H}, nFeatures {NF}, 8 levels, scale 1.2, FAST 20/7, "
+                                  f"bf {BF}, fx {FX}", "frames_per_step": B, "images_per_step": 2 * B,
+                      "run_mode": args.mode,
+                      "mean_keypoints_per_image": round(float(np.mean(kps)), 1),
+                      "mean_stereo_matches": round(float(np.mean(matched)), 1)},
+           "roofline": roofline}
+    if not args.no_cpu:
+        print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(frames[:8], args.cpu_seconds)
+        out["cpu_baseline"] = cpu
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -43,6 +43,9 @@ typedef struct orb_extractor orb_extractor;
 int orb_context_create(int device, orb_context** out);
 void orb_context_destroy(orb_context* ctx);
 int orb_context_synchronize(orb_context* ctx);
+/* The context's hipStream_t (as void*), for callers that time or order work
+ * around the library's launches. */
+void* orb_context_stream(orb_context* ctx);
 
 /* ORBextractor::ORBextractor(nfeatures, scaleFactor, nlevels, iniThFAST,
  * minThFAST)  (include/ORBextractor.h:93, src/ORBextractor.cc:469-571).
@@ -112,6 +115,37 @@ int orb_stereo_match(orb_extractor* left, orb_extractor* right, const orb_keypoi
 int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* im_left, const uint8_t* im_right,
                      int width, int height, size_t stride, float bf, float b, orb_keypoint* kl, uint8_t* dl,
                      int* nl, orb_keypoint* kr, uint8_t* dr, int* nr, int capacity, float* uRight, float* depth);
+
+/* ---------------------------------------------------------------- batches
+ * A fixed set of image slots (each with its own size and nfeatures) extracted
+ * together.  No reference counterpart: the reference extracts one image per
+ * call; this is the batched form of ORBextractor::operator() for many frames
+ * and cameras.  Images are written into the slots' device buffers (resident
+ * in HBM), then orb_batch_run enqueues the whole pipeline:
+ *   mode 0  one launch per stage covering every slot
+ *   mode 1  the mode-0 launches captured once into a hipGraph and replayed
+ *   mode 2  per-image launches of the same kernels (the unbatched baseline)
+ * Stereo pairs declared with orb_batch_set_stereo are matched after
+ * extraction (Frame::ComputeStereoMatches, src/Frame.cc:831-1082). */
+typedef struct orb_batch orb_batch;
+int orb_batch_create(orb_context* ctx, int n, const int* widths, const int* heights, const int* nfeatures,
+                     float scaleFactor, int nlevels, int iniThFAST, int minThFAST, orb_batch** out);
+void orb_batch_destroy(orb_batch* b);
+int orb_batch_size(const orb_batch* b);
+/* Device pointer, pitch and size of a slot's level-0 image buffer. */
+int orb_batch_image(orb_batch* b, int slot, uint8_t** d_image, size_t* pitch, int* width, int* height);
+int orb_batch_upload(orb_batch* b, int slot, const uint8_t* image, size_t stride);
+int orb_batch_set_stereo(orb_batch* b, int npairs, const int* left_slots, const int* right_slots, float bf,
+                         float baseline);
+int orb_batch_run(orb_batch* b, int mode);
+int orb_batch_synchronize(orb_batch* b);
+/* Stage timing with HIP events on the batch's stream, averaged over `steps`
+ * mode-0 runs.  ms receives orb_batch_stage_count() floats: pyramid levels
+ * 0..L-1, FAST cells, quadtree, describe, stereo match, stereo median. */
+int orb_batch_stage_count(const orb_batch* b);
+int orb_batch_stage_times(orb_batch* b, int steps, float* ms);
+int orb_batch_results(orb_batch* b, int slot, orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n);
+int orb_batch_stereo_results(orb_batch* b, int pair, float* uRight, float* depth, int capacity, int* n);
 
 /* Debug accessors used by the parity tests.
  * kind 0: pyramid level, 1: its 7x7 blur, 2: FAST arc strength map. */

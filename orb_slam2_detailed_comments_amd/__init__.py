@@ -20,8 +20,8 @@ import os
 
 import numpy as np
 
-__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "match_bf", "compute_stereo_matches",
-           "stereo_frame", "lib", "library_path", "OrbError"]
+__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "match_bf",
+           "compute_stereo_matches", "stereo_frame", "lib", "library_path", "OrbError"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_PKG, "liborb_gpu.so")
@@ -54,6 +54,19 @@ SIGNATURES = {
     "orb_match_bf": (_I, [_P, _P, _I, _P, _I, _I, _F, _P, _P]),
     "orb_stereo_match": (_I, [_P, _P, _P, _P, _I, _P, _P, _I, _F, _F, _P, _P]),
     "orb_stereo_frame": (_I, [_P, _P, _P, _P, _I, _I, _S, _F, _F, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
+    "orb_context_stream": (_P, [_P]),
+    "orb_batch_create": (_I, [_P, _I, _P, _P, _P, _F, _I, _I, _I, _P]),
+    "orb_batch_destroy": (_V, [_P]),
+    "orb_batch_size": (_I, [_P]),
+    "orb_batch_image": (_I, [_P, _I, _P, _P, _P, _P]),
+    "orb_batch_upload": (_I, [_P, _I, _P, _S]),
+    "orb_batch_set_stereo": (_I, [_P, _I, _P, _P, _F, _F]),
+    "orb_batch_run": (_I, [_P, _I]),
+    "orb_batch_synchronize": (_I, [_P]),
+    "orb_batch_stage_count": (_I, [_P]),
+    "orb_batch_stage_times": (_I, [_P, _I, _P]),
+    "orb_batch_results": (_I, [_P, _I, _P, _I, _P, _P]),
+    "orb_batch_stereo_results": (_I, [_P, _I, _P, _P, _I, _P]),
     "orb_extractor_debug_image": (_I, [_P, _I, _I, _P, _S]),
     "orb_extractor_debug_candidates": (_I, [_P, _I, _P, _I, _P]),
 }
@@ -296,3 +309,85 @@ def stereo_frame(left, right, im_left, im_right, bf, b):
                                                      _ptr(dep)))
     n = nl.value
     return kl[:n].copy(), dl[:n].copy(), kr[:nr.value].copy(), dr[:nr.value].copy(), uR[:n].copy(), dep[:n].copy()
+
+
+class Batch:
+    """A fixed set of image slots extracted together (orb_batch_* in the C ABI).
+
+    run(mode): 0 = one launch per stage over all slots, 1 = the same captured
+    in a hipGraph and replayed, 2 = per-image launches of the same kernels.
+    """
+    STAGE_LAUNCHES, GRAPH, PER_IMAGE = 0, 1, 2
+
+    def __init__(self, sizes, nfeatures, scaleFactor=1.2, nlevels=8, iniThFAST=20, minThFAST=7, context=None):
+        self.ctx = context or Context.default()
+        n = len(sizes)
+        w = np.array([s[0] for s in sizes], np.int32)
+        h = np.array([s[1] for s in sizes], np.int32)
+        nf = np.broadcast_to(np.asarray(nfeatures, np.int32), (n,)).copy()
+        hb = ctypes.c_void_p()
+        _check("orb_batch_create", lib().orb_batch_create(self.ctx.h, n, _ptr(w), _ptr(h), _ptr(nf),
+                                                          float(scaleFactor), int(nlevels), int(iniThFAST),
+                                                          int(minThFAST), ctypes.byref(hb)))
+        self.h = hb
+        self.sizes = [tuple(s) for s in sizes]
+        self.nfeatures = nf
+        self.nlevels = int(nlevels)
+        self.npairs = 0
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orb_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def upload(self, slot, image):
+        img = np.ascontiguousarray(image, np.uint8)
+        if img.shape != (self.sizes[slot][1], self.sizes[slot][0]):
+            raise ValueError(f"slot {slot} expects {self.sizes[slot]}, got {img.shape[::-1]}")
+        _check("orb_batch_upload", lib().orb_batch_upload(self.h, slot, _ptr(img), img.strides[0]))
+
+    def set_stereo(self, left_slots, right_slots, bf, baseline):
+        l = np.ascontiguousarray(left_slots, np.int32)
+        r = np.ascontiguousarray(right_slots, np.int32)
+        _check("orb_batch_set_stereo", lib().orb_batch_set_stereo(self.h, len(l), _ptr(l), _ptr(r), float(bf),
+                                                                  float(baseline)))
+        self.npairs = len(l)
+
+    def run(self, mode=GRAPH):
+        _check("orb_batch_run", lib().orb_batch_run(self.h, int(mode)))
+
+    def synchronize(self):
+        _check("orb_batch_synchronize", lib().orb_batch_synchronize(self.h))
+
+    def stage_times(self, steps):
+        n = lib().orb_batch_stage_count(self.h)
+        ms = np.zeros(n, np.float32)
+        _check("orb_batch_stage_times", lib().orb_batch_stage_times(self.h, int(steps), _ptr(ms)))
+        names = [f"pyramid_l{l}" for l in range(n - 5)] + ["fast_cells", "octree", "describe", "stereo_match",
+                                                            "stereo_median"]
+        return dict(zip(names, ms.tolist()))
+
+    def results(self, slot):
+        w, h = self.sizes[slot]
+        cap = int(self.nfeatures[slot]) + 64 * self.nlevels
+        kps = np.zeros(cap, KEYPOINT_DTYPE)
+        desc = np.zeros((cap, 32), np.uint8)
+        n = ctypes.c_int()
+        _check("orb_batch_results", lib().orb_batch_results(self.h, slot, _ptr(kps), cap, _ptr(desc),
+                                                            ctypes.byref(n)))
+        return kps[:n.value].copy(), desc[:n.value].copy()
+
+    def stereo_results(self, pair):
+        cap = 1 << 16
+        uR = np.zeros(cap, np.float32)
+        dep = np.zeros(cap, np.float32)
+        n = ctypes.c_int()
+        _check("orb_batch_stereo_results", lib().orb_batch_stereo_results(self.h, pair, _ptr(uR), _ptr(dep), cap,
+                                                                          ctypes.byref(n)))
+        return uR[:n.value].copy(), dep[:n.value].copy()

@@ -827,12 +827,6 @@ __global__ void zero_counters_kernel(const ImageJob* __restrict__ jobs, int njob
     if (j < njobs && threadIdx.x < ORB_CNT_SLOTS) jobs[j].counters[threadIdx.x] = 0;
 }
 
-__global__ void zero_ints_kernel(int* p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 // ========================================================= Hamming helpers
 __device__ __forceinline__ int hamming32(const uint4 a0, const uint4 a1, const uint8_t* b)
 {
@@ -848,10 +842,24 @@ __device__ __forceinline__ int hamming32(const uint4 a0, const uint4 a1, const u
 // smallest (distance, index) below TH_HIGH.  Refinement: 11x11 SAD of
 // centre-subtracted patches over 11 shifts, parabola fit, depth = bf/disp.
 
-__global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs A)
+__device__ __forceinline__ int find_pair(const StereoArgs* args, int n, int x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (args[mid].wave_base <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restrict__ args, int npairs, int nwaves)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int iL = blockIdx.x * 4 + wv;
+    const int wid = blockIdx.x * 4 + wv;
+    if (wid >= nwaves) return;
+    const StereoArgs& A = args[find_pair(args, npairs, wid)];
+    const int iL = wid - A.wave_base;
     const int nl = *A.nl, nr = *A.nr;
     if (iL >= nl || iL >= A.cap_l) return;
     float outU = -1.0f, outD = -1.0f;
@@ -993,9 +1001,10 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs A)
 // Median-based outlier cut (Frame.cc:1066-1081): sort the matched SAD values,
 // th = 1.5f*1.4f*median, drop matches with SAD >= th.
 #define MEDIAN_CAP 4096
-__global__ __launch_bounds__(1024) void stereo_median_kernel(const StereoArgs A)
+__global__ __launch_bounds__(1024) void stereo_median_kernel(const StereoArgs* __restrict__ args)
 {
     __shared__ int v[MEDIAN_CAP];
+    const StereoArgs& A = args[blockIdx.x];
     const int M = min(*A.match_count, MEDIAN_CAP);
     if (M == 0) return;
     int npow = 1;
@@ -1074,27 +1083,46 @@ __global__ __launch_bounds__(256) void match_bf_kernel(const uint8_t* __restrict
 }
 
 // ============================================================= launchers
+// ev (optional): events recorded after each stage -- [0] start, then one per
+// pyramid level (max_levels), cells, octree, describe.
 extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable* d_rt, const RunTable* h_rt,
-                                         hipStream_t st)
+                                         hipStream_t st, hipEvent_t* ev)
 {
+    int e = 0;
+    if (ev) hipEventRecord(ev[e++], st);
     zero_counters_kernel<<<h_rt->njobs, 64, 0, st>>>(d_jobs, h_rt->njobs);
     for (int l = 0; l < h_rt->max_levels; l++) {
         const int nb = h_rt->tile_start[l][h_rt->njobs];
         if (nb > 0) level_kernel<<<nb, 256, 0, st>>>(d_jobs, d_rt, l);
+        if (ev) hipEventRecord(ev[e++], st);
     }
     const int ncells = h_rt->cell_start[h_rt->njobs * h_rt->max_levels];
     if (ncells > 0) cells_kernel<<<(ncells + 3) / 4, 256, 0, st>>>(d_jobs, d_rt);
+    if (ev) hipEventRecord(ev[e++], st);
     octree_kernel<<<h_rt->njobs * h_rt->max_levels, OCT_T, 0, st>>>(d_jobs, h_rt->max_levels);
+    if (ev) hipEventRecord(ev[e++], st);
     const int nslots = h_rt->slot_start[h_rt->njobs];
     if (nslots > 0) describe_kernel<<<(nslots + 3) / 4, 256, 0, st>>>(d_jobs, d_rt);
+    if (ev) hipEventRecord(ev[e++], st);
     return hipGetLastError();
 }
 
-extern "C" hipError_t orb_launch_stereo(const StereoArgs* a, hipStream_t st)
+__global__ void zero_match_counts_kernel(const StereoArgs* __restrict__ args, int npairs)
 {
-    zero_ints_kernel<<<1, 64, 0, st>>>(a->match_count, 1);
-    if (a->cap_l > 0) stereo_kernel<<<(a->cap_l + 3) / 4, 256, 0, st>>>(*a);
-    stereo_median_kernel<<<1, 1024, 0, st>>>(*a);
+    for (int p = threadIdx.x; p < npairs; p += blockDim.x) *args[p].match_count = 0;
+}
+
+// d_args: npairs StereoArgs in device memory with wave_base = prefix of cap_l.
+// ev (optional): events after the match kernel and after the median cut.
+extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, int nwaves, hipStream_t st,
+                                        hipEvent_t* ev)
+{
+    if (npairs <= 0) return hipSuccess;
+    zero_match_counts_kernel<<<1, 256, 0, st>>>(d_args, npairs);
+    if (nwaves > 0) stereo_kernel<<<(nwaves + 3) / 4, 256, 0, st>>>(d_args, npairs, nwaves);
+    if (ev) hipEventRecord(ev[0], st);
+    stereo_median_kernel<<<npairs, 1024, 0, st>>>(d_args);
+    if (ev) hipEventRecord(ev[1], st);
     return hipGetLastError();
 }
 

@@ -18,8 +18,9 @@
 
 extern "C" hipError_t orb_upload_constants(const int* umax16, const int* gauss7);
 extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable* d_rt, const RunTable* h_rt,
-                                         hipStream_t st);
-extern "C" hipError_t orb_launch_stereo(const StereoArgs* a, hipStream_t st);
+                                         hipStream_t st, hipEvent_t* ev);
+extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, int nwaves, hipStream_t st,
+                                        hipEvent_t* ev);
 extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const uint8_t* t, const int* nt,
                                           int th_low, float ratio, int* idx, int* dist, int cap_q,
                                           hipStream_t st);
@@ -129,6 +130,25 @@ struct orb_extractor {
     ImageJob* d_pair_jobs = nullptr;
     RunTable* h_pair_rt = nullptr;
     RunTable* d_pair_rt = nullptr;
+    // device copies of the stereo arguments: [0] extractor-resident keypoints,
+    // [1] caller-uploaded keypoints
+    StereoArgs* d_sargs = nullptr;
+    StereoArgs h_sargs[2];
+    bool sargs_valid[2] = {false, false};
+
+    int upload_stereo_args(int which, const StereoArgs& a)
+    {
+        if (!d_sargs && hipMalloc(&d_sargs, 2 * sizeof(StereoArgs)) != hipSuccess) {
+            d_sargs = nullptr;
+            return ORB_E_OOM;
+        }
+        if (sargs_valid[which] && std::memcmp(&a, &h_sargs[which], sizeof(a)) == 0) return ORB_OK;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(d_sargs + which, &a, sizeof(a), hipMemcpyHostToDevice));
+        h_sargs[which] = a;
+        sargs_valid[which] = true;
+        return ORB_OK;
+    }
 
     // ORBextractor::ORBextractor (ORBextractor.cc:469-526): scale tables in
     // float (mvScaleFactor[i-1]*scaleFactor with the double member), inverse
@@ -426,6 +446,7 @@ struct orb_extractor {
         if (d_pair_jobs) (void)hipFree(d_pair_jobs);
         if (d_pair_rt) (void)hipFree(d_pair_rt);
         if (h_pair_rt) (void)hipHostFree(h_pair_rt);
+        if (d_sargs) (void)hipFree(d_sargs);
     }
 };
 
@@ -543,7 +564,7 @@ int orb_extract_device(orb_extractor* ex, const uint8_t* d_image, int width, int
     if (rc) return rc;
     rc = ex->upload_image(d_image, pitch, hipMemcpyDeviceToDevice);
     if (rc) return rc;
-    HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, ex->ctx->stream));
+    HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, ex->ctx->stream, nullptr));
     return ORB_OK;
 }
 
@@ -568,7 +589,7 @@ int orb_extract(orb_extractor* ex, const uint8_t* image, int width, int height, 
     if (rc) return rc;
     rc = ex->upload_image(image, stride, hipMemcpyHostToDevice);
     if (rc) return rc;
-    HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, ex->ctx->stream));
+    HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, ex->ctx->stream, nullptr));
     return ex->read_results(keypoints, capacity, descriptors, n);
 }
 
@@ -674,7 +695,9 @@ int orb_stereo_match(orb_extractor* left, orb_extractor* right, const orb_keypoi
     a.dr = d_dr;
     a.nr = left->d_misc + 2;
     a.cap_l = nl;
-    HIP_TRY(orb_launch_stereo(&a, st));
+    rc = left->upload_stereo_args(1, a);
+    if (rc) return rc;
+    HIP_TRY(orb_launch_stereo(left->d_sargs + 1, 1, nl, st, nullptr));
     HIP_TRY(hipMemcpyAsync(uRight, left->d_uR, (size_t)nl * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(depth, left->d_depth, (size_t)nl * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -700,7 +723,7 @@ int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* i
     rc = left->upload_image(im_left, stride, hipMemcpyHostToDevice);
     if (!rc) rc = right->upload_image(im_right, stride, hipMemcpyHostToDevice);
     if (rc) return rc;
-    HIP_TRY(orb_launch_extract(left->d_pair_jobs, left->d_pair_rt, left->h_pair_rt, st));
+    HIP_TRY(orb_launch_extract(left->d_pair_jobs, left->d_pair_rt, left->h_pair_rt, st, nullptr));
     StereoArgs a;
     left->stereo_args(a, right, left->d_pair_jobs, left->d_pair_jobs + 1, bf, b);
     a.kl = left->hjob.kps;
@@ -710,7 +733,9 @@ int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* i
     a.dr = right->hjob.desc;
     a.nr = right->hjob.counters + ORB_CNT_NOUT;
     a.cap_l = left->plan.out_cap;
-    HIP_TRY(orb_launch_stereo(&a, st));
+    rc = left->upload_stereo_args(0, a);
+    if (rc) return rc;
+    HIP_TRY(orb_launch_stereo(left->d_sargs, 1, a.cap_l, st, nullptr));
     rc = left->read_results(kl, capacity, dl, nl);
     if (rc) return rc;
     rc = right->read_results(kr, capacity, dr, nr);
@@ -758,6 +783,271 @@ int orb_extractor_debug_candidates(orb_extractor* ex, int level, int* xyr, int c
         xyr[3 * i] = j * L.wCell + xl;
         xyr[3 * i + 1] = r * L.hCell + yl;
         xyr[3 * i + 2] = (int)(keys[i] >> 32);
+    }
+    return ORB_OK;
+}
+
+}  // extern "C"
+
+// ================================================================== batch
+// A fixed set of image slots extracted together: one launch per stage over
+// every slot (mode 0), the same launches replayed from one captured hipGraph
+// (mode 1), or the per-image launch sequence for comparison (mode 2).
+// Optional stereo pairs among the slots are matched in one multi-pair launch.
+struct orb_batch {
+    orb_context* ctx = nullptr;
+    std::vector<orb_extractor*> ex;
+    ImageJob* d_jobs = nullptr;
+    RunTable* h_rt = nullptr;
+    RunTable* d_rt = nullptr;
+    std::vector<int> pl, pr;
+    StereoArgs* d_sargs = nullptr;
+    int nwaves = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    // ev (optional, mode 0): stage events, see orb_batch_stage_times.
+    int enqueue(int mode, hipEvent_t* ev = nullptr)
+    {
+        hipStream_t st = ctx->stream;
+        if (mode == 2) {
+            for (orb_extractor* e : ex) HIP_TRY(orb_launch_extract(e->d_job, e->d_rt, e->h_rt, st, nullptr));
+        } else {
+            HIP_TRY(orb_launch_extract(d_jobs, d_rt, h_rt, st, ev));
+        }
+        if (!pl.empty())
+            HIP_TRY(orb_launch_stereo(d_sargs, (int)pl.size(), nwaves, st, ev ? ev + h_rt->max_levels + 4 : nullptr));
+        return ORB_OK;
+    }
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+
+    ~orb_batch()
+    {
+        drop_graph();
+        for (orb_extractor* e : ex) delete e;
+        if (d_jobs) (void)hipFree(d_jobs);
+        if (d_rt) (void)hipFree(d_rt);
+        if (h_rt) (void)hipHostFree(h_rt);
+        if (d_sargs) (void)hipFree(d_sargs);
+    }
+};
+
+extern "C" {
+
+int orb_batch_create(orb_context* ctx, int n, const int* widths, const int* heights, const int* nfeatures,
+                     float scaleFactor, int nlevels, int iniThFAST, int minThFAST, orb_batch** out)
+{
+    if (!ctx || !out || n <= 0 || n > ORB_MAX_JOBS || !widths || !heights || !nfeatures) return ORB_E_ARG;
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(ctx->device));
+    orb_batch* b = new orb_batch();
+    b->ctx = ctx;
+    for (int i = 0; i < n; i++) {
+        orb_extractor* e = nullptr;
+        int rc = orb_extractor_create(ctx, nfeatures[i], scaleFactor, nlevels, iniThFAST, minThFAST, &e);
+        if (!rc) {
+            b->ex.push_back(e);
+            rc = e->ensure(widths[i], heights[i]);
+        }
+        if (rc) {
+            delete b;
+            return rc;
+        }
+    }
+    std::vector<ImageJob> jobs(n);
+    for (int i = 0; i < n; i++) jobs[i] = b->ex[i]->hjob;
+    if (hipMalloc(&b->d_jobs, n * sizeof(ImageJob)) != hipSuccess ||
+        hipMalloc(&b->d_rt, sizeof(RunTable)) != hipSuccess ||
+        hipHostMalloc(&b->h_rt, sizeof(RunTable)) != hipSuccess) {
+        delete b;
+        return ORB_E_OOM;
+    }
+    orb_extractor::build_table(b->ex.data(), n, *b->h_rt);
+    if (hipMemcpy(b->d_jobs, jobs.data(), n * sizeof(ImageJob), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_rt, b->h_rt, sizeof(RunTable), hipMemcpyHostToDevice) != hipSuccess) {
+        delete b;
+        return ORB_E_DEVICE;
+    }
+    *out = b;
+    return ORB_OK;
+}
+
+void orb_batch_destroy(orb_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int orb_batch_size(const orb_batch* b) { return b ? (int)b->ex.size() : ORB_E_ARG; }
+
+int orb_batch_image(orb_batch* b, int slot, uint8_t** d_image, size_t* pitch, int* width, int* height)
+{
+    if (!b || slot < 0 || slot >= (int)b->ex.size()) return ORB_E_ARG;
+    const orb_extractor* e = b->ex[slot];
+    if (d_image) *d_image = e->hjob.lv[0].lvl;
+    if (pitch) *pitch = (size_t)e->hjob.lv[0].pitch;
+    if (width) *width = e->plan.W;
+    if (height) *height = e->plan.H;
+    return ORB_OK;
+}
+
+int orb_batch_upload(orb_batch* b, int slot, const uint8_t* image, size_t stride)
+{
+    if (!b || slot < 0 || slot >= (int)b->ex.size() || !image) return ORB_E_ARG;
+    orb_extractor* e = b->ex[slot];
+    if (stride < (size_t)e->plan.W) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    int rc = e->upload_image(image, stride, hipMemcpyHostToDevice);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return ORB_OK;
+}
+
+int orb_batch_set_stereo(orb_batch* b, int npairs, const int* left_slots, const int* right_slots, float bf,
+                         float baseline)
+{
+    if (!b || npairs < 0 || (npairs && (!left_slots || !right_slots))) return ORB_E_ARG;
+    const int n = (int)b->ex.size();
+    std::vector<StereoArgs> args(npairs);
+    int waves = 0;
+    for (int p = 0; p < npairs; p++) {
+        const int l = left_slots[p], r = right_slots[p];
+        if (l < 0 || l >= n || r < 0 || r >= n || l == r) return ORB_E_ARG;
+        orb_extractor* L = b->ex[l];
+        orb_extractor* R = b->ex[r];
+        if (L->plan.W != R->plan.W || L->plan.H != R->plan.H || L->nlevels != R->nlevels) return ORB_E_ARG;
+        if (L->plan.out_cap > 4096) return ORB_E_ARG;
+        StereoArgs& a = args[p];
+        L->stereo_args(a, R, b->d_jobs + l, b->d_jobs + r, bf, baseline);
+        a.kl = L->hjob.kps;
+        a.dl = L->hjob.desc;
+        a.nl = L->hjob.counters + ORB_CNT_NOUT;
+        a.kr = R->hjob.kps;
+        a.dr = R->hjob.desc;
+        a.nr = R->hjob.counters + ORB_CNT_NOUT;
+        a.cap_l = L->plan.out_cap;
+        a.wave_base = waves;
+        waves += a.cap_l;
+    }
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    b->drop_graph();
+    if (b->d_sargs) (void)hipFree(b->d_sargs);
+    b->d_sargs = nullptr;
+    b->pl.clear();
+    b->pr.clear();
+    b->nwaves = 0;
+    if (npairs == 0) return ORB_OK;
+    if (hipMalloc(&b->d_sargs, npairs * sizeof(StereoArgs)) != hipSuccess) return ORB_E_OOM;
+    HIP_TRY(hipMemcpy(b->d_sargs, args.data(), npairs * sizeof(StereoArgs), hipMemcpyHostToDevice));
+    b->pl.assign(left_slots, left_slots + npairs);
+    b->pr.assign(right_slots, right_slots + npairs);
+    b->nwaves = waves;
+    return ORB_OK;
+}
+
+int orb_batch_run(orb_batch* b, int mode)
+{
+    if (!b || mode < 0 || mode > 2) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    if (mode != 1) return b->enqueue(mode);
+    hipStream_t st = b->ctx->stream;
+    if (!b->exec) {
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = b->enqueue(0);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : ORB_E_DEVICE;
+        }
+        b->graph = g;
+        if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            b->drop_graph();
+            return ORB_E_DEVICE;
+        }
+    }
+    HIP_TRY(hipGraphLaunch(b->exec, st));
+    return ORB_OK;
+}
+
+int orb_batch_stage_count(const orb_batch* b)
+{
+    return b ? b->h_rt->max_levels + 5 : ORB_E_ARG;
+}
+
+// Per-stage device time (HIP events on the batch's stream) averaged over
+// `steps` mode-0 runs: pyramid levels 0..L-1, cells, octree, describe, stereo
+// match, stereo median.  ms receives orb_batch_stage_count() floats.
+int orb_batch_stage_times(orb_batch* b, int steps, float* ms)
+{
+    if (!b || steps <= 0 || !ms) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    const int L = b->h_rt->max_levels;
+    const int nev = L + 6;  // start, L levels, cells, octree, describe, stereo, median
+    std::vector<hipEvent_t> ev(nev);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    std::vector<double> acc(nev - 1, 0.0);
+    int rc = ORB_OK;
+    for (int s = 0; s < steps && !rc; s++) {
+        rc = b->enqueue(0, ev.data());
+        if (rc) break;
+        if (b->pl.empty()) {
+            (void)hipEventRecord(ev[L + 4], b->ctx->stream);
+            (void)hipEventRecord(ev[L + 5], b->ctx->stream);
+        }
+        if (hipEventSynchronize(ev[nev - 1]) != hipSuccess) rc = ORB_E_DEVICE;
+        for (int i = 0; i + 1 < nev && !rc; i++) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) != hipSuccess) rc = ORB_E_DEVICE;
+            acc[i] += t;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    for (int i = 0; i + 1 < nev; i++) ms[i] = (float)(acc[i] / steps);
+    return ORB_OK;
+}
+
+void* orb_context_stream(orb_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int orb_batch_synchronize(orb_batch* b)
+{
+    if (!b) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return ORB_OK;
+}
+
+int orb_batch_results(orb_batch* b, int slot, orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n)
+{
+    if (!b || slot < 0 || slot >= (int)b->ex.size() || !n) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    return b->ex[slot]->read_results(keypoints, capacity, descriptors, n);
+}
+
+int orb_batch_stereo_results(orb_batch* b, int pair, float* uRight, float* depth, int capacity, int* n)
+{
+    if (!b || pair < 0 || pair >= (int)b->pl.size() || !n) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    orb_extractor* L = b->ex[b->pl[pair]];
+    int c[ORB_CNT_SLOTS];
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    HIP_TRY(hipMemcpy(c, L->hjob.counters, sizeof(c), hipMemcpyDeviceToHost));
+    *n = c[ORB_CNT_NOUT];
+    if (*n > capacity) return ORB_E_CAPACITY;
+    if (*n > 0) {
+        if (uRight) HIP_TRY(hipMemcpy(uRight, L->d_uR, (size_t)*n * 4, hipMemcpyDeviceToHost));
+        if (depth) HIP_TRY(hipMemcpy(depth, L->d_depth, (size_t)*n * 4, hipMemcpyDeviceToHost));
     }
     return ORB_OK;
 }

@@ -64,15 +64,16 @@ struct ImageJob {
     LevelJob lv[ORB_MAX_LEVELS];
 };
 
+#define ORB_MAX_JOBS 256
+
 // Launch table for one run over a set of jobs (prefix sums in block units).
 struct RunTable {
     int njobs;
     int max_levels;
-    int tile_start[ORB_MAX_LEVELS][65];   // per level: first tile block of job j
-    int cell_start[65 * ORB_MAX_LEVELS];  // per (job, level): first cell wave
-    int slot_start[65];                   // describe kernel: first slot of job j
+    int tile_start[ORB_MAX_LEVELS][ORB_MAX_JOBS + 1];   // per level: first tile block of job j
+    int cell_start[(ORB_MAX_JOBS + 1) * ORB_MAX_LEVELS];  // per (job, level): first cell wave
+    int slot_start[ORB_MAX_JOBS + 1];                   // describe kernel: first slot of job j
 };
-#define ORB_MAX_JOBS 64
 
 // Arguments of one stereo match (Frame::ComputeStereoMatches).
 struct StereoArgs {
@@ -85,6 +86,7 @@ struct StereoArgs {
     const uint8_t* dr;
     const int* nr;
     int cap_l;
+    int wave_base;          // first wavefront of this pair in a multi-pair launch
     int nrows;
     float bf, mb;
     float scale[ORB_MAX_LEVELS], inv_scale[ORB_MAX_LEVELS];
