@@ -56,10 +56,8 @@ def stage_bytes(stage, n_images, kps_per_image, cands_per_image):
         l = int(stage[len("pyramid_l"):])
         w, h = sizes[l]
         src = W * H if l == 0 else sizes[l - 1][0] * sizes[l - 1][1]
-        # read the source level; write level (l>0), its blur and FAST strength
-        return n_images * (src + (3 if l > 0 else 2) * w * h)
-    if stage == "fast_cells":
-        return n_images * (sum(w * h for w, h in sizes) + 8 * cands_per_image)
+        # read the source level; write the level (l>0) and its blur
+        return n_images * (src + (2 if l > 0 else 1) * w * h)
     if stage == "octree":
         return n_images * (8 * cands_per_image + 8 * kps_per_image)
     if stage == "describe":

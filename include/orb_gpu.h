@@ -141,14 +141,15 @@ int orb_batch_run(orb_batch* b, int mode);
 int orb_batch_synchronize(orb_batch* b);
 /* Stage timing with HIP events on the batch's stream, averaged over `steps`
  * mode-0 runs.  ms receives orb_batch_stage_count() floats: pyramid levels
- * 0..L-1, FAST cells, quadtree, describe, stereo match, stereo median. */
+ * 0..L-1 (pyramid, blur, FAST with per-cell suppression), quadtree, describe,
+ * stereo match, stereo median. */
 int orb_batch_stage_count(const orb_batch* b);
 int orb_batch_stage_times(orb_batch* b, int steps, float* ms);
 int orb_batch_results(orb_batch* b, int slot, orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n);
 int orb_batch_stereo_results(orb_batch* b, int pair, float* uRight, float* depth, int capacity, int* n);
 
 /* Debug accessors used by the parity tests.
- * kind 0: pyramid level, 1: its 7x7 blur, 2: FAST arc strength map. */
+ * kind 0: pyramid level, 1: its 7x7 blur. */
 int orb_extractor_debug_image(orb_extractor* ex, int level, int kind, uint8_t* dst, size_t dst_stride);
 /* FAST candidates of a level as (x, y, response) triples relative to the
  * level's 16-px border, sorted in the reference's vToDistributeKeys order. */

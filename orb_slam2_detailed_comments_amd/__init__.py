@@ -369,8 +369,7 @@ class Batch:
         n = lib().orb_batch_stage_count(self.h)
         ms = np.zeros(n, np.float32)
         _check("orb_batch_stage_times", lib().orb_batch_stage_times(self.h, int(steps), _ptr(ms)))
-        names = [f"pyramid_l{l}" for l in range(n - 5)] + ["fast_cells", "octree", "describe", "stereo_match",
-                                                            "stereo_median"]
+        names = [f"pyramid_l{l}" for l in range(n - 4)] + ["octree", "describe", "stereo_match", "stereo_median"]
         return dict(zip(names, ms.tolist()))
 
     def results(self, slot):

@@ -3,11 +3,11 @@
 // Pipeline per image (one launch per stage covers every job of a run):
 //   level_kernel   (x nlevels) pyramid level l from level l-1 (8-bit fixed
 //                  point bilinear, ORBextractor.cc:1677), its 7x7 sigma-2
-//                  blur (ORBextractor.cc:1607-1615) and the FAST arc
-//                  strength of every pixel, from one LDS tile with a 3-px halo
-//   cells_kernel   FAST-9 per 30-px cell with the iniThFAST/minThFAST retry
-//                  and 3x3 non-max suppression (ORBextractor.cc:1075-1142)
-//   octree_kernel  DistributeOctTree as a per-level workgroup quadtree
+//                  blur (ORBextractor.cc:1607-1615), FAST-9 with per-cell
+//                  non-max suppression at both thresholds
+//                  (ORBextractor.cc:1075-1142), from one LDS tile
+//   octree_kernel  the per-cell iniThFAST/minThFAST retry, then
+//                  DistributeOctTree as a per-level workgroup quadtree
 //                  (ORBextractor.cc:688-1033)
 //   describe_kernel IC_Angle + rotated BRIEF + output assembly
 //                  (ORBextractor.cc:94-204, 1590-1648)
@@ -66,13 +66,30 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask)
 }
 
 // ================================================================ level
-// Extended tile: output tile plus 3 px on each side (blur radius and FAST
-// ring radius), with coordinates reflected (REFLECT_101) at the level edges.
-#define EXT_W (ORB_TILE_W + 6)
-#define EXT_H (ORB_TILE_H + 6)
-#define EXT_P 72
+// One workgroup per 64x32 output tile of pyramid level l:
+//   1. level pixels for the tile plus a 4-px halo (REFLECT_101 at the level
+//      edges) into LDS: copied for level 0, else cv::resize INTER_LINEAR 8U
+//      from level l-1 (ComputePyramid, ORBextractor.cc:1677);
+//   2. the level and its 7x7 sigma-2 blur (ORBextractor.cc:1607-1615);
+//   3. FAST-9 arc strength for the tile plus a 1-px ring, computed exactly only
+//      for pixels that pass a necessary cardinal-point test, after compaction;
+//   4. per-cell non-max suppression at iniThFAST and at minThFAST: a neighbour
+//      counts only if it lies in the same 30-px cell's detection area, which is
+//      exactly cv::FAST run on that cell's window (ORBextractor.cc:1075-1142);
+//   5. candidates of both thresholds appended to the level's key list, and a
+//      per-cell flag set when the cell has an iniThFAST survivor.  The quadtree
+//      kernel keeps minThFAST candidates only for cells without that flag,
+//      which is the reference's per-cell retry.
+#define LT_W ORB_TILE_W
+#define LT_H ORB_TILE_H
+#define LT_HALO 4
+#define LP_W (LT_W + 2 * LT_HALO)
+#define LP_H (LT_H + 2 * LT_HALO)
+#define LM_W (LT_W + 2)
+#define LM_H (LT_H + 2)
+#define LK_CAP 1536
 
-__device__ __forceinline__ int fast_strength(const uint8_t (*t)[EXT_P], int cx, int cy)
+__device__ __forceinline__ int fast_strength(const uint8_t (*t)[LP_W], int cx, int cy)
 {
     // Bresenham radius-3 circle in circular order (OpenCV makeOffsets(16)).
     const int v = t[cy][cx];
@@ -112,48 +129,82 @@ __device__ __forceinline__ int fast_strength(const uint8_t (*t)[EXT_P], int cx, 
     return s < 0 ? 0 : s;
 }
 
+// Necessary condition for strength > t: a 9-run covers two circularly
+// adjacent cardinal points (ring positions 0, 4, 8, 12).
+__device__ __forceinline__ bool fast_maybe(const uint8_t (*t)[LP_W], int cx, int cy, int thr)
+{
+    const int v = t[cy][cx];
+    const int p0 = t[cy + 3][cx], p4 = t[cy][cx + 3], p8 = t[cy - 3][cx], p12 = t[cy][cx - 3];
+    const int hi = v + thr, lo = v - thr;
+    const unsigned b = (p0 > hi) | ((p4 > hi) << 1) | ((p8 > hi) << 2) | ((p12 > hi) << 3);
+    const unsigned k = (p0 < lo) | ((p4 < lo) << 1) | ((p8 < lo) << 2) | ((p12 < lo) << 3);
+    const unsigned rb = ((b >> 1) | (b << 3)) & 15u, rk = ((k >> 1) | (k << 3)) & 15u;
+    return ((b & rb) | (k & rk)) != 0u;
+}
+
 __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__ jobs,
                                                     const RunTable* __restrict__ rt, int level)
 {
-    __shared__ uint8_t tile[EXT_H][EXT_P];
-    __shared__ int hs[EXT_H][ORB_TILE_W];
-    __shared__ int s_job;
+    __shared__ uint8_t px[LP_H][LP_W];
+    __shared__ int hs[LT_H + 6][LT_W];
+    __shared__ uint8_t mm[LM_H][LM_W];
+    __shared__ unsigned short queue[LM_H * LM_W];
+    __shared__ unsigned long long kbuf[LK_CAP];
+    __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
+    __shared__ int s_job, s_qn, s_kn, s_base;
     const int tid = threadIdx.x;
-    if (tid == 0) s_job = find_segment(rt->tile_start[level], rt->njobs, blockIdx.x);
+    const int lane = tid & 63;
+    if (tid == 0) {
+        s_job = find_segment(rt->tile_start[level], rt->njobs, blockIdx.x);
+        s_qn = 0;
+        s_kn = 0;
+    }
     __syncthreads();
     const ImageJob& J = jobs[s_job];
     const LevelJob& L = J.lv[level];
     const int t = blockIdx.x - rt->tile_start[level][s_job];
-    const int tx0 = (t % L.tiles_x) * ORB_TILE_W;
-    const int ty0 = (t / L.tiles_x) * ORB_TILE_H;
+    const int tx0 = (t % L.tiles_x) * LT_W;
+    const int ty0 = (t / L.tiles_x) * LT_H;
     const int w = L.w, h = L.h, pitch = L.pitch;
 
+    // ---- 1. pixels with halo
     if (level == 0) {
-        for (int e = tid; e < EXT_W * EXT_H; e += 256) {
-            const int ex = e % EXT_W, ey = e / EXT_W;
-            const int x = reflect101(tx0 - 3 + ex, w), y = reflect101(ty0 - 3 + ey, h);
-            tile[ey][ex] = L.lvl[(size_t)y * pitch + x];
+        for (int e = tid; e < LP_W * LP_H; e += 256) {
+            const int ex = e % LP_W, ey = e / LP_W;
+            const int x = reflect101(tx0 - LT_HALO + ex, w), y = reflect101(ty0 - LT_HALO + ey, h);
+            px[ey][ex] = L.lvl[(size_t)y * pitch + x];
         }
     } else {
-        // cv::resize INTER_LINEAR, 8U fixed point, from level-1 (see oracle
-        // spec): double source coordinate, float fraction, 11-bit taps.
+        // cv::resize INTER_LINEAR 8U fixed point from level l-1: double source
+        // coordinate, float fraction, 11-bit taps (half-even), computed once
+        // per tile column and row.
         const LevelJob& P = J.lv[level - 1];
         const int sw = P.w, sh = P.h, sp = P.pitch;
-        const double scale_x = 1. / ((double)w / sw);
-        const double scale_y = 1. / ((double)h / sh);
-        for (int e = tid; e < EXT_W * EXT_H; e += 256) {
-            const int ex = e % EXT_W, ey = e / EXT_W;
-            const int x = reflect101(tx0 - 3 + ex, w), y = reflect101(ty0 - 3 + ey, h);
+        if (tid < LP_W) {
+            const double scale_x = 1. / ((double)w / sw);
+            const int x = reflect101(tx0 - LT_HALO + tid, w);
             float fx = (float)((x + 0.5) * scale_x - 0.5);
             int sx = (int)floorf(fx);
             fx -= sx;
             if (sx < 0) { fx = 0.f; sx = 0; }
             if (sx >= sw - 1) { fx = 0.f; sx = sw - 1; }
-            const int a0 = (int)rintf((1.f - fx) * 2048.f), a1 = (int)rintf(fx * 2048.f);
+            xs0[tid] = sx;
+            xa[tid] = (int)rintf((1.f - fx) * 2048.f) | ((int)rintf(fx * 2048.f) << 16);
+        } else if (tid >= 128 && tid < 128 + LP_H) {
+            const int r = tid - 128;
+            const double scale_y = 1. / ((double)h / sh);
+            const int y = reflect101(ty0 - LT_HALO + r, h);
             float fy = (float)((y + 0.5) * scale_y - 0.5);
             const int sy = (int)floorf(fy);
             fy -= sy;
-            const int b0 = (int)rintf((1.f - fy) * 2048.f), b1 = (int)rintf(fy * 2048.f);
+            ys0[r] = sy;
+            yb[r] = (int)rintf((1.f - fy) * 2048.f) | ((int)rintf(fy * 2048.f) << 16);
+        }
+        __syncthreads();
+        for (int e = tid; e < LP_W * LP_H; e += 256) {
+            const int ex = e % LP_W, ey = e / LP_W;
+            const int sx = xs0[ex], a0 = xa[ex] & 0xFFFF, a1 = xa[ex] >> 16;
+            const int sy = ys0[ey], b0 = yb[ey] & 0xFFFF, b1 = yb[ey] >> 16;
             const int r0 = min(max(sy, 0), sh - 1), r1 = min(max(sy + 1, 0), sh - 1);
             const int nx = sx + 1 < sw ? sx + 1 : sx;
             const uint8_t* p0 = P.lvl + (size_t)r0 * sp;
@@ -161,156 +212,141 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             const int S0 = p0[sx] * a0 + p0[nx] * a1;
             const int S1 = p1[sx] * a0 + p1[nx] * a1;
             const int v = (S0 * b0 + S1 * b1 + (1 << 21)) >> 22;
-            tile[ey][ex] = (uint8_t)(v > 255 ? 255 : (v < 0 ? 0 : v));
+            px[ey][ex] = (uint8_t)(v > 255 ? 255 : (v < 0 ? 0 : v));
         }
     }
     __syncthreads();
 
-    // Each thread owns 4 horizontally adjacent output pixels.
-    const int ly = tid >> 4, lx = (tid & 15) * 4;
-    const int y = ty0 + ly, x0 = tx0 + lx;
-    const bool row_in = y < h;
-
-    if (level > 0 && row_in) {
-        uint8_t* dst = L.lvl + (size_t)y * pitch + x0;
-        if (x0 + 3 < w) {
-            uchar4 v = make_uchar4(tile[ly + 3][lx + 3], tile[ly + 3][lx + 4], tile[ly + 3][lx + 5],
-                                   tile[ly + 3][lx + 6]);
-            *(uchar4*)dst = v;
-        } else {
-            for (int k = 0; k < 4; k++)
-                if (x0 + k < w) dst[k] = tile[ly + 3][lx + 3 + k];
+    // ---- 2. level and blur; thread = 4 adjacent pixels x 2 rows
+    const int bx = (tid & 15) * 4;
+    if (level > 0) {
+        for (int r = tid >> 4; r < LT_H; r += 16) {
+            const int y = ty0 + r, x0 = tx0 + bx;
+            if (y >= h) continue;
+            uint8_t* dst = L.lvl + (size_t)y * pitch + x0;
+            const uint8_t* src = &px[r + LT_HALO][bx + LT_HALO];
+            if (x0 + 3 < w) {
+                *(uchar4*)dst = make_uchar4(src[0], src[1], src[2], src[3]);
+            } else {
+                for (int k = 0; k < 4; k++)
+                    if (x0 + k < w) dst[k] = src[k];
+            }
         }
     }
-
-    // Horizontal blur pass over all extended rows.
-    for (int e = tid; e < EXT_H * ORB_TILE_W; e += 256) {
-        const int ey = e / ORB_TILE_W, ex = e % ORB_TILE_W;
+    for (int e = tid; e < (LT_H + 6) * LT_W; e += 256) {
+        const int ey = e / LT_W, ex = e % LT_W;
         int acc = 0;
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc += c_gauss[k] * tile[ey][ex + k];
+        for (int k = 0; k < 7; k++) acc += c_gauss[k] * px[ey + LT_HALO - 3][ex + LT_HALO - 3 + k];
         hs[ey][ex] = acc;
     }
     __syncthreads();
-
-    if (row_in) {
-        uint8_t bl[4], fm[4];
+    for (int r = tid >> 4; r < LT_H; r += 16) {
+        const int y = ty0 + r, x0 = tx0 + bx;
+        if (y >= h) continue;
+        uint8_t bl[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             int acc = 0;
 #pragma unroll
-            for (int r = 0; r < 7; r++) acc += c_gauss[r] * hs[ly + r][lx + k];
+            for (int q = 0; q < 7; q++) acc += c_gauss[q] * hs[r + q][bx + k];
             const int v = (acc + (1 << 15)) >> 16;
             bl[k] = (uint8_t)(v > 255 ? 255 : v);
-            const int x = x0 + k;
-            fm[k] = (x >= 3 && x < w - 3 && y >= 3 && y < h - 3)
-                        ? (uint8_t)fast_strength(tile, lx + 3 + k, ly + 3) : (uint8_t)0;
         }
         uint8_t* db = L.blur + (size_t)y * pitch + x0;
-        uint8_t* dm = L.fastm + (size_t)y * pitch + x0;
         if (x0 + 3 < w) {
             *(uchar4*)db = make_uchar4(bl[0], bl[1], bl[2], bl[3]);
-            *(uchar4*)dm = make_uchar4(fm[0], fm[1], fm[2], fm[3]);
         } else {
             for (int k = 0; k < 4; k++)
-                if (x0 + k < w) {
-                    db[k] = bl[k];
-                    dm[k] = fm[k];
-                }
+                if (x0 + k < w) db[k] = bl[k];
         }
     }
-}
 
-// ================================================================ cells
-// One wavefront per 30-px cell.  The cell's detection area (window minus the
-// 3-px FAST margin) is staged in LDS with a zero ring, so that neighbours
-// outside the window count as score 0, exactly like cv::FAST on the window.
-#define CELL_P 64
-
-__device__ __forceinline__ bool cell_keep(const uint8_t* s, int cx, int cy, int t)
-{
-    const uint8_t* c = s + (cy + 1) * CELL_P + (cx + 1);
-    const int mc = c[0];
-    if (mc <= t) return false;
-    const int sc = mc - 1;
-    const int off[8] = {-CELL_P - 1, -CELL_P, -CELL_P + 1, -1, 1, CELL_P - 1, CELL_P, CELL_P + 1};
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int mq = c[off[k]];
-        const int sq = mq > t ? mq - 1 : 0;
-        if (sq >= sc) return false;
-    }
-    return true;
-}
-
-__global__ __launch_bounds__(256) void cells_kernel(const ImageJob* __restrict__ jobs,
-                                                    const RunTable* __restrict__ rt)
-{
-    __shared__ uint8_t sm[4][CELL_P * CELL_P];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nseg = rt->njobs * rt->max_levels;
-    const int cell = blockIdx.x * 4 + wv;
-    const bool valid = cell < rt->cell_start[nseg];
-    int f = 0;
-    if (valid) f = find_segment(rt->cell_start, nseg, cell);
-    const int jb = f / rt->max_levels, level = f % rt->max_levels;
-    const ImageJob& J = jobs[jb];
-    const LevelJob& L = J.lv[level];
-    int aw = 0, ah = 0, iniX = 0, iniY = 0, ci = 0, cj = 0;
-    if (valid) {
-        const int local = cell - rt->cell_start[f];
-        ci = local / L.nCols;
-        cj = local % L.nCols;
-        iniY = ORB_MIN_BORDER + ci * L.hCell;
-        iniX = ORB_MIN_BORDER + cj * L.wCell;
-        const int maxY = min(iniY + L.hCell + 6, L.maxBorderY);
-        const int maxX = min(iniX + L.wCell + 6, L.maxBorderX);
-        // Cells skipped by ComputeKeyPointsOctTree (:1083, :1101).
-        if (iniY < L.maxBorderY - 3 && iniX < L.maxBorderX - 6) {
-            aw = max(maxX - iniX - 6, 0);
-            ah = max(maxY - iniY - 6, 0);
+    // ---- 3. FAST strength on the tile plus a 1-px ring, detection areas only
+    const int tA = min(max(J.iniTh, 0), 255), tB = min(max(J.minTh, 0), 255);
+    const int tmin = min(tA, tB);
+    const int ax0 = ORB_EDGE, ay0 = ORB_EDGE;
+    const int ax1 = min(ORB_EDGE + L.nCols * L.wCell, L.maxBorderX - 3);
+    const int ay1 = min(ORB_EDGE + L.nRows * L.hCell, L.maxBorderY - 3);
+    for (int e0 = 0; e0 < LM_W * LM_H; e0 += 256) {
+        const int e = e0 + tid;
+        bool cand = false;
+        if (e < LM_W * LM_H) {
+            const int mx = e % LM_W, my = e / LM_W;
+            const int x = tx0 - 1 + mx, y = ty0 - 1 + my;
+            const bool in = x >= ax0 && x < ax1 && y >= ay0 && y < ay1;
+            cand = in && fast_maybe(px, mx + LT_HALO - 1, my + LT_HALO - 1, tmin);
+            mm[my][mx] = 0;
         }
+        const unsigned long long bal = __ballot(cand);
+        int base = 0;
+        if (lane == 0 && bal) base = atomicAdd(&s_qn, __popcll(bal));
+        base = __shfl(base, 0, 64);
+        if (cand) queue[base + lanes_below(bal)] = (unsigned short)e;
     }
-    uint8_t* s = sm[wv];
-    // Column `lane` of the staged area for every staged row.
-    for (int ry = 0; ry < ah + 2; ry++) {
-        uint8_t v = 0;
-        if (lane >= 1 && lane <= aw && ry >= 1 && ry <= ah)
-            v = L.fastm[(size_t)(iniY + 2 + ry) * L.pitch + iniX + 2 + lane];
-        s[ry * CELL_P + lane] = v;
+    __syncthreads();
+    const int qn = s_qn;
+    for (int q = tid; q < qn; q += 256) {
+        const int e = queue[q];
+        const int mx = e % LM_W, my = e / LM_W;
+        mm[my][mx] = (uint8_t)fast_strength(px, mx + LT_HALO - 1, my + LT_HALO - 1);
     }
     __syncthreads();
 
-    int t = J.iniTh < 0 ? 0 : (J.iniTh > 255 ? 255 : J.iniTh);
-    int n = 0;
-    if (lane < aw)
-        for (int r = 0; r < ah; r++) n += cell_keep(s, lane, r, t);
-    n = wave_sum(n);
-    if (n == 0) {
-        // No corner survived at iniThFAST: redo the cell at minThFAST.
-        t = J.minTh < 0 ? 0 : (J.minTh > 255 ? 255 : J.minTh);
-        if (lane < aw)
-            for (int r = 0; r < ah; r++) n += cell_keep(s, lane, r, t);
-        n = wave_sum(n);
-    }
-    if (n == 0) return;
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&J.counters[ORB_CNT_KEYS + level], n);
-    base = __shfl(base, 0, 64);
-    if (base + n > L.key_cap && lane == 0) atomicOr(&J.counters[ORB_CNT_ERR], ORB_ERR_KEYS);
-    int off = 0;
-    for (int r = 0; r < ah; r++) {
-        const bool k = lane < aw && cell_keep(s, lane, r, t);
-        const unsigned long long m = __ballot(k);
-        if (k) {
-            const int pos = base + off + lanes_below(m);
-            const int xl = lane + 3, yl = r + 3;   // FAST window coordinates
-            const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | ((unsigned)yl << 6) | (unsigned)xl;
-            const unsigned resp = (unsigned)s[(r + 1) * CELL_P + lane + 1] - 1u;
-            if (pos < L.key_cap) L.keys[pos] = ((unsigned long long)resp << 32) | ok;
+    // ---- 4. per-cell non-max suppression at both thresholds
+    for (int e = tid; e < LT_W * LT_H; e += 256) {
+        const int lx = e % LT_W, ly = e / LT_W;
+        const int x = tx0 + lx, y = ty0 + ly;
+        const int mc = mm[ly + 1][lx + 1];
+        if (mc <= tmin || x >= ax1 || y >= ay1 || x < ax0 || y < ay0) continue;
+        const int cj = (x - ORB_EDGE) / L.wCell, ci = (y - ORB_EDGE) / L.hCell;
+        const int cx0 = max(ORB_EDGE + cj * L.wCell, ax0), cx1 = min(ORB_EDGE + (cj + 1) * L.wCell, ax1);
+        const int cy0 = max(ORB_EDGE + ci * L.hCell, ay0), cy1 = min(ORB_EDGE + (ci + 1) * L.hCell, ay1);
+        bool keepA = mc > tA, keepB = mc > tB;
+        const int sA = mc - 1, sB = mc - 1;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            if (k == 4) continue;
+            const int dx = k % 3 - 1, dy = k / 3 - 1;
+            const int qx = x + dx, qy = y + dy;
+            if (qx < cx0 || qx >= cx1 || qy < cy0 || qy >= cy1) {
+                // outside this cell's window: score 0
+                keepA = keepA && sA > 0;
+                keepB = keepB && sB > 0;
+                continue;
+            }
+            const int mq = mm[ly + 1 + dy][lx + 1 + dx];
+            keepA = keepA && sA > (mq > tA ? mq - 1 : 0);
+            keepB = keepB && sB > (mq > tB ? mq - 1 : 0);
         }
-        off += __popcll(m);
+        if (!keepA && !keepB) continue;
+        const unsigned xl = (unsigned)(x - (ORB_MIN_BORDER + cj * L.wCell));
+        const unsigned yl = (unsigned)(y - (ORB_MIN_BORDER + ci * L.hCell));
+        const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | (yl << 6) | xl;
+        const unsigned long long key = ((unsigned long long)(unsigned)(mc - 1) << 32) | ok;
+        if (keepA) {
+            L.cellflag[ci * L.nCols + cj] = 1;
+            const int i = atomicAdd(&s_kn, 1);
+            if (i < LK_CAP) kbuf[i] = key;
+        }
+        if (keepB) {
+            const int i = atomicAdd(&s_kn, 1);
+            if (i < LK_CAP) kbuf[i] = key | ORB_KEY_RETRY;
+        }
     }
+    __syncthreads();
+
+    // ---- 5. append the tile's candidates to the level list
+    const int kn = s_kn;
+    if (kn == 0) return;
+    if (tid == 0) {
+        s_base = atomicAdd(&J.counters[ORB_CNT_KEYS + level], kn);
+        if (kn > LK_CAP || s_base + kn > L.key_cap) atomicOr(&J.counters[ORB_CNT_ERR], ORB_ERR_KEYS);
+    }
+    __syncthreads();
+    const int base = s_base;
+    for (int i = tid; i < min(kn, LK_CAP); i += 256)
+        if (base + i < L.key_cap) L.keys[base + i] = kbuf[i];
 }
 
 // ================================================================ octree
@@ -404,12 +440,31 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
     if (level >= J.nlevels) return;
     const LevelJob& L = J.lv[level];
     int* ctr = J.counters;
-    const int K = min(ctr[ORB_CNT_KEYS + level], L.key_cap);
+    const int K0 = min(ctr[ORB_CNT_KEYS + level], L.key_cap);
     const int N = L.quota;
     const int wC = L.wCell, hC = L.hCell;
-    const unsigned long long* keys = L.keys;
     int* node = L.node;
     int total;
+
+    // ---- per-cell retry (ORBextractor.cc:1111-1124): a cell keeps its
+    // iniThFAST candidates when it has any, else its minThFAST candidates.
+    int K = 0;
+    for (int c0 = 0; c0 < K0; c0 += OCT_T) {
+        const int k = c0 + tid;
+        bool v = false;
+        unsigned long long key = 0;
+        if (k < K0) {
+            key = L.keys[k];
+            const unsigned ok = (unsigned)key;
+            const bool flagged = L.cellflag[((ok >> 20) & 255) * L.nCols + ((ok >> 12) & 255)] != 0;
+            v = (key & ORB_KEY_RETRY) ? !flagged : flagged;
+        }
+        const int pos = block_scan(v ? 1 : 0, S.scan, &total);
+        if (v) L.keys2[K + pos] = key & ~ORB_KEY_RETRY;
+        K += total;
+    }
+    __syncthreads();
+    const unsigned long long* keys = L.keys2;
 
     // ---- roots (ORBextractor.cc:695-763)
     if (tid < ORB_MAX_ROOTS) S.cc[tid][0] = 0;
@@ -821,10 +876,17 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
 }
 
 // ============================================================ zero fill
+// Counters and per-cell flags of every job (one workgroup per job).
 __global__ void zero_counters_kernel(const ImageJob* __restrict__ jobs, int njobs)
 {
     const int j = blockIdx.x;
-    if (j < njobs && threadIdx.x < ORB_CNT_SLOTS) jobs[j].counters[threadIdx.x] = 0;
+    if (j >= njobs) return;
+    const ImageJob& J = jobs[j];
+    if (threadIdx.x < ORB_CNT_SLOTS) J.counters[threadIdx.x] = 0;
+    for (int l = 0; l < J.nlevels; l++) {
+        uint8_t* f = J.lv[l].cellflag;
+        for (int i = threadIdx.x; i < J.lv[l].ncells; i += blockDim.x) f[i] = 0;
+    }
 }
 
 // ========================================================= Hamming helpers
@@ -1084,21 +1146,18 @@ __global__ __launch_bounds__(256) void match_bf_kernel(const uint8_t* __restrict
 
 // ============================================================= launchers
 // ev (optional): events recorded after each stage -- [0] start, then one per
-// pyramid level (max_levels), cells, octree, describe.
+// pyramid level (max_levels), octree, describe.
 extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable* d_rt, const RunTable* h_rt,
                                          hipStream_t st, hipEvent_t* ev)
 {
     int e = 0;
     if (ev) hipEventRecord(ev[e++], st);
-    zero_counters_kernel<<<h_rt->njobs, 64, 0, st>>>(d_jobs, h_rt->njobs);
+    zero_counters_kernel<<<h_rt->njobs, 256, 0, st>>>(d_jobs, h_rt->njobs);
     for (int l = 0; l < h_rt->max_levels; l++) {
         const int nb = h_rt->tile_start[l][h_rt->njobs];
         if (nb > 0) level_kernel<<<nb, 256, 0, st>>>(d_jobs, d_rt, l);
         if (ev) hipEventRecord(ev[e++], st);
     }
-    const int ncells = h_rt->cell_start[h_rt->njobs * h_rt->max_levels];
-    if (ncells > 0) cells_kernel<<<(ncells + 3) / 4, 256, 0, st>>>(d_jobs, d_rt);
-    if (ev) hipEventRecord(ev[e++], st);
     octree_kernel<<<h_rt->njobs * h_rt->max_levels, OCT_T, 0, st>>>(d_jobs, h_rt->max_levels);
     if (ev) hipEventRecord(ev[e++], st);
     const int nslots = h_rt->slot_start[h_rt->njobs];

@@ -77,13 +77,13 @@ struct LevelPlan {
     float hX;
     int root_x[ORB_MAX_ROOTS + 1];
     int key_cap, list_cap;
-    size_t key_off, ret_off;
+    size_t key_off, ret_off, cell_off;
 };
 
 struct Plan {
     int W = 0, H = 0;
     LevelPlan lv[ORB_MAX_LEVELS];
-    size_t img_bytes = 0, nkeys = 0, nret = 0;
+    size_t img_bytes = 0, nkeys = 0, nret = 0, ncells = 0;
     int out_cap = 0;
 };
 
@@ -186,7 +186,7 @@ struct orb_extractor {
         p = Plan();
         p.W = W;
         p.H = H;
-        size_t img = 0, nk = 0, nr = 0;
+        size_t img = 0, nk = 0, nr = 0, ncell = 0;
         int out = 0;
         for (int l = 0; l < nlevels; l++) {
             LevelPlan& L = p.lv[l];
@@ -214,7 +214,11 @@ struct orb_extractor {
             if (L.nIni <= 0 || L.nIni > ORB_MAX_ROOTS) return ORB_E_ARG;
             L.hX = (float)(L.maxBorderX - minBorderX) / L.nIni;
             for (int i = 0; i <= L.nIni; i++) L.root_x[i] = (int)(L.hX * (float)i);
-            L.key_cap = L.ncells * ((L.wCell + 1) / 2) * ((L.hCell + 1) / 2);
+            // Strict 3x3 non-max suppression keeps at most one pixel per 2x2
+            // block of a cell; candidates of both thresholds are stored.
+            L.key_cap = 2 * L.ncells * ((L.wCell + 1) / 2) * ((L.hCell + 1) / 2);
+            L.cell_off = ncell;
+            ncell += (size_t)L.ncells;
             L.list_cap = std::max(quota[l] + 3, 4 * L.nIni);
             if (L.list_cap > ORB_LIST_CAP) return ORB_E_ARG;
             L.key_off = nk;
@@ -226,6 +230,7 @@ struct orb_extractor {
         p.img_bytes = img;
         p.nkeys = nk;
         p.nret = nr;
+        p.ncells = ncell;
         p.out_cap = out;
         return ORB_OK;
     }
@@ -245,8 +250,9 @@ struct orb_extractor {
             off = align_up(off + bytes, 256);
             return o;
         };
-        const size_t o_lvl = take(p.img_bytes), o_blur = take(p.img_bytes), o_fm = take(p.img_bytes);
-        const size_t o_keys = take(p.nkeys * 8), o_node = take(p.nkeys * 4), o_ret = take(p.nret * 8);
+        const size_t o_lvl = take(p.img_bytes), o_blur = take(p.img_bytes), o_cell = take(p.ncells);
+        const size_t o_keys = take(p.nkeys * 8), o_keys2 = take(p.nkeys * 8), o_node = take(p.nkeys * 4);
+        const size_t o_ret = take(p.nret * 8);
         const size_t o_ctr = take(ORB_CNT_SLOTS * 4);
         const size_t o_kps = take((size_t)p.out_cap * sizeof(OrbKeyPoint));
         const size_t o_desc = take((size_t)p.out_cap * 32);
@@ -295,11 +301,13 @@ struct orb_extractor {
             L.scale = scale[l];
             for (int i = 0; i <= P.nIni; i++) L.root_x[i] = P.root_x[i];
             L.key_cap = P.key_cap;
+            L.ncells = P.ncells;
             L.list_cap = P.list_cap;
             L.lvl = (uint8_t*)(d_mem + o_lvl + P.img_off);
             L.blur = (uint8_t*)(d_mem + o_blur + P.img_off);
-            L.fastm = (uint8_t*)(d_mem + o_fm + P.img_off);
+            L.cellflag = (uint8_t*)(d_mem + o_cell + P.cell_off);
             L.keys = (unsigned long long*)(d_mem + o_keys) + P.key_off;
+            L.keys2 = (unsigned long long*)(d_mem + o_keys2) + P.key_off;
             L.node = (int*)(d_mem + o_node) + P.key_off;
             L.ret = (unsigned long long*)(d_mem + o_ret) + P.ret_off;
         }
@@ -334,13 +342,6 @@ struct orb_extractor {
             rt.tile_start[l][n] = acc;
         }
         int acc = 0;
-        for (int j = 0; j < n; j++)
-            for (int l = 0; l < maxl; l++) {
-                rt.cell_start[j * maxl + l] = acc;
-                if (l < exs[j]->nlevels) acc += exs[j]->plan.lv[l].ncells;
-            }
-        rt.cell_start[n * maxl] = acc;
-        acc = 0;
         for (int j = 0; j < n; j++) {
             rt.slot_start[j] = acc;
             acc += exs[j]->plan.out_cap;
@@ -750,10 +751,10 @@ int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* i
 
 int orb_extractor_debug_image(orb_extractor* ex, int level, int kind, uint8_t* dst, size_t dst_stride)
 {
-    if (!ex || !ex->d_mem || level < 0 || level >= ex->nlevels || kind < 0 || kind > 2) return ORB_E_ARG;
+    if (!ex || !ex->d_mem || level < 0 || level >= ex->nlevels || kind < 0 || kind > 1) return ORB_E_ARG;
     if (!dst) return ORB_OK;
     const LevelJob& L = ex->hjob.lv[level];
-    const uint8_t* src = kind == 0 ? L.lvl : (kind == 1 ? L.blur : L.fastm);
+    const uint8_t* src = kind == 0 ? L.lvl : L.blur;
     if (dst_stride < (size_t)L.w) return ORB_E_ARG;
     HIP_TRY(hipSetDevice(ex->ctx->device));
     HIP_TRY(hipStreamSynchronize(ex->ctx->stream));
@@ -769,11 +770,20 @@ int orb_extractor_debug_candidates(orb_extractor* ex, int level, int* xyr, int c
     int c[ORB_CNT_SLOTS];
     HIP_TRY(hipMemcpy(c, ex->hjob.counters, sizeof(c), hipMemcpyDeviceToHost));
     const LevelJob& L = ex->hjob.lv[level];
-    const int k = std::min(c[ORB_CNT_KEYS + level], L.key_cap);
+    int k = std::min(c[ORB_CNT_KEYS + level], L.key_cap);
+    std::vector<unsigned long long> all(k);
+    std::vector<uint8_t> flags(L.ncells);
+    if (k) HIP_TRY(hipMemcpy(all.data(), L.keys, (size_t)k * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(flags.data(), L.cellflag, L.ncells, hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> keys;
+    for (unsigned long long key : all) {
+        const unsigned ok = (unsigned)key;
+        const bool flagged = flags[((ok >> 20) & 255) * L.nCols + ((ok >> 12) & 255)] != 0;
+        if ((key & ORB_KEY_RETRY) ? !flagged : flagged) keys.push_back(key & ~ORB_KEY_RETRY);
+    }
+    k = (int)keys.size();
     *n = k;
     if (k > capacity) return ORB_E_CAPACITY;
-    std::vector<unsigned long long> keys(k);
-    if (k) HIP_TRY(hipMemcpy(keys.data(), L.keys, (size_t)k * 8, hipMemcpyDeviceToHost));
     std::sort(keys.begin(), keys.end(), [](unsigned long long a, unsigned long long b) {
         return (unsigned)a < (unsigned)b;
     });
@@ -816,7 +826,7 @@ struct orb_batch {
             HIP_TRY(orb_launch_extract(d_jobs, d_rt, h_rt, st, ev));
         }
         if (!pl.empty())
-            HIP_TRY(orb_launch_stereo(d_sargs, (int)pl.size(), nwaves, st, ev ? ev + h_rt->max_levels + 4 : nullptr));
+            HIP_TRY(orb_launch_stereo(d_sargs, (int)pl.size(), nwaves, st, ev ? ev + h_rt->max_levels + 3 : nullptr));
         return ORB_OK;
     }
 
@@ -982,18 +992,18 @@ int orb_batch_run(orb_batch* b, int mode)
 
 int orb_batch_stage_count(const orb_batch* b)
 {
-    return b ? b->h_rt->max_levels + 5 : ORB_E_ARG;
+    return b ? b->h_rt->max_levels + 4 : ORB_E_ARG;
 }
 
 // Per-stage device time (HIP events on the batch's stream) averaged over
-// `steps` mode-0 runs: pyramid levels 0..L-1, cells, octree, describe, stereo
-// match, stereo median.  ms receives orb_batch_stage_count() floats.
+// `steps` mode-0 runs: pyramid levels 0..L-1, octree, describe, stereo match,
+// stereo median.  ms receives orb_batch_stage_count() floats.
 int orb_batch_stage_times(orb_batch* b, int steps, float* ms)
 {
     if (!b || steps <= 0 || !ms) return ORB_E_ARG;
     HIP_TRY(hipSetDevice(b->ctx->device));
     const int L = b->h_rt->max_levels;
-    const int nev = L + 6;  // start, L levels, cells, octree, describe, stereo, median
+    const int nev = L + 5;  // start, L levels, octree, describe, stereo, median
     std::vector<hipEvent_t> ev(nev);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     std::vector<double> acc(nev - 1, 0.0);
@@ -1002,8 +1012,8 @@ int orb_batch_stage_times(orb_batch* b, int steps, float* ms)
         rc = b->enqueue(0, ev.data());
         if (rc) break;
         if (b->pl.empty()) {
+            (void)hipEventRecord(ev[L + 3], b->ctx->stream);
             (void)hipEventRecord(ev[L + 4], b->ctx->stream);
-            (void)hipEventRecord(ev[L + 5], b->ctx->stream);
         }
         if (hipEventSynchronize(ev[nev - 1]) != hipSuccess) rc = ORB_E_DEVICE;
         for (int i = 0; i + 1 < nev && !rc; i++) {

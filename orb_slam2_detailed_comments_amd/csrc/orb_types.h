@@ -13,7 +13,7 @@
 #define ORB_MAX_ROOTS 16
 #define ORB_LIST_CAP 512       // quadtree list capacity per level (LDS)
 #define ORB_TILE_W 64          // level kernel output tile
-#define ORB_TILE_H 16
+#define ORB_TILE_H 32
 #define ORB_EDGE 19            // EDGE_THRESHOLD, ORBextractor.cc:81
 #define ORB_MIN_BORDER 16      // EDGE_THRESHOLD - 3, ORBextractor.cc:1049
 
@@ -24,6 +24,8 @@
 #define ORB_CNT_ERR (ORB_CNT_NOUT + 1)       // error bits
 #define ORB_CNT_MATCH (ORB_CNT_ERR + 1)      // stereo matches recorded
 #define ORB_CNT_SLOTS 64
+
+#define ORB_KEY_RETRY (1ull << 40)   // candidate from the minThFAST pass
 
 #define ORB_ERR_KEYS 1        // candidate buffer overflow
 #define ORB_ERR_LIST 2        // quadtree list overflow
@@ -45,12 +47,14 @@ struct LevelJob {
     float hX;                        // root width (float, ORBextractor.cc:701)
     float scale;                     // mvScaleFactor[level]
     int root_x[ORB_MAX_ROOTS + 1];   // (int)(hX*i), i = 0..nIni
-    int key_cap;                     // candidate capacity
+    int key_cap;                     // candidate capacity (both thresholds)
+    int ncells;                      // nCols * nRows
     int list_cap;                    // kept-keypoint capacity (<= ORB_LIST_CAP)
     uint8_t* lvl;                    // level image
     uint8_t* blur;                   // 7x7 sigma-2 blur of the level
-    uint8_t* fastm;                  // FAST arc strength per pixel
+    uint8_t* cellflag;               // per FAST cell: has an iniThFAST survivor
     unsigned long long* keys;        // candidates: response<<32 | order key
+    unsigned long long* keys2;       // candidates after the per-cell retry
     int* node;                       // per-candidate quadtree node (scratch)
     unsigned long long* ret;         // kept keypoints in output order
 };
@@ -71,7 +75,6 @@ struct RunTable {
     int njobs;
     int max_levels;
     int tile_start[ORB_MAX_LEVELS][ORB_MAX_JOBS + 1];   // per level: first tile block of job j
-    int cell_start[(ORB_MAX_JOBS + 1) * ORB_MAX_LEVELS];  // per (job, level): first cell wave
     int slot_start[ORB_MAX_JOBS + 1];                   // describe kernel: first slot of job j
 };
 

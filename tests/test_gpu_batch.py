@@ -65,5 +65,5 @@ def test_stage_times_cover_pipeline(orb):
     batch.upload(1, synth.shapes_frame(640, 480, 2))
     batch.set_stereo([0], [1], 386.1, 0.537)
     t = batch.stage_times(3)
-    assert set(t) >= {"pyramid_l0", "fast_cells", "octree", "describe", "stereo_match"}
+    assert set(t) >= {"pyramid_l0", "octree", "describe", "stereo_match"}
     assert all(v >= 0 for v in t.values())
