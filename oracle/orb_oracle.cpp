@@ -958,6 +958,22 @@ int oracle_level_candidates(const oracle_extractor* ex, int level, int* xyr, int
     return 0;
 }
 
+int oracle_distribute_octree(const int* xyr, int n, int minX, int maxX, int minY, int maxY, int N, int* out,
+                             int cap, int* nout)
+{
+    std::vector<Cand> in(n);
+    for (int i = 0; i < n; i++) in[i] = Cand{(float)xyr[3 * i], (float)xyr[3 * i + 1], (float)xyr[3 * i + 2]};
+    const std::vector<Cand> r = distribute_octree(in, minX, maxX, minY, maxY, N);
+    *nout = (int)r.size();
+    if ((int)r.size() > cap) return -2;
+    for (size_t i = 0; i < r.size(); i++) {
+        out[3 * i] = (int)r[i].x;
+        out[3 * i + 1] = (int)r[i].y;
+        out[3 * i + 2] = (int)r[i].response;
+    }
+    return 0;
+}
+
 int oracle_resize_linear(const uint8_t* src, int sw, int sh, int sstride, uint8_t* dst, int dw, int dh)
 {
     if (sw <= 0 || sh <= 0 || dw <= 0 || dh <= 0) return -1;

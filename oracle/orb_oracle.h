@@ -66,6 +66,11 @@ int oracle_level_image(oracle_extractor* ex, int level, int kind, uint8_t* dst);
  * triples (x, y, response) in the border-relative frame of :1135-1136. */
 int oracle_level_candidates(const oracle_extractor* ex, int level, int* xyr, int cap, int* n);
 
+/* DistributeOctTree on (x, y, response) triples in the border-relative frame;
+ * output triples in list order. */
+int oracle_distribute_octree(const int* xyr, int n, int minX, int maxX, int minY, int maxY, int N, int* out,
+                             int cap, int* nout);
+
 /* Stand-alone primitives. */
 int oracle_resize_linear(const uint8_t* src, int sw, int sh, int sstride,
                          uint8_t* dst, int dw, int dh);

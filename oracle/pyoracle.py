@@ -40,6 +40,7 @@ def lib():
             "oracle_level_image": (I, [P, I, I, P]),
             "oracle_level_candidates": (I, [P, I, P, I, P]),
             "oracle_resize_linear": (I, [P, I, I, I, P, I, I]),
+            "oracle_distribute_octree": (I, [P, I, I, I, I, I, I, P, I, P]),
             "oracle_gaussian7": (I, [P, I, I, I, P]),
             "oracle_gaussian7_kernel": (I, [P]),
             "oracle_fast": (I, [P, I, I, I, I, P, I, P]),
@@ -124,6 +125,17 @@ class OracleExtractor:
         if rc != 0:
             raise RuntimeError(rc)
         return buf[:n.value].copy()
+
+
+def distribute_octree(xyr, minX, maxX, minY, maxY, N):
+    xyr = np.ascontiguousarray(xyr, np.int32).reshape(-1, 3)
+    cap = len(xyr) + 64
+    out = np.zeros((cap, 3), np.int32)
+    n = ctypes.c_int()
+    rc = lib().oracle_distribute_octree(_p(xyr), len(xyr), minX, maxX, minY, maxY, N, _p(out), cap, ctypes.byref(n))
+    if rc != 0:
+        raise RuntimeError(rc)
+    return out[:n.value].copy()
 
 
 def resize_linear(src, dw, dh):

@@ -1,0 +1,480 @@
+"""CPU tier: pins for the oracle (no GPU needed).
+
+The reference ships no tests and no golden vectors, and its OpenCV dependency
+is absent, so each oracle function is pinned here by something a plausible
+mistake would fail: closed forms, brute force on small inputs, an
+independently written numpy/Python statement of the same rule, invariants and
+known answers.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from orb_slam2_detailed_comments_amd import synth
+
+RING = [(0, 3), (1, 3), (2, 2), (3, 1), (3, 0), (3, -1), (2, -2), (1, -3), (0, -3), (-1, -3), (-2, -2), (-3, -1),
+        (-3, 0), (-3, 1), (-2, 2), (-1, 3)]
+
+
+# ------------------------------------------------------ DescriptorDistance
+def test_descriptor_distance_closed_form(oracle):
+    rng = np.random.default_rng(1)
+    a = rng.integers(0, 256, (500, 32), dtype=np.uint8)
+    b = rng.integers(0, 256, (500, 32), dtype=np.uint8)
+    ref = np.unpackbits(a ^ b, axis=1).sum(1)
+    got = np.array([oracle.descriptor_distance(x, y) for x, y in zip(a, b)])
+    np.testing.assert_array_equal(got, ref)
+    assert oracle.descriptor_distance(a[0], a[0]) == 0
+    assert oracle.descriptor_distance(a[0], ~a[0]) == 256
+    for i in range(50):
+        x, y, z = a[i], b[i], a[i + 50]
+        dxy, dyz, dxz = (oracle.descriptor_distance(p, q) for p, q in ((x, y), (y, z), (x, z)))
+        assert dxy == oracle.descriptor_distance(y, x)
+        assert dxz <= dxy + dyz
+
+
+# --------------------------------------------------------------- scales
+def test_scale_tables_and_quotas(oracle):
+    t = oracle.OracleExtractor(2000, 1.2, 8, 20, 7).tables()
+    # quotas of the KITTI configuration (nFeatures 2000, scale 1.2, 8 levels)
+    np.testing.assert_array_equal(t["quota"], [434, 362, 302, 251, 209, 175, 145, 122])
+    assert t["quota"].sum() == 2000
+    t1 = oracle.OracleExtractor(1000, 1.2, 8, 20, 7).tables()
+    np.testing.assert_array_equal(t1["quota"], [217, 181, 151, 126, 105, 87, 73, 60])
+    s = np.float32(1.0)
+    for i in range(8):
+        assert t["scale"][i] == s
+        assert t["inv_scale"][i] == np.float32(1.0) / s
+        assert t["sigma2"][i] == s * s
+        s = np.float32(np.float64(s) * np.float64(np.float32(1.2)))
+    assert abs(t["scale"][7] - 1.2 ** 7) < 1e-5
+
+
+def test_umax_disc_is_symmetric(oracle):
+    umax = oracle.OracleExtractor(1000, 1.2, 8, 20, 7).tables()["umax"]
+    assert umax[0] == 15 and len(umax) == 16
+    inside = lambda u, v: abs(u) <= umax[abs(v)]
+    for u in range(-15, 16):
+        for v in range(-15, 16):
+            assert inside(u, v) == inside(v, u)          # symmetric under transpose
+            if u * u + v * v <= 13 * 13:
+                assert inside(u, v)                       # covers the radius-13 disc
+            if u * u + v * v > 16 * 16:
+                assert not inside(u, v)                   # within radius 16
+
+
+# --------------------------------------------------------------- resize
+def _resize_numpy(src, dw, dh):
+    """Independent numpy statement of OpenCV's 8U INTER_LINEAR fixed point."""
+    sh, sw = src.shape
+
+    def taps(dn, sn):
+        scale = 1.0 / (dn / sn)
+        d = np.arange(dn, dtype=np.float64)
+        f = ((d + 0.5) * scale - 0.5).astype(np.float32)
+        i = np.floor(f).astype(np.int64)
+        f = (f - i.astype(np.float32)).astype(np.float32)
+        return i, f
+
+    sx, fx = taps(dw, sw)
+    lo, hi = sx < 0, sx >= sw - 1
+    fx = np.where(lo | hi, np.float32(0), fx).astype(np.float32)
+    sx = np.where(lo, 0, np.where(hi, sw - 1, sx))
+    a0 = np.rint((np.float32(1) - fx) * np.float32(2048)).astype(np.int64)
+    a1 = np.rint(fx * np.float32(2048)).astype(np.int64)
+    sy, fy = taps(dh, sh)
+    b0 = np.rint((np.float32(1) - fy) * np.float32(2048)).astype(np.int64)
+    b1 = np.rint(fy * np.float32(2048)).astype(np.int64)
+    r0 = np.clip(sy, 0, sh - 1)
+    r1 = np.clip(sy + 1, 0, sh - 1)
+    s = src.astype(np.int64)
+    nx = np.minimum(sx + 1, sw - 1)
+    h0 = s[r0][:, sx] * a0 + s[r0][:, nx] * a1
+    h1 = s[r1][:, sx] * a0 + s[r1][:, nx] * a1
+    v = (h0 * b0[:, None] + h1 * b1[:, None] + (1 << 21)) >> 22
+    return np.clip(v, 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh", [(640, 480, 533, 400), (1241, 376, 1034, 313), (100, 80, 83, 67),
+                                         (83, 67, 100, 80)])
+def test_resize_matches_independent_statement(oracle, sw, sh, dw, dh):
+    src = synth.shapes_frame(sw, sh, 3)
+    np.testing.assert_array_equal(oracle.resize_linear(src, dw, dh), _resize_numpy(src, dw, dh))
+
+
+def test_resize_identity_and_float_bilinear(oracle):
+    src = synth.shapes_frame(200, 150, 4)
+    np.testing.assert_array_equal(oracle.resize_linear(src, 200, 150), src)
+    out = oracle.resize_linear(src, 167, 125).astype(np.float64)
+    sx, sy = 200 / 167, 150 / 125
+    xs = np.clip((np.arange(167) + 0.5) * sx - 0.5, 0, 199)
+    ys = np.clip((np.arange(125) + 0.5) * sy - 0.5, 0, 149)
+    x0 = np.floor(xs).astype(int); y0 = np.floor(ys).astype(int)
+    x1 = np.minimum(x0 + 1, 199); y1 = np.minimum(y0 + 1, 149)
+    fx = xs - x0; fy = (ys - y0)[:, None]
+    f = src.astype(np.float64)
+    ref = (f[y0][:, x0] * (1 - fx) + f[y0][:, x1] * fx) * (1 - fy) + (f[y1][:, x0] * (1 - fx) + f[y1][:, x1] * fx) * fy
+    assert np.max(np.abs(out - ref)) <= 1.0
+
+
+# ----------------------------------------------------------- Gaussian blur
+def test_gaussian_taps(oracle):
+    k = oracle.gaussian7_kernel()
+    x = np.arange(-3, 4, dtype=np.float64)
+    cf = np.exp(-x * x / 8.0).astype(np.float32)
+    cf = (cf.astype(np.float64) / cf.astype(np.float64).sum()).astype(np.float32)
+    np.testing.assert_array_equal(k, np.rint(cf * np.float32(256)).astype(np.int32))
+    np.testing.assert_array_equal(k, k[::-1])
+    # The 8-bit taps do not sum to 256: OpenCV's integer 7x7 sigma-2 blur
+    # brightens by 257^2/2^16.
+    assert k.sum() == 257
+
+
+def test_blur_matches_separable_numpy(oracle):
+    k = oracle.gaussian7_kernel().astype(np.int64)
+    for (w, h, seed) in [(64, 48, 1), (533, 400, 2), (7, 9, 3)]:
+        img = synth.shapes_frame(max(w, 8), max(h, 8), seed)[:h, :w] if w >= 8 else \
+            np.random.default_rng(seed).integers(0, 256, (h, w), dtype=np.uint8)
+        p = np.pad(img.astype(np.int64), 3, mode="reflect")   # gfedcb|abcdefgh|gfedcba
+        rows = sum(k[i] * p[:, i:i + w] for i in range(7))
+        cols = sum(k[i] * rows[i:i + h, :] for i in range(7))
+        ref = np.clip((cols + (1 << 15)) >> 16, 0, 255).astype(np.uint8)
+        np.testing.assert_array_equal(oracle.gaussian7(img), ref)
+
+
+# ------------------------------------------------------------------ FAST
+def _fast_bruteforce(img, t):
+    """Direct definition: 9 contiguous ring pixels all > v+t or all < v-t;
+    score = max over arcs/polarities of min |diff| - 1; strict 3x3 NMS with 0
+    for non-corners and pixels outside [3, dim-3)."""
+    h, w = img.shape
+    score = np.zeros((h, w), np.int64)
+    for y in range(3, h - 3):
+        for x in range(3, w - 3):
+            v = int(img[y, x])
+            d = [int(img[y + dy, x + dx]) - v for dx, dy in RING]
+            best, corner = -10 ** 9, False
+            for k in range(16):
+                arc = [d[(k + j) % 16] for j in range(9)]
+                if all(a > t for a in arc) or all(a < -t for a in arc):
+                    corner = True
+                best = max(best, min(arc), min(-a for a in arc))
+            if corner:
+                score[y, x] = max(t, best) - 1
+    out = []
+    for y in range(3, h - 3):
+        for x in range(3, w - 3):
+            s = score[y, x]
+            if s == 0 and not any(True for _ in []):
+                pass
+            if score[y, x] == 0:
+                # a corner may score 0 only for t=0; it is then suppressed by
+                # any neighbour with score 0, i.e. always
+                continue
+            nb = score[y - 1:y + 2, x - 1:x + 2].copy()
+            nb[1, 1] = -1
+            if s > nb.max():
+                out.append((x, y, s))
+    return np.array(out, np.int64).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("seed,t", [(0, 20), (1, 7), (2, 20), (3, 7), (4, 40)])
+def test_fast_matches_bruteforce(oracle, seed, t):
+    rng = np.random.default_rng(seed)
+    img = synth.shapes_frame(64, 48, seed)[8:28, 10:34].copy()
+    img[rng.integers(0, 20, 6), rng.integers(0, 24, 6)] = rng.integers(0, 256, 6)  # isolated spikes
+    got = oracle.fast(img, t)
+    np.testing.assert_array_equal(got, _fast_bruteforce(img, t))
+
+
+def test_fast_known_answers(oracle):
+    img = np.full((15, 15), 10, np.uint8)
+    img[7, 7] = 200                         # isolated spike: all 16 ring pixels darker by 190
+    np.testing.assert_array_equal(oracle.fast(img, 20), [[7, 7, 189]])
+    img = np.full((15, 15), 10, np.uint8)
+    img[7:, 7:] = 200                       # sharp quadrant: (7,7), (8,7), (7,8)... tie at score 189,
+    assert oracle.fast(img, 20).shape[0] == 0  # and strict non-max suppression keeps none of them
+    assert oracle.fast(np.full((15, 15), 10, np.uint8), 20).shape[0] == 0
+
+
+# -------------------------------------------------- fastAtan2, cos/sin
+def test_fast_atan2_accuracy_and_quadrants(oracle):
+    assert oracle.fast_atan2(0.0, 1.0) == 0.0
+    assert abs(oracle.fast_atan2(1.0, 0.0) - 90.0) < 1e-4
+    assert abs(oracle.fast_atan2(0.0, -1.0) - 180.0) < 1e-4
+    assert abs(oracle.fast_atan2(-1.0, 0.0) - 270.0) < 1e-4
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    for y, x in rng.integers(-20000, 20000, (5000, 2)):
+        if x == 0 and y == 0:
+            continue
+        ref = math.degrees(math.atan2(y, x)) % 360.0
+        got = oracle.fast_atan2(float(y), float(x))
+        assert 0.0 <= got <= 360.0
+        err = min(abs(got - ref), 360.0 - abs(got - ref))
+        worst = max(worst, err)
+    assert worst < 0.02, worst   # the degree-7 polynomial is good to ~0.01 deg
+
+
+def test_canonical_cos_sin_is_correctly_rounded(oracle):
+    rng = np.random.default_rng(3)
+    ang = rng.uniform(0, 360, 20000).astype(np.float32) * np.float32(math.pi / 180.0)
+    mism = 0
+    for a in ang:
+        c, s = oracle.cos_sin(float(a))
+        mism += (np.float32(c) != np.float32(math.cos(float(a)))) + (np.float32(s) != np.float32(math.sin(float(a))))
+    assert mism == 0
+
+
+# ---------------------------------------------------------------- IC_Angle
+def test_ic_angle_moments_bruteforce(oracle):
+    umax = oracle.OracleExtractor(1000, 1.2, 8, 20, 7).tables()["umax"]
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        patch = rng.integers(0, 256, (41, 41), dtype=np.uint8)
+        m01 = m10 = 0
+        for v in range(-15, 16):
+            for u in range(-umax[abs(v)], umax[abs(v)] + 1):
+                val = int(patch[20 + v, 20 + u])
+                m10 += u * val
+                m01 += v * val
+        assert oracle.ic_angle(patch, 20, 20) == oracle.fast_atan2(float(m01), float(m10))
+    ramp = np.tile(np.arange(41, dtype=np.uint8) * 5, (41, 1))      # brighter to the right
+    assert abs(oracle.ic_angle(ramp, 20, 20) - 0.0) < 0.05 or abs(oracle.ic_angle(ramp, 20, 20) - 360) < 0.05
+    assert abs(oracle.ic_angle(ramp.T.copy(), 20, 20) - 90.0) < 0.05  # brighter downwards (+y)
+
+
+# -------------------------------------------------------- DistributeOctTree
+def _distribute_python(keys, minX, maxX, minY, maxY, N):
+    """Second statement of DistributeOctTree (ORBextractor.cc:688-1033) in
+    plain Python; ties in the phase-2 sort by creation order, newer first."""
+    nIni = int(math.floor(float(np.float32(np.float32(maxX - minX) / np.float32(maxY - minY))) + 0.5))
+    hX = np.float32(np.float32(maxX - minX) / np.float32(nIni))
+    seq = [0]
+
+    def node(x0, y0, x1, y1, ks):
+        seq[0] += 1
+        return {"r": (x0, y0, x1, y1), "k": ks, "seq": seq[0], "nomore": len(ks) == 1}
+
+    def divide(n):
+        x0, y0, x1, y1 = n["r"]
+        hx, hy = (x1 - x0 + 1) // 2, (y1 - y0 + 1) // 2
+        mx, my = x0 + hx, y0 + hy
+        parts = [[], [], [], []]
+        for k in n["k"]:
+            parts[(0 if k[0] < mx else 1) + (0 if k[1] < my else 2)].append(k)
+        rects = [(x0, y0, mx, my), (mx, y0, x1, my), (x0, my, mx, y1), (mx, my, x1, y1)]
+        return [(rects[q], parts[q]) for q in range(4)]
+
+    roots = [[] for _ in range(nIni)]
+    for k in keys:
+        roots[int(np.float32(np.float32(k[0]) / hX))].append(k)
+    lst = []
+    for i in range(nIni):
+        x0 = int(np.float32(hX * np.float32(i)))
+        x1 = int(np.float32(hX * np.float32(i + 1)))
+        if roots[i]:
+            lst.append(node(x0, 0, x1, maxY - minY, roots[i]))
+    while True:
+        prev = len(lst)
+        front, keep, expand = [], [], []
+        for n in lst:
+            if n["nomore"]:
+                keep.append(n)
+                continue
+            for r, ks in divide(n):
+                if ks:
+                    c = node(*r, ks)
+                    front.insert(0, c)
+                    if len(ks) > 1:
+                        expand.append(c)
+        lst = front + keep
+        if len(lst) >= N or len(lst) == prev:
+            break
+        if len(lst) + 3 * len(expand) > N:
+            while True:
+                prev = len(lst)
+                order = sorted(expand, key=lambda n: (len(n["k"]), n["seq"]), reverse=True)
+                expand = []
+                for n in order:
+                    kids = []
+                    for r, ks in divide(n):
+                        if ks:
+                            c = node(*r, ks)
+                            kids.insert(0, c)
+                            if len(ks) > 1:
+                                expand.append(c)
+                    lst = kids + [m for m in lst if m is not n]
+                    if len(lst) >= N:
+                        break
+                if len(lst) >= N or len(lst) == prev:
+                    break
+            break
+    out = []
+    for n in lst:
+        best = n["k"][0]
+        for k in n["k"][1:]:
+            if k[2] > best[2]:
+                best = k
+        out.append(best)
+    return np.array(out, np.int64).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("n,N,seed,w,h", [(0, 10, 0, 300, 100), (1, 10, 1, 300, 100), (40, 10, 2, 300, 100),
+                                          (200, 60, 3, 147, 102), (300, 217, 4, 608, 448),
+                                          (120, 122, 5, 314, 73), (500, 87, 6, 225, 161)])
+def test_distribute_octree_matches_python(oracle, n, N, seed, w, h):
+    rng = np.random.default_rng(seed)
+    pts = set()
+    while len(pts) < n:
+        pts.add((int(rng.integers(3, w - 3)), int(rng.integers(3, h - 3))))
+    pts = sorted(pts, key=lambda p: (p[1] // 30, p[0] // 30, p[1], p[0]))
+    keys = np.array([(x, y, int(rng.integers(7, 40))) for x, y in pts], np.int64).reshape(-1, 3)
+    got = oracle.distribute_octree(keys, 16, 16 + w, 16, 16 + h, N)
+    ref = _distribute_python([tuple(k) for k in keys], 16, 16 + w, 16, 16 + h, N)
+    np.testing.assert_array_equal(got, ref)
+    if n:
+        assert min(n, N) <= len(got) <= max(N + 3, 4 * round(w / h))
+        assert len({tuple(k) for k in got}) == len(got)
+
+
+# ---------------------------------------------------------------- extraction
+def test_extract_invariants_and_descriptors(oracle):
+    img = synth.shapes_frame(640, 480, 7)
+    ex = oracle.OracleExtractor(1000, 1.2, 8, 20, 7)
+    k, d = ex(img)
+    t = ex.tables()
+    assert 0 < len(k) <= 1000 + 3 * 8
+    assert np.all(np.diff(k["octave"]) >= 0)
+    assert np.all(k["class_id"] == -1)
+    pattern = _pattern()
+    for lvl in range(8):
+        sel = np.nonzero(k["octave"] == lvl)[0]
+        assert len(sel) <= t["quota"][lvl] + 3
+        lw, lh = ex.level_size(lvl)
+        blur = ex.level_image(lvl, blurred=True)
+        assert np.all(k["size"][sel] == np.float32(int(np.float32(31) * t["scale"][lvl])))
+        for i in sel[:40]:
+            x = int(round(float(k["x"][i] / t["scale"][lvl]))) if lvl else int(k["x"][i])
+            y = int(round(float(k["y"][i] / t["scale"][lvl]))) if lvl else int(k["y"][i])
+            assert 19 <= x < lw - 19 and 19 <= y < lh - 19
+            ang = np.float32(k["angle"][i]) * np.float32(math.pi / 180.0)
+            a = np.float32(math.cos(float(ang)))
+            b = np.float32(math.sin(float(ang)))
+            bits = []
+            for p in range(256):
+                vals = []
+                for q in (2 * p, 2 * p + 1):
+                    px, py = np.float32(pattern[2 * q]), np.float32(pattern[2 * q + 1])
+                    ry = int(np.rint(np.float32(px * b) + np.float32(py * a)))
+                    rx = int(np.rint(np.float32(px * a) - np.float32(py * b)))
+                    vals.append(int(blur[y + ry, x + rx]))
+                bits.append(vals[0] < vals[1])
+            ref = np.packbits(np.array(bits, np.uint8), bitorder="little")
+            np.testing.assert_array_equal(d[i], ref)
+
+
+def _pattern():
+    path = os.path.join(os.path.dirname(__file__), "..", "orb_slam2_detailed_comments_amd", "csrc", "orb_pattern.inc")
+    vals = []
+    for line in open(path):
+        if line.startswith("//"):
+            continue
+        vals += [int(v) for v in line.replace(",", " ").split()]
+    assert len(vals) == 1024
+    return vals
+
+
+def test_pattern_table_integrity():
+    import hashlib
+    vals = _pattern()
+    path = os.path.join(os.path.dirname(__file__), "..", "orb_slam2_detailed_comments_amd", "csrc", "orb_pattern.inc")
+    header = open(path).read()
+    digest = hashlib.sha256(bytes(v & 0xFF for v in vals)).hexdigest()
+    assert f"sha256(int8 bytes) = {digest}" in header
+    assert max(map(abs, vals)) == 13
+    # first pairs of ORB's learned pattern
+    assert vals[:8] == [8, -3, 9, 5, 4, 2, 7, -12]
+    ref = "/root/reference/src/ORBextractor.cc"
+    if os.path.exists(ref):
+        import importlib.util
+        spec = importlib.util.spec_from_file_location(
+            "gen_pattern", os.path.join(os.path.dirname(path), "gen_pattern.py"))
+        gp = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(gp)
+        assert gp.parse_pattern(open(ref, encoding="utf-8", errors="replace").read()) == vals
+
+
+def test_descriptor_straddles_are_rare(oracle):
+    """Descriptor bits that flip between the canonical statement and the
+    variants a reference build may use (libm cosf, FMA-contracted x*b+y*a)."""
+    img = synth.shapes_frame(1241, 376, 5)
+    base = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    k0, d0 = base(img)
+    for v in (1, 2, 3):
+        ex = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+        ex.set_variant(v)
+        k, d = ex(img)
+        np.testing.assert_array_equal(k["x"], k0["x"])
+        diff = np.any(d != d0, axis=1).sum()
+        assert diff <= 0.02 * len(k0), (v, diff)
+
+
+# ------------------------------------------------------------------ matching
+def test_match_bf_matches_numpy(oracle):
+    rng = np.random.default_rng(5)
+    q = rng.integers(0, 256, (200, 32), dtype=np.uint8)
+    t = np.concatenate([q[:80] ^ (rng.random((80, 32)) < 0.02).astype(np.uint8), rng.integers(0, 256, (150, 32),
+                                                                                             dtype=np.uint8)])
+    D = np.unpackbits(q[:, None, :] ^ t[None, :, :], axis=2).sum(2)
+    idx, dist = oracle.match_bf(q, t, 50, 0.6)
+    for i in range(len(q)):
+        order = np.lexsort((np.arange(len(t)), D[i]))
+        best, second = D[i, order[0]], D[i, order[1]]
+        ok = best <= 50 and best < np.float32(second) * np.float32(0.6)
+        assert idx[i] == (order[0] if ok else -1)
+        assert dist[i] == best
+    assert (idx[:80] == np.arange(80)).all()
+
+
+def test_knn2_finds_planted(oracle):
+    q, m, planted = synth.descriptor_sets(50, 5000, 11)
+    idx, dist, acc = oracle.knn2(q, m, 0.7, nthreads=4)
+    D = np.unpackbits(q[:, None, :] ^ m[None, :, :], axis=2).sum(2)
+    order = np.argsort(D, axis=1, kind="stable")
+    np.testing.assert_array_equal(idx, order[:, :2])
+    np.testing.assert_array_equal(dist, np.take_along_axis(D, order[:, :2], 1))
+    for i in range(50):
+        rows = np.nonzero(planted == i)[0]
+        if len(rows):
+            assert idx[i, 0] in rows
+
+
+# ------------------------------------------------------------------- stereo
+def test_stereo_recovers_planted_disparity(oracle):
+    L, R, disp = synth.stereo_pair(1241, 376, 8)
+    bf, fx = 386.1, 718.856
+    b = float(np.float32(bf) / np.float32(fx))
+    ol = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    kl, dl, kr, dr, uR, dep = oracle.stereo_frame(ol, orr, L, R, bf, b)
+    ok = uR >= 0
+    assert ok.sum() > 500
+    d_est = kl["x"][ok] - uR[ok]
+    # The right view is defined on its own grid: right(x, y) = left(x + d(x, y), y),
+    # so the planted disparity of a match is read at the right position.
+    d_true = disp[kl["y"][ok].astype(int), np.clip(np.rint(uR[ok]).astype(int), 0, L.shape[1] - 1)]
+    tol = 1.5 * kl["size"][ok] / 31.0          # one pixel of the keypoint's level, plus rounding
+    assert np.mean(np.abs(d_est - d_true) < tol) > 0.95
+    assert np.median(np.abs(d_est - d_true)) < 0.6
+    np.testing.assert_allclose(dep[ok], bf / d_est.astype(np.float32), rtol=1e-5)
+
+
+def test_stereo_no_right_keypoints(oracle):
+    L = synth.shapes_frame(1241, 376, 9)
+    ol = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    kl, dl, kr, dr, uR, dep = oracle.stereo_frame(ol, orr, L, np.full_like(L, 50), 386.1, 0.537)
+    assert len(kr) == 0 and np.all(uR == -1) and np.all(dep == -1)
