@@ -101,6 +101,25 @@ int orb_descriptor_distance(const uint8_t* a, const uint8_t* b);
 int orb_match_bf(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* train, int nt, int th_low,
                  float ratio, int32_t* idx, int32_t* dist);
 
+/* Dense k=2 search: for each of nq query rows the two map rows nearest in
+ * Hamming distance, ordered by (distance, index) -- the order of the
+ * best/second-best update of ORBmatcher (src/ORBmatcher.cc:614-647, distance
+ * of ORBmatcher::DescriptorDistance) -- over all nm map rows.
+ * idx2/dist2: 2 ints per query (-1 / INT_MAX when fewer than two rows);
+ * accept[i] = best < (float)second * ratio (may be NULL).
+ * method 0: XOR + popcount kernel; 1: int8 +-1 MFMA kernel (dot = 256 - 2*d).
+ * Both give identical results.  nm < 2^31. */
+int orb_knn2(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* map, int64_t nm, int method, float ratio,
+             int32_t* idx2, int32_t* dist2, uint8_t* accept);
+/* Same on device-resident buffers; returns once enqueued on the context stream. */
+int orb_knn2_device(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm, int method,
+                    float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept);
+/* Average device milliseconds of the search and merge kernels over `steps`
+ * runs (HIP events on the context stream). */
+int orb_knn2_time(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm, int method,
+                  float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept, int steps, float* ms_search,
+                  float* ms_merge);
+
 /* Frame::ComputeStereoMatches() (include/Frame.h:249, src/Frame.cc:831-1082)
  * on keypoints/descriptors of a left and a right image, reading the two
  * extractors' pyramids from their last calls.  bf = Camera.bf, b = bf/fx.

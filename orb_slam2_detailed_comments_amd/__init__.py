@@ -20,7 +20,7 @@ import os
 
 import numpy as np
 
-__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "match_bf",
+__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "match_bf", "knn2",
            "compute_stereo_matches", "stereo_frame", "lib", "library_path", "OrbError"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
@@ -55,6 +55,9 @@ SIGNATURES = {
     "orb_stereo_match": (_I, [_P, _P, _P, _P, _I, _P, _P, _I, _F, _F, _P, _P]),
     "orb_stereo_frame": (_I, [_P, _P, _P, _P, _I, _I, _S, _F, _F, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "orb_context_stream": (_P, [_P]),
+    "orb_knn2": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
+    "orb_knn2_device": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
+    "orb_knn2_time": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P, _I, _P, _P]),
     "orb_batch_create": (_I, [_P, _I, _P, _P, _P, _F, _I, _I, _I, _P]),
     "orb_batch_destroy": (_V, [_P]),
     "orb_batch_size": (_I, [_P]),
@@ -270,6 +273,20 @@ def match_bf(query, train, th_low=ORBmatcher.TH_LOW, ratio=0.6, context=None):
     _check("orb_match_bf", lib().orb_match_bf(ctx.h, _ptr(q), len(q), _ptr(t), len(t), int(th_low), float(ratio),
                                              _ptr(idx), _ptr(dist)))
     return idx, dist
+
+
+def knn2(query, map_rows, ratio=0.7, method="mfma", context=None):
+    """Dense k=2 Hamming search: (idx[nq, 2], dist[nq, 2], accept[nq])."""
+    ctx = context or Context.default()
+    q = np.ascontiguousarray(query, np.uint8)
+    m = np.ascontiguousarray(map_rows, np.uint8)
+    idx = np.zeros((len(q), 2), np.int32)
+    dist = np.zeros((len(q), 2), np.int32)
+    acc = np.zeros(len(q), np.uint8)
+    meth = {"popcount": 0, "mfma": 1}[method]
+    _check("orb_knn2", lib().orb_knn2(ctx.h, _ptr(q), len(q), _ptr(m), len(m), meth, float(ratio), _ptr(idx),
+                                      _ptr(dist), _ptr(acc)))
+    return idx, dist, acc.astype(bool)
 
 
 def compute_stereo_matches(left, right, kl, dl, kr, dr, bf, b):

@@ -1151,18 +1151,18 @@ extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable*
                                          hipStream_t st, hipEvent_t* ev)
 {
     int e = 0;
-    if (ev) hipEventRecord(ev[e++], st);
+    if (ev) (void)hipEventRecord(ev[e++], st);
     zero_counters_kernel<<<h_rt->njobs, 256, 0, st>>>(d_jobs, h_rt->njobs);
     for (int l = 0; l < h_rt->max_levels; l++) {
         const int nb = h_rt->tile_start[l][h_rt->njobs];
         if (nb > 0) level_kernel<<<nb, 256, 0, st>>>(d_jobs, d_rt, l);
-        if (ev) hipEventRecord(ev[e++], st);
+        if (ev) (void)hipEventRecord(ev[e++], st);
     }
     octree_kernel<<<h_rt->njobs * h_rt->max_levels, OCT_T, 0, st>>>(d_jobs, h_rt->max_levels);
-    if (ev) hipEventRecord(ev[e++], st);
+    if (ev) (void)hipEventRecord(ev[e++], st);
     const int nslots = h_rt->slot_start[h_rt->njobs];
     if (nslots > 0) describe_kernel<<<(nslots + 3) / 4, 256, 0, st>>>(d_jobs, d_rt);
-    if (ev) hipEventRecord(ev[e++], st);
+    if (ev) (void)hipEventRecord(ev[e++], st);
     return hipGetLastError();
 }
 
@@ -1179,9 +1179,9 @@ extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, in
     if (npairs <= 0) return hipSuccess;
     zero_match_counts_kernel<<<1, 256, 0, st>>>(d_args, npairs);
     if (nwaves > 0) stereo_kernel<<<(nwaves + 3) / 4, 256, 0, st>>>(d_args, npairs, nwaves);
-    if (ev) hipEventRecord(ev[0], st);
+    if (ev) (void)hipEventRecord(ev[0], st);
     stereo_median_kernel<<<npairs, 1024, 0, st>>>(d_args);
-    if (ev) hipEventRecord(ev[1], st);
+    if (ev) (void)hipEventRecord(ev[1], st);
     return hipGetLastError();
 }
 
@@ -1190,5 +1190,240 @@ extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const
                                           hipStream_t st)
 {
     if (cap_q > 0) match_bf_kernel<<<(cap_q + 3) / 4, 256, 0, st>>>(q, nq, t, nt, th_low, ratio, idx, dist, cap_q);
+    return hipGetLastError();
+}
+
+// ============================================================ dense k=2
+// All-pairs Hamming search of nq queries against nm map rows, k = 2 nearest
+// by (distance, index), the order of ORBmatcher's best/second-best update
+// (ORBmatcher.cc:614-647).  The map is split into chunks of KNN_CHUNK rows;
+// each (chunk, query tile) workgroup keeps per query the two smallest keys
+//   key = distance << 20 | row-in-chunk
+// (lexicographic (distance, index) is numeric key order, so the update is
+// k1 = med3(k0, k1, key); k0 = min(k0, key)), and a merge kernel reduces the
+// chunks.  Two kernels produce the same partial keys:
+//   knn_popc_kernel  XOR + v_bcnt popcount, one query per lane, map rows
+//                    broadcast from LDS
+//   knn_mfma_kernel  bits expanded to int8 +-1; v_mfma_i32_16x16x64_i8 gives
+//                    dot = 256 - 2*Hamming exactly (int32 accumulate)
+#define KNN_CHUNK 8192
+#define KNN_NONE 0xFFFFFFFFu
+
+// Median of three (the compiler emits v_med3_u32).
+__device__ __forceinline__ unsigned med3u(unsigned a, unsigned b, unsigned c)
+{
+    return max(min(a, b), min(max(a, b), c));
+}
+
+__global__ __launch_bounds__(256) void knn_popc_kernel(const uint8_t* __restrict__ q, int nq,
+                                                       const uint8_t* __restrict__ m, long long nm,
+                                                       uint2* __restrict__ part, int nchunks)
+{
+    __shared__ uint4 rows[256][2];
+    const int tid = threadIdx.x;
+    const int chunk = blockIdx.x;
+    const int qi = blockIdx.y * 256 + tid;
+    uint4 a0 = make_uint4(0, 0, 0, 0), a1 = a0;
+    if (qi < nq) {
+        a0 = *(const uint4*)(q + (size_t)qi * 32);
+        a1 = *(const uint4*)(q + (size_t)qi * 32 + 16);
+    }
+    const long long c0 = (long long)chunk * KNN_CHUNK;
+    const int len = (int)min((long long)KNN_CHUNK, nm - c0);
+    unsigned k0 = KNN_NONE, k1 = KNN_NONE;
+    for (int t0 = 0; t0 < len; t0 += 256) {
+        const int n = min(256, len - t0);
+        __syncthreads();
+        if (tid < n) {
+            const uint8_t* r = m + (size_t)(c0 + t0 + tid) * 32;
+            rows[tid][0] = *(const uint4*)r;
+            rows[tid][1] = *(const uint4*)(r + 16);
+        }
+        __syncthreads();
+        for (int r = 0; r < n; r++) {
+            const uint4 b0 = rows[r][0], b1 = rows[r][1];
+            unsigned d = __popc(a0.x ^ b0.x);
+            d += __popc(a0.y ^ b0.y);
+            d += __popc(a0.z ^ b0.z);
+            d += __popc(a0.w ^ b0.w);
+            d += __popc(a1.x ^ b1.x);
+            d += __popc(a1.y ^ b1.y);
+            d += __popc(a1.z ^ b1.z);
+            d += __popc(a1.w ^ b1.w);
+            const unsigned key = (d << 20) | (unsigned)(t0 + r);
+            k1 = med3u(k0, k1, key);
+            k0 = min(k0, key);
+        }
+    }
+    if (qi < nq) part[(size_t)qi * nchunks + chunk] = make_uint2(k0, k1);
+}
+
+// Bit -> int8 +-1 expansion shared by both MFMA operands.  Flat dword F
+// (0..63) of a descriptor holds, in bytes 0..3, bits {i, i+8, i+16, i+24} of
+// 32-bit word u = F/8, with i = F%8: one shift, one mask and one v_perm.
+__device__ __forceinline__ unsigned pm1_dword(unsigned w, int i)
+{
+    const unsigned sel = (i >= 2 ? (w >> (i - 2)) : (w << (2 - i))) & 0x04040404u;
+    return __builtin_amdgcn_perm(0x01010101u, 0xFFFFFFFFu, sel);
+}
+
+typedef int knn_v4i __attribute__((ext_vector_type(4)));
+
+// Workgroup: 4 wavefronts x 64 queries, one map chunk.  Map rows are
+// expanded 64 at a time into LDS in operand order [subtile][k-chunk][lane];
+// each wavefront runs 4 (k-chunks) x 4 (query groups) MFMAs per 16 rows.
+__global__ __launch_bounds__(256) void knn_mfma_kernel(const uint8_t* __restrict__ q, int nq,
+                                                       const uint8_t* __restrict__ m, long long nm,
+                                                       uint2* __restrict__ part, int nchunks)
+{
+    __shared__ uint4 A[4][4][64];   // 16 KB: 64 rows of +-1 bytes
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane >> 4, i16 = lane & 15;
+    const int chunk = blockIdx.x;
+    const int qbase = blockIdx.y * 256 + wv * 64;
+
+    // Query operands: group gq, k-chunk c -> this lane's 4 dwords F = 16c+4g+d.
+    knn_v4i B[4][4];
+#pragma unroll
+    for (int gq = 0; gq < 4; gq++) {
+        const int qi = qbase + 16 * gq + i16;
+        unsigned w[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) w[u] = qi < nq ? *(const unsigned*)(q + (size_t)qi * 32 + 4 * u) : 0u;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const int F = 16 * c + 4 * g + d;
+                B[gq][c][d] = (int)pm1_dword(w[F >> 3], F & 7);
+            }
+        }
+    }
+    const long long c0 = (long long)chunk * KNN_CHUNK;
+    const int len = (int)min((long long)KNN_CHUNK, nm - c0);
+    unsigned k0[4], k1[4];
+#pragma unroll
+    for (int gq = 0; gq < 4; gq++) k0[gq] = k1[gq] = KNN_NONE;
+
+    // Fill role of this thread: row fr (0..63) of the tile, k-chunk fc.
+    const int fr = tid >> 2, fc = tid & 3;
+    for (int t0 = 0; t0 < len; t0 += 64) {
+        const int n = min(64, len - t0);
+        __syncthreads();
+        {
+            uint2 wv2 = make_uint2(0, 0);
+            if (fr < n) wv2 = *(const uint2*)(m + (size_t)(c0 + t0 + fr) * 32 + 8 * fc);
+            const int s = fr >> 4, ii = fr & 15;
+#pragma unroll
+            for (int gg = 0; gg < 4; gg++) {
+                uint4 v;
+                const int F0 = 16 * fc + 4 * gg;   // dwords F0..F0+3 of word 2fc + gg/2
+                const unsigned w = ((F0 >> 3) & 1) ? wv2.y : wv2.x;
+                v.x = pm1_dword(w, (F0 + 0) & 7);
+                v.y = pm1_dword(w, (F0 + 1) & 7);
+                v.z = pm1_dword(w, (F0 + 2) & 7);
+                v.w = pm1_dword(w, (F0 + 3) & 7);
+                A[s][fc][ii + 16 * gg] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            if (16 * s >= n) break;
+            knn_v4i acc[4];
+#pragma unroll
+            for (int gq = 0; gq < 4; gq++) acc[gq] = (knn_v4i){0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint4 av = A[s][c][lane];
+                const knn_v4i a = (knn_v4i){(int)av.x, (int)av.y, (int)av.z, (int)av.w};
+#pragma unroll
+                for (int gq = 0; gq < 4; gq++)
+                    acc[gq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, B[gq][c], acc[gq], 0, 0, 0);
+            }
+            // Accumulator (lane, r) = (map row 16s + 4g + r, query 16gq + i16).
+            // key = (256 - dot) << 19 | row = Hamming << 20 | row.
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = t0 + 16 * s + 4 * g + r;
+                const int cbase = (256 << 19) + row;
+                const bool ok = row < len;
+#pragma unroll
+                for (int gq = 0; gq < 4; gq++) {
+                    const unsigned key = ok ? (unsigned)(__mul24(acc[gq][r], -(1 << 19)) + cbase)
+                                            : KNN_NONE;
+                    k1[gq] = med3u(k0[gq], k1[gq], key);
+                    k0[gq] = min(k0[gq], key);
+                }
+            }
+        }
+    }
+    // Merge the four lanes that hold the same query (lanes i16 + 16g).
+#pragma unroll
+    for (int gq = 0; gq < 4; gq++) {
+#pragma unroll
+        for (int o = 16; o <= 32; o <<= 1) {
+            const unsigned b0 = __shfl_xor(k0[gq], o, 64), b1 = __shfl_xor(k1[gq], o, 64);
+            const unsigned n0 = min(k0[gq], b0);
+            const unsigned n1 = min(max(k0[gq], b0), min(k1[gq], b1));
+            k0[gq] = n0;
+            k1[gq] = n1;
+        }
+        const int qi = qbase + 16 * gq + i16;
+        if (g == 0 && qi < nq) part[(size_t)qi * nchunks + chunk] = make_uint2(k0[gq], k1[gq]);
+    }
+}
+
+// Reduce the per-chunk keys of each query to the global two nearest and apply
+// the ratio test (best < ratio * second, in float).
+__global__ __launch_bounds__(256) void knn_merge_kernel(const uint2* __restrict__ part, int nq, int nchunks,
+                                                        float ratio, int* __restrict__ idx2, int* __restrict__ dist2,
+                                                        uint8_t* __restrict__ accept)
+{
+    const int qi = blockIdx.x * 256 + threadIdx.x;
+    if (qi >= nq) return;
+    unsigned long long b0 = ~0ull, b1 = ~0ull;
+    for (int c = 0; c < nchunks; c++) {
+        const uint2 p = part[(size_t)qi * nchunks + c];
+        const unsigned ks[2] = {p.x, p.y};
+        for (int t = 0; t < 2; t++) {
+            if (ks[t] == KNN_NONE) continue;
+            const unsigned long long k = ((unsigned long long)(ks[t] >> 20) << 40) |
+                                         (unsigned long long)((long long)c * KNN_CHUNK + (ks[t] & 0xFFFFF));
+            if (k < b0) {
+                b1 = b0;
+                b0 = k;
+            } else if (k < b1) {
+                b1 = k;
+            }
+        }
+    }
+    const int d0 = b0 == ~0ull ? INT_MAX : (int)(b0 >> 40), d1 = b1 == ~0ull ? INT_MAX : (int)(b1 >> 40);
+    idx2[2 * qi] = b0 == ~0ull ? -1 : (int)(b0 & 0xFFFFFFFFFFull);
+    idx2[2 * qi + 1] = b1 == ~0ull ? -1 : (int)(b1 & 0xFFFFFFFFFFull);
+    dist2[2 * qi] = d0;
+    dist2[2 * qi + 1] = d1;
+    if (accept) accept[qi] = (uint8_t)(b0 != ~0ull && b1 != ~0ull && (float)d0 < (float)d1 * ratio);
+}
+
+extern "C" int orb_knn_chunks(long long nm) { return (int)((nm + KNN_CHUNK - 1) / KNN_CHUNK); }
+
+extern "C" hipError_t orb_launch_knn2(const uint8_t* q, int nq, const uint8_t* m, long long nm, int method,
+                                      float ratio, uint2* part, int* idx2, int* dist2, uint8_t* accept,
+                                      hipStream_t st, hipEvent_t* ev)
+{
+    const int nchunks = orb_knn_chunks(nm);
+    if (nq <= 0) return hipSuccess;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    if (nchunks > 0) {
+        const dim3 grid(nchunks, (nq + 255) / 256);
+        if (method == 1)
+            knn_mfma_kernel<<<grid, 256, 0, st>>>(q, nq, m, nm, part, nchunks);
+        else
+            knn_popc_kernel<<<grid, 256, 0, st>>>(q, nq, m, nm, part, nchunks);
+    }
+    if (ev) (void)hipEventRecord(ev[1], st);
+    knn_merge_kernel<<<(nq + 255) / 256, 256, 0, st>>>(part, nq, nchunks, ratio, idx2, dist2, accept);
+    if (ev) (void)hipEventRecord(ev[2], st);
     return hipGetLastError();
 }

@@ -25,8 +25,12 @@ extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const
                                           int th_low, float ratio, int* idx, int* dist, int cap_q,
                                           hipStream_t st);
 
-namespace {
+extern "C" int orb_knn_chunks(long long nm);
+extern "C" hipError_t orb_launch_knn2(const uint8_t* q, int nq, const uint8_t* m, long long nm, int method,
+                                      float ratio, uint2* part, int* idx2, int* dist2, uint8_t* accept,
+                                      hipStream_t st, hipEvent_t* ev);
 
+namespace {
 
 inline int cv_round(float v) { return (int)std::lrintf(v); }
 
@@ -97,6 +101,27 @@ struct orb_context {
     uint8_t* d_t = nullptr;
     int* d_out = nullptr;
     size_t q_cap = 0, t_cap = 0;
+    // dense k=2 search scratch
+    uint2* d_part = nullptr;
+    size_t part_cap = 0;
+    uint8_t* d_kq = nullptr;
+    uint8_t* d_km = nullptr;
+    int* d_kout = nullptr;
+    size_t kq_cap = 0, km_cap = 0;
+
+    int grow(void** p, size_t* cap, size_t bytes)
+    {
+        if (bytes <= *cap) return ORB_OK;
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            return ORB_E_OOM;
+        }
+        *cap = bytes;
+        return ORB_OK;
+    }
 };
 
 struct orb_extractor {
@@ -487,6 +512,10 @@ void orb_context_destroy(orb_context* ctx)
     if (ctx->d_q) (void)hipFree(ctx->d_q);
     if (ctx->d_t) (void)hipFree(ctx->d_t);
     if (ctx->d_out) (void)hipFree(ctx->d_out);
+    if (ctx->d_part) (void)hipFree(ctx->d_part);
+    if (ctx->d_kq) (void)hipFree(ctx->d_kq);
+    if (ctx->d_km) (void)hipFree(ctx->d_km);
+    if (ctx->d_kout) (void)hipFree(ctx->d_kout);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1059,6 +1088,89 @@ int orb_batch_stereo_results(orb_batch* b, int pair, float* uRight, float* depth
         if (uRight) HIP_TRY(hipMemcpy(uRight, L->d_uR, (size_t)*n * 4, hipMemcpyDeviceToHost));
         if (depth) HIP_TRY(hipMemcpy(depth, L->d_depth, (size_t)*n * 4, hipMemcpyDeviceToHost));
     }
+    return ORB_OK;
+}
+
+}  // extern "C"
+
+// ============================================================= dense k=2
+extern "C" {
+
+int orb_knn2_device(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm, int method,
+                    float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept)
+{
+    if (!ctx || nq < 0 || nm < 0 || nm >= (1ll << 31) || (method != 0 && method != 1)) return ORB_E_ARG;
+    if (nq == 0) return ORB_OK;
+    if (!d_query || !d_idx2 || !d_dist2 || (nm && !d_map)) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t nchunks = (size_t)orb_knn_chunks(nm);
+    int rc = ctx->grow((void**)&ctx->d_part, &ctx->part_cap, std::max<size_t>(1, (size_t)nq * nchunks) * sizeof(uint2));
+    if (rc) return rc;
+    HIP_TRY(orb_launch_knn2(d_query, nq, d_map, nm, method, ratio, ctx->d_part, d_idx2, d_dist2, d_accept,
+                            ctx->stream, nullptr));
+    return ORB_OK;
+}
+
+int orb_knn2(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* map, int64_t nm, int method, float ratio,
+             int32_t* idx2, int32_t* dist2, uint8_t* accept)
+{
+    if (!ctx || nq < 0 || nm < 0 || (nq && (!query || !idx2 || !dist2)) || (nm && !map)) return ORB_E_ARG;
+    if (nq == 0) return ORB_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = ctx->grow((void**)&ctx->d_kq, &ctx->kq_cap, (size_t)nq * 32);
+    if (!rc) rc = ctx->grow((void**)&ctx->d_km, &ctx->km_cap, std::max<size_t>(1, (size_t)nm * 32));
+    if (rc) return rc;
+    int* d_out = nullptr;
+    if (hipMalloc(&d_out, (size_t)nq * 17) != hipSuccess) return ORB_E_OOM;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(ctx->d_kq, query, (size_t)nq * 32, hipMemcpyHostToDevice, st));
+    if (nm) HIP_TRY(hipMemcpyAsync(ctx->d_km, map, (size_t)nm * 32, hipMemcpyHostToDevice, st));
+    rc = orb_knn2_device(ctx, ctx->d_kq, nq, ctx->d_km, nm, method, ratio, d_out, d_out + 2 * (size_t)nq,
+                         (uint8_t*)(d_out + 4 * (size_t)nq));
+    if (!rc) {
+        if (hipMemcpyAsync(idx2, d_out, (size_t)nq * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(dist2, d_out + 2 * (size_t)nq, (size_t)nq * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            (accept && hipMemcpyAsync(accept, d_out + 4 * (size_t)nq, (size_t)nq, hipMemcpyDeviceToHost, st) !=
+                           hipSuccess) ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = ORB_E_DEVICE;
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+// Average device time of the search kernel and of the merge kernel over
+// `steps` runs on device-resident data (HIP events on the context stream).
+int orb_knn2_time(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm, int method,
+                  float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept, int steps, float* ms_search,
+                  float* ms_merge)
+{
+    if (!ctx || steps <= 0 || nq <= 0 || !d_query || !d_map || nm <= 0 || !d_idx2 || !d_dist2) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t nchunks = (size_t)orb_knn_chunks(nm);
+    int rc = ctx->grow((void**)&ctx->d_part, &ctx->part_cap, (size_t)nq * nchunks * sizeof(uint2));
+    if (rc) return rc;
+    hipEvent_t ev[3];
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    double a = 0, b = 0;
+    for (int s = 0; s < steps && !rc; s++) {
+        if (orb_launch_knn2(d_query, nq, d_map, nm, method, ratio, ctx->d_part, d_idx2, d_dist2, d_accept,
+                            ctx->stream, ev) != hipSuccess ||
+            hipEventSynchronize(ev[2]) != hipSuccess) {
+            rc = ORB_E_DEVICE;
+            break;
+        }
+        float t1 = 0, t2 = 0;
+        (void)hipEventElapsedTime(&t1, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&t2, ev[1], ev[2]);
+        a += t1;
+        b += t2;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    if (ms_search) *ms_search = (float)(a / steps);
+    if (ms_merge) *ms_merge = (float)(b / steps);
     return ORB_OK;
 }
 

@@ -1,0 +1,55 @@
+"""Dense k=2 Hamming search: popcount and int8 +-1 MFMA kernels, bit-exact
+against the oracle (distances and indices, ties broken by index)."""
+import numpy as np
+import pytest
+
+from orb_slam2_detailed_comments_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("method", ["popcount", "mfma"])
+@pytest.mark.parametrize("nq,nm,seed", [(1, 1, 0), (3, 2, 1), (17, 100, 2), (65, 8193, 3), (300, 20000, 4),
+                                        (257, 16384, 5)])
+def test_knn2_small_exact(orb, oracle, method, nq, nm, seed):
+    rng = np.random.default_rng(seed)
+    q = rng.integers(0, 256, (nq, 32), dtype=np.uint8)
+    m = rng.integers(0, 256, (nm, 32), dtype=np.uint8)
+    # ties: duplicated rows and rows equal to queries
+    if nm > 4:
+        m[nm // 2] = m[1]
+        m[nm - 1] = q[0]
+        m[0] = q[0]
+    gi, gd, ga = orb.knn2(q, m, 0.7, method)
+    oi, od, oa = oracle.knn2(q, m, 0.7, nthreads=8)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(ga, oa)
+
+
+def test_knn2_planted_config3_subset(orb, oracle):
+    q, m, planted = synth.descriptor_sets(2000, 200_000, 33)
+    oi, od, oa = oracle.knn2(q, m, 0.7, nthreads=16)
+    for method in ("popcount", "mfma"):
+        gi, gd, ga = orb.knn2(q, m, 0.7, method)
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gd, od)
+        np.testing.assert_array_equal(ga, oa)
+
+
+def test_knn2_full_size_methods_agree(orb):
+    q, m, planted = synth.descriptor_sets(2000, 1_000_000, 3)
+    pi, pd, pa = orb.knn2(q, m, 0.7, "popcount")
+    mi, md, ma = orb.knn2(q, m, 0.7, "mfma")
+    np.testing.assert_array_equal(mi, pi)
+    np.testing.assert_array_equal(md, pd)
+    np.testing.assert_array_equal(ma, pa)
+    # every query with a planted copy finds one as its nearest neighbour
+    has = np.zeros(2000, bool)
+    has[planted[planted >= 0]] = True
+    assert np.all(planted[mi[has, 0]] == np.nonzero(has)[0])
+    # distances match the rows they name
+    rows = np.arange(2000)
+    d0 = np.unpackbits(q ^ m[mi[:, 0]], axis=1).sum(1)
+    np.testing.assert_array_equal(d0, md[:, 0])
+    assert np.all(md[:, 0] <= md[:, 1])
