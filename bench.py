@@ -68,6 +68,76 @@ def stage_bytes(stage, n_images, kps_per_image, cands_per_image):
     return None
 
 
+def knn_extra(orb, ctx, steps):
+    """Dense k=2 search, 2000 queries x 1,000,000 map rows resident in HBM."""
+    import ctypes
+    from orb_slam2_detailed_comments_amd import synth
+    Q, M = 2000, 1_000_000
+    q, m, _ = synth.descriptor_sets(Q, M, 3)
+    out = {"workload": "k=2 Hamming search, 2000 queries x 1,000,000 map rows (10% planted), ratio 0.7",
+           "int8_ops": 2 * Q * M * 256}
+    for name, meth in (("popcount", 0), ("mfma", 1)):
+        ms_s, ms_m = ctypes.c_float(), ctypes.c_float()
+        rc = orb.lib().orb_knn2_time(ctx.h, ctypes.c_void_p(q.ctypes.data), Q, ctypes.c_void_p(m.ctypes.data), M,
+                                     meth, 0.7, steps, ctypes.byref(ms_s), ctypes.byref(ms_m))
+        if rc != 0:
+            raise RuntimeError(f"orb_knn2_time {rc}")
+        out[f"{name}_search_ms"] = round(ms_s.value, 4)
+        out[f"{name}_merge_ms"] = round(ms_m.value, 4)
+        out[f"{name}_searches_per_s"] = round(1e3 / (ms_s.value + ms_m.value), 1)
+    tops = out["int8_ops"] / (out["mfma_search_ms"] * 1e-3) / 1e12
+    out["mfma_roofline"] = {"bound": "mfma", "achieved": round(tops, 1), "peak": 5000.0, "unit": "TOP/s",
+                            "frac": round(tops / 5000.0, 4)}
+    # the popcount form does 8 XOR + 8 v_bcnt + 3 select ops per pair
+    gops = Q * M * 19 / (out["popcount_search_ms"] * 1e-3) / 1e12
+    out["popcount_valu_tops"] = round(gops, 1)
+    return out
+
+
+def serving_extra(orb, ctx):
+    """16 camera streams (4 each of 640x480, 752x480, 1241x376, 1280x720),
+    Poisson arrivals; extract + match against the stream's resident previous
+    descriptors.  Also the plain per-frame GPU path on the same frames."""
+    from orb_slam2_detailed_comments_amd import serving
+    streams = serving.MIXED_16
+    frames = serving.make_stream_frames(streams, 4, seed=300)
+    srv = orb.Server(streams, max_inflight=3, context=ctx)
+    out = {"workload": "16 streams (4x 640x480, 4x 752x480, 4x 1241x376, 4x 1280x720), Poisson arrivals, "
+                       "extract + match vs the stream's previous frame (TH_LOW 50, ratio 0.6)"}
+    recs, _ = serving.run(srv, frames, fps=30.0, duration=3.0, seed=1)
+    out["at_30fps_per_stream"] = serving.summarize(recs)
+    print(f"[bench] serving 30 fps/stream: {out['at_30fps_per_stream']}", file=sys.stderr, flush=True)
+    sweep = []
+    for fps in (120.0, 240.0, 480.0, 960.0):
+        recs, _ = serving.run(srv, frames, fps=fps, duration=1.0, seed=2)
+        r = serving.summarize(recs)
+        r["offered_fps_per_stream"] = fps
+        sweep.append(r)
+        print(f"[bench] serving {fps} fps/stream: {r}", file=sys.stderr, flush=True)
+    out["sweep"] = sweep
+    ok = [r for r in sweep if r["p99_ms"] <= 10.0 and r["errors"] == 0 and r["refused"] == 0]
+    out["max_frames_per_s_at_p99_le_10ms"] = max((r["frames_per_s"] for r in ok), default=None)
+    out["server_stats"] = srv.stats()
+    srv.close()
+    # plain per-frame GPU path: one ORBextractor call and one match per frame
+    exs = [orb.ORBextractor(nf, 1.2, 8, 20, 7, context=ctx) for (_, _, nf) in streams]
+    prev = [None] * len(streams)
+    for i, f in enumerate(frames):       # warm-up: one frame per stream
+        prev[i] = exs[i](f[0])[1]
+    n, t0 = 0, time.perf_counter()
+    for k in range(1, 4):
+        for i, f in enumerate(frames):
+            _, d = exs[i](f[k])
+            orb.match_bf(d, prev[i], 50, 0.6, context=ctx)
+            prev[i] = d
+            n += 1
+    plain = n / (time.perf_counter() - t0)
+    out["plain_per_frame_frames_per_s"] = round(plain, 1)
+    if out["max_frames_per_s_at_p99_le_10ms"]:
+        out["served_over_plain"] = round(out["max_frames_per_s_at_p99_le_10ms"] / plain, 2)
+    return out
+
+
 def cpu_baseline(frames, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -117,6 +187,7 @@ def main():
     ap.add_argument("--mode", choices=["graph", "stages", "per-image"], default="graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the dense k=2 search timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box in this build has one MI355X, and the ORB front end "
@@ -180,6 +251,11 @@ def main():
                       "mean_keypoints_per_image": round(float(np.mean(kps)), 1),
                       "mean_stereo_matches": round(float(np.mean(matched)), 1)},
            "roofline": roofline}
+    if not args.no_extras:
+        print("[bench] dense k=2 search", file=sys.stderr, flush=True)
+        out["knn2"] = knn_extra(orb, ctx, max(3, args.steps // 2))
+        print("[bench] serving", file=sys.stderr, flush=True)
+        out["serving"] = serving_extra(orb, ctx)
     if not args.no_cpu:
         print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
         cpu = cpu_baseline(frames[:8], args.cpu_seconds)

@@ -114,11 +114,11 @@ int orb_knn2(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* map,
 /* Same on device-resident buffers; returns once enqueued on the context stream. */
 int orb_knn2_device(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm, int method,
                     float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept);
-/* Average device milliseconds of the search and merge kernels over `steps`
- * runs (HIP events on the context stream). */
-int orb_knn2_time(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm, int method,
-                  float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept, int steps, float* ms_search,
-                  float* ms_merge);
+/* Copies query and map rows into HBM, then returns the average device
+ * milliseconds of the search and merge kernels over `steps` runs (HIP events
+ * on the context stream). */
+int orb_knn2_time(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* map, int64_t nm, int method,
+                  float ratio, int steps, float* ms_search, float* ms_merge);
 
 /* Frame::ComputeStereoMatches() (include/Frame.h:249, src/Frame.cc:831-1082)
  * on keypoints/descriptors of a left and a right image, reading the two
@@ -166,6 +166,43 @@ int orb_batch_stage_count(const orb_batch* b);
 int orb_batch_stage_times(orb_batch* b, int steps, float* ms);
 int orb_batch_results(orb_batch* b, int slot, orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n);
 int orb_batch_stereo_results(orb_batch* b, int pair, float* uRight, float* depth, int capacity, int* n);
+
+/* ---------------------------------------------------------------- serving
+ * Continuous-batching request loop for many camera streams.  No reference
+ * counterpart: it serves the per-frame hot path (ORB extraction as in the
+ * Frame constructors, src/Frame.cc:121-170, plus a brute-force Hamming match
+ * against the stream's previous frame, best/second rule of
+ * src/ORBmatcher.cc:614-647) for many cameras at once.  A server thread admits
+ * pending frames, oldest first, into extractor slots while the HBM budget
+ * allows (one frame per stream in flight, so each stream stays ordered),
+ * launches them as one batch on a free HIP stream (up to max_inflight batches
+ * overlap), keeps every stream's last descriptors resident in HBM, and stamps
+ * latencies with orb_server_clock() (CLOCK_MONOTONIC seconds).
+ * A frame whose size can never fit the budget completes with
+ * status ORB_E_ADMISSION; submit refuses (ORB_E_ADMISSION) beyond 4096
+ * pending frames. */
+typedef struct orb_server orb_server;
+typedef struct orb_serve_info {
+    uint64_t ticket;
+    int stream, status, n_keypoints, n_matches;
+    double t_arrival, t_admit, t_done;
+} orb_serve_info;
+int orb_server_create(orb_context* ctx, int nstreams, const int* widths, const int* heights, const int* nfeatures,
+                      float scaleFactor, int nlevels, int iniThFAST, int minThFAST, int th_low, float ratio,
+                      size_t hbm_budget, int max_inflight, int max_batch, orb_server** out);
+void orb_server_destroy(orb_server* s);
+double orb_server_clock(void);
+int orb_server_submit(orb_server* s, int stream, const uint8_t* image, size_t stride, double t_arrival,
+                      uint64_t* ticket);
+/* Completed frames, oldest completion first. */
+int orb_server_poll(orb_server* s, orb_serve_info* out, int max, int* n);
+/* Results of a polled frame (then released): keypoints, descriptors and the
+ * match of each keypoint into the stream's previous frame (-1 = none). */
+int orb_server_fetch(orb_server* s, uint64_t ticket, orb_keypoint* keypoints, int capacity, uint8_t* descriptors,
+                     int32_t* match_idx, int32_t* match_dist, int* n);
+int orb_server_release(orb_server* s, uint64_t ticket);
+int orb_server_drain(orb_server* s, double timeout_s);
+int orb_server_stats(orb_server* s, int64_t* batches, int64_t* requests, size_t* hbm_used, int* slots);
 
 /* Debug accessors used by the parity tests.
  * kind 0: pyramid level, 1: its 7x7 blur. */

@@ -20,7 +20,7 @@ import os
 
 import numpy as np
 
-__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "match_bf", "knn2",
+__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "match_bf", "knn2",
            "compute_stereo_matches", "stereo_frame", "lib", "library_path", "OrbError"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
@@ -57,7 +57,7 @@ SIGNATURES = {
     "orb_context_stream": (_P, [_P]),
     "orb_knn2": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
     "orb_knn2_device": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
-    "orb_knn2_time": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P, _I, _P, _P]),
+    "orb_knn2_time": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _I, _P, _P]),
     "orb_batch_create": (_I, [_P, _I, _P, _P, _P, _F, _I, _I, _I, _P]),
     "orb_batch_destroy": (_V, [_P]),
     "orb_batch_size": (_I, [_P]),
@@ -70,6 +70,15 @@ SIGNATURES = {
     "orb_batch_stage_times": (_I, [_P, _I, _P]),
     "orb_batch_results": (_I, [_P, _I, _P, _I, _P, _P]),
     "orb_batch_stereo_results": (_I, [_P, _I, _P, _P, _I, _P]),
+    "orb_server_create": (_I, [_P, _I, _P, _P, _P, _F, _I, _I, _I, _I, _F, _S, _I, _I, _P]),
+    "orb_server_destroy": (_V, [_P]),
+    "orb_server_clock": (ctypes.c_double, []),
+    "orb_server_submit": (_I, [_P, _I, _P, _S, ctypes.c_double, _P]),
+    "orb_server_poll": (_I, [_P, _P, _I, _P]),
+    "orb_server_fetch": (_I, [_P, ctypes.c_uint64, _P, _I, _P, _P, _P, _P]),
+    "orb_server_release": (_I, [_P, ctypes.c_uint64]),
+    "orb_server_drain": (_I, [_P, ctypes.c_double]),
+    "orb_server_stats": (_I, [_P, _P, _P, _P, _P]),
     "orb_extractor_debug_image": (_I, [_P, _I, _I, _P, _S]),
     "orb_extractor_debug_candidates": (_I, [_P, _I, _P, _I, _P]),
 }
@@ -407,3 +416,88 @@ class Batch:
         _check("orb_batch_stereo_results", lib().orb_batch_stereo_results(self.h, pair, _ptr(uR), _ptr(dep), cap,
                                                                           ctypes.byref(n)))
         return uR[:n.value].copy(), dep[:n.value].copy()
+
+
+class _ServeInfo(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("stream", ctypes.c_int), ("status", ctypes.c_int),
+                ("n_keypoints", ctypes.c_int), ("n_matches", ctypes.c_int), ("t_arrival", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class Server:
+    """Continuous-batching serving loop for camera streams (orb_server_* in the C ABI).
+
+    streams: list of (width, height, nfeatures).  Each submitted frame is
+    extracted and matched (best/second, th_low, ratio) against its stream's
+    previous frame, whose descriptors stay resident in HBM.
+    """
+
+    def __init__(self, streams, scaleFactor=1.2, nlevels=8, iniThFAST=20, minThFAST=7, th_low=50, ratio=0.6,
+                 hbm_budget=2 << 30, max_inflight=3, max_batch=64, context=None):
+        self.ctx = context or Context.default()
+        n = len(streams)
+        w = np.array([s[0] for s in streams], np.int32)
+        h = np.array([s[1] for s in streams], np.int32)
+        nf = np.array([s[2] for s in streams], np.int32)
+        hs = ctypes.c_void_p()
+        _check("orb_server_create", lib().orb_server_create(self.ctx.h, n, _ptr(w), _ptr(h), _ptr(nf),
+                                                            float(scaleFactor), int(nlevels), int(iniThFAST),
+                                                            int(minThFAST), int(th_low), float(ratio),
+                                                            int(hbm_budget), int(max_inflight), int(max_batch),
+                                                            ctypes.byref(hs)))
+        self.h = hs
+        self.streams = [tuple(s) for s in streams]
+        self._info = (_ServeInfo * 1024)()
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orb_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def clock():
+        return lib().orb_server_clock()
+
+    def submit(self, stream, image, t_arrival=None):
+        img = np.ascontiguousarray(image, np.uint8)
+        w, h, _ = self.streams[stream]
+        if img.shape != (h, w):
+            raise ValueError(f"stream {stream} expects {w}x{h}")
+        t = ctypes.c_uint64()
+        ta = self.clock() if t_arrival is None else float(t_arrival)
+        _check("orb_server_submit", lib().orb_server_submit(self.h, stream, _ptr(img), img.strides[0], ta,
+                                                            ctypes.byref(t)))
+        return t.value
+
+    def poll(self):
+        n = ctypes.c_int()
+        _check("orb_server_poll", lib().orb_server_poll(self.h, self._info, len(self._info), ctypes.byref(n)))
+        return [dict(ticket=i.ticket, stream=i.stream, status=i.status, n_keypoints=i.n_keypoints,
+                     n_matches=i.n_matches, t_arrival=i.t_arrival, t_admit=i.t_admit, t_done=i.t_done)
+                for i in self._info[:n.value]]
+
+    def fetch(self, ticket, capacity=8192):
+        kps = np.zeros(capacity, KEYPOINT_DTYPE)
+        desc = np.zeros((capacity, 32), np.uint8)
+        idx = np.zeros(capacity, np.int32)
+        dist = np.zeros(capacity, np.int32)
+        n = ctypes.c_int()
+        _check("orb_server_fetch", lib().orb_server_fetch(self.h, ticket, _ptr(kps), capacity, _ptr(desc), _ptr(idx),
+                                                          _ptr(dist), ctypes.byref(n)))
+        k = n.value
+        return kps[:k].copy(), desc[:k].copy(), idx[:k].copy(), dist[:k].copy()
+
+    def release(self, ticket):
+        _check("orb_server_release", lib().orb_server_release(self.h, ticket))
+
+    def drain(self, timeout=60.0):
+        _check("orb_server_drain", lib().orb_server_drain(self.h, float(timeout)))
+
+    def stats(self):
+        b, r, u, sl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_size_t(), ctypes.c_int()
+        _check("orb_server_stats", lib().orb_server_stats(self.h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(u),
+                                                          ctypes.byref(sl)))
+        return dict(batches=b.value, requests=r.value, hbm_used=u.value, slots=sl.value)

@@ -169,10 +169,12 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
 
     // ---- 1. pixels with halo
     if (level == 0) {
+        const uint8_t* src = J.src;
+        const int sp = J.src_pitch;
         for (int e = tid; e < LP_W * LP_H; e += 256) {
             const int ex = e % LP_W, ey = e / LP_W;
             const int x = reflect101(tx0 - LT_HALO + ex, w), y = reflect101(ty0 - LT_HALO + ey, h);
-            px[ey][ex] = L.lvl[(size_t)y * pitch + x];
+            px[ey][ex] = src[(size_t)y * sp + x];
         }
     } else {
         // cv::resize INTER_LINEAR 8U fixed point from level l-1: double source
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
 
     // ---- 2. level and blur; thread = 4 adjacent pixels x 2 rows
     const int bx = (tid & 15) * 4;
-    if (level > 0) {
+    if (level > 0 || J.src != L.lvl) {
         for (int r = tid >> 4; r < LT_H; r += 16) {
             const int y = ty0 + r, x0 = tx0 + bx;
             if (y >= h) continue;
@@ -1144,6 +1146,70 @@ __global__ __launch_bounds__(256) void match_bf_kernel(const uint8_t* __restrict
     }
 }
 
+// Many (query set, train set) matches in one launch, one wavefront per query.
+__device__ __forceinline__ int find_match(const MatchArgs* args, int n, int x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (args[mid].wave_base <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void match_pairs_kernel(const MatchArgs* __restrict__ args, int npairs, int nwaves)
+{
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= nwaves) return;
+    const MatchArgs& A = args[find_match(args, npairs, wid)];
+    const int iq = wid - A.wave_base;
+    const int nq = *A.nq, nt = *A.nt;
+    if (iq >= nq || iq >= A.cap_q) return;
+    const uint4 a0 = *(const uint4*)(A.q + (size_t)iq * 32);
+    const uint4 a1 = *(const uint4*)(A.q + (size_t)iq * 32 + 16);
+    int d0 = INT_MAX, i0 = INT_MAX, d1 = INT_MAX;
+    for (int j = lane; j < nt; j += 64) {
+        const int d = hamming32(a0, a1, A.t + (size_t)j * 32);
+        if (d < d0) {
+            d1 = d0;
+            d0 = d;
+            i0 = j;
+        } else if (d < d1) {
+            d1 = d;
+        }
+    }
+    int bd = d0, bi = i0;
+    for (int o = 32; o > 0; o >>= 1) {
+        const int od = __shfl_xor(bd, o, 64), oi = __shfl_xor(bi, o, 64);
+        if (od < bd || (od == bd && oi < bi)) {
+            bd = od;
+            bi = oi;
+        }
+    }
+    int sec = (i0 == bi) ? d1 : d0;
+    for (int o = 32; o > 0; o >>= 1) sec = min(sec, __shfl_xor(sec, o, 64));
+    if (lane == 0) {
+        const bool ok = bi != INT_MAX && bd <= A.th_low && (float)bd < (float)sec * A.ratio;
+        A.idx[iq] = ok ? bi : -1;
+        A.dist[iq] = bd;
+    }
+}
+
+// After the matches of a launch: each pair's query rows become the resident
+// "previous" set of its stream (one workgroup per pair).
+__global__ __launch_bounds__(256) void save_prev_kernel(const MatchArgs* __restrict__ args, int npairs)
+{
+    const MatchArgs& A = args[blockIdx.x];
+    if (!A.save_desc) return;
+    const int n = min(*A.nq, A.save_cap);
+    const uint4* src = (const uint4*)A.q;
+    uint4* dst = (uint4*)A.save_desc;
+    for (int i = threadIdx.x; i < 2 * n; i += 256) dst[i] = src[i];
+    if (threadIdx.x == 0) *A.save_n = n;
+}
+
 // ============================================================= launchers
 // ev (optional): events recorded after each stage -- [0] start, then one per
 // pyramid level (max_levels), octree, describe.
@@ -1425,5 +1491,13 @@ extern "C" hipError_t orb_launch_knn2(const uint8_t* q, int nq, const uint8_t* m
     if (ev) (void)hipEventRecord(ev[1], st);
     knn_merge_kernel<<<(nq + 255) / 256, 256, 0, st>>>(part, nq, nchunks, ratio, idx2, dist2, accept);
     if (ev) (void)hipEventRecord(ev[2], st);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t orb_launch_match_pairs(const MatchArgs* d_args, int npairs, int nwaves, hipStream_t st)
+{
+    if (npairs <= 0) return hipSuccess;
+    if (nwaves > 0) match_pairs_kernel<<<(nwaves + 3) / 4, 256, 0, st>>>(d_args, npairs, nwaves);
+    save_prev_kernel<<<npairs, 256, 0, st>>>(d_args, npairs);
     return hipGetLastError();
 }

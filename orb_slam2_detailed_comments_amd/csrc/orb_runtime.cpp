@@ -6,475 +6,7 @@
 // level.  Those values are computed here with the same float statements as the
 // reference and packed into ImageJob descriptors; the pixel and keypoint work
 // runs in the gfx950 kernels of orb_kernels.hip.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstring>
-#include <vector>
-
-#include "orb_gpu.h"
-#include "orb_types.h"
-
-extern "C" hipError_t orb_upload_constants(const int* umax16, const int* gauss7);
-extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable* d_rt, const RunTable* h_rt,
-                                         hipStream_t st, hipEvent_t* ev);
-extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, int nwaves, hipStream_t st,
-                                        hipEvent_t* ev);
-extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const uint8_t* t, const int* nt,
-                                          int th_low, float ratio, int* idx, int* dist, int cap_q,
-                                          hipStream_t st);
-
-extern "C" int orb_knn_chunks(long long nm);
-extern "C" hipError_t orb_launch_knn2(const uint8_t* q, int nq, const uint8_t* m, long long nm, int method,
-                                      float ratio, uint2* part, int* idx2, int* dist2, uint8_t* accept,
-                                      hipStream_t st, hipEvent_t* ev);
-
-namespace {
-
-inline int cv_round(float v) { return (int)std::lrintf(v); }
-
-#define HIP_TRY(x)                               \
-    do {                                         \
-        if ((x) != hipSuccess) return ORB_E_DEVICE; \
-    } while (0)
-
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-// Integer taps of getGaussianKernel(7, 2) scaled by 256 (OpenCV's 8U
-// separable path): float kernel normalised in double, rounded half-even.
-void gaussian_taps(int k[7])
-{
-    float cf[7];
-    double sum = 0;
-    for (int i = 0; i < 7; i++) {
-        const double x = i - 3.0;
-        cf[i] = (float)std::exp((-0.5 / (2.0 * 2.0)) * x * x);
-        sum += cf[i];
-    }
-    sum = 1. / sum;
-    for (int i = 0; i < 7; i++) cf[i] = (float)(cf[i] * sum);
-    for (int i = 0; i < 7; i++) k[i] = cv_round(cf[i] * 256.f);
-}
-
-// umax: half-widths of the radius-15 orientation disc (ORBextractor.cc:542-570).
-void disc_umax(int umax[16])
-{
-    const int half = 15;
-    const int vmax = (int)std::floor(half * std::sqrt(2.f) / 2 + 1);
-    const int vmin = (int)std::ceil(half * std::sqrt(2.f) / 2);
-    const double hp2 = half * half;
-    for (int v = 0; v <= vmax; ++v) umax[v] = (int)std::lrint(std::sqrt(hp2 - v * v));
-    for (int v = half, v0 = 0; v >= vmin; --v) {
-        while (umax[v0] == umax[v0 + 1]) ++v0;
-        umax[v] = v0;
-        ++v0;
-    }
-}
-
-struct LevelPlan {
-    int w, h, pitch;
-    size_t img_off;
-    int tiles_x, tiles_y, ncells;
-    int nCols, nRows, wCell, hCell, maxBorderX, maxBorderY;
-    int nIni;
-    float hX;
-    int root_x[ORB_MAX_ROOTS + 1];
-    int key_cap, list_cap;
-    size_t key_off, ret_off, cell_off;
-};
-
-struct Plan {
-    int W = 0, H = 0;
-    LevelPlan lv[ORB_MAX_LEVELS];
-    size_t img_bytes = 0, nkeys = 0, nret = 0, ncells = 0;
-    int out_cap = 0;
-};
-
-}  // namespace
-
-struct orb_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    // brute-force matcher scratch
-    uint8_t* d_q = nullptr;
-    uint8_t* d_t = nullptr;
-    int* d_out = nullptr;
-    size_t q_cap = 0, t_cap = 0;
-    // dense k=2 search scratch
-    uint2* d_part = nullptr;
-    size_t part_cap = 0;
-    uint8_t* d_kq = nullptr;
-    uint8_t* d_km = nullptr;
-    int* d_kout = nullptr;
-    size_t kq_cap = 0, km_cap = 0;
-
-    int grow(void** p, size_t* cap, size_t bytes)
-    {
-        if (bytes <= *cap) return ORB_OK;
-        if (*p) (void)hipFree(*p);
-        *p = nullptr;
-        *cap = 0;
-        if (hipMalloc(p, bytes) != hipSuccess) {
-            *p = nullptr;
-            return ORB_E_OOM;
-        }
-        *cap = bytes;
-        return ORB_OK;
-    }
-};
-
-struct orb_extractor {
-    orb_context* ctx = nullptr;
-    int nfeatures = 0, nlevels = 0, iniTh = 0, minTh = 0;
-    float scaleFactorF = 1.2f;
-    double scaleFactor = 1.2;
-    std::vector<float> scale, invScale, sigma2, invSigma2;
-    std::vector<int> quota;
-
-    Plan plan;
-    unsigned gen = 0;             // bumped whenever buffers or plan change
-    char* d_mem = nullptr;
-    size_t mem_bytes = 0;
-    ImageJob hjob{};
-    ImageJob* d_job = nullptr;
-    RunTable* h_rt = nullptr;     // pinned
-    RunTable* d_rt = nullptr;
-    // stereo state (this extractor as the left view)
-    float* d_uR = nullptr;
-    float* d_depth = nullptr;
-    int* d_msad = nullptr;
-    int* d_midx = nullptr;
-    int* d_misc = nullptr;        // [0] match count, [1] nl, [2] nr
-    OrbKeyPoint* d_kin = nullptr; // uploaded keypoints: left then right
-    uint8_t* d_din = nullptr;
-    int in_cap = 0;
-    // two-view run (this = left, pair_right = right)
-    const orb_extractor* pair_right = nullptr;
-    unsigned pair_gen_l = 0, pair_gen_r = 0;
-    ImageJob* d_pair_jobs = nullptr;
-    RunTable* h_pair_rt = nullptr;
-    RunTable* d_pair_rt = nullptr;
-    // device copies of the stereo arguments: [0] extractor-resident keypoints,
-    // [1] caller-uploaded keypoints
-    StereoArgs* d_sargs = nullptr;
-    StereoArgs h_sargs[2];
-    bool sargs_valid[2] = {false, false};
-
-    int upload_stereo_args(int which, const StereoArgs& a)
-    {
-        if (!d_sargs && hipMalloc(&d_sargs, 2 * sizeof(StereoArgs)) != hipSuccess) {
-            d_sargs = nullptr;
-            return ORB_E_OOM;
-        }
-        if (sargs_valid[which] && std::memcmp(&a, &h_sargs[which], sizeof(a)) == 0) return ORB_OK;
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        HIP_TRY(hipMemcpy(d_sargs + which, &a, sizeof(a), hipMemcpyHostToDevice));
-        h_sargs[which] = a;
-        sargs_valid[which] = true;
-        return ORB_OK;
-    }
-
-    // ORBextractor::ORBextractor (ORBextractor.cc:469-526): scale tables in
-    // float (mvScaleFactor[i-1]*scaleFactor with the double member), inverse
-    // tables, and the geometric per-level quota with the remainder on top.
-    void init_tables()
-    {
-        scale.assign(nlevels, 1.f);
-        sigma2.assign(nlevels, 1.f);
-        for (int i = 1; i < nlevels; i++) {
-            scale[i] = (float)(scale[i - 1] * scaleFactor);
-            sigma2[i] = scale[i] * scale[i];
-        }
-        invScale.resize(nlevels);
-        invSigma2.resize(nlevels);
-        for (int i = 0; i < nlevels; i++) {
-            invScale[i] = 1.0f / scale[i];
-            invSigma2[i] = 1.0f / sigma2[i];
-        }
-        quota.assign(nlevels, 0);
-        const float factor = (float)(1.0f / scaleFactor);
-        float nDesired = nfeatures * (1 - factor) / (1 - (float)std::pow((double)factor, (double)nlevels));
-        int sum = 0;
-        for (int l = 0; l < nlevels - 1; l++) {
-            quota[l] = cv_round(nDesired);
-            sum += quota[l];
-            nDesired *= factor;
-        }
-        quota[nlevels - 1] = std::max(nfeatures - sum, 0);
-    }
-
-    // Level sizes (ComputePyramid, :1663), FAST cell grids (:1047-1071) and
-    // quadtree roots (DistributeOctTree, :695-720) for a W x H image.
-    int make_plan(int W, int H, Plan& p) const
-    {
-        p = Plan();
-        p.W = W;
-        p.H = H;
-        size_t img = 0, nk = 0, nr = 0, ncell = 0;
-        int out = 0;
-        for (int l = 0; l < nlevels; l++) {
-            LevelPlan& L = p.lv[l];
-            L.w = cv_round((float)W * invScale[l]);
-            L.h = cv_round((float)H * invScale[l]);
-            if (L.w < 40 || L.h < 40) return ORB_E_ARG;
-            L.pitch = (int)align_up((size_t)L.w, 64);
-            L.img_off = img;
-            img += align_up((size_t)L.pitch * L.h, 256);
-            L.tiles_x = (L.w + ORB_TILE_W - 1) / ORB_TILE_W;
-            L.tiles_y = (L.h + ORB_TILE_H - 1) / ORB_TILE_H;
-            const int minBorderX = ORB_MIN_BORDER, minBorderY = ORB_MIN_BORDER;
-            L.maxBorderX = L.w - ORB_EDGE + 3;
-            L.maxBorderY = L.h - ORB_EDGE + 3;
-            const float width = (float)(L.maxBorderX - minBorderX);
-            const float height = (float)(L.maxBorderY - minBorderY);
-            L.nCols = (int)(width / 30.f);
-            L.nRows = (int)(height / 30.f);
-            if (L.nCols <= 0 || L.nRows <= 0 || L.nCols > 255 || L.nRows > 255) return ORB_E_ARG;
-            L.wCell = (int)std::ceil(width / L.nCols);
-            L.hCell = (int)std::ceil(height / L.nRows);
-            if (L.wCell > 58 || L.hCell > 58) return ORB_E_ARG;
-            L.ncells = L.nCols * L.nRows;
-            L.nIni = (int)std::round((float)(L.maxBorderX - minBorderX) / (L.maxBorderY - minBorderY));
-            if (L.nIni <= 0 || L.nIni > ORB_MAX_ROOTS) return ORB_E_ARG;
-            L.hX = (float)(L.maxBorderX - minBorderX) / L.nIni;
-            for (int i = 0; i <= L.nIni; i++) L.root_x[i] = (int)(L.hX * (float)i);
-            // Strict 3x3 non-max suppression keeps at most one pixel per 2x2
-            // block of a cell; candidates of both thresholds are stored.
-            L.key_cap = 2 * L.ncells * ((L.wCell + 1) / 2) * ((L.hCell + 1) / 2);
-            L.cell_off = ncell;
-            ncell += (size_t)L.ncells;
-            L.list_cap = std::max(quota[l] + 3, 4 * L.nIni);
-            if (L.list_cap > ORB_LIST_CAP) return ORB_E_ARG;
-            L.key_off = nk;
-            nk += (size_t)L.key_cap;
-            L.ret_off = nr;
-            nr += (size_t)L.list_cap;
-            out += L.list_cap;
-        }
-        p.img_bytes = img;
-        p.nkeys = nk;
-        p.nret = nr;
-        p.ncells = ncell;
-        p.out_cap = out;
-        return ORB_OK;
-    }
-
-    // Device buffers and the job descriptor for the current image size.
-    int ensure(int W, int H)
-    {
-        if (plan.W == W && plan.H == H && d_mem) return ORB_OK;
-        Plan p;
-        int rc = make_plan(W, H, p);
-        if (rc) return rc;
-        HIP_TRY(hipSetDevice(ctx->device));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        size_t off = 0;
-        auto take = [&](size_t bytes) {
-            const size_t o = off;
-            off = align_up(off + bytes, 256);
-            return o;
-        };
-        const size_t o_lvl = take(p.img_bytes), o_blur = take(p.img_bytes), o_cell = take(p.ncells);
-        const size_t o_keys = take(p.nkeys * 8), o_keys2 = take(p.nkeys * 8), o_node = take(p.nkeys * 4);
-        const size_t o_ret = take(p.nret * 8);
-        const size_t o_ctr = take(ORB_CNT_SLOTS * 4);
-        const size_t o_kps = take((size_t)p.out_cap * sizeof(OrbKeyPoint));
-        const size_t o_desc = take((size_t)p.out_cap * 32);
-        const size_t o_uR = take((size_t)p.out_cap * 4), o_dep = take((size_t)p.out_cap * 4);
-        const size_t o_msad = take((size_t)p.out_cap * 4), o_midx = take((size_t)p.out_cap * 4);
-        const size_t o_misc = take(64);
-        const size_t total = off;
-        if (total > mem_bytes) {
-            if (d_mem) (void)hipFree(d_mem);
-            d_mem = nullptr;
-            mem_bytes = 0;
-            if (hipMalloc(&d_mem, total) != hipSuccess) {
-                d_mem = nullptr;
-                return ORB_E_OOM;
-            }
-            mem_bytes = total;
-        }
-        plan = p;
-        gen++;
-        ImageJob& J = hjob;
-        std::memset(&J, 0, sizeof(J));
-        J.nlevels = nlevels;
-        J.iniTh = iniTh;
-        J.minTh = minTh;
-        J.out_cap = p.out_cap;
-        J.counters = (int*)(d_mem + o_ctr);
-        J.kps = (OrbKeyPoint*)(d_mem + o_kps);
-        J.desc = (uint8_t*)(d_mem + o_desc);
-        for (int l = 0; l < nlevels; l++) {
-            const LevelPlan& P = p.lv[l];
-            LevelJob& L = J.lv[l];
-            L.w = P.w;
-            L.h = P.h;
-            L.pitch = P.pitch;
-            L.tiles_x = P.tiles_x;
-            L.tiles_y = P.tiles_y;
-            L.nCols = P.nCols;
-            L.nRows = P.nRows;
-            L.wCell = P.wCell;
-            L.hCell = P.hCell;
-            L.maxBorderX = P.maxBorderX;
-            L.maxBorderY = P.maxBorderY;
-            L.quota = quota[l];
-            L.nIni = P.nIni;
-            L.hX = P.hX;
-            L.scale = scale[l];
-            for (int i = 0; i <= P.nIni; i++) L.root_x[i] = P.root_x[i];
-            L.key_cap = P.key_cap;
-            L.ncells = P.ncells;
-            L.list_cap = P.list_cap;
-            L.lvl = (uint8_t*)(d_mem + o_lvl + P.img_off);
-            L.blur = (uint8_t*)(d_mem + o_blur + P.img_off);
-            L.cellflag = (uint8_t*)(d_mem + o_cell + P.cell_off);
-            L.keys = (unsigned long long*)(d_mem + o_keys) + P.key_off;
-            L.keys2 = (unsigned long long*)(d_mem + o_keys2) + P.key_off;
-            L.node = (int*)(d_mem + o_node) + P.key_off;
-            L.ret = (unsigned long long*)(d_mem + o_ret) + P.ret_off;
-        }
-        d_uR = (float*)(d_mem + o_uR);
-        d_depth = (float*)(d_mem + o_dep);
-        d_msad = (int*)(d_mem + o_msad);
-        d_midx = (int*)(d_mem + o_midx);
-        d_misc = (int*)(d_mem + o_misc);
-        if (!d_job && hipMalloc(&d_job, sizeof(ImageJob)) != hipSuccess) return ORB_E_OOM;
-        if (!d_rt && hipMalloc(&d_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
-        if (!h_rt && hipHostMalloc(&h_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
-        const orb_extractor* one[1] = {this};
-        build_table(one, 1, *h_rt);
-        HIP_TRY(hipMemcpy(d_job, &hjob, sizeof(ImageJob), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(d_rt, h_rt, sizeof(RunTable), hipMemcpyHostToDevice));
-        return ORB_OK;
-    }
-
-    static void build_table(const orb_extractor* const* exs, int n, RunTable& rt)
-    {
-        std::memset(&rt, 0, sizeof(rt));
-        rt.njobs = n;
-        int maxl = 0;
-        for (int j = 0; j < n; j++) maxl = std::max(maxl, exs[j]->nlevels);
-        rt.max_levels = maxl;
-        for (int l = 0; l < maxl; l++) {
-            int acc = 0;
-            for (int j = 0; j < n; j++) {
-                rt.tile_start[l][j] = acc;
-                if (l < exs[j]->nlevels) acc += exs[j]->plan.lv[l].tiles_x * exs[j]->plan.lv[l].tiles_y;
-            }
-            rt.tile_start[l][n] = acc;
-        }
-        int acc = 0;
-        for (int j = 0; j < n; j++) {
-            rt.slot_start[j] = acc;
-            acc += exs[j]->plan.out_cap;
-        }
-        rt.slot_start[n] = acc;
-    }
-
-    int upload_image(const uint8_t* img, size_t stride, hipMemcpyKind kind)
-    {
-        HIP_TRY(hipMemcpy2DAsync(hjob.lv[0].lvl, hjob.lv[0].pitch, img, stride, plan.W, plan.H, kind,
-                                 ctx->stream));
-        return ORB_OK;
-    }
-
-    int check_errors()
-    {
-        int c[ORB_CNT_SLOTS];
-        HIP_TRY(hipMemcpy(c, hjob.counters, sizeof(c), hipMemcpyDeviceToHost));
-        return c[ORB_CNT_ERR] ? ORB_E_OVERFLOW : ORB_OK;
-    }
-
-    int read_results(orb_keypoint* kps, int cap, uint8_t* desc, int* n)
-    {
-        int c[ORB_CNT_SLOTS];
-        HIP_TRY(hipMemcpyAsync(c, hjob.counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (c[ORB_CNT_ERR]) return ORB_E_OVERFLOW;
-        const int cnt = c[ORB_CNT_NOUT];
-        *n = cnt;
-        if (cnt > cap) return ORB_E_CAPACITY;
-        if (cnt > 0) {
-            if (kps)
-                HIP_TRY(hipMemcpyAsync(kps, hjob.kps, (size_t)cnt * sizeof(orb_keypoint), hipMemcpyDeviceToHost,
-                                       ctx->stream));
-            if (desc)
-                HIP_TRY(hipMemcpyAsync(desc, hjob.desc, (size_t)cnt * 32, hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-        }
-        return ORB_OK;
-    }
-
-    // Two-view run table (left = this, right = r) for the stereo frame.
-    int ensure_pair(const orb_extractor* r)
-    {
-        if (pair_right == r && pair_gen_l == gen && pair_gen_r == r->gen) return ORB_OK;
-        if (!d_pair_jobs && hipMalloc(&d_pair_jobs, 2 * sizeof(ImageJob)) != hipSuccess) return ORB_E_OOM;
-        if (!d_pair_rt && hipMalloc(&d_pair_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
-        if (!h_pair_rt && hipHostMalloc(&h_pair_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
-        const orb_extractor* two[2] = {this, r};
-        build_table(two, 2, *h_pair_rt);
-        ImageJob jobs[2] = {hjob, r->hjob};
-        HIP_TRY(hipMemcpy(d_pair_jobs, jobs, sizeof(jobs), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(d_pair_rt, h_pair_rt, sizeof(RunTable), hipMemcpyHostToDevice));
-        pair_right = r;
-        pair_gen_l = gen;
-        pair_gen_r = r->gen;
-        return ORB_OK;
-    }
-
-    int ensure_inputs(int cap)
-    {
-        if (cap <= in_cap) return ORB_OK;
-        if (d_kin) (void)hipFree(d_kin);
-        if (d_din) (void)hipFree(d_din);
-        d_kin = nullptr;
-        d_din = nullptr;
-        in_cap = 0;
-        if (hipMalloc(&d_kin, 2 * (size_t)cap * sizeof(OrbKeyPoint)) != hipSuccess) return ORB_E_OOM;
-        if (hipMalloc(&d_din, 2 * (size_t)cap * 32) != hipSuccess) return ORB_E_OOM;
-        in_cap = cap;
-        return ORB_OK;
-    }
-
-    void stereo_args(StereoArgs& a, const orb_extractor* r, const ImageJob* dl_job, const ImageJob* dr_job,
-                     float bf, float b) const
-    {
-        std::memset(&a, 0, sizeof(a));
-        a.left = dl_job;
-        a.right = dr_job;
-        a.nrows = plan.lv[0].h;
-        a.bf = bf;
-        a.mb = b;
-        for (int l = 0; l < nlevels; l++) {
-            a.scale[l] = scale[l];
-            a.inv_scale[l] = invScale[l];
-        }
-        a.uRight = d_uR;
-        a.depth = d_depth;
-        a.match_sad = d_msad;
-        a.match_idx = d_midx;
-        a.match_count = d_misc;
-        (void)r;
-    }
-
-    ~orb_extractor()
-    {
-        if (d_mem) (void)hipFree(d_mem);
-        if (d_job) (void)hipFree(d_job);
-        if (d_rt) (void)hipFree(d_rt);
-        if (h_rt) (void)hipHostFree(h_rt);
-        if (d_kin) (void)hipFree(d_kin);
-        if (d_din) (void)hipFree(d_din);
-        if (d_pair_jobs) (void)hipFree(d_pair_jobs);
-        if (d_pair_rt) (void)hipFree(d_pair_rt);
-        if (h_pair_rt) (void)hipHostFree(h_pair_rt);
-        if (d_sargs) (void)hipFree(d_sargs);
-    }
-};
+#include "orb_internal.h"
 
 // ====================================================================== C ABI
 extern "C" {
@@ -932,8 +464,8 @@ int orb_batch_image(orb_batch* b, int slot, uint8_t** d_image, size_t* pitch, in
 {
     if (!b || slot < 0 || slot >= (int)b->ex.size()) return ORB_E_ARG;
     const orb_extractor* e = b->ex[slot];
-    if (d_image) *d_image = e->hjob.lv[0].lvl;
-    if (pitch) *pitch = (size_t)e->hjob.lv[0].pitch;
+    if (d_image) *d_image = const_cast<uint8_t*>(e->hjob.src);
+    if (pitch) *pitch = (size_t)e->hjob.src_pitch;
     if (width) *width = e->plan.W;
     if (height) *height = e->plan.H;
     return ORB_OK;
@@ -1141,22 +673,33 @@ int orb_knn2(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* map,
 }
 
 // Average device time of the search kernel and of the merge kernel over
-// `steps` runs on device-resident data (HIP events on the context stream).
-int orb_knn2_time(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm, int method,
-                  float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept, int steps, float* ms_search,
-                  float* ms_merge)
+// `steps` runs.  The host query and map rows are first copied into
+// context-resident device buffers; only the kernels are timed (HIP events on
+// the context stream).
+int orb_knn2_time(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* map, int64_t nm, int method,
+                  float ratio, int steps, float* ms_search, float* ms_merge)
 {
-    if (!ctx || steps <= 0 || nq <= 0 || !d_query || !d_map || nm <= 0 || !d_idx2 || !d_dist2) return ORB_E_ARG;
+    if (!ctx || steps <= 0 || nq <= 0 || !query || !map || nm <= 0 || nm >= (1ll << 31) ||
+        (method != 0 && method != 1))
+        return ORB_E_ARG;
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t nchunks = (size_t)orb_knn_chunks(nm);
     int rc = ctx->grow((void**)&ctx->d_part, &ctx->part_cap, (size_t)nq * nchunks * sizeof(uint2));
+    if (!rc) rc = ctx->grow((void**)&ctx->d_kq, &ctx->kq_cap, (size_t)nq * 32);
+    if (!rc) rc = ctx->grow((void**)&ctx->d_km, &ctx->km_cap, (size_t)nm * 32);
     if (rc) return rc;
-    hipEvent_t ev[3];
-    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    int* d_out = nullptr;
+    if (hipMalloc(&d_out, (size_t)nq * 17) != hipSuccess) return ORB_E_OOM;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = ORB_E_DEVICE;
+    if (!rc && (hipMemcpyAsync(ctx->d_kq, query, (size_t)nq * 32, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+                hipMemcpyAsync(ctx->d_km, map, (size_t)nm * 32, hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
+        rc = ORB_E_DEVICE;
     double a = 0, b = 0;
-    for (int s = 0; s < steps && !rc; s++) {
-        if (orb_launch_knn2(d_query, nq, d_map, nm, method, ratio, ctx->d_part, d_idx2, d_dist2, d_accept,
-                            ctx->stream, ev) != hipSuccess ||
+    for (int s = -1; s < steps && !rc; s++) {   // s = -1: untimed warm-up
+        if (orb_launch_knn2(ctx->d_kq, nq, ctx->d_km, nm, method, ratio, ctx->d_part, d_out, d_out + 2 * (size_t)nq,
+                            (uint8_t*)(d_out + 4 * (size_t)nq), ctx->stream, ev) != hipSuccess ||
             hipEventSynchronize(ev[2]) != hipSuccess) {
             rc = ORB_E_DEVICE;
             break;
@@ -1164,10 +707,15 @@ int orb_knn2_time(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_
         float t1 = 0, t2 = 0;
         (void)hipEventElapsedTime(&t1, ev[0], ev[1]);
         (void)hipEventElapsedTime(&t2, ev[1], ev[2]);
-        a += t1;
-        b += t2;
+        if (s >= 0) {
+            a += t1;
+            b += t2;
+        }
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_out);
     if (rc) return rc;
     if (ms_search) *ms_search = (float)(a / steps);
     if (ms_merge) *ms_merge = (float)(b / steps);

@@ -60,6 +60,8 @@ struct LevelJob {
 };
 
 struct ImageJob {
+    const uint8_t* src;              // input image (level 0), src_pitch bytes per row;
+    int src_pitch;                   // the level-0 kernel copies it into lv[0].lvl
     int nlevels, iniTh, minTh;
     int out_cap;                     // keypoint output capacity
     int* counters;                   // ORB_CNT_SLOTS ints
@@ -98,4 +100,22 @@ struct StereoArgs {
     int* match_sad;
     int* match_idx;
     int* match_count;
+};
+
+// Arguments of one brute-force match of a query set against a train set
+// (serving: this frame's descriptors against the stream's previous ones).
+struct MatchArgs {
+    const uint8_t* q;
+    const int* nq;
+    const uint8_t* t;
+    const int* nt;
+    int th_low;
+    float ratio;
+    int* idx;
+    int* dist;
+    int cap_q;
+    int wave_base;          // first wavefront of this pair in a multi-pair launch
+    uint8_t* save_desc;     // after matching: q copied here ...
+    int* save_n;            // ... and *nq here (the stream's resident state)
+    int save_cap;
 };
