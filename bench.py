@@ -252,6 +252,12 @@ def main():
                       "mean_stereo_matches": round(float(np.mean(matched)), 1)},
            "roofline": roofline}
     if not args.no_extras:
+        import ctypes
+        ms = ctypes.c_float()
+        nbytes = 1 << 30
+        if orb.lib().orb_hbm_copy_time(ctx.h, nbytes, 10, ctypes.byref(ms)) == 0:
+            out["hbm_copy_microbench"] = {"bytes_moved": 2 * nbytes, "ms": round(ms.value, 4),
+                                          "GB_per_s": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
         print("[bench] dense k=2 search", file=sys.stderr, flush=True)
         out["knn2"] = knn_extra(orb, ctx, max(3, args.steps // 2))
         print("[bench] serving", file=sys.stderr, flush=True)

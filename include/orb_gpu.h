@@ -204,6 +204,11 @@ int orb_server_release(orb_server* s, uint64_t ticket);
 int orb_server_drain(orb_server* s, double timeout_s);
 int orb_server_stats(orb_server* s, int64_t* batches, int64_t* requests, size_t* hbm_used, int* slots);
 
+/* Measurement aid, no reference counterpart: average milliseconds of a
+ * device-to-device copy of `bytes` bytes by a plain streaming kernel (HBM
+ * bandwidth check and hardware-counter calibration). */
+int orb_hbm_copy_time(orb_context* ctx, size_t bytes, int steps, float* ms);
+
 /* Debug accessors used by the parity tests.
  * kind 0: pyramid level, 1: its 7x7 blur. */
 int orb_extractor_debug_image(orb_extractor* ex, int level, int kind, uint8_t* dst, size_t dst_stride);

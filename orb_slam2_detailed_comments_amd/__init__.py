@@ -55,6 +55,7 @@ SIGNATURES = {
     "orb_stereo_match": (_I, [_P, _P, _P, _P, _I, _P, _P, _I, _F, _F, _P, _P]),
     "orb_stereo_frame": (_I, [_P, _P, _P, _P, _I, _I, _S, _F, _F, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "orb_context_stream": (_P, [_P]),
+    "orb_hbm_copy_time": (_I, [_P, _S, _I, _P]),
     "orb_knn2": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
     "orb_knn2_device": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
     "orb_knn2_time": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _I, _P, _P]),

@@ -21,6 +21,7 @@ extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const
                                           hipStream_t st);
 
 extern "C" hipError_t orb_launch_match_pairs(const MatchArgs* d_args, int npairs, int nwaves, hipStream_t st);
+extern "C" hipError_t orb_launch_hbm_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 extern "C" int orb_knn_chunks(long long nm);
 extern "C" hipError_t orb_launch_knn2(const uint8_t* q, int nq, const uint8_t* m, long long nm, int method,
                                       float ratio, uint2* part, int* idx2, int* dist2, uint8_t* accept,
@@ -143,6 +144,9 @@ struct orb_extractor {
     int* d_msad = nullptr;
     int* d_midx = nullptr;
     int* d_misc = nullptr;        // [0] match count, [1] nl, [2] nr
+    int* d_sidx = nullptr;        // right keypoints sorted by y (stereo scratch)
+    float* d_sy = nullptr;
+    int* d_rows = nullptr;        // per-row start into the sorted right keypoints
     OrbKeyPoint* d_kin = nullptr; // uploaded keypoints: left then right
     uint8_t* d_din = nullptr;
     int in_cap = 0;
@@ -282,6 +286,8 @@ struct orb_extractor {
         const size_t o_uR = take((size_t)p.out_cap * 4), o_dep = take((size_t)p.out_cap * 4);
         const size_t o_msad = take((size_t)p.out_cap * 4), o_midx = take((size_t)p.out_cap * 4);
         const size_t o_misc = take(64);
+        const size_t o_sidx = take((size_t)p.out_cap * 4), o_sy = take((size_t)p.out_cap * 4);
+        const size_t o_rows = take(((size_t)H + 2) * 4);
         const size_t total = off;
         if (total > mem_bytes) {
             if (d_mem) (void)hipFree(d_mem);
@@ -341,6 +347,9 @@ struct orb_extractor {
         d_msad = (int*)(d_mem + o_msad);
         d_midx = (int*)(d_mem + o_midx);
         d_misc = (int*)(d_mem + o_misc);
+        d_sidx = (int*)(d_mem + o_sidx);
+        d_sy = (float*)(d_mem + o_sy);
+        d_rows = (int*)(d_mem + o_rows);
         if (!d_job && hipMalloc(&d_job, sizeof(ImageJob)) != hipSuccess) return ORB_E_OOM;
         if (!d_rt && hipMalloc(&d_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
         if (!h_rt && hipHostMalloc(&h_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
@@ -367,11 +376,12 @@ struct orb_extractor {
             rt.tile_start[l][n] = acc;
         }
         int acc = 0;
-        for (int j = 0; j < n; j++) {
-            rt.slot_start[j] = acc;
-            acc += exs[j]->plan.out_cap;
-        }
-        rt.slot_start[n] = acc;
+        for (int j = 0; j < n; j++)
+            for (int l = 0; l < maxl; l++) {
+                rt.desc_start[j * maxl + l] = acc;
+                if (l < exs[j]->nlevels) acc += (exs[j]->plan.lv[l].list_cap + 3) / 4;
+            }
+        rt.desc_start[n * maxl] = acc;
     }
 
     // Input frame into the contiguous staging buffer hjob.src (one linear
@@ -465,7 +475,13 @@ struct orb_extractor {
         a.match_sad = d_msad;
         a.match_idx = d_midx;
         a.match_count = d_misc;
-        (void)r;
+        a.sorted_idx = d_sidx;
+        a.sorted_y = d_sy;
+        a.row_start = d_rows;
+        a.cap_r = std::min(plan.out_cap, r->plan.out_cap);
+        float band = 0.f;
+        for (int l = 0; l < nlevels; l++) band = std::max(band, 2.0f * scale[l]);
+        a.band = band;
     }
 
     ~orb_extractor()

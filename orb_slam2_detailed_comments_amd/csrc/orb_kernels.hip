@@ -784,31 +784,26 @@ __device__ __forceinline__ void sincos_canonical_dev(double x, double* s, double
     }
 }
 
-// One wavefront per output keypoint.
+// One wavefront per kept keypoint; a workgroup serves one (image, level), so
+// the level's geometry and output offset are uniform across it.
 __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restrict__ jobs,
                                                        const RunTable* __restrict__ rt)
 {
     const int lane = threadIdx.x & 63;
-    const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (slot >= rt->slot_start[rt->njobs]) return;
-    const int jb = find_segment(rt->slot_start, rt->njobs, slot);
+    const int nseg = rt->njobs * rt->max_levels;
+    const int f = find_segment(rt->desc_start, nseg, blockIdx.x);
+    const int jb = f / rt->max_levels, level = f % rt->max_levels;
     const ImageJob& J = jobs[jb];
-    const int s = slot - rt->slot_start[jb];
     const int* ctr = J.counters;
-    int level = -1, idx = 0, total = 0;
+    int base = 0, total = 0;
     for (int l = 0; l < J.nlevels; l++) {
         const int c = ctr[ORB_CNT_RET + l];
-        if (level < 0 && s < total + c) {
-            level = l;
-            idx = s - total;
-        }
+        base += l < level ? c : 0;
         total += c;
     }
-    if (s == 0 && lane == 0) {
-        J.counters[ORB_CNT_NOUT] = total <= J.out_cap ? total : 0;
-        if (total > J.out_cap) atomicOr(&J.counters[ORB_CNT_ERR], ORB_ERR_OUT);
-    }
-    if (level < 0 || total > J.out_cap) return;
+    const int idx = (blockIdx.x - rt->desc_start[f]) * 4 + (threadIdx.x >> 6);
+    if (level == 0 && idx == 0 && lane == 0) J.counters[ORB_CNT_NOUT] = total;
+    if (level >= J.nlevels || idx >= ctr[ORB_CNT_RET + level]) return;
     const LevelJob& L = J.lv[level];
     const unsigned long long key = L.ret[idx];
     int xr, yr;
@@ -817,22 +812,26 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     const int pitch = L.pitch;
 
     // IC_Angle (ORBextractor.cc:94-141): m10 = sum u*I, m01 = sum v*I over
-    // the disc |u| <= umax[|v|].  Lane = column u, the two half-waves split
-    // the rows.
+    // the disc |u| <= umax[|v|].  Lane = column u (lane 31 idle), the two
+    // half-waves take rows 0..15 and -15..-1; all 16 loads are issued before
+    // use (every address is inside the level: keypoints are >= 19 px from
+    // its edges).
     const uint8_t* img = L.lvl + (size_t)cy * pitch + cx;
     const int u = (lane & 31) - 15;
+    const int au = u < 0 ? -u : u;
+    const int vb = lane < 32 ? 0 : -16;
+    int px[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) px[t] = img[(vb + t) * pitch + u];
     int m10 = 0, m01 = 0;
-    if ((lane & 31) < 31) {
-        const int au = u < 0 ? -u : u;
-        const int v0 = lane < 32 ? 0 : -15, v1 = lane < 32 ? 15 : -1;
-        for (int v = v0; v <= v1; v++) {
-            const int av = v < 0 ? -v : v;
-            if (au <= c_umax[av]) {
-                const int val = img[v * pitch + u];
-                m10 += u * val;
-                m01 += v * val;
-            }
-        }
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int v = vb + t;
+        const int av = v < 0 ? -v : v;
+        const bool in = (lane & 31) < 31 && v >= -15 && au <= c_umax[av < 16 ? av : 15];
+        const int val = in ? px[t] : 0;
+        m10 += u * val;
+        m01 += v * val;
     }
     m10 = wave_sum(m10);
     m01 = wave_sum(m01);
@@ -845,18 +844,20 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     sincos_canonical_dev((double)angle, &sd, &cd);
     const float a = (float)cd, b = (float)sd;
     const uint8_t* center = L.blur + (size_t)cy * pitch + cx;
-    int nib = 0;
+    int i0[4], i1[4];
 #pragma unroll
     for (int bit = 0; bit < 4; bit++) {
         const int p = lane * 4 + bit;   // pair index: byte p/8, bit p%8
         const float x0 = (float)c_pattern[4 * p], y0 = (float)c_pattern[4 * p + 1];
         const float x1 = (float)c_pattern[4 * p + 2], y1 = (float)c_pattern[4 * p + 3];
-        const int i0 = center[(int)rintf(x0 * b + y0 * a) * pitch + (int)rintf(x0 * a - y0 * b)];
-        const int i1 = center[(int)rintf(x1 * b + y1 * a) * pitch + (int)rintf(x1 * a - y1 * b)];
-        nib |= (i0 < i1) << bit;
+        i0[bit] = center[(int)rintf(x0 * b + y0 * a) * pitch + (int)rintf(x0 * a - y0 * b)];
+        i1[bit] = center[(int)rintf(x1 * b + y1 * a) * pitch + (int)rintf(x1 * a - y1 * b)];
     }
+    int nib = 0;
+#pragma unroll
+    for (int bit = 0; bit < 4; bit++) nib |= (i0[bit] < i1[bit]) << bit;
     const int hi = __shfl_down(nib, 1, 64);
-    const int out = s;
+    const int out = base + idx;
     if ((lane & 1) == 0) J.desc[(size_t)out * 32 + (lane >> 1)] = (uint8_t)(nib | (hi << 4));
     if (lane == 0) {
         const float sc = L.scale;
@@ -906,6 +907,51 @@ __device__ __forceinline__ int hamming32(const uint4 a0, const uint4 a1, const u
 // smallest (distance, index) below TH_HIGH.  Refinement: 11x11 SAD of
 // centre-subtracted patches over 11 shifts, parabola fit, depth = bf/disp.
 
+// Sort each pair's right keypoints by (y, index) (one workgroup per pair,
+// bitonic sort in LDS); the match kernel binary-searches the row band.
+#define STEREO_SORT_CAP 4096
+__global__ __launch_bounds__(1024) void stereo_prep_kernel(const StereoArgs* __restrict__ args)
+{
+    __shared__ unsigned long long v[STEREO_SORT_CAP];
+    const StereoArgs& A = args[blockIdx.x];
+    const int n = min(*A.nr, min(A.cap_r, STEREO_SORT_CAP));
+    int npow = 1;
+    while (npow < n) npow <<= 1;
+    for (int i = threadIdx.x; i < npow; i += 1024)
+        v[i] = i < n ? ((unsigned long long)__float_as_uint(A.kr[i].y) << 32) | (unsigned)i : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= npow; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int a = threadIdx.x; a < npow; a += 1024) {
+                const int b = a ^ j;
+                if (b > a) {
+                    const unsigned long long va = v[a], vb = v[b];
+                    const bool up = (a & k) == 0;
+                    if (up ? (va > vb) : (va < vb)) {
+                        v[a] = vb;
+                        v[b] = va;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        A.sorted_idx[i] = (int)(unsigned)v[i];
+        A.sorted_y[i] = __uint_as_float((unsigned)(v[i] >> 32));
+    }
+    // Per level-0 row: first sorted entry that can hold the row in its band.
+    for (int row = threadIdx.x; row <= A.nrows; row += 1024) {
+        const float ylo = (float)row - 1.0f - A.band;
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (__uint_as_float((unsigned)(v[mid] >> 32)) <= ylo) lo = mid + 1;
+            else hi = mid;
+        }
+        A.row_start[row] = lo;
+    }
+}
+
 __device__ __forceinline__ int find_pair(const StereoArgs* args, int n, int x)
 {
     int lo = 0, hi = n - 1;
@@ -941,7 +987,15 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     if (go) {
         const uint4 a0 = *(const uint4*)(A.dl + (size_t)iL * 32);
         const uint4 a1 = *(const uint4*)(A.dl + (size_t)iL * 32 + 16);
-        for (int iR = lane; iR < nr; iR += 64) {
+        // Right keypoints sorted by y: every candidate of row `row` has
+        // y in (row - 1 - band, row + 1 + band); the exact band, octave and
+        // uR tests follow.
+        const int n = min(nr, A.cap_r);
+        const float yhi = (float)row + 1.0f + A.band;
+        const int lo = A.row_start[row];
+        for (int c = lo + lane; c < n; c += 64) {
+            if (A.sorted_y[c] >= yhi) break;
+            const int iR = A.sorted_idx[c];
             const OrbKeyPoint kpR = A.kr[iR];
             const float r = 2.0f * A.scale[kpR.octave];
             const int maxr = (int)ceilf(kpR.y + r);
@@ -951,7 +1005,7 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
             const float uR = kpR.x;
             if (uR >= minU && uR <= maxU) {
                 const int dist = hamming32(a0, a1, A.dr + (size_t)iR * 32);
-                if (dist < bestDist) {
+                if (dist < bestDist || (dist == bestDist && iR < bestIdx)) {
                     bestDist = dist;
                     bestIdx = iR;
                 }
@@ -1226,8 +1280,8 @@ extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable*
     }
     octree_kernel<<<h_rt->njobs * h_rt->max_levels, OCT_T, 0, st>>>(d_jobs, h_rt->max_levels);
     if (ev) (void)hipEventRecord(ev[e++], st);
-    const int nslots = h_rt->slot_start[h_rt->njobs];
-    if (nslots > 0) describe_kernel<<<(nslots + 3) / 4, 256, 0, st>>>(d_jobs, d_rt);
+    const int nblocks = h_rt->desc_start[h_rt->njobs * h_rt->max_levels];
+    if (nblocks > 0) describe_kernel<<<nblocks, 256, 0, st>>>(d_jobs, d_rt);
     if (ev) (void)hipEventRecord(ev[e++], st);
     return hipGetLastError();
 }
@@ -1244,6 +1298,7 @@ extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, in
 {
     if (npairs <= 0) return hipSuccess;
     zero_match_counts_kernel<<<1, 256, 0, st>>>(d_args, npairs);
+    stereo_prep_kernel<<<npairs, 1024, 0, st>>>(d_args);
     if (nwaves > 0) stereo_kernel<<<(nwaves + 3) / 4, 256, 0, st>>>(d_args, npairs, nwaves);
     if (ev) (void)hipEventRecord(ev[0], st);
     stereo_median_kernel<<<npairs, 1024, 0, st>>>(d_args);
@@ -1499,5 +1554,23 @@ extern "C" hipError_t orb_launch_match_pairs(const MatchArgs* d_args, int npairs
     if (npairs <= 0) return hipSuccess;
     if (nwaves > 0) match_pairs_kernel<<<(nwaves + 3) / 4, 256, 0, st>>>(d_args, npairs, nwaves);
     save_prev_kernel<<<npairs, 256, 0, st>>>(d_args, npairs);
+    return hipGetLastError();
+}
+
+// ======================================================= HBM copy microbench
+// Plain streaming copy (16-byte loads and stores, grid-stride): the measured
+// HBM rate of this kernel re-confirms the bandwidth peak, and its counter
+// readings calibrate FETCH_SIZE / WRITE_SIZE for a known byte count.
+__global__ __launch_bounds__(256) void hbm_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                       size_t n16)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+extern "C" hipError_t orb_launch_hbm_copy(const void* src, void* dst, size_t bytes, hipStream_t st)
+{
+    const size_t n16 = bytes / 16;
+    hbm_copy_kernel<<<8192, 256, 0, st>>>((const uint4*)src, (uint4*)dst, n16);
     return hipGetLastError();
 }

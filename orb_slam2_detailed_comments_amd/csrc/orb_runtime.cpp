@@ -228,7 +228,11 @@ int orb_stereo_match(orb_extractor* left, orb_extractor* right, const orb_keypoi
         if (kl[i].octave < 0 || kl[i].octave >= left->nlevels) return ORB_E_ARG;
     for (int i = 0; i < nr; i++)
         if (kr[i].octave < 0 || kr[i].octave >= left->nlevels) return ORB_E_ARG;
-    if (nl > left->plan.out_cap) return ORB_E_CAPACITY;
+    // stereo scratch: right keypoints sorted in LDS (<= 4096) and the
+    // left extractor's per-keypoint arrays
+    if (nl > left->plan.out_cap || nr > std::min(left->plan.out_cap, right->plan.out_cap) || nl > 4096 ||
+        nr > 4096)
+        return ORB_E_CAPACITY;
     HIP_TRY(hipSetDevice(left->ctx->device));
     hipStream_t st = left->ctx->stream;
     int rc = left->ensure_pair(right);
@@ -282,6 +286,7 @@ int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* i
     if (!rc) rc = right->ensure(width, height);
     if (!rc) rc = left->ensure_pair(right);
     if (rc) return rc;
+    if (left->plan.out_cap > 4096 || right->plan.out_cap > 4096) return ORB_E_ARG;
     rc = left->upload_image(im_left, stride, hipMemcpyHostToDevice);
     if (!rc) rc = right->upload_image(im_right, stride, hipMemcpyHostToDevice);
     if (rc) return rc;
@@ -496,7 +501,7 @@ int orb_batch_set_stereo(orb_batch* b, int npairs, const int* left_slots, const 
         orb_extractor* L = b->ex[l];
         orb_extractor* R = b->ex[r];
         if (L->plan.W != R->plan.W || L->plan.H != R->plan.H || L->nlevels != R->nlevels) return ORB_E_ARG;
-        if (L->plan.out_cap > 4096) return ORB_E_ARG;
+        if (L->plan.out_cap > 4096 || R->plan.out_cap > 4096) return ORB_E_ARG;
         StereoArgs& a = args[p];
         L->stereo_args(a, R, b->d_jobs + l, b->d_jobs + r, bf, baseline);
         a.kl = L->hjob.kps;
@@ -723,3 +728,37 @@ int orb_knn2_time(orb_context* ctx, const uint8_t* query, int nq, const uint8_t*
 }
 
 }  // extern "C"
+
+// HBM copy microbenchmark: average milliseconds of a device-to-device copy of
+// `bytes` bytes by a plain streaming kernel over `steps` runs.
+extern "C" int orb_hbm_copy_time(orb_context* ctx, size_t bytes, int steps, float* ms)
+{
+    if (!ctx || bytes < 16 || steps <= 0 || !ms) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return ORB_E_OOM;
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return ORB_E_OOM;
+    }
+    int rc = ORB_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMemsetAsync(a, 1, bytes, ctx->stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        rc = ORB_E_DEVICE;
+    if (!rc && orb_launch_hbm_copy(a, b, bytes, ctx->stream) != hipSuccess) rc = ORB_E_DEVICE;   // warm-up
+    if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = ORB_E_DEVICE;
+    for (int s = 0; s < steps && !rc; s++)
+        if (orb_launch_hbm_copy(a, b, bytes, ctx->stream) != hipSuccess) rc = ORB_E_DEVICE;
+    if (!rc && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = ORB_E_DEVICE;
+    float t = 0.f;
+    if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = ORB_E_DEVICE;
+    if (!rc) *ms = t / steps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}

@@ -77,7 +77,7 @@ struct RunTable {
     int njobs;
     int max_levels;
     int tile_start[ORB_MAX_LEVELS][ORB_MAX_JOBS + 1];   // per level: first tile block of job j
-    int slot_start[ORB_MAX_JOBS + 1];                   // describe kernel: first slot of job j
+    int desc_start[(ORB_MAX_JOBS + 1) * ORB_MAX_LEVELS];  // describe kernel: first block of (job, level)
 };
 
 // Arguments of one stereo match (Frame::ComputeStereoMatches).
@@ -100,6 +100,11 @@ struct StereoArgs {
     int* match_sad;
     int* match_idx;
     int* match_count;
+    int* sorted_idx;        // scratch: right keypoints sorted by y ...
+    float* sorted_y;        // ... (cap_r entries each)
+    int* row_start;         // first sorted entry with y > row - 1 - band (nrows + 1)
+    int cap_r;
+    float band;             // max over levels of 2*scale: widest row band
 };
 
 // Arguments of one brute-force match of a query set against a train set
