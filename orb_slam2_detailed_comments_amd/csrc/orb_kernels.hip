@@ -83,25 +83,27 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask)
 #define LT_W ORB_TILE_W
 #define LT_H ORB_TILE_H
 #define LT_HALO 4
-#define LP_W (LT_W + 2 * LT_HALO)
-#define LP_H (LT_H + 2 * LT_HALO)
-#define LM_W (LT_W + 2)
-#define LM_H (LT_H + 2)
+#define LP_W (LT_W + 2 * LT_HALO)      // 72 bytes = 18 dwords
+#define LP_H (LT_H + 2 * LT_HALO)      // 40 rows
+#define LP_WD (LP_W / 4)
+#define LM_H (LT_H + 2)                // FAST strength rows: tile -1 .. 32
 #define LK_CAP 1536
 
-__device__ __forceinline__ int fast_strength(const uint8_t (*t)[LP_W], int cx, int cy)
+typedef unsigned short orb_us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int fast_strength(const uint8_t* p, int stride)
 {
     // Bresenham radius-3 circle in circular order (OpenCV makeOffsets(16)).
-    const int v = t[cy][cx];
+    const int v = p[0];
     int d[16];
-    d[0] = t[cy + 3][cx + 0] - v;  d[1] = t[cy + 3][cx + 1] - v;
-    d[2] = t[cy + 2][cx + 2] - v;  d[3] = t[cy + 1][cx + 3] - v;
-    d[4] = t[cy + 0][cx + 3] - v;  d[5] = t[cy - 1][cx + 3] - v;
-    d[6] = t[cy - 2][cx + 2] - v;  d[7] = t[cy - 3][cx + 1] - v;
-    d[8] = t[cy - 3][cx + 0] - v;  d[9] = t[cy - 3][cx - 1] - v;
-    d[10] = t[cy - 2][cx - 2] - v; d[11] = t[cy - 1][cx - 3] - v;
-    d[12] = t[cy + 0][cx - 3] - v; d[13] = t[cy + 1][cx - 3] - v;
-    d[14] = t[cy + 2][cx - 2] - v; d[15] = t[cy + 3][cx - 1] - v;
+    d[0] = p[3 * stride] - v;       d[1] = p[3 * stride + 1] - v;
+    d[2] = p[2 * stride + 2] - v;   d[3] = p[stride + 3] - v;
+    d[4] = p[3] - v;                d[5] = p[-stride + 3] - v;
+    d[6] = p[-2 * stride + 2] - v;  d[7] = p[-3 * stride + 1] - v;
+    d[8] = p[-3 * stride] - v;      d[9] = p[-3 * stride - 1] - v;
+    d[10] = p[-2 * stride - 2] - v; d[11] = p[-stride - 3] - v;
+    d[12] = p[-3] - v;              d[13] = p[stride - 3] - v;
+    d[14] = p[2 * stride - 2] - v;  d[15] = p[3 * stride - 1] - v;
     // Minimum and maximum over every run of 9 contiguous ring pixels.
     int mn[16], mx[16];
 #pragma unroll
@@ -129,29 +131,47 @@ __device__ __forceinline__ int fast_strength(const uint8_t (*t)[LP_W], int cx, i
     return s < 0 ? 0 : s;
 }
 
-// Necessary condition for strength > t: a 9-run covers two circularly
-// adjacent cardinal points (ring positions 0, 4, 8, 12).
-__device__ __forceinline__ bool fast_maybe(const uint8_t (*t)[LP_W], int cx, int cy, int thr)
+// Necessary condition for strength > t, on two pixels at once (16-bit
+// lanes): a 9-run covers two circularly adjacent cardinal points (ring
+// positions 0, 4, 8, 12), all brighter than v+t or all darker than v-t.
+__device__ __forceinline__ unsigned fast_maybe2(unsigned v, unsigned p0, unsigned p4, unsigned p8, unsigned p12,
+                                                unsigned t2)
 {
-    const int v = t[cy][cx];
-    const int p0 = t[cy + 3][cx], p4 = t[cy][cx + 3], p8 = t[cy - 3][cx], p12 = t[cy][cx - 3];
-    const int hi = v + thr, lo = v - thr;
-    const unsigned b = (p0 > hi) | ((p4 > hi) << 1) | ((p8 > hi) << 2) | ((p12 > hi) << 3);
-    const unsigned k = (p0 < lo) | ((p4 < lo) << 1) | ((p8 < lo) << 2) | ((p12 < lo) << 3);
-    const unsigned rb = ((b >> 1) | (b << 3)) & 15u, rk = ((k >> 1) | (k << 3)) & 15u;
-    return ((b & rb) | (k & rk)) != 0u;
+    const orb_us2 V = __builtin_bit_cast(orb_us2, v), T = __builtin_bit_cast(orb_us2, t2);
+    const orb_us2 hi = V + T;                                   // <= 510, no wrap
+    const orb_us2 lo = __builtin_elementwise_sub_sat(V, T);     // max(v - t, 0)
+    const orb_us2 a = __builtin_bit_cast(orb_us2, p0), b = __builtin_bit_cast(orb_us2, p4);
+    const orb_us2 c = __builtin_bit_cast(orb_us2, p8), d = __builtin_bit_cast(orb_us2, p12);
+    const orb_us2 ba = __builtin_elementwise_sub_sat(a, hi), bb = __builtin_elementwise_sub_sat(b, hi);
+    const orb_us2 bc = __builtin_elementwise_sub_sat(c, hi), bd = __builtin_elementwise_sub_sat(d, hi);
+    const orb_us2 ka = __builtin_elementwise_sub_sat(lo, a), kb = __builtin_elementwise_sub_sat(lo, b);
+    const orb_us2 kc = __builtin_elementwise_sub_sat(lo, c), kd = __builtin_elementwise_sub_sat(lo, d);
+    orb_us2 r = __builtin_elementwise_max(__builtin_elementwise_min(ba, bb), __builtin_elementwise_min(bb, bc));
+    r = __builtin_elementwise_max(r, __builtin_elementwise_max(__builtin_elementwise_min(bc, bd),
+                                                               __builtin_elementwise_min(bd, ba)));
+    r = __builtin_elementwise_max(r, __builtin_elementwise_max(__builtin_elementwise_min(ka, kb),
+                                                               __builtin_elementwise_min(kb, kc)));
+    r = __builtin_elementwise_max(r, __builtin_elementwise_max(__builtin_elementwise_min(kc, kd),
+                                                               __builtin_elementwise_min(kd, ka)));
+    return __builtin_bit_cast(unsigned, r);
 }
+
+// Bytes 0 and 2 (sel 0x0C0A0C08 style) of a dword pair into 16-bit lanes.
+__device__ __forceinline__ unsigned lo_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C020C00u); }
+__device__ __forceinline__ unsigned hi_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C030C01u); }
 
 __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__ jobs,
                                                     const RunTable* __restrict__ rt, int level)
 {
-    __shared__ uint8_t px[LP_H][LP_W];
-    __shared__ int hs[LT_H + 6][LT_W];
-    __shared__ uint8_t mm[LM_H][LM_W];
-    __shared__ unsigned short queue[LM_H * LM_W];
+    __shared__ unsigned pxw[LP_H][LP_WD];         // level pixels, tile + 4-px halo
+    __shared__ uint4 hs[LT_H + 6][LT_W / 4];      // horizontal blur sums
+    __shared__ unsigned mmw[LM_H][LP_WD];         // FAST strength, tile + 1-px ring
+    __shared__ unsigned short queue[LM_H * (LT_W + 2)];
     __shared__ unsigned long long kbuf[LK_CAP];
     __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
     __shared__ int s_job, s_qn, s_kn, s_base;
+    const uint8_t* px = (const uint8_t*)pxw;
+    uint8_t* mm = (uint8_t*)mmw;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     if (tid == 0) {
@@ -159,6 +179,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         s_qn = 0;
         s_kn = 0;
     }
+    for (int e = tid; e < LM_H * LP_WD; e += 256) mmw[e / LP_WD][e % LP_WD] = 0u;
     __syncthreads();
     const ImageJob& J = jobs[s_job];
     const LevelJob& L = J.lv[level];
@@ -166,15 +187,26 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const int tx0 = (t % L.tiles_x) * LT_W;
     const int ty0 = (t / L.tiles_x) * LT_H;
     const int w = L.w, h = L.h, pitch = L.pitch;
+    const bool interior = tx0 >= LT_HALO && tx0 + LT_W + LT_HALO <= w && ty0 >= LT_HALO &&
+                          ty0 + LT_H + LT_HALO <= h;
 
-    // ---- 1. pixels with halo
+    // ---- 1. pixels with halo, one dword (4 columns) per slot
     if (level == 0) {
         const uint8_t* src = J.src;
         const int sp = J.src_pitch;
-        for (int e = tid; e < LP_W * LP_H; e += 256) {
-            const int ex = e % LP_W, ey = e / LP_W;
-            const int x = reflect101(tx0 - LT_HALO + ex, w), y = reflect101(ty0 - LT_HALO + ey, h);
-            px[ey][ex] = src[(size_t)y * sp + x];
+        for (int e = tid; e < LP_H * LP_WD; e += 256) {
+            const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
+            unsigned d = 0;
+            if (interior) {
+                const uint8_t* r = src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex);
+                d = r[0] | (r[1] << 8) | (r[2] << 16) | ((unsigned)r[3] << 24);
+            } else {
+                const int y = reflect101(ty0 - LT_HALO + ey, h);
+                const uint8_t* r = src + (size_t)y * sp;
+#pragma unroll
+                for (int k = 0; k < 4; k++) d |= (unsigned)r[reflect101(tx0 - LT_HALO + ex + k, w)] << (8 * k);
+            }
+            pxw[ey][ex >> 2] = d;
         }
     } else {
         // cv::resize INTER_LINEAR 8U fixed point from level l-1: double source
@@ -203,137 +235,218 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             yb[r] = (int)rintf((1.f - fy) * 2048.f) | ((int)rintf(fy * 2048.f) << 16);
         }
         __syncthreads();
-        for (int e = tid; e < LP_W * LP_H; e += 256) {
-            const int ex = e % LP_W, ey = e / LP_W;
-            const int sx = xs0[ex], a0 = xa[ex] & 0xFFFF, a1 = xa[ex] >> 16;
+        for (int e = tid; e < LP_H * LP_WD; e += 256) {
+            const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
             const int sy = ys0[ey], b0 = yb[ey] & 0xFFFF, b1 = yb[ey] >> 16;
-            const int r0 = min(max(sy, 0), sh - 1), r1 = min(max(sy + 1, 0), sh - 1);
-            const int nx = sx + 1 < sw ? sx + 1 : sx;
-            const uint8_t* p0 = P.lvl + (size_t)r0 * sp;
-            const uint8_t* p1 = P.lvl + (size_t)r1 * sp;
-            const int S0 = p0[sx] * a0 + p0[nx] * a1;
-            const int S1 = p1[sx] * a0 + p1[nx] * a1;
-            const int v = (S0 * b0 + S1 * b1 + (1 << 21)) >> 22;
-            px[ey][ex] = (uint8_t)(v > 255 ? 255 : (v < 0 ? 0 : v));
+            const uint8_t* p0 = P.lvl + (size_t)min(max(sy, 0), sh - 1) * sp;
+            const uint8_t* p1 = P.lvl + (size_t)min(max(sy + 1, 0), sh - 1) * sp;
+            unsigned d = 0;
+            // All terms are non-negative, so the rounding shift is done
+            // unsigned.  (A signed shift with a [0, 255] clamp of lanes 0 and
+            // 1 is lowered by the gfx950 backend to v_ashr_pk_u8_i32, whose
+            // upper result half then leaked into bytes 2 and 3 of d.)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int sx = xs0[ex + k], a0 = xa[ex + k] & 0xFFFF, a1 = xa[ex + k] >> 16;
+                const int nx = sx + 1 < sw ? sx + 1 : sx;
+                const unsigned S0 = p0[sx] * a0 + p0[nx] * a1;
+                const unsigned S1 = p1[sx] * a0 + p1[nx] * a1;
+                const unsigned v = (S0 * b0 + S1 * b1 + (1u << 21)) >> 22;
+                d |= min(v, 255u) << (8 * k);
+            }
+            pxw[ey][ex >> 2] = d;
         }
     }
     __syncthreads();
 
-    // ---- 2. level and blur; thread = 4 adjacent pixels x 2 rows
-    const int bx = (tid & 15) * 4;
+    // ---- 2. the level (copied out) and its 7x7 blur.  Horizontal taps as
+    // two v_dot4_u32_u8 per output on byte quads (v_alignbyte); the vertical
+    // pass on the row sums, symmetric taps.
+    const int gc = tid & 15;                      // 4-column group of the tile
     if (level > 0 || J.src != L.lvl) {
         for (int r = tid >> 4; r < LT_H; r += 16) {
-            const int y = ty0 + r, x0 = tx0 + bx;
+            const int y = ty0 + r, x0 = tx0 + 4 * gc;
             if (y >= h) continue;
+            const unsigned d = pxw[r + LT_HALO][gc + 1];
             uint8_t* dst = L.lvl + (size_t)y * pitch + x0;
-            const uint8_t* src = &px[r + LT_HALO][bx + LT_HALO];
             if (x0 + 3 < w) {
-                *(uchar4*)dst = make_uchar4(src[0], src[1], src[2], src[3]);
+                *(unsigned*)dst = d;
             } else {
                 for (int k = 0; k < 4; k++)
-                    if (x0 + k < w) dst[k] = src[k];
+                    if (x0 + k < w) dst[k] = (uint8_t)(d >> (8 * k));
             }
         }
     }
-    for (int e = tid; e < (LT_H + 6) * LT_W; e += 256) {
-        const int ey = e / LT_W, ex = e % LT_W;
-        int acc = 0;
-#pragma unroll
-        for (int k = 0; k < 7; k++) acc += c_gauss[k] * px[ey + LT_HALO - 3][ex + LT_HALO - 3 + k];
-        hs[ey][ex] = acc;
+    const unsigned T1 = (unsigned)c_gauss[0] | (c_gauss[1] << 8) | (c_gauss[2] << 16) | ((unsigned)c_gauss[3] << 24);
+    const unsigned T2 = (unsigned)c_gauss[4] | (c_gauss[5] << 8) | (c_gauss[6] << 16);
+    for (int e = tid; e < (LT_H + 6) * (LT_W / 4); e += 256) {
+        const int rr = e >> 4, c = e & 15;       // blur row rr <-> px row rr+1
+        const unsigned D0 = pxw[rr + 1][c], D1 = pxw[rr + 1][c + 1], D2 = pxw[rr + 1][c + 2];
+        uint4 o;
+        o.x = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 1), T1,
+                                     __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 1), T2, 0u, false),
+                                     false);
+        o.y = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 2), T1,
+                                     __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 2), T2, 0u, false),
+                                     false);
+        o.z = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 3), T1,
+                                     __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 3), T2, 0u, false),
+                                     false);
+        o.w = __builtin_amdgcn_udot4(D1, T1, __builtin_amdgcn_udot4(D2, T2, 0u, false), false);
+        hs[rr][c] = o;
     }
     __syncthreads();
-    for (int r = tid >> 4; r < LT_H; r += 16) {
-        const int y = ty0 + r, x0 = tx0 + bx;
-        if (y >= h) continue;
-        uint8_t bl[4];
+    {
+        const int k0 = c_gauss[3], k1 = c_gauss[4], k2 = c_gauss[5], k3 = c_gauss[6];
+        for (int r = tid >> 4; r < LT_H; r += 16) {
+            const int y = ty0 + r, x0 = tx0 + 4 * gc;
+            if (y >= h) continue;
+            uint4 q[7];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            int acc = 0;
-#pragma unroll
-            for (int q = 0; q < 7; q++) acc += c_gauss[q] * hs[r + q][bx + k];
-            const int v = (acc + (1 << 15)) >> 16;
-            bl[k] = (uint8_t)(v > 255 ? 255 : v);
-        }
-        uint8_t* db = L.blur + (size_t)y * pitch + x0;
-        if (x0 + 3 < w) {
-            *(uchar4*)db = make_uchar4(bl[0], bl[1], bl[2], bl[3]);
-        } else {
-            for (int k = 0; k < 4; k++)
-                if (x0 + k < w) db[k] = bl[k];
+            for (int i = 0; i < 7; i++) q[i] = hs[r + i][gc];
+            unsigned out = 0;
+#define ORB_VBLUR(comp, k)                                                                              \
+            {                                                                                            \
+                const unsigned acc = q[3].comp * k0 + (q[2].comp + q[4].comp) * k1 +                    \
+                                     (q[1].comp + q[5].comp) * k2 + (q[0].comp + q[6].comp) * k3;       \
+                const unsigned v = (acc + (1u << 15)) >> 16;                                            \
+                out |= (v > 255u ? 255u : v) << (8 * (k));                                              \
+            }
+            ORB_VBLUR(x, 0) ORB_VBLUR(y, 1) ORB_VBLUR(z, 2) ORB_VBLUR(w, 3)
+#undef ORB_VBLUR
+            uint8_t* db = L.blur + (size_t)y * pitch + x0;
+            if (x0 + 3 < w) {
+                *(unsigned*)db = out;
+            } else {
+                for (int k = 0; k < 4; k++)
+                    if (x0 + k < w) db[k] = (uint8_t)(out >> (8 * k));
+            }
         }
     }
 
-    // ---- 3. FAST strength on the tile plus a 1-px ring, detection areas only
+    // ---- 3. FAST strength on the tile plus a 1-px ring, detection areas
+    // only.  The cardinal pre-test runs on 4-pixel groups (two pixels per
+    // 16-bit lane pair); survivors are compacted, then scored exactly.
     const int tA = min(max(J.iniTh, 0), 255), tB = min(max(J.minTh, 0), 255);
     const int tmin = min(tA, tB);
     const int ax0 = ORB_EDGE, ay0 = ORB_EDGE;
     const int ax1 = min(ORB_EDGE + L.nCols * L.wCell, L.maxBorderX - 3);
     const int ay1 = min(ORB_EDGE + L.nRows * L.hCell, L.maxBorderY - 3);
-    for (int e0 = 0; e0 < LM_W * LM_H; e0 += 256) {
+    const unsigned t2 = (unsigned)tmin | ((unsigned)tmin << 16);
+    // groups: rows tile -1..32, 4-column groups 0..15; plus the ring columns
+    // tile -1 and 64 (16 extra slots per row, 2 used)
+    for (int e0 = 0; e0 < LM_H * 17; e0 += 256) {
         const int e = e0 + tid;
-        bool cand = false;
-        if (e < LM_W * LM_H) {
-            const int mx = e % LM_W, my = e / LM_W;
-            const int x = tx0 - 1 + mx, y = ty0 - 1 + my;
-            const bool in = x >= ax0 && x < ax1 && y >= ay0 && y < ay1;
-            cand = in && fast_maybe(px, mx + LT_HALO - 1, my + LT_HALO - 1, tmin);
-            mm[my][mx] = 0;
+        unsigned cand = 0u;                       // bit k: pixel k of the group
+        int my = 0, c = 0;
+        if (e < LM_H * 17) {
+            my = e / 17;
+            c = e - my * 17;
+            const int y = ty0 - 1 + my;
+            const int py = my + 3;                // px row of tile row y
+            if (c < 16) {
+                const int x = tx0 + 4 * c;
+                const unsigned V = pxw[py][c + 1], Up = pxw[py - 3][c + 1], Dn = pxw[py + 3][c + 1];
+                const unsigned Lw = pxw[py][c], Rw = pxw[py][c + 2];
+                const unsigned P4 = __builtin_amdgcn_alignbyte(Rw, V, 3);
+                const unsigned P12 = __builtin_amdgcn_alignbyte(V, Lw, 1);
+                const unsigned m02 = fast_maybe2(lo_bytes(V), lo_bytes(Dn), lo_bytes(P4), lo_bytes(Up),
+                                                 lo_bytes(P12), t2);
+                const unsigned m13 = fast_maybe2(hi_bytes(V), hi_bytes(Dn), hi_bytes(P4), hi_bytes(Up),
+                                                 hi_bytes(P12), t2);
+                const bool yin = y >= ay0 && y < ay1;
+                cand = ((m02 & 0xFFFFu) ? 1u : 0u) | ((m13 & 0xFFFFu) ? 2u : 0u) | ((m02 >> 16) ? 4u : 0u) |
+                       ((m13 >> 16) ? 8u : 0u);
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (!yin || x + k < ax0 || x + k >= ax1) cand &= ~(1u << k);
+            } else {
+                // ring columns x = tx0 - 1 and tx0 + 64
+                for (int side = 0; side < 2; side++) {
+                    const int x = side ? tx0 + LT_W : tx0 - 1;
+                    const bool in = x >= ax0 && x < ax1 && y >= ay0 && y < ay1;
+                    if (!in) continue;
+                    const int pxc = x - tx0 + LT_HALO;
+                    const uint8_t* p = px + py * LP_W + pxc;
+                    const int v = p[0], hi = v + tmin, lo = v - tmin;
+                    const int q0 = p[3 * LP_W], q4 = p[3], q8 = p[-3 * LP_W], q12 = p[-3];
+                    const unsigned b = (q0 > hi) | ((q4 > hi) << 1) | ((q8 > hi) << 2) | ((q12 > hi) << 3);
+                    const unsigned kk = (q0 < lo) | ((q4 < lo) << 1) | ((q8 < lo) << 2) | ((q12 < lo) << 3);
+                    const unsigned rb = ((b >> 1) | (b << 3)) & 15u, rk = ((kk >> 1) | (kk << 3)) & 15u;
+                    if ((b & rb) | (kk & rk)) cand |= side ? 2u : 1u;
+                }
+            }
         }
-        const unsigned long long bal = __ballot(cand);
-        int base = 0;
-        if (lane == 0 && bal) base = atomicAdd(&s_qn, __popcll(bal));
-        base = __shfl(base, 0, 64);
-        if (cand) queue[base + lanes_below(bal)] = (unsigned short)e;
+        // queue entry: m-row << 8 | byte column in mmw (tile col + 4)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool ck = (cand >> k) & 1u;
+            const unsigned long long bal = __ballot(ck);
+            if (!bal) continue;
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&s_qn, __popcll(bal));
+            base = __shfl(base, 0, 64);
+            if (ck) {
+                const int col = c < 16 ? 4 + 4 * c + k : (k == 0 ? 3 : 4 + LT_W);
+                queue[base + lanes_below(bal)] = (unsigned short)((my << 8) | col);
+            }
+        }
     }
     __syncthreads();
     const int qn = s_qn;
     for (int q = tid; q < qn; q += 256) {
         const int e = queue[q];
-        const int mx = e % LM_W, my = e / LM_W;
-        mm[my][mx] = (uint8_t)fast_strength(px, mx + LT_HALO - 1, my + LT_HALO - 1);
+        const int my = e >> 8, col = e & 255;
+        mm[my * LP_W + col] = (uint8_t)fast_strength(px + (my + 3) * LP_W + col, LP_W);
     }
     __syncthreads();
 
-    // ---- 4. per-cell non-max suppression at both thresholds
-    for (int e = tid; e < LT_W * LT_H; e += 256) {
-        const int lx = e % LT_W, ly = e / LT_W;
-        const int x = tx0 + lx, y = ty0 + ly;
-        const int mc = mm[ly + 1][lx + 1];
-        if (mc <= tmin || x >= ax1 || y >= ay1 || x < ax0 || y < ay0) continue;
-        const int cj = (x - ORB_EDGE) / L.wCell, ci = (y - ORB_EDGE) / L.hCell;
-        const int cx0 = max(ORB_EDGE + cj * L.wCell, ax0), cx1 = min(ORB_EDGE + (cj + 1) * L.wCell, ax1);
-        const int cy0 = max(ORB_EDGE + ci * L.hCell, ay0), cy1 = min(ORB_EDGE + (ci + 1) * L.hCell, ay1);
-        bool keepA = mc > tA, keepB = mc > tB;
-        const int sA = mc - 1, sB = mc - 1;
+    // ---- 4. per-cell non-max suppression at both thresholds; 4-pixel
+    // groups whose strengths are all <= min threshold are skipped at once.
+    for (int e = tid; e < LT_H * (LT_W / 4); e += 256) {
+        const int ly = e >> 4, c = e & 15;
+        const unsigned grp = mmw[ly + 1][c + 1];
+        if (grp == 0u) continue;
+        for (int k = 0; k < 4; k++) {
+            const int lx = 4 * c + k;
+            const int x = tx0 + lx, y = ty0 + ly;
+            const int mc = (grp >> (8 * k)) & 255;
+            if (mc <= tmin || x >= ax1 || y >= ay1 || x < ax0 || y < ay0) continue;
+            const int cj = (x - ORB_EDGE) / L.wCell, ci = (y - ORB_EDGE) / L.hCell;
+            const int cx0 = max(ORB_EDGE + cj * L.wCell, ax0), cx1 = min(ORB_EDGE + (cj + 1) * L.wCell, ax1);
+            const int cy0 = max(ORB_EDGE + ci * L.hCell, ay0), cy1 = min(ORB_EDGE + (ci + 1) * L.hCell, ay1);
+            bool keepA = mc > tA, keepB = mc > tB;
+            const int sA = mc - 1, sB = mc - 1;
+            const uint8_t* mrow = mm + (ly + 1) * LP_W + (lx + 4);
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            if (k == 4) continue;
-            const int dx = k % 3 - 1, dy = k / 3 - 1;
-            const int qx = x + dx, qy = y + dy;
-            if (qx < cx0 || qx >= cx1 || qy < cy0 || qy >= cy1) {
-                // outside this cell's window: score 0
-                keepA = keepA && sA > 0;
-                keepB = keepB && sB > 0;
-                continue;
+            for (int n = 0; n < 9; n++) {
+                if (n == 4) continue;
+                const int dx = n % 3 - 1, dy = n / 3 - 1;
+                const int qx = x + dx, qy = y + dy;
+                if (qx < cx0 || qx >= cx1 || qy < cy0 || qy >= cy1) {
+                    // outside this cell's window: score 0
+                    keepA = keepA && sA > 0;
+                    keepB = keepB && sB > 0;
+                    continue;
+                }
+                const int mq = mrow[dy * LP_W + dx];
+                keepA = keepA && sA > (mq > tA ? mq - 1 : 0);
+                keepB = keepB && sB > (mq > tB ? mq - 1 : 0);
             }
-            const int mq = mm[ly + 1 + dy][lx + 1 + dx];
-            keepA = keepA && sA > (mq > tA ? mq - 1 : 0);
-            keepB = keepB && sB > (mq > tB ? mq - 1 : 0);
-        }
-        if (!keepA && !keepB) continue;
-        const unsigned xl = (unsigned)(x - (ORB_MIN_BORDER + cj * L.wCell));
-        const unsigned yl = (unsigned)(y - (ORB_MIN_BORDER + ci * L.hCell));
-        const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | (yl << 6) | xl;
-        const unsigned long long key = ((unsigned long long)(unsigned)(mc - 1) << 32) | ok;
-        if (keepA) {
-            L.cellflag[ci * L.nCols + cj] = 1;
-            const int i = atomicAdd(&s_kn, 1);
-            if (i < LK_CAP) kbuf[i] = key;
-        }
-        if (keepB) {
-            const int i = atomicAdd(&s_kn, 1);
-            if (i < LK_CAP) kbuf[i] = key | ORB_KEY_RETRY;
+            if (!keepA && !keepB) continue;
+            const unsigned xl = (unsigned)(x - (ORB_MIN_BORDER + cj * L.wCell));
+            const unsigned yl = (unsigned)(y - (ORB_MIN_BORDER + ci * L.hCell));
+            const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | (yl << 6) | xl;
+            const unsigned long long key = ((unsigned long long)(unsigned)(mc - 1) << 32) | ok;
+            if (keepA) {
+                L.cellflag[ci * L.nCols + cj] = 1;
+                const int i = atomicAdd(&s_kn, 1);
+                if (i < LK_CAP) kbuf[i] = key;
+            }
+            if (keepB) {
+                const int i = atomicAdd(&s_kn, 1);
+                if (i < LK_CAP) kbuf[i] = key | ORB_KEY_RETRY;
+            }
         }
     }
     __syncthreads();
