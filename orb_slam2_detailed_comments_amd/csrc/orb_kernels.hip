@@ -87,7 +87,7 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask)
 #define LP_H (LT_H + 2 * LT_HALO)      // 40 rows
 #define LP_WD (LP_W / 4)
 #define LM_H (LT_H + 2)                // FAST strength rows: tile -1 .. 32
-#define LK_CAP 1536
+#define LK_CAP 1536                  // per-tile candidates, both thresholds
 
 typedef unsigned short orb_us2 __attribute__((ext_vector_type(2)));
 
@@ -160,16 +160,27 @@ __device__ __forceinline__ unsigned fast_maybe2(unsigned v, unsigned p0, unsigne
 __device__ __forceinline__ unsigned lo_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C020C00u); }
 __device__ __forceinline__ unsigned hi_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C030C01u); }
 
+// LDS that the phases use one after another: the resize source window
+// (phase 1), the horizontal blur sums (phase 2), then the FAST queue and the
+// tile's candidate list (phases 3-5).
+#define LW_MAX 2656                  // source window dwords: scale <= ~1.9
+union LevelScratch {
+    unsigned win[LW_MAX];
+    uint4 hs[LT_H + 6][LT_W / 4];
+    struct {
+        unsigned short queue[LM_H * (LT_W + 2)];
+        unsigned kloc[LK_CAP];       // tile-local candidate: ly | lx | score | retry
+    } f;
+};
+
 __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__ jobs,
                                                     const RunTable* __restrict__ rt, int level)
 {
     __shared__ unsigned pxw[LP_H][LP_WD];         // level pixels, tile + 4-px halo
-    __shared__ uint4 hs[LT_H + 6][LT_W / 4];      // horizontal blur sums
     __shared__ unsigned mmw[LM_H][LP_WD];         // FAST strength, tile + 1-px ring
-    __shared__ unsigned short queue[LM_H * (LT_W + 2)];
-    __shared__ unsigned long long kbuf[LK_CAP];
+    __shared__ LevelScratch u;
     __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
-    __shared__ int s_job, s_qn, s_kn, s_base;
+    __shared__ int s_job, s_qn, s_kn, s_base, s_wx0, s_wx1, s_wy0, s_wy1;
     const uint8_t* px = (const uint8_t*)pxw;
     uint8_t* mm = (uint8_t*)mmw;
     const int tid = threadIdx.x;
@@ -178,6 +189,8 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         s_job = find_segment(rt->tile_start[level], rt->njobs, blockIdx.x);
         s_qn = 0;
         s_kn = 0;
+        s_wx0 = s_wy0 = 0x7FFFFFFF;
+        s_wx1 = s_wy1 = -1;
     }
     for (int e = tid; e < LM_H * LP_WD; e += 256) mmw[e / LP_WD][e % LP_WD] = 0u;
     __syncthreads();
@@ -198,8 +211,8 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
             unsigned d = 0;
             if (interior) {
-                const uint8_t* r = src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex);
-                d = r[0] | (r[1] << 8) | (r[2] << 16) | ((unsigned)r[3] << 24);
+                // unaligned dword load (rows of the input are W bytes apart)
+                __builtin_memcpy(&d, src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex), 4);
             } else {
                 const int y = reflect101(ty0 - LT_HALO + ey, h);
                 const uint8_t* r = src + (size_t)y * sp;
@@ -224,6 +237,8 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             if (sx >= sw - 1) { fx = 0.f; sx = sw - 1; }
             xs0[tid] = sx;
             xa[tid] = (int)rintf((1.f - fx) * 2048.f) | ((int)rintf(fx * 2048.f) << 16);
+            atomicMin(&s_wx0, sx);
+            atomicMax(&s_wx1, sx + 1 < sw ? sx + 1 : sx);
         } else if (tid >= 128 && tid < 128 + LP_H) {
             const int r = tid - 128;
             const double scale_y = 1. / ((double)h / sh);
@@ -233,28 +248,64 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             fy -= sy;
             ys0[r] = sy;
             yb[r] = (int)rintf((1.f - fy) * 2048.f) | ((int)rintf(fy * 2048.f) << 16);
+            atomicMin(&s_wy0, min(max(sy, 0), sh - 1));
+            atomicMax(&s_wy1, min(max(sy + 1, 0), sh - 1));
         }
         __syncthreads();
-        for (int e = tid; e < LP_H * LP_WD; e += 256) {
-            const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
-            const int sy = ys0[ey], b0 = yb[ey] & 0xFFFF, b1 = yb[ey] >> 16;
-            const uint8_t* p0 = P.lvl + (size_t)min(max(sy, 0), sh - 1) * sp;
-            const uint8_t* p1 = P.lvl + (size_t)min(max(sy + 1, 0), sh - 1) * sp;
-            unsigned d = 0;
-            // All terms are non-negative, so the rounding shift is done
-            // unsigned.  (A signed shift with a [0, 255] clamp of lanes 0 and
-            // 1 is lowered by the gfx950 backend to v_ashr_pk_u8_i32, whose
-            // upper result half then leaked into bytes 2 and 3 of d.)
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int sx = xs0[ex + k], a0 = xa[ex + k] & 0xFFFF, a1 = xa[ex + k] >> 16;
-                const int nx = sx + 1 < sw ? sx + 1 : sx;
-                const unsigned S0 = p0[sx] * a0 + p0[nx] * a1;
-                const unsigned S1 = p1[sx] * a0 + p1[nx] * a1;
-                const unsigned v = (S0 * b0 + S1 * b1 + (1u << 21)) >> 22;
-                d |= min(v, 255u) << (8 * k);
+        // The source rectangle the tile reads, in whole aligned dwords
+        // (level rows start 64-byte aligned, so no read leaves its row's
+        // pitch).  It is staged in LDS when it fits; otherwise (scale factors
+        // near 2 and above) the taps are read from global memory directly.
+        const int wx0 = s_wx0 & ~3, wy0 = s_wy0;
+        const int nwd = (s_wx1 >> 2) - (wx0 >> 2) + 1, nwr = s_wy1 - wy0 + 1;
+        if (nwd * nwr <= LW_MAX) {
+            const unsigned* srow = (const unsigned*)(P.lvl + (size_t)wy0 * sp + wx0);
+            for (int e = tid; e < nwd * nwr; e += 256) {
+                const int r = e / nwd, c = e - r * nwd;
+                u.win[e] = srow[(size_t)r * (sp >> 2) + c];
             }
-            pxw[ey][ex >> 2] = d;
+            __syncthreads();
+            const uint8_t* wb = (const uint8_t*)u.win;
+            const int wstride = nwd * 4;
+            for (int e = tid; e < LP_H * LP_WD; e += 256) {
+                const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
+                const int sy = ys0[ey], b0 = yb[ey] & 0xFFFF, b1 = yb[ey] >> 16;
+                const uint8_t* p0 = wb + (min(max(sy, 0), sh - 1) - wy0) * wstride - wx0;
+                const uint8_t* p1 = wb + (min(max(sy + 1, 0), sh - 1) - wy0) * wstride - wx0;
+                unsigned d = 0;
+                // All terms are non-negative, so the rounding shift is done
+                // unsigned.  (A signed shift with a [0, 255] clamp of lanes 0
+                // and 1 is lowered by the gfx950 backend to v_ashr_pk_u8_i32,
+                // whose upper result half then leaked into bytes 2 and 3.)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int sx = xs0[ex + k], a0 = xa[ex + k] & 0xFFFF, a1 = xa[ex + k] >> 16;
+                    const int nx = sx + 1 < sw ? sx + 1 : sx;
+                    const unsigned S0 = p0[sx] * a0 + p0[nx] * a1;
+                    const unsigned S1 = p1[sx] * a0 + p1[nx] * a1;
+                    const unsigned v = (S0 * b0 + S1 * b1 + (1u << 21)) >> 22;
+                    d |= min(v, 255u) << (8 * k);
+                }
+                pxw[ey][ex >> 2] = d;
+            }
+        } else {
+            for (int e = tid; e < LP_H * LP_WD; e += 256) {
+                const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
+                const int sy = ys0[ey], b0 = yb[ey] & 0xFFFF, b1 = yb[ey] >> 16;
+                const uint8_t* p0 = P.lvl + (size_t)min(max(sy, 0), sh - 1) * sp;
+                const uint8_t* p1 = P.lvl + (size_t)min(max(sy + 1, 0), sh - 1) * sp;
+                unsigned d = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int sx = xs0[ex + k], a0 = xa[ex + k] & 0xFFFF, a1 = xa[ex + k] >> 16;
+                    const int nx = sx + 1 < sw ? sx + 1 : sx;
+                    const unsigned S0 = p0[sx] * a0 + p0[nx] * a1;
+                    const unsigned S1 = p1[sx] * a0 + p1[nx] * a1;
+                    const unsigned v = (S0 * b0 + S1 * b1 + (1u << 21)) >> 22;
+                    d |= min(v, 255u) << (8 * k);
+                }
+                pxw[ey][ex >> 2] = d;
+            }
         }
     }
     __syncthreads();
@@ -293,7 +344,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                                      __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 3), T2, 0u, false),
                                      false);
         o.w = __builtin_amdgcn_udot4(D1, T1, __builtin_amdgcn_udot4(D2, T2, 0u, false), false);
-        hs[rr][c] = o;
+        u.hs[rr][c] = o;
     }
     __syncthreads();
     {
@@ -303,7 +354,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             if (y >= h) continue;
             uint4 q[7];
 #pragma unroll
-            for (int i = 0; i < 7; i++) q[i] = hs[r + i][gc];
+            for (int i = 0; i < 7; i++) q[i] = u.hs[r + i][gc];
             unsigned out = 0;
 #define ORB_VBLUR(comp, k)                                                                              \
             {                                                                                            \
@@ -323,6 +374,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             }
         }
     }
+    __syncthreads();                              // hs is reused below
 
     // ---- 3. FAST strength on the tile plus a 1-px ring, detection areas
     // only.  The cardinal pre-test runs on 4-pixel groups (two pixels per
@@ -334,7 +386,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const int ay1 = min(ORB_EDGE + L.nRows * L.hCell, L.maxBorderY - 3);
     const unsigned t2 = (unsigned)tmin | ((unsigned)tmin << 16);
     // groups: rows tile -1..32, 4-column groups 0..15; plus the ring columns
-    // tile -1 and 64 (16 extra slots per row, 2 used)
+    // tile -1 and 64 (slot 16 of each row)
     for (int e0 = 0; e0 < LM_H * 17; e0 += 256) {
         const int e = e0 + tid;
         unsigned cand = 0u;                       // bit k: pixel k of the group
@@ -388,14 +440,14 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             base = __shfl(base, 0, 64);
             if (ck) {
                 const int col = c < 16 ? 4 + 4 * c + k : (k == 0 ? 3 : 4 + LT_W);
-                queue[base + lanes_below(bal)] = (unsigned short)((my << 8) | col);
+                u.f.queue[base + lanes_below(bal)] = (unsigned short)((my << 8) | col);
             }
         }
     }
     __syncthreads();
     const int qn = s_qn;
     for (int q = tid; q < qn; q += 256) {
-        const int e = queue[q];
+        const int e = u.f.queue[q];
         const int my = e >> 8, col = e & 255;
         mm[my * LP_W + col] = (uint8_t)fast_strength(px + (my + 3) * LP_W + col, LP_W);
     }
@@ -434,24 +486,22 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 keepB = keepB && sB > (mq > tB ? mq - 1 : 0);
             }
             if (!keepA && !keepB) continue;
-            const unsigned xl = (unsigned)(x - (ORB_MIN_BORDER + cj * L.wCell));
-            const unsigned yl = (unsigned)(y - (ORB_MIN_BORDER + ci * L.hCell));
-            const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | (yl << 6) | xl;
-            const unsigned long long key = ((unsigned long long)(unsigned)(mc - 1) << 32) | ok;
+            const unsigned loc = ((unsigned)ly << 6) | (unsigned)lx | ((unsigned)mc << 11);
             if (keepA) {
                 L.cellflag[ci * L.nCols + cj] = 1;
                 const int i = atomicAdd(&s_kn, 1);
-                if (i < LK_CAP) kbuf[i] = key;
+                if (i < LK_CAP) u.f.kloc[i] = loc;
             }
             if (keepB) {
                 const int i = atomicAdd(&s_kn, 1);
-                if (i < LK_CAP) kbuf[i] = key | ORB_KEY_RETRY;
+                if (i < LK_CAP) u.f.kloc[i] = loc | (1u << 19);
             }
         }
     }
     __syncthreads();
 
-    // ---- 5. append the tile's candidates to the level list
+    // ---- 5. append the tile's candidates to the level list as
+    // (score << 32 | cell row, cell col, y, x in cell) [| retry]
     const int kn = s_kn;
     if (kn == 0) return;
     if (tid == 0) {
@@ -460,8 +510,19 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     }
     __syncthreads();
     const int base = s_base;
-    for (int i = tid; i < min(kn, LK_CAP); i += 256)
-        if (base + i < L.key_cap) L.keys[base + i] = kbuf[i];
+    for (int i = tid; i < min(kn, LK_CAP); i += 256) {
+        if (base + i >= L.key_cap) break;
+        const unsigned loc = u.f.kloc[i];
+        const int x = tx0 + (int)(loc & 63u), y = ty0 + (int)((loc >> 6) & 31u);
+        const int mc = (int)((loc >> 11) & 255u);
+        const int cj = (x - ORB_EDGE) / L.wCell, ci = (y - ORB_EDGE) / L.hCell;
+        const unsigned xl = (unsigned)(x - (ORB_MIN_BORDER + cj * L.wCell));
+        const unsigned yl = (unsigned)(y - (ORB_MIN_BORDER + ci * L.hCell));
+        const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | (yl << 6) | xl;
+        unsigned long long key = ((unsigned long long)(unsigned)(mc - 1) << 32) | ok;
+        if (loc & (1u << 19)) key |= ORB_KEY_RETRY;
+        L.keys[base + i] = key;
+    }
 }
 
 // ================================================================ octree
