@@ -142,7 +142,11 @@ int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* i
  * and cameras.  Images are written into the slots' device buffers (resident
  * in HBM), then orb_batch_run enqueues the whole pipeline:
  *   mode 0  one launch per stage covering every slot
- *   mode 1  the mode-0 launches captured once into a hipGraph and replayed
+ *   mode 1  the mode-0 launches captured once into a hipGraph and replayed;
+ *           with orb_batch_set_branches(b, k > 1) the slots are split into k
+ *           groups (whole stereo pairs) whose launches form k independent
+ *           graph branches, so one group's small, short launches overlap
+ *           another's large ones
  *   mode 2  per-image launches of the same kernels (the unbatched baseline)
  * Stereo pairs declared with orb_batch_set_stereo are matched after
  * extraction (Frame::ComputeStereoMatches, src/Frame.cc:831-1082). */
@@ -158,6 +162,10 @@ int orb_batch_set_stereo(orb_batch* b, int npairs, const int* left_slots, const 
                          float baseline);
 int orb_batch_run(orb_batch* b, int mode);
 int orb_batch_synchronize(orb_batch* b);
+/* Number of graph branches for mode 1 (1..8, default 1).  Slots are cut into
+ * contiguous groups of near-equal size that keep every stereo pair whole; if
+ * no such cut exists the graph has fewer branches.  Results are identical. */
+int orb_batch_set_branches(orb_batch* b, int branches);
 /* Stage timing with HIP events on the batch's stream, averaged over `steps`
  * mode-0 runs.  ms receives orb_batch_stage_count() floats: pyramid levels
  * 0..L-1 (pyramid, blur, FAST with per-cell suppression), quadtree, describe,

@@ -66,6 +66,7 @@ SIGNATURES = {
     "orb_batch_upload": (_I, [_P, _I, _P, _S]),
     "orb_batch_set_stereo": (_I, [_P, _I, _P, _P, _F, _F]),
     "orb_batch_run": (_I, [_P, _I]),
+    "orb_batch_set_branches": (_I, [_P, _I]),
     "orb_batch_synchronize": (_I, [_P]),
     "orb_batch_stage_count": (_I, [_P]),
     "orb_batch_stage_times": (_I, [_P, _I, _P]),
@@ -385,6 +386,10 @@ class Batch:
         _check("orb_batch_set_stereo", lib().orb_batch_set_stereo(self.h, len(l), _ptr(l), _ptr(r), float(bf),
                                                                   float(baseline)))
         self.npairs = len(l)
+
+    def set_branches(self, branches):
+        """Split the mode-1 graph into this many independent branches."""
+        _check("orb_batch_set_branches", lib().orb_batch_set_branches(self.h, int(branches)))
 
     def run(self, mode=GRAPH):
         _check("orb_batch_run", lib().orb_batch_run(self.h, int(mode)))

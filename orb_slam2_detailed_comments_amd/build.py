@@ -5,6 +5,9 @@ Flags:
                       written, on the host and on the device
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       no packed FP32 instructions in device code
+  -mllvm -amdgpu-mfma-vgpr-form
+                      MFMA accumulators in VGPRs, so the Hamming kernel's
+                      epilogue reads them without v_accvgpr_read copies
 """
 import os
 import subprocess
@@ -29,6 +32,7 @@ def hipcc():
 def command(out=LIB):
     return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-amdgpu-mfma-vgpr-form",
             "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC, "-o", out] + \
            [os.path.join(CSRC, s) for s in SOURCES]
 

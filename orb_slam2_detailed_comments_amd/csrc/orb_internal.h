@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "orb_gpu.h"
@@ -14,7 +15,7 @@
 extern "C" hipError_t orb_upload_constants(const int* umax16, const int* gauss7);
 extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable* d_rt, const RunTable* h_rt,
                                          hipStream_t st, hipEvent_t* ev);
-extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, int nwaves, hipStream_t st,
+extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, int max_left, hipStream_t st,
                                         hipEvent_t* ev);
 extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const uint8_t* t, const int* nt,
                                           int th_low, float ratio, int* idx, int* dist, int cap_q,
@@ -141,11 +142,10 @@ struct orb_extractor {
     // stereo state (this extractor as the left view)
     float* d_uR = nullptr;
     float* d_depth = nullptr;
-    int* d_msad = nullptr;
-    int* d_midx = nullptr;
-    int* d_misc = nullptr;        // [0] match count, [1] nl, [2] nr
-    int* d_sidx = nullptr;        // right keypoints sorted by y (stereo scratch)
-    float* d_sy = nullptr;
+    int* d_msad = nullptr;        // per left keypoint: SAD of its stereo match
+    int* d_misc = nullptr;        // [1] nl, [2] nr of caller-supplied stereo inputs
+    float4* d_srec = nullptr;     // right keypoints sorted by y (stereo scratch) ...
+    uint8_t* d_sdesc = nullptr;   // ... and their descriptors
     int* d_rows = nullptr;        // per-row start into the sorted right keypoints
     OrbKeyPoint* d_kin = nullptr; // uploaded keypoints: left then right
     uint8_t* d_din = nullptr;
@@ -284,9 +284,9 @@ struct orb_extractor {
         const size_t o_kps = take((size_t)p.out_cap * sizeof(OrbKeyPoint));
         const size_t o_desc = take((size_t)p.out_cap * 32);
         const size_t o_uR = take((size_t)p.out_cap * 4), o_dep = take((size_t)p.out_cap * 4);
-        const size_t o_msad = take((size_t)p.out_cap * 4), o_midx = take((size_t)p.out_cap * 4);
+        const size_t o_msad = take((size_t)p.out_cap * 4);
         const size_t o_misc = take(64);
-        const size_t o_sidx = take((size_t)p.out_cap * 4), o_sy = take((size_t)p.out_cap * 4);
+        const size_t o_srec = take((size_t)p.out_cap * 16), o_sdesc = take((size_t)p.out_cap * 32);
         const size_t o_rows = take(((size_t)H + 2) * 4);
         const size_t total = off;
         if (total > mem_bytes) {
@@ -345,10 +345,9 @@ struct orb_extractor {
         d_uR = (float*)(d_mem + o_uR);
         d_depth = (float*)(d_mem + o_dep);
         d_msad = (int*)(d_mem + o_msad);
-        d_midx = (int*)(d_mem + o_midx);
         d_misc = (int*)(d_mem + o_misc);
-        d_sidx = (int*)(d_mem + o_sidx);
-        d_sy = (float*)(d_mem + o_sy);
+        d_srec = (float4*)(d_mem + o_srec);
+        d_sdesc = (uint8_t*)(d_mem + o_sdesc);
         d_rows = (int*)(d_mem + o_rows);
         if (!d_job && hipMalloc(&d_job, sizeof(ImageJob)) != hipSuccess) return ORB_E_OOM;
         if (!d_rt && hipMalloc(&d_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
@@ -473,10 +472,8 @@ struct orb_extractor {
         a.uRight = d_uR;
         a.depth = d_depth;
         a.match_sad = d_msad;
-        a.match_idx = d_midx;
-        a.match_count = d_misc;
-        a.sorted_idx = d_sidx;
-        a.sorted_y = d_sy;
+        a.srec = d_srec;
+        a.sdesc = d_sdesc;
         a.row_start = d_rows;
         a.cap_r = std::min(plan.out_cap, r->plan.out_cap);
         float band = 0.f;

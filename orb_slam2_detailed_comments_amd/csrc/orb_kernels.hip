@@ -54,10 +54,30 @@ __device__ __forceinline__ int find_segment(const int* starts, int n, int x)
     return lo;
 }
 
+// Sum over the 64 lanes of a wavefront (every lane must be active): DPP
+// quad swaps and row mirrors give each 16-lane row its total, then the four
+// row totals are read into scalar registers.  Returns the same value in
+// every lane.
 __device__ __forceinline__ int wave_sum(int v)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    v += __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false);   // row_mirror
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+           __builtin_amdgcn_readlane(v, 48);
+}
+
+// The dispatcher sends workgroup b to XCD b % 8, and each XCD has its own
+// L2.  This bijection on [0, n) hands XCD x the contiguous logical range
+// [x*q + min(x, r), ...) (q = n / 8, r = n % 8), so neighbouring tiles, whose
+// halos share cache lines, and the keypoints of one image meet in one L2.
+#define ORB_NUM_XCD 8
+__device__ __forceinline__ int xcd_block(int b, int n)
+{
+    const int q = n / ORB_NUM_XCD, r = n % ORB_NUM_XCD;
+    const int x = b % ORB_NUM_XCD, i = b / ORB_NUM_XCD;
+    return x * q + min(x, r) + i;
 }
 
 __device__ __forceinline__ int lanes_below(unsigned long long mask)
@@ -160,6 +180,28 @@ __device__ __forceinline__ unsigned fast_maybe2(unsigned v, unsigned p0, unsigne
 __device__ __forceinline__ unsigned lo_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C020C00u); }
 __device__ __forceinline__ unsigned hi_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C030C01u); }
 
+// Four adjacent pixels of cv::resize INTER_LINEAR 8U from source rows p0,
+// p1: horizontal 11-bit taps (a0 | a1 << 16) per column, vertical taps
+// yb = b0 | b1 << 16, rounding shift by 22.  All terms are non-negative, so
+// the shift is done unsigned.  (A signed shift with a [0, 255] clamp of lanes
+// 0 and 1 is lowered by the gfx950 backend to v_ashr_pk_u8_i32, whose upper
+// result half then leaked into bytes 2 and 3.)
+__device__ __forceinline__ unsigned resize4(const uint8_t* p0, const uint8_t* p1, const int* sxk, const int* nxk,
+                                            const int* tap, int yb)
+{
+    const unsigned b0 = (unsigned)yb & 0xFFFFu, b1 = (unsigned)yb >> 16;
+    unsigned d = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const unsigned a0 = (unsigned)tap[k] & 0xFFFFu, a1 = (unsigned)tap[k] >> 16;
+        const unsigned S0 = p0[sxk[k]] * a0 + p0[nxk[k]] * a1;
+        const unsigned S1 = p1[sxk[k]] * a0 + p1[nxk[k]] * a1;
+        const unsigned v = (S0 * b0 + S1 * b1 + (1u << 21)) >> 22;
+        d |= min(v, 255u) << (8 * k);
+    }
+    return d;
+}
+
 // LDS that the phases use one after another: the resize source window
 // (phase 1), the horizontal blur sums (phase 2), then the FAST queue and the
 // tile's candidate list (phases 3-5).
@@ -180,14 +222,16 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     __shared__ unsigned mmw[LM_H][LP_WD];         // FAST strength, tile + 1-px ring
     __shared__ LevelScratch u;
     __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
-    __shared__ int s_job, s_qn, s_kn, s_base, s_wx0, s_wx1, s_wy0, s_wy1;
+    __shared__ int s_job, s_blk, s_qn, s_nq, s_kn, s_base, s_wx0, s_wx1, s_wy0, s_wy1;
     const uint8_t* px = (const uint8_t*)pxw;
     uint8_t* mm = (uint8_t*)mmw;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     if (tid == 0) {
-        s_job = find_segment(rt->tile_start[level], rt->njobs, blockIdx.x);
+        s_blk = xcd_block(blockIdx.x, gridDim.x);
+        s_job = find_segment(rt->tile_start[level], rt->njobs, s_blk);
         s_qn = 0;
+        s_nq = 0;
         s_kn = 0;
         s_wx0 = s_wy0 = 0x7FFFFFFF;
         s_wx1 = s_wy1 = -1;
@@ -196,7 +240,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     __syncthreads();
     const ImageJob& J = jobs[s_job];
     const LevelJob& L = J.lv[level];
-    const int t = blockIdx.x - rt->tile_start[level][s_job];
+    const int t = s_blk - rt->tile_start[level][s_job];
     const int tx0 = (t % L.tiles_x) * LT_W;
     const int ty0 = (t / L.tiles_x) * LT_H;
     const int w = L.w, h = L.h, pitch = L.pitch;
@@ -258,53 +302,42 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // near 2 and above) the taps are read from global memory directly.
         const int wx0 = s_wx0 & ~3, wy0 = s_wy0;
         const int nwd = (s_wx1 >> 2) - (wx0 >> 2) + 1, nwr = s_wy1 - wy0 + 1;
-        if (nwd * nwr <= LW_MAX) {
+        // Each of 252 threads owns one 4-column group (taps kept in
+        // registers) and rows r0, r0 + 14, r0 + 28 of the tile.
+        const bool staged = nwd <= 64 && nwd * nwr <= LW_MAX;
+        if (staged) {
             const unsigned* srow = (const unsigned*)(P.lvl + (size_t)wy0 * sp + wx0);
-            for (int e = tid; e < nwd * nwr; e += 256) {
-                const int r = e / nwd, c = e - r * nwd;
-                u.win[e] = srow[(size_t)r * (sp >> 2) + c];
-            }
+            const int lg = nwd <= 16 ? 4 : (nwd <= 32 ? 5 : 6);     // threads per row: 2^lg >= nwd
+            const int tc = tid & ((1 << lg) - 1);
+            for (int r = tid >> lg; r < nwr; r += 256 >> lg)
+                if (tc < nwd) u.win[r * nwd + tc] = srow[(size_t)r * (sp >> 2) + tc];
             __syncthreads();
-            const uint8_t* wb = (const uint8_t*)u.win;
-            const int wstride = nwd * 4;
-            for (int e = tid; e < LP_H * LP_WD; e += 256) {
-                const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
-                const int sy = ys0[ey], b0 = yb[ey] & 0xFFFF, b1 = yb[ey] >> 16;
-                const uint8_t* p0 = wb + (min(max(sy, 0), sh - 1) - wy0) * wstride - wx0;
-                const uint8_t* p1 = wb + (min(max(sy + 1, 0), sh - 1) - wy0) * wstride - wx0;
-                unsigned d = 0;
-                // All terms are non-negative, so the rounding shift is done
-                // unsigned.  (A signed shift with a [0, 255] clamp of lanes 0
-                // and 1 is lowered by the gfx950 backend to v_ashr_pk_u8_i32,
-                // whose upper result half then leaked into bytes 2 and 3.)
+        }
+        if (tid < 14 * LP_WD) {
+            const int c = tid % LP_WD, r0 = tid / LP_WD;
+            const int xoff = staged ? wx0 : 0;
+            int sxk[4], nxk[4], tap[4];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int sx = xs0[ex + k], a0 = xa[ex + k] & 0xFFFF, a1 = xa[ex + k] >> 16;
-                    const int nx = sx + 1 < sw ? sx + 1 : sx;
-                    const unsigned S0 = p0[sx] * a0 + p0[nx] * a1;
-                    const unsigned S1 = p1[sx] * a0 + p1[nx] * a1;
-                    const unsigned v = (S0 * b0 + S1 * b1 + (1u << 21)) >> 22;
-                    d |= min(v, 255u) << (8 * k);
-                }
-                pxw[ey][ex >> 2] = d;
+            for (int k = 0; k < 4; k++) {
+                const int sx = xs0[4 * c + k];
+                sxk[k] = sx - xoff;
+                nxk[k] = (sx + 1 < sw ? sx + 1 : sx) - xoff;
+                tap[k] = xa[4 * c + k];
             }
-        } else {
-            for (int e = tid; e < LP_H * LP_WD; e += 256) {
-                const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
-                const int sy = ys0[ey], b0 = yb[ey] & 0xFFFF, b1 = yb[ey] >> 16;
-                const uint8_t* p0 = P.lvl + (size_t)min(max(sy, 0), sh - 1) * sp;
-                const uint8_t* p1 = P.lvl + (size_t)min(max(sy + 1, 0), sh - 1) * sp;
-                unsigned d = 0;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int sx = xs0[ex + k], a0 = xa[ex + k] & 0xFFFF, a1 = xa[ex + k] >> 16;
-                    const int nx = sx + 1 < sw ? sx + 1 : sx;
-                    const unsigned S0 = p0[sx] * a0 + p0[nx] * a1;
-                    const unsigned S1 = p1[sx] * a0 + p1[nx] * a1;
-                    const unsigned v = (S0 * b0 + S1 * b1 + (1u << 21)) >> 22;
-                    d |= min(v, 255u) << (8 * k);
+            if (staged) {
+                const uint8_t* wb = (const uint8_t*)u.win;
+                const int ws = nwd * 4;
+                for (int ey = r0; ey < LP_H; ey += 14) {
+                    const int sy = ys0[ey];
+                    pxw[ey][c] = resize4(wb + (min(max(sy, 0), sh - 1) - wy0) * ws,
+                                         wb + (min(max(sy + 1, 0), sh - 1) - wy0) * ws, sxk, nxk, tap, yb[ey]);
                 }
-                pxw[ey][ex >> 2] = d;
+            } else {
+                for (int ey = r0; ey < LP_H; ey += 14) {
+                    const int sy = ys0[ey];
+                    pxw[ey][c] = resize4(P.lvl + (size_t)min(max(sy, 0), sh - 1) * sp,
+                                         P.lvl + (size_t)min(max(sy + 1, 0), sh - 1) * sp, sxk, nxk, tap, yb[ey]);
+                }
             }
         }
     }
@@ -445,25 +478,41 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         }
     }
     __syncthreads();
+    // Exact strengths of the queued pixels.  Pixels of the tile proper that
+    // beat the lower threshold are compacted again, in place (a slot is only
+    // rewritten after the barrier that ends its iteration's reads), so that
+    // suppression runs on dense lanes.
     const int qn = s_qn;
-    for (int q = tid; q < qn; q += 256) {
-        const int e = u.f.queue[q];
-        const int my = e >> 8, col = e & 255;
-        mm[my * LP_W + col] = (uint8_t)fast_strength(px + (my + 3) * LP_W + col, LP_W);
+    for (int q0 = 0; q0 < qn; q0 += 256) {
+        const int q = q0 + tid;
+        int e = 0;
+        bool keep = false;
+        if (q < qn) {
+            e = u.f.queue[q];
+            const int my = e >> 8, col = e & 255;
+            const int sv = fast_strength(px + (my + 3) * LP_W + col, LP_W);
+            mm[my * LP_W + col] = (uint8_t)sv;
+            keep = sv > tmin && my >= 1 && my <= LT_H && col >= 4 && col < 4 + LT_W;
+        }
+        __syncthreads();
+        const unsigned long long bal = __ballot(keep);
+        if (bal) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&s_nq, __popcll(bal));
+            base = __shfl(base, 0, 64);
+            if (keep) u.f.queue[base + lanes_below(bal)] = (unsigned short)e;
+        }
     }
     __syncthreads();
 
-    // ---- 4. per-cell non-max suppression at both thresholds; 4-pixel
-    // groups whose strengths are all <= min threshold are skipped at once.
-    for (int e = tid; e < LT_H * (LT_W / 4); e += 256) {
-        const int ly = e >> 4, c = e & 15;
-        const unsigned grp = mmw[ly + 1][c + 1];
-        if (grp == 0u) continue;
-        for (int k = 0; k < 4; k++) {
-            const int lx = 4 * c + k;
-            const int x = tx0 + lx, y = ty0 + ly;
-            const int mc = (grp >> (8 * k)) & 255;
-            if (mc <= tmin || x >= ax1 || y >= ay1 || x < ax0 || y < ay0) continue;
+    // ---- 4. per-cell non-max suppression at both thresholds
+    const int nq = s_nq;
+    for (int q = tid; q < nq; q += 256) {
+        const int e = u.f.queue[q];
+        const int ly = (e >> 8) - 1, lx = (e & 255) - 4;
+        const int x = tx0 + lx, y = ty0 + ly;
+        const int mc = mm[(ly + 1) * LP_W + lx + 4];
+        {
             const int cj = (x - ORB_EDGE) / L.wCell, ci = (y - ORB_EDGE) / L.hCell;
             const int cx0 = max(ORB_EDGE + cj * L.wCell, ax0), cx1 = min(ORB_EDGE + (cj + 1) * L.wCell, ax1);
             const int cy0 = max(ORB_EDGE + ci * L.hCell, ay0), cy1 = min(ORB_EDGE + (ci + 1) * L.hCell, ay1);
@@ -963,9 +1012,11 @@ __device__ __forceinline__ void sincos_canonical_dev(double x, double* s, double
 __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restrict__ jobs,
                                                        const RunTable* __restrict__ rt)
 {
+    __shared__ unsigned s_brief[4][37 * 10];      // per wavefront: 37 rows x 40 bytes
     const int lane = threadIdx.x & 63;
     const int nseg = rt->njobs * rt->max_levels;
-    const int f = find_segment(rt->desc_start, nseg, blockIdx.x);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int f = find_segment(rt->desc_start, nseg, blk);
     const int jb = f / rt->max_levels, level = f % rt->max_levels;
     const ImageJob& J = jobs[jb];
     const int* ctr = J.counters;
@@ -975,7 +1026,7 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
         base += l < level ? c : 0;
         total += c;
     }
-    const int idx = (blockIdx.x - rt->desc_start[f]) * 4 + (threadIdx.x >> 6);
+    const int idx = (blk - rt->desc_start[f]) * 4 + (threadIdx.x >> 6);
     if (level == 0 && idx == 0 && lane == 0) J.counters[ORB_CNT_NOUT] = total;
     if (level >= J.nlevels || idx >= ctr[ORB_CNT_RET + level]) return;
     const LevelJob& L = J.lv[level];
@@ -985,47 +1036,80 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     const int cx = xr + ORB_MIN_BORDER, cy = yr + ORB_MIN_BORDER;
     const int pitch = L.pitch;
 
-    // IC_Angle (ORBextractor.cc:94-141): m10 = sum u*I, m01 = sum v*I over
-    // the disc |u| <= umax[|v|].  Lane = column u (lane 31 idle), the two
-    // half-waves take rows 0..15 and -15..-1; all 16 loads are issued before
-    // use (every address is inside the level: keypoints are >= 19 px from
-    // its edges).
-    const uint8_t* img = L.lvl + (size_t)cy * pitch + cx;
-    const int u = (lane & 31) - 15;
-    const int au = u < 0 ? -u : u;
-    const int vb = lane < 32 ? 0 : -16;
-    int px[16];
+    // Both patches are read with aligned dword loads issued together, before
+    // any use: the 31x31 orientation patch of the level and the 37x37 window
+    // of the blurred level that every rotated BRIEF sample falls in (pattern
+    // radius <= 18.39 px).  Keypoints lie >= 19 px inside the level, so every
+    // dword is inside the level's rows (at most 3 bytes past a row's last
+    // pixel, still inside the image allocation).
+    const int xa0 = (cx - 15) & ~3, nad = ((cx + 15) >> 2) - (xa0 >> 2) + 1;     // <= 9 dwords/row
+    const int xb0 = (cx - 18) & ~3, nbd = ((cx + 18) >> 2) - (xb0 >> 2) + 1;     // <= 10 dwords/row
+    unsigned pa[5], pb[6];
 #pragma unroll
-    for (int t = 0; t < 16; t++) px[t] = img[(vb + t) * pitch + u];
+    for (int t = 0; t < 5; t++) {
+        const int sl = lane + 64 * t;
+        const int r = sl / nad, c = sl - r * nad;
+        pa[t] = sl < 31 * nad ? *(const unsigned*)(L.lvl + (size_t)(cy - 15 + r) * pitch + xa0 + 4 * c) : 0u;
+    }
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        const int sl = lane + 64 * t;
+        const int r = sl / nbd, c = sl - r * nbd;
+        pb[t] = sl < 37 * nbd ? *(const unsigned*)(L.blur + (size_t)(cy - 18 + r) * pitch + xb0 + 4 * c) : 0u;
+    }
+    unsigned* win = s_brief[threadIdx.x >> 6];
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        const int sl = lane + 64 * t;
+        const int r = sl / nbd, c = sl - r * nbd;
+        if (sl < 37 * nbd) win[r * 10 + c] = pb[t];
+    }
+
+    // IC_Angle (ORBextractor.cc:94-141): m10 = sum u*I, m01 = sum v*I over
+    // the disc |u| <= umax[|v|], from the dwords each lane holds.
+    unsigned long long umt = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) umt |= (unsigned long long)c_umax[i] << (4 * i);
     int m10 = 0, m01 = 0;
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-        const int v = vb + t;
-        const int av = v < 0 ? -v : v;
-        const bool in = (lane & 31) < 31 && v >= -15 && au <= c_umax[av < 16 ? av : 15];
-        const int val = in ? px[t] : 0;
-        m10 += u * val;
-        m01 += v * val;
+    for (int t = 0; t < 5; t++) {
+        const int sl = lane + 64 * t;
+        const int r = sl / nad, c = sl - r * nad;
+        const int v = r - 15, av = v < 0 ? -v : v;
+        const int um = (int)((umt >> (4 * (av & 15))) & 15u);
+        const int u0 = xa0 + 4 * c - cx;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int uu = u0 + k;
+            const int val = (int)((pa[t] >> (8 * k)) & 255u);
+            const bool in = sl < 31 * nad && uu >= -um && uu <= um;
+            m10 += in ? uu * val : 0;
+            m01 += in ? v * val : 0;
+        }
     }
     m10 = wave_sum(m10);
     m01 = wave_sum(m01);
     const float angle_deg = fast_atan2_dev((float)m01, (float)m10);
 
-    // Rotated BRIEF (ORBextractor.cc:153-204) on the blurred level.
+    // Rotated BRIEF (ORBextractor.cc:153-204) on the blurred level, sampled
+    // from the staged window (row stride 40 bytes, centre at row 18).
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float factorPI = (float)(3.1415926535897932384626433832795 / 180.f);
     const float angle = angle_deg * factorPI;
     double sd, cd;
     sincos_canonical_dev((double)angle, &sd, &cd);
     const float a = (float)cd, b = (float)sd;
-    const uint8_t* center = L.blur + (size_t)cy * pitch + cx;
+    const uint8_t* center = (const uint8_t*)win + 18 * 40 + (cx - xb0);
     int i0[4], i1[4];
 #pragma unroll
     for (int bit = 0; bit < 4; bit++) {
         const int p = lane * 4 + bit;   // pair index: byte p/8, bit p%8
         const float x0 = (float)c_pattern[4 * p], y0 = (float)c_pattern[4 * p + 1];
         const float x1 = (float)c_pattern[4 * p + 2], y1 = (float)c_pattern[4 * p + 3];
-        i0[bit] = center[(int)rintf(x0 * b + y0 * a) * pitch + (int)rintf(x0 * a - y0 * b)];
-        i1[bit] = center[(int)rintf(x1 * b + y1 * a) * pitch + (int)rintf(x1 * a - y1 * b)];
+        i0[bit] = center[(int)rintf(x0 * b + y0 * a) * 40 + (int)rintf(x0 * a - y0 * b)];
+        i1[bit] = center[(int)rintf(x1 * b + y1 * a) * 40 + (int)rintf(x1 * a - y1 * b)];
     }
     int nib = 0;
 #pragma unroll
@@ -1082,7 +1166,9 @@ __device__ __forceinline__ int hamming32(const uint4 a0, const uint4 a1, const u
 // centre-subtracted patches over 11 shifts, parabola fit, depth = bf/disp.
 
 // Sort each pair's right keypoints by (y, index) (one workgroup per pair,
-// bitonic sort in LDS); the match kernel binary-searches the row band.
+// bitonic sort in LDS) and lay them out in that order as 16-byte records and
+// 32-byte descriptors, so a candidate is two independent loads at its sorted
+// position; plus, per level-0 row, the first entry of its band.
 #define STEREO_SORT_CAP 4096
 __global__ __launch_bounds__(1024) void stereo_prep_kernel(const StereoArgs* __restrict__ args)
 {
@@ -1110,8 +1196,14 @@ __global__ __launch_bounds__(1024) void stereo_prep_kernel(const StereoArgs* __r
             __syncthreads();
         }
     for (int i = threadIdx.x; i < n; i += 1024) {
-        A.sorted_idx[i] = (int)(unsigned)v[i];
-        A.sorted_y[i] = __uint_as_float((unsigned)(v[i] >> 32));
+        const int iR = (int)(unsigned)v[i];
+        const OrbKeyPoint& k = A.kr[iR];
+        A.srec[i] = make_float4(k.y, k.x, __int_as_float(k.octave), __int_as_float(iR));
+    }
+    // descriptors: two 16-byte halves per entry
+    for (int i = threadIdx.x; i < 2 * n; i += 1024) {
+        const int iR = (int)(unsigned)v[i >> 1];
+        ((uint4*)A.sdesc)[i] = ((const uint4*)A.dr)[2 * iR + (i & 1)];
     }
     // Per level-0 row: first sorted entry that can hold the row in its band.
     for (int row = threadIdx.x; row <= A.nrows; row += 1024) {
@@ -1126,28 +1218,19 @@ __global__ __launch_bounds__(1024) void stereo_prep_kernel(const StereoArgs* __r
     }
 }
 
-__device__ __forceinline__ int find_pair(const StereoArgs* args, int n, int x)
+// grid (ceil(max left capacity / 4), pairs), one wavefront per left keypoint
+__global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restrict__ args)
 {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (args[mid].wave_base <= x) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-__global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restrict__ args, int npairs, int nwaves)
-{
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int wid = blockIdx.x * 4 + wv;
-    if (wid >= nwaves) return;
-    const StereoArgs& A = args[find_pair(args, npairs, wid)];
-    const int iL = wid - A.wave_base;
+    const int lane = threadIdx.x & 63;
+    const StereoArgs& A = args[blockIdx.y];
+    const int iL = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int nl = *A.nl, nr = *A.nr;
     if (iL >= nl || iL >= A.cap_l) return;
     float outU = -1.0f, outD = -1.0f;
+    int outSad = -1;
     const OrbKeyPoint kpL = A.kl[iL];
+    const uint4 a0 = *(const uint4*)(A.dl + (size_t)iL * 32);
+    const uint4 a1 = *(const uint4*)(A.dl + (size_t)iL * 32 + 16);
     const int levelL = kpL.octave;
     const float vL = kpL.y, uL = kpL.x;
     const int row = (int)vL;
@@ -1157,70 +1240,70 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     const float minU = uL - maxD;
     const float maxU = uL - minD;
     bool go = row >= 0 && row < A.nrows && !(maxU < 0);
+    // The level both patches are read from (octaves are validated on entry
+    // and produced in range by the extractor); fetched ahead of the search.
+    const LevelJob& LL = A.left->lv[levelL];
+    const LevelJob& LR = A.right->lv[levelL];
+    const uint8_t* pl = LL.lvl;
+    const uint8_t* pr = LR.lvl;
+    const int pitchL = LL.pitch, pitchR = LR.pitch, wL = LL.w, hL = LL.h, wR = LR.w, hR = LR.h;
     int bestDist = 100, bestIdx = INT_MAX;
+    float bestU = 0.f;
     if (go) {
-        const uint4 a0 = *(const uint4*)(A.dl + (size_t)iL * 32);
-        const uint4 a1 = *(const uint4*)(A.dl + (size_t)iL * 32 + 16);
         // Right keypoints sorted by y: every candidate of row `row` has
         // y in (row - 1 - band, row + 1 + band); the exact band, octave and
         // uR tests follow.
         const int n = min(nr, A.cap_r);
         const float yhi = (float)row + 1.0f + A.band;
-        const int lo = A.row_start[row];
-        for (int c = lo + lane; c < n; c += 64) {
-            if (A.sorted_y[c] >= yhi) break;
-            const int iR = A.sorted_idx[c];
-            const OrbKeyPoint kpR = A.kr[iR];
-            const float r = 2.0f * A.scale[kpR.octave];
-            const int maxr = (int)ceilf(kpR.y + r);
-            const int minr = (int)floorf(kpR.y - r);
+        for (int c = A.row_start[row] + lane; c < n; c += 64) {
+            const float4 rec = A.srec[c];
+            if (rec.x >= yhi) break;
+            const int oct = __float_as_int(rec.z), iR = __float_as_int(rec.w);
+            const float r = 2.0f * A.scale[oct];
+            const int maxr = (int)ceilf(rec.x + r);
+            const int minr = (int)floorf(rec.x - r);
             if (row < minr || row > maxr) continue;
-            if (kpR.octave < levelL - 1 || kpR.octave > levelL + 1) continue;
-            const float uR = kpR.x;
+            if (oct < levelL - 1 || oct > levelL + 1) continue;
+            const float uR = rec.y;
             if (uR >= minU && uR <= maxU) {
-                const int dist = hamming32(a0, a1, A.dr + (size_t)iR * 32);
+                const int dist = hamming32(a0, a1, A.sdesc + (size_t)c * 32);
                 if (dist < bestDist || (dist == bestDist && iR < bestIdx)) {
                     bestDist = dist;
                     bestIdx = iR;
+                    bestU = uR;
                 }
             }
         }
         // Smallest (distance, index) across the wavefront.
         for (int o = 32; o > 0; o >>= 1) {
             const int od = __shfl_xor(bestDist, o, 64), oi = __shfl_xor(bestIdx, o, 64);
+            const float ou = __shfl_xor(bestU, o, 64);
             if (od < bestDist || (od == bestDist && oi < bestIdx)) {
                 bestDist = od;
                 bestIdx = oi;
+                bestU = ou;
             }
         }
         go = bestDist < 75 && bestIdx != INT_MAX;
     }
     int sx0 = 0, sy0 = 0, sxR = 0;
     float scaleduR0 = 0.f;
-    const uint8_t *pl = nullptr, *pr = nullptr;
-    int pitchL = 0, pitchR = 0;
     if (go) {
-        const float uR0 = A.kr[bestIdx].x;
+        const float uR0 = bestU;
         const float scaleFactor = A.inv_scale[levelL];
         const float scaleduL = roundf(kpL.x * scaleFactor);
         const float scaledvL = roundf(kpL.y * scaleFactor);
         scaleduR0 = roundf(uR0 * scaleFactor);
-        const LevelJob& LL = A.left->lv[levelL];
-        const LevelJob& LR = A.right->lv[levelL];
         const float iniu = scaleduR0 - 5 - 5;
         const float endu = scaleduR0 + 5 + 5 + 1;
-        if (iniu < 0 || endu >= LR.w) go = false;
+        if (iniu < 0 || endu >= wR) go = false;
         sx0 = (int)scaleduL;
         sy0 = (int)scaledvL;
         sxR = (int)scaleduR0;
         // Keypoints of this extractor always leave room for the 11x11
         // patches; caller-supplied ones are checked instead of read past the
         // level (the reference would index outside the cv::Mat).
-        if (sy0 - 5 < 0 || sy0 + 5 >= LL.h || sy0 + 5 >= LR.h || sx0 - 5 < 0 || sx0 + 5 >= LL.w) go = false;
-        pl = LL.lvl;
-        pr = LR.lvl;
-        pitchL = LL.pitch;
-        pitchR = LR.pitch;
+        if (sy0 - 5 < 0 || sy0 + 5 >= hL || sy0 + 5 >= hR || sx0 - 5 < 0 || sx0 + 5 >= wL) go = false;
     }
     if (go) {
         // 121 (shift, row) tasks over 64 lanes; per task 11 absolute
@@ -1276,9 +1359,7 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
                         }
                         outD = __fdiv_rn(A.bf, disparity);
                         outU = bestuR;
-                        const int m = atomicAdd(A.match_count, 1);
-                        A.match_sad[m] = best;
-                        A.match_idx[m] = iL;
+                        outSad = best;
                     }
                 }
             }
@@ -1287,21 +1368,32 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     if (lane == 0) {
         A.uRight[iL] = outU;
         A.depth[iL] = outD;
+        A.match_sad[iL] = outSad;
     }
 }
 
-// Median-based outlier cut (Frame.cc:1066-1081): sort the matched SAD values,
-// th = 1.5f*1.4f*median, drop matches with SAD >= th.
+// Median cut (Frame.cc:1066-1081): matches whose SAD is not below
+// 1.5 * 1.4 * median are dropped.  One workgroup per pair gathers the kept
+// SADs, sorts them in LDS and clears the rejected keypoints.
 #define MEDIAN_CAP 4096
 __global__ __launch_bounds__(1024) void stereo_median_kernel(const StereoArgs* __restrict__ args)
 {
     __shared__ int v[MEDIAN_CAP];
+    __shared__ int s_m;
     const StereoArgs& A = args[blockIdx.x];
-    const int M = min(*A.match_count, MEDIAN_CAP);
+    const int n = min(min(*A.nl, A.cap_l), MEDIAN_CAP);
+    if (threadIdx.x == 0) s_m = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const int sad = A.match_sad[i];
+        if (sad >= 0) v[atomicAdd(&s_m, 1)] = sad;
+    }
+    __syncthreads();
+    const int M = s_m;
     if (M == 0) return;
     int npow = 1;
     while (npow < M) npow <<= 1;
-    for (int i = threadIdx.x; i < npow; i += 1024) v[i] = i < M ? A.match_sad[i] : INT_MAX;
+    for (int i = M + threadIdx.x; i < npow; i += 1024) v[i] = INT_MAX;
     __syncthreads();
     for (int k = 2; k <= npow; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -1321,11 +1413,11 @@ __global__ __launch_bounds__(1024) void stereo_median_kernel(const StereoArgs* _
     const float median = (float)v[M / 2];
     const float k = 1.5f * 1.4f;
     const float thDist = k * median;
-    for (int i = threadIdx.x; i < M; i += 1024) {
-        if (!((float)A.match_sad[i] < thDist)) {
-            const int iL = A.match_idx[i];
-            A.uRight[iL] = -1;
-            A.depth[iL] = -1;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const int sad = A.match_sad[i];
+        if (sad >= 0 && !((float)sad < thDist)) {
+            A.uRight[i] = -1;
+            A.depth[i] = -1;
         }
     }
 }
@@ -1460,20 +1552,14 @@ extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable*
     return hipGetLastError();
 }
 
-__global__ void zero_match_counts_kernel(const StereoArgs* __restrict__ args, int npairs)
-{
-    for (int p = threadIdx.x; p < npairs; p += blockDim.x) *args[p].match_count = 0;
-}
-
-// d_args: npairs StereoArgs in device memory with wave_base = prefix of cap_l.
+// d_args: npairs StereoArgs in device memory; max_left: the largest cap_l.
 // ev (optional): events after the match kernel and after the median cut.
-extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, int nwaves, hipStream_t st,
+extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, int max_left, hipStream_t st,
                                         hipEvent_t* ev)
 {
     if (npairs <= 0) return hipSuccess;
-    zero_match_counts_kernel<<<1, 256, 0, st>>>(d_args, npairs);
     stereo_prep_kernel<<<npairs, 1024, 0, st>>>(d_args);
-    if (nwaves > 0) stereo_kernel<<<(nwaves + 3) / 4, 256, 0, st>>>(d_args, npairs, nwaves);
+    if (max_left > 0) stereo_kernel<<<dim3((max_left + 3) / 4, npairs), 256, 0, st>>>(d_args);
     if (ev) (void)hipEventRecord(ev[0], st);
     stereo_median_kernel<<<npairs, 1024, 0, st>>>(d_args);
     if (ev) (void)hipEventRecord(ev[1], st);
@@ -1562,6 +1648,14 @@ __device__ __forceinline__ unsigned pm1_dword(unsigned w, int i)
     return __builtin_amdgcn_perm(0x01010101u, 0xFFFFFFFFu, sel);
 }
 
+// The same expansion with the signs flipped (bit set -> -1), used for the
+// map operand so that the MFMA accumulates -dot directly.
+__device__ __forceinline__ unsigned pm1n_dword(unsigned w, int i)
+{
+    const unsigned sel = (i >= 2 ? (w >> (i - 2)) : (w << (2 - i))) & 0x04040404u;
+    return __builtin_amdgcn_perm(0xFFFFFFFFu, 0x01010101u, sel);
+}
+
 typedef int knn_v4i __attribute__((ext_vector_type(4)));
 
 // Workgroup: 4 wavefronts x 64 queries, one map chunk.  Map rows are
@@ -1600,55 +1694,85 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const uint8_t* __restrict
 #pragma unroll
     for (int gq = 0; gq < 4; gq++) k0[gq] = k1[gq] = KNN_NONE;
 
-    // Fill role of this thread: row fr (0..63) of the tile, k-chunk fc.
+    // Fill role of this thread: row fr (0..63) of the tile, k-chunk fc.  The
+    // map operand is expanded with flipped signs, so the accumulator holds
+    // -dot and key = (256 - dot) << 19 | row = (acc << 19) + (256 << 19 | row)
+    // is one shift-add.  Full 64-row tiles run without bounds checks; the
+    // chunk's last, partial tile takes the checked form.
     const int fr = tid >> 2, fc = tid & 3;
+    const int nfull = len & ~63;
     for (int t0 = 0; t0 < len; t0 += 64) {
-        const int n = min(64, len - t0);
+        const bool full = t0 < nfull;
+        const int n = full ? 64 : len - t0;
         __syncthreads();
         {
             uint2 wv2 = make_uint2(0, 0);
-            if (fr < n) wv2 = *(const uint2*)(m + (size_t)(c0 + t0 + fr) * 32 + 8 * fc);
+            if (full || fr < n) wv2 = *(const uint2*)(m + (size_t)(c0 + t0 + fr) * 32 + 8 * fc);
             const int s = fr >> 4, ii = fr & 15;
 #pragma unroll
             for (int gg = 0; gg < 4; gg++) {
                 uint4 v;
                 const int F0 = 16 * fc + 4 * gg;   // dwords F0..F0+3 of word 2fc + gg/2
                 const unsigned w = ((F0 >> 3) & 1) ? wv2.y : wv2.x;
-                v.x = pm1_dword(w, (F0 + 0) & 7);
-                v.y = pm1_dword(w, (F0 + 1) & 7);
-                v.z = pm1_dword(w, (F0 + 2) & 7);
-                v.w = pm1_dword(w, (F0 + 3) & 7);
+                v.x = pm1n_dword(w, (F0 + 0) & 7);
+                v.y = pm1n_dword(w, (F0 + 1) & 7);
+                v.z = pm1n_dword(w, (F0 + 2) & 7);
+                v.w = pm1n_dword(w, (F0 + 3) & 7);
                 A[s][fc][ii + 16 * gg] = v;
             }
         }
         __syncthreads();
+        if (full) {
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            if (16 * s >= n) break;
-            knn_v4i acc[4];
+            for (int s = 0; s < 4; s++) {
+                knn_v4i acc[4];
 #pragma unroll
-            for (int gq = 0; gq < 4; gq++) acc[gq] = (knn_v4i){0, 0, 0, 0};
+                for (int gq = 0; gq < 4; gq++) acc[gq] = (knn_v4i){0, 0, 0, 0};
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const uint4 av = A[s][c][lane];
-                const knn_v4i a = (knn_v4i){(int)av.x, (int)av.y, (int)av.z, (int)av.w};
+                for (int c = 0; c < 4; c++) {
+                    const uint4 av = A[s][c][lane];
+                    const knn_v4i a = (knn_v4i){(int)av.x, (int)av.y, (int)av.z, (int)av.w};
 #pragma unroll
-                for (int gq = 0; gq < 4; gq++)
-                    acc[gq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, B[gq][c], acc[gq], 0, 0, 0);
+                    for (int gq = 0; gq < 4; gq++)
+                        acc[gq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, B[gq][c], acc[gq], 0, 0, 0);
+                }
+                // Accumulator (lane, r) = (map row 16s + 4g + r, query 16gq + i16).
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const unsigned cbase = (256u << 19) + (unsigned)(t0 + 16 * s + 4 * g + r);
+#pragma unroll
+                    for (int gq = 0; gq < 4; gq++) {
+                        const unsigned key = ((unsigned)acc[gq][r] << 19) + cbase;
+                        k1[gq] = med3u(k0[gq], k1[gq], key);
+                        k0[gq] = min(k0[gq], key);
+                    }
+                }
             }
-            // Accumulator (lane, r) = (map row 16s + 4g + r, query 16gq + i16).
-            // key = (256 - dot) << 19 | row = Hamming << 20 | row.
+        } else {
+            for (int s = 0; s < 4; s++) {
+                if (16 * s >= n) break;
+                knn_v4i acc[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = t0 + 16 * s + 4 * g + r;
-                const int cbase = (256 << 19) + row;
-                const bool ok = row < len;
+                for (int gq = 0; gq < 4; gq++) acc[gq] = (knn_v4i){0, 0, 0, 0};
 #pragma unroll
-                for (int gq = 0; gq < 4; gq++) {
-                    const unsigned key = ok ? (unsigned)(__mul24(acc[gq][r], -(1 << 19)) + cbase)
-                                            : KNN_NONE;
-                    k1[gq] = med3u(k0[gq], k1[gq], key);
-                    k0[gq] = min(k0[gq], key);
+                for (int c = 0; c < 4; c++) {
+                    const uint4 av = A[s][c][lane];
+                    const knn_v4i a = (knn_v4i){(int)av.x, (int)av.y, (int)av.z, (int)av.w};
+#pragma unroll
+                    for (int gq = 0; gq < 4; gq++)
+                        acc[gq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, B[gq][c], acc[gq], 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = t0 + 16 * s + 4 * g + r;
+                    const unsigned cbase = (256u << 19) + (unsigned)row;
+                    const bool ok = row < len;
+#pragma unroll
+                    for (int gq = 0; gq < 4; gq++) {
+                        const unsigned key = ok ? ((unsigned)acc[gq][r] << 19) + cbase : KNN_NONE;
+                        k1[gq] = med3u(k0[gq], k1[gq], key);
+                        k0[gq] = min(k0[gq], key);
+                    }
                 }
             }
         }

@@ -377,10 +377,24 @@ struct orb_batch {
     RunTable* h_rt = nullptr;
     RunTable* d_rt = nullptr;
     std::vector<int> pl, pr;
+    std::vector<StereoArgs> h_sargs;
     StereoArgs* d_sargs = nullptr;
-    int nwaves = 0;
+    int nwaves = 0;                // largest left capacity among the pairs
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+
+    // Graph branches (mode 1): contiguous slot groups holding whole pairs.
+    struct Branch {
+        int s = 0, n = 0;                  // slots [s, s + n)
+        std::unique_ptr<RunTable> h_rt;
+        RunTable* d_rt = nullptr;
+        StereoArgs* d_sargs = nullptr;
+        int npairs = 0, nwaves = 0;
+    };
+    int want_branches = 1;
+    std::vector<Branch> br;
+    std::vector<hipStream_t> br_st;        // side streams, default priority
+    std::vector<hipEvent_t> br_ev;         // fork + one join per side stream
 
     // ev (optional, mode 0): stage events, see orb_batch_stage_times.
     int enqueue(int mode, hipEvent_t* ev = nullptr)
@@ -396,17 +410,114 @@ struct orb_batch {
         return ORB_OK;
     }
 
+    // Cut the slots into up to want_branches groups of whole pairs.
+    std::vector<int> cuts() const
+    {
+        const int n = (int)ex.size();
+        std::vector<int> c{0};
+        for (int k = 1; k < want_branches; k++) {
+            const int target = (int)((long long)n * k / want_branches);
+            int best = -1;
+            for (int d = 0; d < n && best < 0; d++)
+                for (int m : {target + d, target - d}) {
+                    if (m <= c.back() || m >= n) continue;
+                    bool ok = true;
+                    for (size_t p = 0; p < pl.size() && ok; p++)
+                        ok = (pl[p] < m) == (pr[p] < m);
+                    if (ok) {
+                        best = m;
+                        break;
+                    }
+                }
+            if (best > c.back()) c.push_back(best);
+        }
+        c.push_back(n);
+        return c;
+    }
+
+    int build_branches()
+    {
+        drop_branches();
+        const std::vector<int> c = cuts();
+        if (c.size() <= 2) return ORB_OK;
+        for (size_t g = 0; g + 1 < c.size(); g++) {
+            Branch B;
+            B.s = c[g];
+            B.n = c[g + 1] - c[g];
+            B.h_rt.reset(new RunTable());
+            orb_extractor::build_table(ex.data() + B.s, B.n, *B.h_rt);
+            std::vector<StereoArgs> sub;
+            for (size_t p = 0; p < pl.size(); p++)
+                if (pl[p] >= B.s && pl[p] < B.s + B.n) {
+                    sub.push_back(h_sargs[p]);
+                    B.nwaves = std::max(B.nwaves, h_sargs[p].cap_l);
+                }
+            B.npairs = (int)sub.size();
+            br.push_back(std::move(B));
+            Branch& R = br.back();
+            if (hipMalloc(&R.d_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
+            HIP_TRY(hipMemcpy(R.d_rt, R.h_rt.get(), sizeof(RunTable), hipMemcpyHostToDevice));
+            if (R.npairs) {
+                if (hipMalloc(&R.d_sargs, R.npairs * sizeof(StereoArgs)) != hipSuccess) return ORB_E_OOM;
+                HIP_TRY(hipMemcpy(R.d_sargs, sub.data(), R.npairs * sizeof(StereoArgs), hipMemcpyHostToDevice));
+            }
+        }
+        while (br_st.size() + 1 < br.size()) {
+            hipStream_t s2 = nullptr;
+            HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            br_st.push_back(s2);
+        }
+        while (br_ev.size() < br.size()) {
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            br_ev.push_back(e);
+        }
+        return ORB_OK;
+    }
+
+    // Captured body of mode 1 with branches: fork from the batch stream, one
+    // group per stream, join back.
+    int enqueue_branches()
+    {
+        hipStream_t st = ctx->stream;
+        HIP_TRY(hipEventRecord(br_ev[0], st));
+        for (size_t g = 1; g < br.size(); g++) HIP_TRY(hipStreamWaitEvent(br_st[g - 1], br_ev[0], 0));
+        for (size_t g = 0; g < br.size(); g++) {
+            const Branch& B = br[g];
+            hipStream_t s = g == 0 ? st : br_st[g - 1];
+            HIP_TRY(orb_launch_extract(d_jobs + B.s, B.d_rt, B.h_rt.get(), s, nullptr));
+            if (B.npairs) HIP_TRY(orb_launch_stereo(B.d_sargs, B.npairs, B.nwaves, s, nullptr));
+        }
+        for (size_t g = 1; g < br.size(); g++) {
+            HIP_TRY(hipEventRecord(br_ev[g], br_st[g - 1]));
+            HIP_TRY(hipStreamWaitEvent(st, br_ev[g], 0));
+        }
+        return ORB_OK;
+    }
+
+    void drop_branches()
+    {
+        for (Branch& B : br) {
+            if (B.d_rt) (void)hipFree(B.d_rt);
+            if (B.d_sargs) (void)hipFree(B.d_sargs);
+        }
+        br.clear();
+    }
+
     void drop_graph()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
+        drop_branches();
     }
 
     ~orb_batch()
     {
         drop_graph();
+        for (hipStream_t s2 : br_st) (void)hipStreamDestroy(s2);
+        for (hipEvent_t e : br_ev) (void)hipEventDestroy(e);
         for (orb_extractor* e : ex) delete e;
         if (d_jobs) (void)hipFree(d_jobs);
         if (d_rt) (void)hipFree(d_rt);
@@ -511,8 +622,7 @@ int orb_batch_set_stereo(orb_batch* b, int npairs, const int* left_slots, const 
         a.dr = R->hjob.desc;
         a.nr = R->hjob.counters + ORB_CNT_NOUT;
         a.cap_l = L->plan.out_cap;
-        a.wave_base = waves;
-        waves += a.cap_l;
+        waves = std::max(waves, a.cap_l);
     }
     HIP_TRY(hipSetDevice(b->ctx->device));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
@@ -521,13 +631,25 @@ int orb_batch_set_stereo(orb_batch* b, int npairs, const int* left_slots, const 
     b->d_sargs = nullptr;
     b->pl.clear();
     b->pr.clear();
+    b->h_sargs.clear();
     b->nwaves = 0;
     if (npairs == 0) return ORB_OK;
     if (hipMalloc(&b->d_sargs, npairs * sizeof(StereoArgs)) != hipSuccess) return ORB_E_OOM;
     HIP_TRY(hipMemcpy(b->d_sargs, args.data(), npairs * sizeof(StereoArgs), hipMemcpyHostToDevice));
     b->pl.assign(left_slots, left_slots + npairs);
     b->pr.assign(right_slots, right_slots + npairs);
+    b->h_sargs = args;
     b->nwaves = waves;
+    return ORB_OK;
+}
+
+int orb_batch_set_branches(orb_batch* b, int branches)
+{
+    if (!b || branches < 1 || branches > 8) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    b->drop_graph();
+    b->want_branches = branches;
     return ORB_OK;
 }
 
@@ -538,8 +660,13 @@ int orb_batch_run(orb_batch* b, int mode)
     if (mode != 1) return b->enqueue(mode);
     hipStream_t st = b->ctx->stream;
     if (!b->exec) {
+        int rc = b->want_branches > 1 ? b->build_branches() : ORB_OK;
+        if (rc) {
+            b->drop_branches();
+            return rc;
+        }
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const int rc = b->enqueue(0);
+        rc = b->br.size() > 1 ? b->enqueue_branches() : b->enqueue(0);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(st, &g);
         if (rc || e != hipSuccess) {

@@ -91,17 +91,15 @@ struct StereoArgs {
     const uint8_t* dr;
     const int* nr;
     int cap_l;
-    int wave_base;          // first wavefront of this pair in a multi-pair launch
     int nrows;
     float bf, mb;
     float scale[ORB_MAX_LEVELS], inv_scale[ORB_MAX_LEVELS];
     float* uRight;
     float* depth;
-    int* match_sad;
-    int* match_idx;
-    int* match_count;
-    int* sorted_idx;        // scratch: right keypoints sorted by y ...
-    float* sorted_y;        // ... (cap_r entries each)
+    int* match_sad;         // per left keypoint: SAD of its match, or -1
+    float4* srec;           // scratch: right keypoints sorted by (y, index) as
+                            // {y, x, octave bits, index bits} ...
+    uint8_t* sdesc;         // ... and their descriptors in the same order
     int* row_start;         // first sorted entry with y > row - 1 - band (nrows + 1)
     int cap_r;
     float band;             // max over levels of 2*scale: widest row band

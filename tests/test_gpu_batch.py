@@ -67,3 +67,47 @@ def test_stage_times_cover_pipeline(orb):
     t = batch.stage_times(3)
     assert set(t) >= {"pyramid_l0", "octree", "describe", "stereo_match"}
     assert all(v >= 0 for v in t.values())
+
+
+def test_batch_graph_branches_match_oracle(mixed):
+    batch, ref = mixed
+    for k in (3, 8, 1):
+        batch.set_branches(k)
+        for _ in range(2):
+            batch.run(1)
+        batch.synchronize()
+        for s, (ok, od) in enumerate(ref):
+            gk, gd = batch.results(s)
+            assert len(gk) == len(ok), (k, s, len(gk), len(ok))
+            np.testing.assert_array_equal(gk.view(np.uint8), ok.view(np.uint8), err_msg=f"branches {k} slot {s}")
+            np.testing.assert_array_equal(gd, od, err_msg=f"branches {k} slot {s} descriptors")
+
+
+@pytest.mark.parametrize("left,right", [([0, 2, 4], [1, 3, 5]), ([0, 1, 2], [5, 4, 3])])
+def test_batch_stereo_branches(orb, oracle, left, right):
+    """Pairs stay whole across branches; a nesting with no valid cut keeps
+    one branch.  Either way the results are the single-branch ones."""
+    bf, fx = 386.1, 718.856
+    b = float(np.float32(bf) / np.float32(fx))
+    pairs = [synth.stereo_pair(1241, 376, 60 + i)[:2] for i in range(3)]
+    batch = orb.Batch([(1241, 376)] * 6, 2000)
+    for i, (l, r) in enumerate(pairs):
+        batch.upload(left[i], l)
+        batch.upload(right[i], r)
+    batch.set_stereo(left, right, bf, b)
+    batch.run(0)
+    batch.synchronize()
+    want = [batch.stereo_results(i) for i in range(3)]
+    batch.set_branches(2)
+    for _ in range(3):
+        batch.run(1)
+    batch.synchronize()
+    for i in range(3):
+        uR, dep = batch.stereo_results(i)
+        np.testing.assert_array_equal(uR.view(np.uint32), want[i][0].view(np.uint32))
+        np.testing.assert_array_equal(dep.view(np.uint32), want[i][1].view(np.uint32))
+    l, r = pairs[0]
+    ol = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    okl, odl, okr, odr, ouR, odep = oracle.stereo_frame(ol, orr, l, r, bf, b)
+    np.testing.assert_array_equal(want[0][0].view(np.uint32), ouR.view(np.uint32))
