@@ -94,6 +94,41 @@ def knn_extra(orb, ctx, steps):
     return out
 
 
+def bow_extra(orb, ctx, steps, cpu_seconds):
+    """DBoW2 transform (Frame::ComputeBoW, levelsup 4) of 64 images x 2000
+    descriptors against a k=10, L=6 tree (1,111,111 nodes, ORBvoc.txt's
+    shape), descriptors resident in HBM; the oracle on one host core beside."""
+    import ctypes
+    from orb_slam2_detailed_comments_amd import synth
+    voc = synth.vocabulary(10, 6, 41)
+    n_img, per = 64, 2000
+    q = synth.vocabulary_queries(voc, n_img * per, 43, flip_ands=3)
+    offs = np.arange(0, n_img * per + 1, per, dtype=np.int32)
+    v = orb.Vocabulary(context=ctx, **voc)
+    ms = ctypes.c_float()
+    rc = orb.lib().orb_vocabulary_bow_time(v.h, n_img, ctypes.c_void_p(q.ctypes.data),
+                                           ctypes.c_void_p(offs.ctypes.data), 4, steps, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"orb_vocabulary_bow_time {rc}")
+    out = {"workload": "DBoW2 transform + BowVector + FeatureVector, 64 images x 2000 descriptors, k=10 L=6 "
+                       "synthetic tree (1,111,111 nodes), levelsup 4",
+           "ms_per_batch": round(ms.value, 4), "images_per_s": round(n_img / (ms.value * 1e-3), 1),
+           "descriptors_per_s": round(n_img * per / (ms.value * 1e-3), 1)}
+    v.close()
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.OracleVocabulary(**voc)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            o.bow(q[(n % n_img) * per:(n % n_img + 1) * per], 4)
+            n += 1
+        cpu = n / (time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": round(cpu, 2), "unit": "images/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} images of 2000 descriptors through the oracle in ~{cpu_seconds:.0f}s"}
+    return out
+
+
 def serving_extra(orb, ctx):
     """16 camera streams (4 each of 640x480, 752x480, 1241x376, 1280x720),
     Poisson arrivals; extract + match against the stream's resident previous
@@ -264,6 +299,8 @@ def main():
         out["knn2"] = knn_extra(orb, ctx, max(3, args.steps // 2))
         print("[bench] serving", file=sys.stderr, flush=True)
         out["serving"] = serving_extra(orb, ctx)
+        print("[bench] vocabulary transform", file=sys.stderr, flush=True)
+        out["bow"] = bow_extra(orb, ctx, max(args.steps, 5), 0.0 if args.no_cpu else min(args.cpu_seconds, 5.0))
     if not args.no_cpu:
         print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
         cpu = cpu_baseline(frames[:8], args.cpu_seconds)

@@ -212,6 +212,44 @@ int orb_server_release(orb_server* s, uint64_t ticket);
 int orb_server_drain(orb_server* s, double timeout_s);
 int orb_server_stats(orb_server* s, int64_t* batches, int64_t* requests, size_t* hbm_used, int* slots);
 
+/* ---------------------------------------------------------------- vocabulary
+ * DBoW2 bag of words (Thirdparty/DBoW2/DBoW2/TemplatedVocabulary.h), as
+ * Frame::ComputeBoW / KeyFrame::ComputeBoW call it (src/Frame.cc:704-718,
+ * levelsup 4).  The tree lives in HBM, renumbered breadth-first; node ids in
+ * the outputs are the file's. */
+typedef struct orb_vocabulary orb_vocabulary;
+/* TemplatedVocabulary::loadFromTextFile (TemplatedVocabulary.h:1368): the
+ * ORBvoc.txt format.  Blank lines are skipped. */
+int orb_vocabulary_load_text(orb_context* ctx, const char* path, orb_vocabulary** out);
+/* The same tree from arrays in file order: n_nodes entries, entry 0 the root
+ * (its fields ignored), parent[i] < i, is_word[i] > 0 for words, 32-byte
+ * descriptors, weights.  scoring 0..5 (L1_NORM .. DOT_PRODUCT), weighting
+ * 0..3 (TF_IDF, TF, IDF, BINARY), as DBoW2's enums. */
+int orb_vocabulary_create(orb_context* ctx, int k, int L, int scoring, int weighting, int n_nodes,
+                          const int32_t* parent, const uint8_t* is_word, const uint8_t* desc, const double* weight,
+                          orb_vocabulary** out);
+void orb_vocabulary_destroy(orb_vocabulary* v);
+int orb_vocabulary_info(const orb_vocabulary* v, int* k, int* L, int* scoring, int* weighting, int* n_nodes,
+                        int* n_words);
+/* TemplatedVocabulary::transform(feature, word_id, weight, nid, levelsup)
+ * (TemplatedVocabulary.h:1240-1290) for n descriptors: word id, word weight
+ * and the node at depth L - levelsup (node_id may be NULL). */
+int orb_vocabulary_transform(orb_vocabulary* v, const uint8_t* desc, int n, int levelsup, int32_t* word_id,
+                             double* weight, int32_t* node_id);
+/* TemplatedVocabulary::transform(features, BowVector, FeatureVector, levelsup)
+ * (TemplatedVocabulary.h:1136-1206) for n_images images at once.  Image i is
+ * rows offsets[i] .. offsets[i+1]-1 of desc (at most 4096 rows); its outputs
+ * are written from the same offset: the BowVector as n_bow[i] (word id,
+ * value) pairs in word order, the FeatureVector as n_fv[i] (node id, feature
+ * index) pairs in (node, feature) order. */
+int orb_vocabulary_bow(orb_vocabulary* v, int n_images, const uint8_t* desc, const int32_t* offsets, int levelsup,
+                       int32_t* bow_word, double* bow_value, int32_t* n_bow, int32_t* fv_node, int32_t* fv_feature,
+                       int32_t* n_fv);
+/* Measurement aid, no reference counterpart: average milliseconds of the
+ * device work of orb_vocabulary_bow with the descriptors resident in HBM. */
+int orb_vocabulary_bow_time(orb_vocabulary* v, int n_images, const uint8_t* desc, const int32_t* offsets,
+                            int levelsup, int steps, float* ms);
+
 /* Measurement aid, no reference counterpart: average milliseconds of a
  * device-to-device copy of `bytes` bytes by a plain streaming kernel (HBM
  * bandwidth check and hardware-counter calibration). */

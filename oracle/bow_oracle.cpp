@@ -1,0 +1,307 @@
+// bow_oracle.cpp -- CPU oracle for the DBoW2 vocabulary transform.
+//
+// TEST INFRASTRUCTURE ONLY (see orb_oracle.h): the checker for the GPU
+// vocabulary path and the CPU baseline that bench.py times.
+//
+// Restates, from Thirdparty/DBoW2/DBoW2:
+//   TemplatedVocabulary::loadFromTextFile   TemplatedVocabulary.h:1368-1470
+//   TemplatedVocabulary::saveToTextFile     TemplatedVocabulary.h:1475-1497
+//   TemplatedVocabulary::transform(feature, word_id, weight, nid, levelsup)
+//                                           TemplatedVocabulary.h:1240-1290
+//   TemplatedVocabulary::transform(features, BowVector, FeatureVector,
+//                                  levelsup) TemplatedVocabulary.h:1136-1206
+//   BowVector::addWeight / addIfNotExist / normalize   BowVector.cpp
+//   FeatureVector::addFeature                          FeatureVector.cpp:37-53
+//   FORB::distance (popcount of XOR)                   FORB.cpp:81-101
+//   ScoringObject mustNormalize per scoring type       ScoringObject.h
+// with the same containers (std::map) so that iteration and summation order
+// are the reference's.
+//
+// One deliberate difference: loadFromTextFile's `while(!f.eof())` turns a
+// trailing empty line into an extra child of the root whose descriptor bytes
+// are never written (uninitialised memory).  Blank lines are skipped here.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "orb_oracle.h"
+
+namespace {
+
+struct VNode {
+    int parent = 0;
+    std::vector<int> children;
+    uint8_t desc[32] = {0};
+    double weight = 0.0;
+    int word_id = 0;          // Node() default for non-word nodes
+};
+
+struct Voc {
+    int k = 0, L = 0, scoring = 0, weighting = 0;
+    std::vector<VNode> nodes;
+    std::vector<int> words;   // word id -> node id
+};
+
+// FORB::distance: popcount of the XOR of 8 32-bit words.
+int forb_distance(const uint8_t* a, const uint8_t* b)
+{
+    int d = 0;
+    for (int i = 0; i < 32; i++) d += __builtin_popcount((unsigned)(a[i] ^ b[i]));
+    return d;
+}
+
+// ScoringObject.h: L1_NORM, L2_NORM, CHI_SQUARE, KL, BHATTACHARYYA normalise
+// (L2 for L2_NORM, L1 otherwise); DOT_PRODUCT does not.
+bool must_normalize(int scoring, bool* l2)
+{
+    *l2 = scoring == 1;
+    return scoring != 5;
+}
+
+void add_node(Voc& v, int pid, int is_word, const uint8_t* desc, double weight)
+{
+    const int nid = (int)v.nodes.size();
+    v.nodes.emplace_back();
+    VNode& n = v.nodes.back();
+    n.parent = pid;
+    std::memcpy(n.desc, desc, 32);
+    n.weight = weight;
+    v.nodes[pid].children.push_back(nid);
+    if (is_word > 0) {
+        v.nodes[nid].word_id = (int)v.words.size();
+        v.words.push_back(nid);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+oracle_vocabulary* oracle_voc_create(int k, int L, int scoring, int weighting, int n_nodes, const int32_t* parent,
+                                     const uint8_t* is_word, const uint8_t* desc, const double* weight)
+{
+    if (n_nodes < 1) return nullptr;
+    Voc* v = new Voc();
+    v->k = k;
+    v->L = L;
+    v->scoring = scoring;
+    v->weighting = weighting;
+    v->nodes.resize(1);
+    for (int i = 1; i < n_nodes; i++) {
+        if (parent[i] < 0 || parent[i] >= i) {
+            delete v;
+            return nullptr;
+        }
+        add_node(*v, parent[i], is_word[i], desc + (size_t)i * 32, weight[i]);
+    }
+    return reinterpret_cast<oracle_vocabulary*>(v);
+}
+
+// loadFromTextFile: header "k L scoring weighting" with the reference's range
+// checks, then one line per node: parent, is-leaf flag, 32 descriptor bytes
+// (parsed as int, stored as unsigned char), weight (double).
+oracle_vocabulary* oracle_voc_load_text(const char* path)
+{
+    std::ifstream f(path);
+    if (!f.is_open()) return nullptr;
+    std::string s;
+    if (!std::getline(f, s)) return nullptr;
+    std::stringstream ss(s);
+    int k = -1, L = -1, n1 = -1, n2 = -1;
+    ss >> k >> L >> n1 >> n2;
+    if (k < 0 || k > 20 || L < 1 || L > 10 || n1 < 0 || n1 > 5 || n2 < 0 || n2 > 3) return nullptr;
+    Voc* v = new Voc();
+    v->k = k;
+    v->L = L;
+    v->scoring = n1;
+    v->weighting = n2;
+    v->nodes.resize(1);
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        std::stringstream sn(line);
+        int pid = 0, leaf = 0;
+        sn >> pid >> leaf;
+        uint8_t d[32] = {0};
+        for (int i = 0; i < 32; i++) {
+            int x;
+            sn >> x;
+            if (!sn.fail()) d[i] = (uint8_t)x;
+        }
+        double w = 0.0;
+        sn >> w;
+        if (pid < 0 || pid >= (int)v->nodes.size()) {
+            delete v;
+            return nullptr;
+        }
+        add_node(*v, pid, leaf, d, w);
+    }
+    return reinterpret_cast<oracle_vocabulary*>(v);
+}
+
+// saveToTextFile: "k L  scoring weighting", then per node: parent, isLeaf()
+// (= no children), each descriptor byte as an int followed by a space, a
+// space, and the weight through a default-formatted ostream.
+int oracle_voc_save_text(const oracle_vocabulary* voc, const char* path)
+{
+    const Voc& v = *reinterpret_cast<const Voc*>(voc);
+    std::ofstream f(path);
+    if (!f.is_open()) return -1;
+    f << v.k << " " << v.L << " " << " " << v.scoring << " " << v.weighting << std::endl;
+    for (size_t i = 1; i < v.nodes.size(); i++) {
+        const VNode& n = v.nodes[i];
+        f << n.parent << " " << (n.children.empty() ? 1 : 0) << " ";
+        std::stringstream sd;
+        for (int b = 0; b < 32; b++) sd << (int)n.desc[b] << " ";
+        f << sd.str() << " " << (double)n.weight << std::endl;
+    }
+    return 0;
+}
+
+void oracle_voc_destroy(oracle_vocabulary* voc) { delete reinterpret_cast<Voc*>(voc); }
+
+int oracle_voc_info(const oracle_vocabulary* voc, int* k, int* L, int* scoring, int* weighting, int* n_nodes,
+                    int* n_words)
+{
+    const Voc& v = *reinterpret_cast<const Voc*>(voc);
+    if (k) *k = v.k;
+    if (L) *L = v.L;
+    if (scoring) *scoring = v.scoring;
+    if (weighting) *weighting = v.weighting;
+    if (n_nodes) *n_nodes = (int)v.nodes.size();
+    if (n_words) *n_words = (int)v.words.size();
+    return 0;
+}
+
+// Node table in id order: parent, word id (0 for non-words), descriptor,
+// weight -- what a loader produced, for comparison.
+int oracle_voc_nodes(const oracle_vocabulary* voc, int32_t* parent, int32_t* word_id, uint8_t* desc, double* weight)
+{
+    const Voc& v = *reinterpret_cast<const Voc*>(voc);
+    for (size_t i = 0; i < v.nodes.size(); i++) {
+        if (parent) parent[i] = v.nodes[i].parent;
+        if (word_id) word_id[i] = v.nodes[i].word_id;
+        if (desc) std::memcpy(desc + i * 32, v.nodes[i].desc, 32);
+        if (weight) weight[i] = v.nodes[i].weight;
+    }
+    return 0;
+}
+
+// transform(feature, word_id, weight, nid, levelsup), TemplatedVocabulary.h
+// :1240-1290: from the root, take the child at the smallest distance (the
+// first one on ties), until a node without children; nid is the node chosen
+// at depth L - levelsup (the root when that is <= 0).
+void oracle_voc_transform_one(const oracle_vocabulary* voc, const uint8_t* f, int levelsup, int* word_id,
+                              double* weight, int* nid)
+{
+    const Voc& v = *reinterpret_cast<const Voc*>(voc);
+    const int nid_level = v.L - levelsup;
+    if (nid_level <= 0 && nid) *nid = 0;
+    int final_id = 0, current_level = 0;
+    if (v.nodes[0].children.empty()) {       // a vocabulary of the root alone
+        *word_id = v.nodes[0].word_id;
+        *weight = v.nodes[0].weight;
+        return;
+    }
+    do {
+        ++current_level;
+        const std::vector<int>& ch = v.nodes[final_id].children;
+        final_id = ch[0];
+        double best_d = forb_distance(f, v.nodes[final_id].desc);
+        for (size_t j = 1; j < ch.size(); j++) {
+            const double d = forb_distance(f, v.nodes[ch[j]].desc);
+            if (d < best_d) {
+                best_d = d;
+                final_id = ch[j];
+            }
+        }
+        if (nid && current_level == nid_level) *nid = final_id;
+    } while (!v.nodes[final_id].children.empty());
+    *word_id = v.nodes[final_id].word_id;
+    *weight = v.nodes[final_id].weight;
+}
+
+int oracle_voc_transform(const oracle_vocabulary* voc, const uint8_t* desc, int n, int levelsup, int32_t* word_id,
+                         double* weight, int32_t* node_id)
+{
+    for (int i = 0; i < n; i++) {
+        int w = 0, nid = 0;
+        double wt = 0.0;
+        oracle_voc_transform_one(voc, desc + (size_t)i * 32, levelsup, &w, &wt, &nid);
+        word_id[i] = w;
+        weight[i] = wt;
+        if (node_id) node_id[i] = nid;
+    }
+    return 0;
+}
+
+// transform(features, BowVector, FeatureVector, levelsup),
+// TemplatedVocabulary.h:1136-1206.  Outputs in map order: BowVector as
+// (word id, value) sorted by word id; FeatureVector as (node id, feature
+// index) sorted by node id, features in insertion (= index) order.
+int oracle_voc_bow(const oracle_vocabulary* voc, const uint8_t* desc, int n, int levelsup, int32_t* bow_word,
+                   double* bow_value, int* n_bow, int32_t* fv_node, int32_t* fv_feature, int* n_fv)
+{
+    const Voc& v = *reinterpret_cast<const Voc*>(voc);
+    std::map<unsigned, double> bow;
+    std::map<unsigned, std::vector<unsigned>> fv;
+    *n_bow = 0;
+    *n_fv = 0;
+    if (v.words.empty()) return 0;
+    bool l2 = false;
+    const bool must = must_normalize(v.scoring, &l2);
+    const bool tf = v.weighting == 0 || v.weighting == 1;     // TF_IDF, TF
+    for (int i = 0; i < n; i++) {
+        int id = 0, nid = 0;
+        double w = 0.0;
+        oracle_voc_transform_one(voc, desc + (size_t)i * 32, levelsup, &id, &w, &nid);
+        if (w > 0) {
+            auto it = bow.lower_bound((unsigned)id);
+            if (it != bow.end() && it->first == (unsigned)id) {
+                if (tf) it->second += w;                         // addWeight
+            } else {
+                bow.insert(it, {(unsigned)id, w});               // addWeight / addIfNotExist
+            }
+            fv[(unsigned)nid].push_back((unsigned)i);            // addFeature
+        }
+    }
+    if (tf && !bow.empty() && !must) {
+        const double nd = (double)bow.size();
+        for (auto& e : bow) e.second /= nd;
+    }
+    if (must) {                                                  // BowVector::normalize
+        double norm = 0.0;
+        if (!l2) {
+            for (auto& e : bow) norm += std::fabs(e.second);
+        } else {
+            for (auto& e : bow) norm += e.second * e.second;
+            norm = std::sqrt(norm);
+        }
+        if (norm > 0.0)
+            for (auto& e : bow) e.second /= norm;
+    }
+    int nb = 0;
+    for (auto& e : bow) {
+        bow_word[nb] = (int32_t)e.first;
+        bow_value[nb] = e.second;
+        nb++;
+    }
+    int nf = 0;
+    for (auto& e : fv)
+        for (unsigned f : e.second) {
+            fv_node[nf] = (int32_t)e.first;
+            fv_feature[nf] = (int32_t)f;
+            nf++;
+        }
+    *n_bow = nb;
+    *n_fv = nf;
+    return 0;
+}
+
+}  // extern "C"

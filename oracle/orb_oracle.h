@@ -114,6 +114,27 @@ int oracle_stereo_frame(oracle_extractor* left, oracle_extractor* right,
                         oracle_keypoint* kr, uint8_t* dr, int* nr, int cap,
                         float* uRight, float* depth);
 
+/* ---- DBoW2 vocabulary (bow_oracle.cpp): Thirdparty/DBoW2/DBoW2/
+ * TemplatedVocabulary.h loadFromTextFile :1368, saveToTextFile :1475,
+ * transform :1136 and :1240; BowVector.cpp; FeatureVector.cpp.  Nodes are in
+ * file order, node 0 is the root; parent[i] < i. */
+typedef struct oracle_vocabulary oracle_vocabulary;
+oracle_vocabulary* oracle_voc_create(int k, int L, int scoring, int weighting, int n_nodes,
+                                     const int32_t* parent, const uint8_t* is_word,
+                                     const uint8_t* desc, const double* weight);
+oracle_vocabulary* oracle_voc_load_text(const char* path);
+int oracle_voc_save_text(const oracle_vocabulary* voc, const char* path);
+void oracle_voc_destroy(oracle_vocabulary* voc);
+int oracle_voc_info(const oracle_vocabulary* voc, int* k, int* L, int* scoring, int* weighting,
+                    int* n_nodes, int* n_words);
+int oracle_voc_nodes(const oracle_vocabulary* voc, int32_t* parent, int32_t* word_id,
+                     uint8_t* desc, double* weight);
+int oracle_voc_transform(const oracle_vocabulary* voc, const uint8_t* desc, int n, int levelsup,
+                         int32_t* word_id, double* weight, int32_t* node_id);
+int oracle_voc_bow(const oracle_vocabulary* voc, const uint8_t* desc, int n, int levelsup,
+                   int32_t* bow_word, double* bow_value, int* n_bow,
+                   int32_t* fv_node, int32_t* fv_feature, int* n_fv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,14 @@ def lib():
             "oracle_knn2": (I, [P, I, P, ctypes.c_int64, F, P, P, P, I]),
             "oracle_stereo_match": (I, [P, P, P, P, I, P, P, I, F, F, P, P]),
             "oracle_stereo_frame": (I, [P, P, P, P, I, I, F, F, P, P, P, P, P, P, I, P, P]),
+            "oracle_voc_create": (P, [I, I, I, I, I, P, P, P, P]),
+            "oracle_voc_load_text": (P, [ctypes.c_char_p]),
+            "oracle_voc_save_text": (I, [P, ctypes.c_char_p]),
+            "oracle_voc_destroy": (V, [P]),
+            "oracle_voc_info": (I, [P, P, P, P, P, P, P]),
+            "oracle_voc_nodes": (I, [P, P, P, P, P]),
+            "oracle_voc_transform": (I, [P, P, I, I, P, P, P]),
+            "oracle_voc_bow": (I, [P, P, I, I, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -239,3 +247,68 @@ def stereo_frame(left, right, imL, imR, bf, b):
         raise RuntimeError(rc)
     n = nl.value
     return kl[:n], dl[:n], kr[:nr.value], dr[:nr.value], uR[:n], dep[:n]
+
+
+class OracleVocabulary:
+    """DBoW2 TemplatedVocabulary restated (bow_oracle.cpp).  Node arrays are
+    in file order with node 0 the root: parent[i] < i, is_word, desc (n x 32
+    u8), weight (f64)."""
+
+    def __init__(self, k=None, L=None, scoring=0, weighting=0, parent=None, is_word=None, desc=None,
+                 weight=None, path=None):
+        if path is not None:
+            self.h = lib().oracle_voc_load_text(os.fsencode(path))
+        else:
+            parent = np.ascontiguousarray(parent, np.int32)
+            n = len(parent)
+            is_word = np.ascontiguousarray(is_word, np.uint8)
+            desc = np.ascontiguousarray(desc, np.uint8).reshape(n, 32)
+            weight = np.ascontiguousarray(weight, np.float64)
+            self.h = lib().oracle_voc_create(k, L, scoring, weighting, n, _p(parent), _p(is_word), _p(desc),
+                                             _p(weight))
+        if not self.h:
+            raise ValueError("vocabulary rejected")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_voc_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(6)]
+        lib().oracle_voc_info(self.h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("k", "L", "scoring", "weighting", "n_nodes", "n_words"), (x.value for x in v)))
+
+    def nodes(self):
+        n = self.info()["n_nodes"]
+        parent = np.zeros(n, np.int32)
+        word = np.zeros(n, np.int32)
+        desc = np.zeros((n, 32), np.uint8)
+        weight = np.zeros(n, np.float64)
+        lib().oracle_voc_nodes(self.h, _p(parent), _p(word), _p(desc), _p(weight))
+        return parent, word, desc, weight
+
+    def save_text(self, path):
+        if lib().oracle_voc_save_text(self.h, os.fsencode(path)) != 0:
+            raise OSError(path)
+
+    def transform(self, desc, levelsup=0):
+        desc = np.ascontiguousarray(desc, np.uint8).reshape(-1, 32)
+        n = len(desc)
+        wid = np.zeros(n, np.int32)
+        w = np.zeros(n, np.float64)
+        nid = np.zeros(n, np.int32)
+        lib().oracle_voc_transform(self.h, _p(desc), n, levelsup, _p(wid), _p(w), _p(nid))
+        return wid, w, nid
+
+    def bow(self, desc, levelsup=0):
+        desc = np.ascontiguousarray(desc, np.uint8).reshape(-1, 32)
+        n = len(desc)
+        bw = np.zeros(max(n, 1), np.int32)
+        bv = np.zeros(max(n, 1), np.float64)
+        fn = np.zeros(max(n, 1), np.int32)
+        ff = np.zeros(max(n, 1), np.int32)
+        nb, nf = ctypes.c_int(), ctypes.c_int()
+        lib().oracle_voc_bow(self.h, _p(desc), n, levelsup, _p(bw), _p(bv), ctypes.byref(nb), _p(fn), _p(ff),
+                             ctypes.byref(nf))
+        return bw[:nb.value].copy(), bv[:nb.value].copy(), fn[:nf.value].copy(), ff[:nf.value].copy()

@@ -11,6 +11,8 @@ Python mirror of the reference interface over the C ABI in include/orb_gpu.h
   ORBmatcher::SearchForInitialization, src/ORBmatcher.cc:614-647)
 * ``compute_stereo_matches`` / ``stereo_frame`` (Frame::ComputeStereoMatches,
   src/Frame.cc:831-1082, and the stereo Frame constructor)
+* ``Vocabulary`` with ``transform`` / ``bow`` (DBoW2 TemplatedVocabulary,
+  Thirdparty/DBoW2/DBoW2/TemplatedVocabulary.h, as Frame::ComputeBoW uses it)
 
 Keypoints are numpy structured arrays with cv::KeyPoint's fields.  There is no
 CPU fallback: if the native library is missing every call raises.
@@ -20,8 +22,8 @@ import os
 
 import numpy as np
 
-__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "match_bf", "knn2",
-           "compute_stereo_matches", "stereo_frame", "lib", "library_path", "OrbError"]
+__all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "Vocabulary", "match_bf",
+           "knn2", "compute_stereo_matches", "stereo_frame", "lib", "library_path", "OrbError"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_PKG, "liborb_gpu.so")
@@ -81,6 +83,13 @@ SIGNATURES = {
     "orb_server_release": (_I, [_P, ctypes.c_uint64]),
     "orb_server_drain": (_I, [_P, ctypes.c_double]),
     "orb_server_stats": (_I, [_P, _P, _P, _P, _P]),
+    "orb_vocabulary_load_text": (_I, [_P, ctypes.c_char_p, _P]),
+    "orb_vocabulary_create": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
+    "orb_vocabulary_destroy": (_V, [_P]),
+    "orb_vocabulary_info": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "orb_vocabulary_transform": (_I, [_P, _P, _I, _I, _P, _P, _P]),
+    "orb_vocabulary_bow": (_I, [_P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P]),
+    "orb_vocabulary_bow_time": (_I, [_P, _I, _P, _P, _I, _I, _P]),
     "orb_extractor_debug_image": (_I, [_P, _I, _I, _P, _S]),
     "orb_extractor_debug_candidates": (_I, [_P, _I, _P, _I, _P]),
 }
@@ -507,3 +516,82 @@ class Server:
         _check("orb_server_stats", lib().orb_server_stats(self.h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(u),
                                                           ctypes.byref(sl)))
         return dict(batches=b.value, requests=r.value, hbm_used=u.value, slots=sl.value)
+
+
+class Vocabulary:
+    """DBoW2 vocabulary tree in HBM (TemplatedVocabulary,
+    Thirdparty/DBoW2/DBoW2/TemplatedVocabulary.h).
+
+    ``Vocabulary(path=...)`` reads the ORBvoc.txt text format
+    (loadFromTextFile, :1368); ``Vocabulary(k, L, scoring, weighting, parent,
+    is_word, desc, weight)`` builds the same tree from file-order arrays
+    (node 0 the root, parent[i] < i).
+    """
+
+    TF_IDF, TF, IDF, BINARY = 0, 1, 2, 3
+    L1_NORM, L2_NORM, CHI_SQUARE, KL, BHATTACHARYYA, DOT_PRODUCT = 0, 1, 2, 3, 4, 5
+
+    def __init__(self, k=None, L=None, scoring=0, weighting=0, parent=None, is_word=None, desc=None, weight=None,
+                 path=None, context=None):
+        self.ctx = context or Context.default()
+        h = ctypes.c_void_p()
+        if path is not None:
+            _check("orb_vocabulary_load_text", lib().orb_vocabulary_load_text(self.ctx.h, os.fsencode(path),
+                                                                              ctypes.byref(h)))
+        else:
+            parent = np.ascontiguousarray(parent, np.int32)
+            n = len(parent)
+            is_word = np.ascontiguousarray(is_word, np.uint8)
+            desc = np.ascontiguousarray(desc, np.uint8).reshape(n, 32)
+            weight = np.ascontiguousarray(weight, np.float64)
+            _check("orb_vocabulary_create",
+                   lib().orb_vocabulary_create(self.ctx.h, int(k), int(L), int(scoring), int(weighting), n,
+                                               _ptr(parent), _ptr(is_word), _ptr(desc), _ptr(weight),
+                                               ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orb_vocabulary_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(6)]
+        _check("orb_vocabulary_info", lib().orb_vocabulary_info(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("k", "L", "scoring", "weighting", "n_nodes", "n_words"), (x.value for x in v)))
+
+    def transform(self, desc, levelsup=0):
+        """Per descriptor: (word id, weight, node id at depth L - levelsup)."""
+        desc = np.ascontiguousarray(desc, np.uint8).reshape(-1, 32)
+        n = len(desc)
+        wid = np.zeros(n, np.int32)
+        w = np.zeros(n, np.float64)
+        nid = np.zeros(n, np.int32)
+        _check("orb_vocabulary_transform",
+               lib().orb_vocabulary_transform(self.h, _ptr(desc), n, int(levelsup), _ptr(wid), _ptr(w), _ptr(nid)))
+        return wid, w, nid
+
+    def bow(self, images, levelsup=4):
+        """transform(features, BowVector, FeatureVector, levelsup) for a list
+        of N x 32 descriptor arrays: per image (words, values, fv_nodes,
+        fv_features), in the reference's map order."""
+        mats = [np.ascontiguousarray(d, np.uint8).reshape(-1, 32) for d in images]
+        offs = np.zeros(len(mats) + 1, np.int32)
+        offs[1:] = np.cumsum([len(d) for d in mats])
+        rows = int(offs[-1])
+        desc = np.concatenate(mats) if rows else np.zeros((0, 32), np.uint8)
+        cap = max(rows, 1)
+        bw, bv = np.zeros(cap, np.int32), np.zeros(cap, np.float64)
+        fn, ff = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        nb, nf = np.zeros(len(mats), np.int32), np.zeros(len(mats), np.int32)
+        _check("orb_vocabulary_bow",
+               lib().orb_vocabulary_bow(self.h, len(mats), _ptr(desc), _ptr(offs), int(levelsup), _ptr(bw), _ptr(bv),
+                                        _ptr(nb), _ptr(fn), _ptr(ff), _ptr(nf)))
+        out = []
+        for i in range(len(mats)):
+            o = int(offs[i])
+            out.append((bw[o:o + nb[i]].copy(), bv[o:o + nb[i]].copy(), fn[o:o + nf[i]].copy(),
+                        ff[o:o + nf[i]].copy()))
+        return out

@@ -418,63 +418,51 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const int ax1 = min(ORB_EDGE + L.nCols * L.wCell, L.maxBorderX - 3);
     const int ay1 = min(ORB_EDGE + L.nRows * L.hCell, L.maxBorderY - 3);
     const unsigned t2 = (unsigned)tmin | ((unsigned)tmin << 16);
-    // groups: rows tile -1..32, 4-column groups 0..15; plus the ring columns
-    // tile -1 and 64 (slot 16 of each row)
-    for (int e0 = 0; e0 < LM_H * 17; e0 += 256) {
+    // Groups: rows tile -1..32, 4-column groups c = -1..16.  Group -1 only
+    // contributes its pixel 3 (tile column -1) and group 16 its pixel 0
+    // (column 64): the 1-px ring.  Their missing neighbour dwords (columns -8
+    // and 68 onwards) feed only the bytes that are masked off.
+    for (int e0 = 0; e0 < LM_H * 18; e0 += 256) {
         const int e = e0 + tid;
         unsigned cand = 0u;                       // bit k: pixel k of the group
         int my = 0, c = 0;
-        if (e < LM_H * 17) {
-            my = e / 17;
-            c = e - my * 17;
+        if (e < LM_H * 18) {
+            my = e / 18;
+            c = e - my * 18 - 1;
             const int y = ty0 - 1 + my;
             const int py = my + 3;                // px row of tile row y
-            if (c < 16) {
-                const int x = tx0 + 4 * c;
-                const unsigned V = pxw[py][c + 1], Up = pxw[py - 3][c + 1], Dn = pxw[py + 3][c + 1];
-                const unsigned Lw = pxw[py][c], Rw = pxw[py][c + 2];
-                const unsigned P4 = __builtin_amdgcn_alignbyte(Rw, V, 3);
-                const unsigned P12 = __builtin_amdgcn_alignbyte(V, Lw, 1);
-                const unsigned m02 = fast_maybe2(lo_bytes(V), lo_bytes(Dn), lo_bytes(P4), lo_bytes(Up),
-                                                 lo_bytes(P12), t2);
-                const unsigned m13 = fast_maybe2(hi_bytes(V), hi_bytes(Dn), hi_bytes(P4), hi_bytes(Up),
-                                                 hi_bytes(P12), t2);
-                const bool yin = y >= ay0 && y < ay1;
-                cand = ((m02 & 0xFFFFu) ? 1u : 0u) | ((m13 & 0xFFFFu) ? 2u : 0u) | ((m02 >> 16) ? 4u : 0u) |
-                       ((m13 >> 16) ? 8u : 0u);
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (!yin || x + k < ax0 || x + k >= ax1) cand &= ~(1u << k);
-            } else {
-                // ring columns x = tx0 - 1 and tx0 + 64
-                for (int side = 0; side < 2; side++) {
-                    const int x = side ? tx0 + LT_W : tx0 - 1;
-                    const bool in = x >= ax0 && x < ax1 && y >= ay0 && y < ay1;
-                    if (!in) continue;
-                    const int pxc = x - tx0 + LT_HALO;
-                    const uint8_t* p = px + py * LP_W + pxc;
-                    const int v = p[0], hi = v + tmin, lo = v - tmin;
-                    const int q0 = p[3 * LP_W], q4 = p[3], q8 = p[-3 * LP_W], q12 = p[-3];
-                    const unsigned b = (q0 > hi) | ((q4 > hi) << 1) | ((q8 > hi) << 2) | ((q12 > hi) << 3);
-                    const unsigned kk = (q0 < lo) | ((q4 < lo) << 1) | ((q8 < lo) << 2) | ((q12 < lo) << 3);
-                    const unsigned rb = ((b >> 1) | (b << 3)) & 15u, rk = ((kk >> 1) | (kk << 3)) & 15u;
-                    if ((b & rb) | (kk & rk)) cand |= side ? 2u : 1u;
-                }
-            }
+            const int x = tx0 + 4 * c;
+            const unsigned V = pxw[py][c + 1], Up = pxw[py - 3][c + 1], Dn = pxw[py + 3][c + 1];
+            const unsigned Lw = c >= 0 ? pxw[py][c] : 0u;
+            const unsigned Rw = c <= 15 ? pxw[py][c + 2] : 0u;
+            const unsigned P4 = __builtin_amdgcn_alignbyte(Rw, V, 3);
+            const unsigned P12 = __builtin_amdgcn_alignbyte(V, Lw, 1);
+            const unsigned m02 = fast_maybe2(lo_bytes(V), lo_bytes(Dn), lo_bytes(P4), lo_bytes(Up),
+                                             lo_bytes(P12), t2);
+            const unsigned m13 = fast_maybe2(hi_bytes(V), hi_bytes(Dn), hi_bytes(P4), hi_bytes(Up),
+                                             hi_bytes(P12), t2);
+            // pixels k in [klo, khi): detection area, and the ring rule
+            const int klo = min(max(max(ax0 - x, c < 0 ? 3 : 0), 0), 4);
+            const int khi = max(min(min(ax1 - x, c > 15 ? 1 : 4), 4), 0);
+            const bool yin = y >= ay0 && y < ay1;
+            const unsigned mask = (yin && klo < khi) ? ((1u << khi) - (1u << klo)) : 0u;
+            cand = (((m02 & 0xFFFFu) ? 1u : 0u) | ((m13 & 0xFFFFu) ? 2u : 0u) | ((m02 >> 16) ? 4u : 0u) |
+                    ((m13 >> 16) ? 8u : 0u)) & mask;
         }
-        // queue entry: m-row << 8 | byte column in mmw (tile col + 4)
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const bool ck = (cand >> k) & 1u;
-            const unsigned long long bal = __ballot(ck);
-            if (!bal) continue;
+        // Append: the wave's exclusive prefix of the per-lane counts (0..4)
+        // from three ballots of the count's bits, one LDS atomic per wave.
+        const int cnt = __popc(cand);
+        const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
+        const int tot = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+        if (tot) {
             int base = 0;
-            if (lane == 0) base = atomicAdd(&s_qn, __popcll(bal));
+            if (lane == 0) base = atomicAdd(&s_qn, tot);
             base = __shfl(base, 0, 64);
-            if (ck) {
-                const int col = c < 16 ? 4 + 4 * c + k : (k == 0 ? 3 : 4 + LT_W);
-                u.f.queue[base + lanes_below(bal)] = (unsigned short)((my << 8) | col);
-            }
+            int pos = base + lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2);
+            // queue entry: m-row << 8 | byte column in mmw (tile col + 4)
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if ((cand >> k) & 1u) u.f.queue[pos++] = (unsigned short)((my << 8) | (4 + 4 * c + k));
         }
     }
     __syncthreads();
@@ -1039,34 +1027,34 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     // Both patches are read with aligned dword loads issued together, before
     // any use: the 31x31 orientation patch of the level and the 37x37 window
     // of the blurred level that every rotated BRIEF sample falls in (pattern
-    // radius <= 18.39 px).  Keypoints lie >= 19 px inside the level, so every
-    // dword is inside the level's rows (at most 3 bytes past a row's last
-    // pixel, still inside the image allocation).
-    const int xa0 = (cx - 15) & ~3, nad = ((cx + 15) >> 2) - (xa0 >> 2) + 1;     // <= 9 dwords/row
-    const int xb0 = (cx - 18) & ~3, nbd = ((cx + 18) >> 2) - (xb0 >> 2) + 1;     // <= 10 dwords/row
+    // radius <= 18.39 px).  Keypoints lie >= 19 px inside the level (x <= w-20,
+    // y <= h-20), so every dword lies in rows that have a successor and ends
+    // at most 2 bytes past the row's last pixel: inside the level's buffer.
+    const int xa0 = (cx - 15) & ~3;               // 31 px + <= 3 of alignment: 9 dwords a row
+    const int xb0 = (cx - 18) & ~3;               // 37 px + <= 3 of alignment: 10 dwords a row
     unsigned pa[5], pb[6];
 #pragma unroll
     for (int t = 0; t < 5; t++) {
         const int sl = lane + 64 * t;
-        const int r = sl / nad, c = sl - r * nad;
-        pa[t] = sl < 31 * nad ? *(const unsigned*)(L.lvl + (size_t)(cy - 15 + r) * pitch + xa0 + 4 * c) : 0u;
+        const int r = sl / 9, c = sl - r * 9;
+        pa[t] = sl < 31 * 9 ? *(const unsigned*)(L.lvl + (size_t)(cy - 15 + r) * pitch + xa0 + 4 * c) : 0u;
     }
 #pragma unroll
     for (int t = 0; t < 6; t++) {
         const int sl = lane + 64 * t;
-        const int r = sl / nbd, c = sl - r * nbd;
-        pb[t] = sl < 37 * nbd ? *(const unsigned*)(L.blur + (size_t)(cy - 18 + r) * pitch + xb0 + 4 * c) : 0u;
+        const int r = sl / 10, c = sl - r * 10;
+        pb[t] = sl < 37 * 10 ? *(const unsigned*)(L.blur + (size_t)(cy - 18 + r) * pitch + xb0 + 4 * c) : 0u;
     }
     unsigned* win = s_brief[threadIdx.x >> 6];
 #pragma unroll
     for (int t = 0; t < 6; t++) {
         const int sl = lane + 64 * t;
-        const int r = sl / nbd, c = sl - r * nbd;
-        if (sl < 37 * nbd) win[r * 10 + c] = pb[t];
+        if (sl < 37 * 10) win[sl] = pb[t];
     }
 
     // IC_Angle (ORBextractor.cc:94-141): m10 = sum u*I, m01 = sum v*I over
-    // the disc |u| <= umax[|v|], from the dwords each lane holds.
+    // the disc |u| <= umax[|v|].  Per dword of 4 pixels: a byte mask of the
+    // disc, then v_dot4_u32_u8 for the masked sum and for sum (u+20)*I.
     unsigned long long umt = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) umt |= (unsigned long long)c_umax[i] << (4 * i);
@@ -1074,18 +1062,20 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
 #pragma unroll
     for (int t = 0; t < 5; t++) {
         const int sl = lane + 64 * t;
-        const int r = sl / nad, c = sl - r * nad;
+        const int r = sl / 9, c = sl - r * 9;
         const int v = r - 15, av = v < 0 ? -v : v;
         const int um = (int)((umt >> (4 * (av & 15))) & 15u);
         const int u0 = xa0 + 4 * c - cx;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int uu = u0 + k;
-            const int val = (int)((pa[t] >> (8 * k)) & 255u);
-            const bool in = sl < 31 * nad && uu >= -um && uu <= um;
-            m10 += in ? uu * val : 0;
-            m01 += in ? v * val : 0;
-        }
+        const int lo = -um - u0, hi = um - u0;    // byte k is in the disc iff lo <= k <= hi
+        const int kl = min(max(lo, 0), 3), kh = max(min(hi, 3), 0);
+        const bool any = sl < 31 * 9 && lo <= 3 && hi >= 0;
+        const unsigned mask = any ? ((0xFFFFFFFFu << (8 * kl)) & (0xFFFFFFFFu >> (8 * (3 - kh)))) : 0u;
+        const unsigned md = pa[t] & mask;
+        const unsigned wts = (unsigned)(u0 + 20) * 0x01010101u + 0x03020100u;   // bytes u+20 in [2, 40]
+        const int sum = (int)__builtin_amdgcn_udot4(md, 0x01010101u, 0u, false);
+        const int sumu = (int)__builtin_amdgcn_udot4(md, wts, 0u, false);
+        m01 += v * sum;
+        m10 += sumu - 20 * sum;
     }
     m10 = wave_sum(m10);
     m01 = wave_sum(m01);
@@ -1221,6 +1211,7 @@ __global__ __launch_bounds__(1024) void stereo_prep_kernel(const StereoArgs* __r
 // grid (ceil(max left capacity / 4), pairs), one wavefront per left keypoint
 __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restrict__ args)
 {
+    __shared__ unsigned s_patch[4][44 + 77];     // per wavefront: left 11x16 B, right 11x28 B
     const int lane = threadIdx.x & 63;
     const StereoArgs& A = args[blockIdx.y];
     const int iL = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1306,19 +1297,51 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
         if (sy0 - 5 < 0 || sy0 + 5 >= hL || sy0 + 5 >= hR || sx0 - 5 < 0 || sx0 + 5 >= wL) go = false;
     }
     if (go) {
-        // 121 (shift, row) tasks over 64 lanes; per task 11 absolute
-        // differences of centre-subtracted pixels, then one wave sum per shift.
-        const int cL = pl[(size_t)sy0 * pitchL + sx0];
+        // Stage the 11x11 left patch and the 11x21 right strip (every shift's
+        // patch) in LDS: rows of aligned dwords, two per lane, bytewise only
+        // where a dword would pass the level's right edge.  Then 121 (shift,
+        // row) tasks over 64 lanes; per task 11 absolute differences of
+        // centre-subtracted pixels, then one wave sum per shift.
+        const int xl0 = (sx0 - 5) & ~3, nld = ((sx0 + 5) >> 2) - (xl0 >> 2) + 1;     // <= 4 dwords
+        const int xr0 = (sxR - 10) & ~3, nrd = ((sxR + 10) >> 2) - (xr0 >> 2) + 1;   // <= 7 dwords
+        unsigned* pw = s_patch[threadIdx.x >> 6];
+#pragma unroll
+        for (int h2 = 0; h2 < 2; h2++) {
+            const int sl = lane + 64 * h2;
+            const bool isL = sl < 11 * nld;
+            const int k = isL ? sl : sl - 11 * nld;
+            const int nd = isL ? nld : nrd;
+            const int r = k / nd, c = k - r * nd;
+            if (isL || k < 11 * nrd) {
+                const uint8_t* img = isL ? pl : pr;
+                const int x = (isL ? xl0 : xr0) + 4 * c, w = isL ? wL : wR;
+                const uint8_t* rp = img + (size_t)(sy0 - 5 + r) * (isL ? pitchL : pitchR);
+                unsigned d = 0;
+                if (x + 3 < w) {
+                    d = *(const unsigned*)(rp + x);
+                } else {
+                    for (int b2 = 0; b2 < 4; b2++)
+                        if (x + b2 < w) d |= (unsigned)rp[x + b2] << (8 * b2);
+                }
+                pw[isL ? r * 4 + c : 44 + r * 7 + c] = d;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint8_t* lb = (const uint8_t*)pw;              // 11 rows x 16 bytes
+        const uint8_t* rb = (const uint8_t*)(pw + 44);       // 11 rows x 28 bytes
+        const int cL = lb[5 * 16 + sx0 - xl0];
         int acc[2] = {0, 0};
 #pragma unroll
         for (int h2 = 0; h2 < 2; h2++) {
             const int task = lane + 64 * h2;
             if (task < 121) {
-                const int inc = task / 11 - 5, dy = task % 11 - 5;
-                const int xc = sxR + inc;
-                const int cR = pr[(size_t)sy0 * pitchR + xc];
-                const uint8_t* rl = pl + (size_t)(sy0 + dy) * pitchL + sx0 - 5;
-                const uint8_t* rr = pr + (size_t)(sy0 + dy) * pitchR + xc - 5;
+                const int inc = task / 11 - 5, dy = task % 11;
+                const int xc = sxR + inc - xr0;
+                const int cR = rb[5 * 28 + xc];
+                const uint8_t* rl = lb + dy * 16 + sx0 - 5 - xl0;
+                const uint8_t* rr = rb + dy * 28 + xc - 5;
                 int a = 0;
 #pragma unroll
                 for (int dx = 0; dx < 11; dx++) a += abs((rl[dx] - cL) - (rr[dx] - cR));

@@ -122,3 +122,23 @@ struct MatchArgs {
     int* save_n;            // ... and *nq here (the stream's resident state)
     int save_cap;
 };
+
+// A DBoW2 vocabulary tree on the device, nodes renumbered breadth-first so
+// that every node's children are contiguous (in their file order).
+struct VocDev {
+    const int2* meta;       // per node: first child, number of children
+    const uint4* desc;      // per node: 32-byte descriptor as two uint4
+    const int* orig_id;     // per node: id in the file (node 0 = root)
+    const int* word_id;     // per node: word id (0 for non-words)
+    const double* weight;   // per node: weight from the file
+    int n_nodes;
+    int L;                  // depth levels from the header (nid level = L - levelsup)
+    int max_depth;          // deepest node, bounds every descent
+    int scoring, weighting;
+};
+
+// One image of a BoW batch: its rows [off, off + n) of the descriptor and
+// per-descriptor arrays; outputs at the same offsets.
+struct BowJob {
+    int off, n;
+};

@@ -158,3 +158,58 @@ def stream_frames(width, height, seed, n_frames, max_rot=2.0, max_shift=8.0, noi
         cur = np.clip(np.rint(moved), 0, 255).astype(np.uint8)
         frames.append(cur)
     return frames
+
+
+def _sparse_bits(rng, shape, ands):
+    """Random bytes whose bits are set with probability 2**-ands."""
+    out = rng.integers(0, 256, size=shape, dtype=np.uint8)
+    for _ in range(ands - 1):
+        out &= rng.integers(0, 256, size=shape, dtype=np.uint8)
+    return out
+
+
+def vocabulary(k=10, L=6, seed=0, flip_ands=3, stop_frac=0.02, scoring=0, weighting=0):
+    """A DBoW2-shaped vocabulary tree in file order, breadth first: node 0 is
+    the root, the k**l nodes of level l follow level l-1, node j's children
+    are consecutive.  Level-1 descriptors are uniform; a child is its parent
+    with each bit flipped with probability 2**-flip_ands.  Leaves are the
+    words, with idf-like weights in [0.5, 8) and a stop_frac share of zero
+    weights (stopped words).  Returns a dict of the constructor arguments of
+    Vocabulary / OracleVocabulary."""
+    rng = np.random.default_rng(seed + 7919)
+    parents = [np.zeros(1, np.int32)]
+    descs = [np.zeros((1, 32), np.uint8)]
+    prev = np.zeros(1, np.int64)
+    first = 1
+    for level in range(1, L + 1):
+        n = len(prev) * k
+        par = np.repeat(prev, k).astype(np.int32)
+        if level == 1:
+            d = rng.integers(0, 256, size=(n, 32), dtype=np.uint8)
+        else:
+            d = np.repeat(descs[-1], k, axis=0) ^ _sparse_bits(rng, (n, 32), flip_ands)
+        parents.append(par)
+        descs.append(d)
+        prev = np.arange(first, first + n, dtype=np.int64)
+        first += n
+    parent = np.concatenate(parents)
+    desc = np.concatenate(descs)
+    nn = len(parent)
+    is_word = np.zeros(nn, np.uint8)
+    is_word[nn - k ** L:] = 1
+    weight = np.zeros(nn, np.float64)
+    nw = k ** L
+    w = rng.uniform(0.5, 8.0, size=nw)
+    w[rng.random(nw) < stop_frac] = 0.0
+    weight[nn - nw:] = w
+    return dict(k=k, L=L, scoring=scoring, weighting=weighting, parent=parent, is_word=is_word, desc=desc,
+                weight=weight)
+
+
+def vocabulary_queries(voc, n, seed, flip_ands=4):
+    """n descriptors near random words of a vocabulary() tree: a word's
+    descriptor with each bit flipped with probability 2**-flip_ands."""
+    rng = np.random.default_rng(seed + 104723)
+    words = np.flatnonzero(voc["is_word"])
+    pick = voc["desc"][rng.choice(words, size=n)]
+    return pick ^ _sparse_bits(rng, pick.shape, flip_ands)
