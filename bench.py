@@ -223,7 +223,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the dense k=2 search timing")
-    ap.add_argument("--branches", type=int, default=1, help="independent hipGraph branches (mode graph)")
+    ap.add_argument("--branches", type=int, default=2, help="independent hipGraph branches (mode graph)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (--gpus 1): frames and streams shard across GPUs with no collective, "
