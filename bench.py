@@ -129,6 +129,43 @@ def bow_extra(orb, ctx, steps, cpu_seconds):
     return out
 
 
+def mappoint_extra(orb, ctx, steps, cpu_seconds):
+    """MapPoint::ComputeDistinctiveDescriptors for 20,000 map points with
+    2..40 observations each (clustered descriptors), rows resident in HBM."""
+    import ctypes
+    rng = np.random.default_rng(77)
+    sizes = rng.integers(2, 41, size=20000)
+    pts = []
+    for n in sizes:
+        base = rng.integers(0, 256, size=32, dtype=np.uint8)
+        pts.append(base ^ (rng.integers(0, 256, size=(n, 32), dtype=np.uint8) &
+                           rng.integers(0, 256, size=(n, 32), dtype=np.uint8) &
+                           rng.integers(0, 256, size=(n, 32), dtype=np.uint8)))
+    offs = np.zeros(len(pts) + 1, np.int32)
+    offs[1:] = np.cumsum(sizes)
+    desc = np.concatenate(pts)
+    ms = ctypes.c_float()
+    rc = orb.lib().orb_distinctive_descriptors_time(ctx.h, len(pts), ctypes.c_void_p(desc.ctypes.data),
+                                                    ctypes.c_void_p(offs.ctypes.data), steps, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"orb_distinctive_descriptors_time {rc}")
+    out = {"workload": "ComputeDistinctiveDescriptors, 20,000 map points x U[2,40] observations",
+           "ms_per_batch": round(ms.value, 4), "points_per_s": round(len(pts) / (ms.value * 1e-3), 1)}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        n, t0 = 0, time.perf_counter()
+        chunk = 500
+        while time.perf_counter() - t0 < cpu_seconds:
+            k = (n // chunk) % (len(pts) // chunk)
+            pyoracle.distinctive_descriptors(pts[k * chunk:(k + 1) * chunk])
+            n += chunk
+        cpu = n / (time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": round(cpu, 1), "unit": "points/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} points through the oracle in ~{cpu_seconds:.0f}s"}
+    return out
+
+
 def serving_extra(orb, ctx):
     """16 camera streams (4 each of 640x480, 752x480, 1241x376, 1280x720),
     Poisson arrivals; extract + match against the stream's resident previous
@@ -301,6 +338,9 @@ def main():
         out["serving"] = serving_extra(orb, ctx)
         print("[bench] vocabulary transform", file=sys.stderr, flush=True)
         out["bow"] = bow_extra(orb, ctx, max(args.steps, 5), 0.0 if args.no_cpu else min(args.cpu_seconds, 5.0))
+        print("[bench] map-point descriptors", file=sys.stderr, flush=True)
+        out["distinctive_descriptors"] = mappoint_extra(orb, ctx, max(args.steps, 5),
+                                                        0.0 if args.no_cpu else min(args.cpu_seconds, 3.0))
     if not args.no_cpu:
         print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
         cpu = cpu_baseline(frames[:8], args.cpu_seconds)

@@ -250,6 +250,21 @@ int orb_vocabulary_bow(orb_vocabulary* v, int n_images, const uint8_t* desc, con
 int orb_vocabulary_bow_time(orb_vocabulary* v, int n_images, const uint8_t* desc, const int32_t* offsets,
                             int levelsup, int steps, float* ms);
 
+/* ---------------------------------------------------------------- map points
+ * MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450) for
+ * n_points points at once.  Point p's observation descriptors are rows
+ * offsets[p] .. offsets[p+1]-1 of desc (at most 1024), in the order the
+ * reference visits its observations.  best[p] receives the index, within
+ * the point's rows, of the descriptor with the least median distance to the
+ * others (the first on ties), or -1 for a point without rows; out_desc
+ * (optional, n_points x 32) receives that descriptor (zeros for -1). */
+int orb_distinctive_descriptors(orb_context* ctx, int n_points, const uint8_t* desc, const int32_t* offsets,
+                                int32_t* best, uint8_t* out_desc);
+/* Measurement aid, no reference counterpart: average milliseconds of the
+ * device work of orb_distinctive_descriptors with the rows resident in HBM. */
+int orb_distinctive_descriptors_time(orb_context* ctx, int n_points, const uint8_t* desc, const int32_t* offsets,
+                                     int steps, float* ms);
+
 /* Measurement aid, no reference counterpart: average milliseconds of a
  * device-to-device copy of `bytes` bytes by a plain streaming kernel (HBM
  * bandwidth check and hardware-counter calibration). */

@@ -14,12 +14,14 @@
 //   FeatureVector::addFeature                          FeatureVector.cpp:37-53
 //   FORB::distance (popcount of XOR)                   FORB.cpp:81-101
 //   ScoringObject mustNormalize per scoring type       ScoringObject.h
+// and MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450),
 // with the same containers (std::map) so that iteration and summation order
 // are the reference's.
 //
 // One deliberate difference: loadFromTextFile's `while(!f.eof())` turns a
 // trailing empty line into an extra child of the root whose descriptor bytes
 // are never written (uninitialised memory).  Blank lines are skipped here.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -301,6 +303,42 @@ int oracle_voc_bow(const oracle_vocabulary* voc, const uint8_t* desc, int n, int
         }
     *n_bow = nb;
     *n_fv = nf;
+    return 0;
+}
+
+// MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450) for
+// many points: point p owns rows offsets[p] .. offsets[p+1]-1 (its
+// observations in the caller's order).  The full distance matrix, each row
+// sorted, median = row[0.5*(N-1)] (index truncated), the first row with the
+// smallest median wins.  best[p] is that row's index within the point's
+// rows, or -1 for a point without rows.
+int oracle_distinctive_descriptors(const uint8_t* desc, const int32_t* offsets, int n_points, int32_t* best)
+{
+    for (int p = 0; p < n_points; p++) {
+        const int a = offsets[p], N = offsets[p + 1] - offsets[p];
+        if (N <= 0) {
+            best[p] = -1;
+            continue;
+        }
+        std::vector<std::vector<float>> D(N, std::vector<float>(N, 0.f));
+        for (int i = 0; i < N; i++)
+            for (int j = i + 1; j < N; j++) {
+                const int d = forb_distance(desc + (size_t)(a + i) * 32, desc + (size_t)(a + j) * 32);
+                D[i][j] = (float)d;
+                D[j][i] = (float)d;
+            }
+        int bestMedian = 0x7FFFFFFF, bestIdx = 0;
+        for (int i = 0; i < N; i++) {
+            std::vector<int> v(D[i].begin(), D[i].end());
+            std::sort(v.begin(), v.end());
+            const int median = v[(size_t)(0.5 * (N - 1))];
+            if (median < bestMedian) {
+                bestMedian = median;
+                bestIdx = i;
+            }
+        }
+        best[p] = bestIdx;
+    }
     return 0;
 }
 

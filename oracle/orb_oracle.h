@@ -135,6 +135,12 @@ int oracle_voc_bow(const oracle_vocabulary* voc, const uint8_t* desc, int n, int
                    int32_t* bow_word, double* bow_value, int* n_bow,
                    int32_t* fv_node, int32_t* fv_feature, int* n_fv);
 
+/* MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450), many
+ * points at once: best[p] indexes point p's rows offsets[p]..offsets[p+1]-1
+ * (-1 when it has none). */
+int oracle_distinctive_descriptors(const uint8_t* desc, const int32_t* offsets, int n_points,
+                                   int32_t* best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,7 @@ def lib():
             "oracle_voc_nodes": (I, [P, P, P, P, P]),
             "oracle_voc_transform": (I, [P, P, I, I, P, P, P]),
             "oracle_voc_bow": (I, [P, P, I, I, P, P, P, P, P, P]),
+            "oracle_distinctive_descriptors": (I, [P, P, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -312,3 +313,15 @@ class OracleVocabulary:
         lib().oracle_voc_bow(self.h, _p(desc), n, levelsup, _p(bw), _p(bv), ctypes.byref(nb), _p(fn), _p(ff),
                              ctypes.byref(nf))
         return bw[:nb.value].copy(), bv[:nb.value].copy(), fn[:nf.value].copy(), ff[:nf.value].copy()
+
+
+def distinctive_descriptors(points):
+    """MapPoint::ComputeDistinctiveDescriptors for a list of N_p x 32 arrays:
+    the chosen row of each (-1 for an empty one)."""
+    mats = [np.ascontiguousarray(d, np.uint8).reshape(-1, 32) for d in points]
+    offs = np.zeros(len(mats) + 1, np.int32)
+    offs[1:] = np.cumsum([len(d) for d in mats])
+    desc = np.concatenate(mats) if offs[-1] else np.zeros((1, 32), np.uint8)
+    best = np.zeros(max(len(mats), 1), np.int32)
+    lib().oracle_distinctive_descriptors(_p(desc), _p(offs), len(mats), _p(best))
+    return best[:len(mats)]

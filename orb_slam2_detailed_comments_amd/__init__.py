@@ -11,6 +11,8 @@ Python mirror of the reference interface over the C ABI in include/orb_gpu.h
   ORBmatcher::SearchForInitialization, src/ORBmatcher.cc:614-647)
 * ``compute_stereo_matches`` / ``stereo_frame`` (Frame::ComputeStereoMatches,
   src/Frame.cc:831-1082, and the stereo Frame constructor)
+* ``distinctive_descriptors`` (MapPoint::ComputeDistinctiveDescriptors,
+  src/MapPoint.cc:365-450, for many map points at once)
 * ``Vocabulary`` with ``transform`` / ``bow`` (DBoW2 TemplatedVocabulary,
   Thirdparty/DBoW2/DBoW2/TemplatedVocabulary.h, as Frame::ComputeBoW uses it)
 
@@ -23,7 +25,8 @@ import os
 import numpy as np
 
 __all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "Vocabulary", "match_bf",
-           "knn2", "compute_stereo_matches", "stereo_frame", "lib", "library_path", "OrbError"]
+           "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "lib", "library_path",
+           "OrbError"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_PKG, "liborb_gpu.so")
@@ -90,6 +93,8 @@ SIGNATURES = {
     "orb_vocabulary_transform": (_I, [_P, _P, _I, _I, _P, _P, _P]),
     "orb_vocabulary_bow": (_I, [_P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P]),
     "orb_vocabulary_bow_time": (_I, [_P, _I, _P, _P, _I, _I, _P]),
+    "orb_distinctive_descriptors": (_I, [_P, _I, _P, _P, _P, _P]),
+    "orb_distinctive_descriptors_time": (_I, [_P, _I, _P, _P, _I, _P]),
     "orb_extractor_debug_image": (_I, [_P, _I, _I, _P, _S]),
     "orb_extractor_debug_candidates": (_I, [_P, _I, _P, _I, _P]),
 }
@@ -516,6 +521,22 @@ class Server:
         _check("orb_server_stats", lib().orb_server_stats(self.h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(u),
                                                           ctypes.byref(sl)))
         return dict(batches=b.value, requests=r.value, hbm_used=u.value, slots=sl.value)
+
+
+def distinctive_descriptors(points, context=None):
+    """MapPoint::ComputeDistinctiveDescriptors for a list of N_p x 32
+    observation-descriptor arrays: (chosen row per point, -1 if empty;
+    the chosen descriptors, n_points x 32)."""
+    ctx = context or Context.default()
+    mats = [np.ascontiguousarray(d, np.uint8).reshape(-1, 32) for d in points]
+    offs = np.zeros(len(mats) + 1, np.int32)
+    offs[1:] = np.cumsum([len(d) for d in mats])
+    desc = np.concatenate(mats) if offs[-1] else np.zeros((1, 32), np.uint8)
+    best = np.zeros(max(len(mats), 1), np.int32)
+    out = np.zeros((max(len(mats), 1), 32), np.uint8)
+    _check("orb_distinctive_descriptors",
+           lib().orb_distinctive_descriptors(ctx.h, len(mats), _ptr(desc), _ptr(offs), _ptr(best), _ptr(out)))
+    return best[:len(mats)], out[:len(mats)]
 
 
 class Vocabulary:

@@ -22,6 +22,8 @@ extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const
                                           hipStream_t st);
 
 extern "C" hipError_t orb_launch_match_pairs(const MatchArgs* d_args, int npairs, int nwaves, hipStream_t st);
+extern "C" hipError_t orb_launch_distinctive(const uint8_t* desc, const int* offsets, int n_points, int* best,
+                                             uint8_t* out_desc, hipStream_t st);
 extern "C" hipError_t orb_launch_hbm_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 extern "C" int orb_knn_chunks(long long nm);
 extern "C" hipError_t orb_launch_knn2(const uint8_t* q, int nq, const uint8_t* m, long long nm, int method,
@@ -107,6 +109,12 @@ struct orb_context {
     uint8_t* d_km = nullptr;
     int* d_kout = nullptr;
     size_t kq_cap = 0, km_cap = 0;
+    // map-point scratch: observation descriptors, offsets, chosen rows
+    uint8_t* d_mp = nullptr;
+    int* d_mpo = nullptr;
+    int* d_mpb = nullptr;
+    uint8_t* d_mpd = nullptr;
+    size_t mp_cap = 0, mpo_cap = 0, mpb_cap = 0, mpd_cap = 0;
 
     int grow(void** p, size_t* cap, size_t bytes)
     {

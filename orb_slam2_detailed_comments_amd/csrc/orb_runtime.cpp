@@ -48,6 +48,9 @@ void orb_context_destroy(orb_context* ctx)
     if (ctx->d_kq) (void)hipFree(ctx->d_kq);
     if (ctx->d_km) (void)hipFree(ctx->d_km);
     if (ctx->d_kout) (void)hipFree(ctx->d_kout);
+    void* mp[] = {ctx->d_mp, ctx->d_mpo, ctx->d_mpb, ctx->d_mpd};
+    for (void* q : mp)
+        if (q) (void)hipFree(q);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -851,6 +854,74 @@ int orb_knn2_time(orb_context* ctx, const uint8_t* query, int nq, const uint8_t*
     if (rc) return rc;
     if (ms_search) *ms_search = (float)(a / steps);
     if (ms_merge) *ms_merge = (float)(b / steps);
+    return ORB_OK;
+}
+
+}  // extern "C"
+
+// MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450) for
+// n_points points; point p's observations are rows offsets[p] ..
+// offsets[p+1]-1 of desc, in the order the reference would visit them.
+static int distinctive_stage(orb_context* ctx, int n_points, const uint8_t* desc, const int32_t* offsets)
+{
+    const int rows = offsets[n_points];
+    if (offsets[0] < 0 || rows < offsets[0]) return ORB_E_ARG;
+    for (int p = 0; p < n_points; p++) {
+        if (offsets[p + 1] < offsets[p]) return ORB_E_ARG;
+        if (offsets[p + 1] - offsets[p] > 1024) return ORB_E_CAPACITY;
+    }
+    int rc = ctx->grow((void**)&ctx->d_mp, &ctx->mp_cap, (size_t)std::max(rows, 1) * 32);
+    if (!rc) rc = ctx->grow((void**)&ctx->d_mpo, &ctx->mpo_cap, (size_t)(n_points + 1) * 4);
+    if (!rc) rc = ctx->grow((void**)&ctx->d_mpb, &ctx->mpb_cap, (size_t)n_points * 4);
+    if (!rc) rc = ctx->grow((void**)&ctx->d_mpd, &ctx->mpd_cap, (size_t)n_points * 32);
+    if (rc) return rc;
+    if (rows) HIP_TRY(hipMemcpyAsync(ctx->d_mp, desc, (size_t)rows * 32, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->d_mpo, offsets, (size_t)(n_points + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    return ORB_OK;
+}
+
+extern "C" {
+
+int orb_distinctive_descriptors(orb_context* ctx, int n_points, const uint8_t* desc, const int32_t* offsets,
+                                int32_t* best, uint8_t* out_desc)
+{
+    if (!ctx || n_points < 0 || (n_points && (!offsets || !best))) return ORB_E_ARG;
+    if (n_points == 0) return ORB_OK;
+    if (offsets[n_points] > 0 && !desc) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = distinctive_stage(ctx, n_points, desc, offsets);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(orb_launch_distinctive(ctx->d_mp, ctx->d_mpo, n_points, ctx->d_mpb, out_desc ? ctx->d_mpd : nullptr, st));
+    HIP_TRY(hipMemcpyAsync(best, ctx->d_mpb, (size_t)n_points * 4, hipMemcpyDeviceToHost, st));
+    if (out_desc) HIP_TRY(hipMemcpyAsync(out_desc, ctx->d_mpd, (size_t)n_points * 32, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return ORB_OK;
+}
+
+int orb_distinctive_descriptors_time(orb_context* ctx, int n_points, const uint8_t* desc, const int32_t* offsets,
+                                     int steps, float* ms)
+{
+    if (!ctx || n_points <= 0 || !desc || !offsets || steps <= 0 || !ms) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = distinctive_stage(ctx, n_points, desc, offsets);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(orb_launch_distinctive(ctx->d_mp, ctx->d_mpo, n_points, ctx->d_mpb, ctx->d_mpd, st));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    (void)hipEventRecord(e0, st);
+    for (int s = 0; s < steps; s++)
+        (void)orb_launch_distinctive(ctx->d_mp, ctx->d_mpo, n_points, ctx->d_mpb, ctx->d_mpd, st);
+    (void)hipEventRecord(e1, st);
+    const hipError_t se = hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (se != hipSuccess) return ORB_E_DEVICE;
+    *ms = t / steps;
     return ORB_OK;
 }
 
