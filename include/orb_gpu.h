@@ -85,6 +85,16 @@ int orb_extract_device(orb_extractor* ex, const uint8_t* d_image, int width, int
 int orb_extractor_results(orb_extractor* ex, orb_keypoint* keypoints, int capacity, uint8_t* descriptors,
                           int* n);
 
+/* Colour input, converted on the device in front of the pyramid exactly as
+ * cvtColor does for 8-bit images (src/Tracking.cc:250-276): channels 3 or 4,
+ * rgb = 1 for RGB/RGBA (CV_RGB2GRAY, CV_RGBA2GRAY), 0 for BGR/BGRA;
+ * gray = (R*4899 + G*9617 + B*1868 + 8192) >> 14.  Otherwise as orb_extract. */
+int orb_extract_color(orb_extractor* ex, const uint8_t* image, int width, int height, size_t stride, int channels,
+                      int rgb, orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n);
+/* The conversion alone, into a caller's gray buffer. */
+int orb_cvt_color_gray(orb_context* ctx, const uint8_t* image, int width, int height, size_t stride, int channels,
+                       int rgb, uint8_t* gray, size_t gray_stride);
+
 /* mvImagePyramid[level] (ORBextractor.h:162) of the last call, copied to a
  * host buffer (dst may be NULL to query the size). */
 int orb_extractor_pyramid_level(orb_extractor* ex, int level, uint8_t* dst, size_t dst_stride, int* width,

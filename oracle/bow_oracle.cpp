@@ -342,4 +342,20 @@ int oracle_distinctive_descriptors(const uint8_t* desc, const int32_t* offsets, 
     return 0;
 }
 
+// cvtColor(CV_{RGB,BGR,RGBA,BGRA}2GRAY) for 8-bit images, OpenCV 2.4/3.x
+// (absent here) as ORB-SLAM2 calls it in Tracking::GrabImage*
+// (src/Tracking.cc:250-276): 14-bit fixed-point weights R 4899, G 9617,
+// B 1868 (round(16384 * {0.299, 0.587, 0.114})), rounding by + 2^13.
+// bidx: the channel holding blue (0 for BGR/BGRA, 2 for RGB/RGBA).
+int oracle_cvt_gray(const uint8_t* src, int w, int h, int stride, int scn, int bidx, uint8_t* dst)
+{
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const uint8_t* p = src + (size_t)y * stride + (size_t)x * scn;
+            const int b = p[bidx], g = p[1], r = p[bidx ^ 2];
+            dst[(size_t)y * w + x] = (uint8_t)((b * 1868 + g * 9617 + r * 4899 + (1 << 13)) >> 14);
+        }
+    return 0;
+}
+
 }  // extern "C"

@@ -135,6 +135,9 @@ int oracle_voc_bow(const oracle_vocabulary* voc, const uint8_t* desc, int n, int
                    int32_t* bow_word, double* bow_value, int* n_bow,
                    int32_t* fv_node, int32_t* fv_feature, int* n_fv);
 
+/* cvtColor to gray, 8-bit (Tracking.cc:250-276): bidx = channel of blue. */
+int oracle_cvt_gray(const uint8_t* src, int w, int h, int stride, int scn, int bidx, uint8_t* dst);
+
 /* MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450), many
  * points at once: best[p] indexes point p's rows offsets[p]..offsets[p+1]-1
  * (-1 when it has none). */

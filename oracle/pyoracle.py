@@ -61,6 +61,7 @@ def lib():
             "oracle_voc_transform": (I, [P, P, I, I, P, P, P]),
             "oracle_voc_bow": (I, [P, P, I, I, P, P, P, P, P, P]),
             "oracle_distinctive_descriptors": (I, [P, P, I, P]),
+            "oracle_cvt_gray": (I, [P, I, I, I, I, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -325,3 +326,12 @@ def distinctive_descriptors(points):
     best = np.zeros(max(len(mats), 1), np.int32)
     lib().oracle_distinctive_descriptors(_p(desc), _p(offs), len(mats), _p(best))
     return best[:len(mats)]
+
+
+def cvt_gray(img, rgb=True):
+    """cvtColor(CV_RGB2GRAY / CV_BGR2GRAY / ...A2GRAY) of an h x w x 3|4 u8 image."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w, scn = img.shape
+    out = np.zeros((h, w), np.uint8)
+    lib().oracle_cvt_gray(_p(img), w, h, w * scn, scn, 2 if rgb else 0, _p(out))
+    return out

@@ -25,7 +25,7 @@ import os
 import numpy as np
 
 __all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "Vocabulary", "match_bf",
-           "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "lib", "library_path",
+           "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "cvt_color_gray", "lib", "library_path",
            "OrbError"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
@@ -55,6 +55,8 @@ SIGNATURES = {
     "orb_extract_device": (_I, [_P, _P, _I, _I, _S]),
     "orb_extractor_results": (_I, [_P, _P, _I, _P, _P]),
     "orb_extractor_pyramid_level": (_I, [_P, _I, _P, _S, _P, _P]),
+    "orb_extract_color": (_I, [_P, _P, _I, _I, _S, _I, _I, _P, _I, _P, _P]),
+    "orb_cvt_color_gray": (_I, [_P, _P, _I, _I, _S, _I, _I, _P, _S]),
     "orb_descriptor_distance": (_I, [_P, _P]),
     "orb_match_bf": (_I, [_P, _P, _I, _P, _I, _I, _F, _P, _P]),
     "orb_stereo_match": (_I, [_P, _P, _P, _P, _I, _P, _P, _I, _F, _F, _P, _P]),
@@ -238,6 +240,24 @@ class ORBextractor:
         n = ctypes.c_int()
         _check("orb_extract", lib().orb_extract(self.h, _ptr(img), w, h, img.strides[0], _ptr(kps), cap,
                                                 _ptr(desc), ctypes.byref(n)))
+        self._size = (w, h)
+        return kps[:n.value].copy(), desc[:n.value].copy()
+
+    def extract_color(self, image, rgb=True):
+        """The same for an h x w x 3|4 colour image, converted on the device as
+        cvtColor(CV_RGB2GRAY / CV_BGR2GRAY / ...A2GRAY) does (Tracking.cc:250-276)."""
+        img = np.ascontiguousarray(image, np.uint8)
+        if img.ndim != 3 or img.shape[2] not in (3, 4):
+            raise ValueError("image must be h x w x 3 or 4 uint8")
+        h, w, c = img.shape
+        if img.size == 0:
+            return np.zeros(0, KEYPOINT_DTYPE), np.zeros((0, 32), np.uint8)
+        cap = self.capacity(w, h)
+        kps = np.zeros(cap, KEYPOINT_DTYPE)
+        desc = np.zeros((cap, 32), np.uint8)
+        n = ctypes.c_int()
+        _check("orb_extract_color", lib().orb_extract_color(self.h, _ptr(img), w, h, w * c, c, 1 if rgb else 0,
+                                                            _ptr(kps), cap, _ptr(desc), ctypes.byref(n)))
         self._size = (w, h)
         return kps[:n.value].copy(), desc[:n.value].copy()
 
@@ -521,6 +541,17 @@ class Server:
         _check("orb_server_stats", lib().orb_server_stats(self.h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(u),
                                                           ctypes.byref(sl)))
         return dict(batches=b.value, requests=r.value, hbm_used=u.value, slots=sl.value)
+
+
+def cvt_color_gray(image, rgb=True, context=None):
+    """cvtColor to gray of an h x w x 3|4 uint8 image on the device."""
+    ctx = context or Context.default()
+    img = np.ascontiguousarray(image, np.uint8)
+    h, w, c = img.shape
+    out = np.zeros((h, w), np.uint8)
+    _check("orb_cvt_color_gray", lib().orb_cvt_color_gray(ctx.h, _ptr(img), w, h, w * c, c, 1 if rgb else 0,
+                                                          _ptr(out), w))
+    return out
 
 
 def distinctive_descriptors(points, context=None):

@@ -22,6 +22,8 @@ extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const
                                           hipStream_t st);
 
 extern "C" hipError_t orb_launch_match_pairs(const MatchArgs* d_args, int npairs, int nwaves, hipStream_t st);
+extern "C" hipError_t orb_launch_gray(const uint8_t* src, size_t sstride, int scn, int bidx, int w, int h,
+                                      uint8_t* dst, size_t dstride, hipStream_t st);
 extern "C" hipError_t orb_launch_distinctive(const uint8_t* desc, const int* offsets, int n_points, int* best,
                                              uint8_t* out_desc, hipStream_t st);
 extern "C" hipError_t orb_launch_hbm_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
@@ -109,6 +111,9 @@ struct orb_context {
     uint8_t* d_km = nullptr;
     int* d_kout = nullptr;
     size_t kq_cap = 0, km_cap = 0;
+    // colour input staging
+    uint8_t* d_color = nullptr;
+    size_t color_cap = 0;
     // map-point scratch: observation descriptors, offsets, chosen rows
     uint8_t* d_mp = nullptr;
     int* d_mpo = nullptr;

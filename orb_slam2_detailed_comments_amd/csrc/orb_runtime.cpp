@@ -48,7 +48,7 @@ void orb_context_destroy(orb_context* ctx)
     if (ctx->d_kq) (void)hipFree(ctx->d_kq);
     if (ctx->d_km) (void)hipFree(ctx->d_km);
     if (ctx->d_kout) (void)hipFree(ctx->d_kout);
-    void* mp[] = {ctx->d_mp, ctx->d_mpo, ctx->d_mpb, ctx->d_mpd};
+    void* mp[] = {ctx->d_mp, ctx->d_mpo, ctx->d_mpb, ctx->d_mpd, ctx->d_color};
     for (void* q : mp)
         if (q) (void)hipFree(q);
     (void)hipStreamDestroy(ctx->stream);
@@ -156,6 +156,62 @@ int orb_extract(orb_extractor* ex, const uint8_t* image, int width, int height, 
     if (rc) return rc;
     HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, ex->ctx->stream, nullptr));
     return ex->read_results(keypoints, capacity, descriptors, n);
+}
+
+}  // extern "C"
+
+// Colour rows -> the gray staging buffer of an extractor (or any pitched
+// device destination), through the context's colour staging buffer.
+static int upload_color(orb_context* ctx, const uint8_t* image, int width, int height, size_t stride, int channels,
+                        int rgb, uint8_t* d_dst, size_t dst_pitch)
+{
+    const size_t row = (size_t)width * channels;
+    int rc = ctx->grow((void**)&ctx->d_color, &ctx->color_cap, row * height);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    if (stride == row)
+        HIP_TRY(hipMemcpyAsync(ctx->d_color, image, row * height, hipMemcpyHostToDevice, st));
+    else
+        HIP_TRY(hipMemcpy2DAsync(ctx->d_color, row, image, stride, row, height, hipMemcpyHostToDevice, st));
+    HIP_TRY(orb_launch_gray(ctx->d_color, row, channels, rgb ? 2 : 0, width, height, d_dst, dst_pitch, st));
+    return ORB_OK;
+}
+
+extern "C" {
+
+int orb_extract_color(orb_extractor* ex, const uint8_t* image, int width, int height, size_t stride, int channels,
+                      int rgb, orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n)
+{
+    if (!ex || !n || width < 0 || height < 0 || (channels != 3 && channels != 4)) return ORB_E_ARG;
+    *n = 0;
+    if (width == 0 || height == 0) return ORB_OK;
+    if (!image || stride < (size_t)width * channels) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ex->ctx->device));
+    int rc = ex->ensure(width, height);
+    if (rc) return rc;
+    rc = upload_color(ex->ctx, image, width, height, stride, channels, rgb,
+                      const_cast<uint8_t*>(ex->hjob.src), (size_t)ex->hjob.src_pitch);
+    if (rc) return rc;
+    HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, ex->ctx->stream, nullptr));
+    return ex->read_results(keypoints, capacity, descriptors, n);
+}
+
+int orb_cvt_color_gray(orb_context* ctx, const uint8_t* image, int width, int height, size_t stride, int channels,
+                       int rgb, uint8_t* gray, size_t gray_stride)
+{
+    if (!ctx || width < 0 || height < 0 || (channels != 3 && channels != 4)) return ORB_E_ARG;
+    if (width == 0 || height == 0) return ORB_OK;
+    if (!image || !gray || stride < (size_t)width * channels || gray_stride < (size_t)width) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint8_t* d_gray = nullptr;
+    if (hipMalloc(&d_gray, (size_t)width * height) != hipSuccess) return ORB_E_OOM;
+    int rc = upload_color(ctx, image, width, height, stride, channels, rgb, d_gray, (size_t)width);
+    if (!rc && hipMemcpy2DAsync(gray, gray_stride, d_gray, width, width, height, hipMemcpyDeviceToHost,
+                                ctx->stream) != hipSuccess)
+        rc = ORB_E_DEVICE;
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ORB_E_DEVICE;
+    (void)hipFree(d_gray);
+    return rc;
 }
 
 int orb_extractor_pyramid_level(orb_extractor* ex, int level, uint8_t* dst, size_t dst_stride, int* width,
