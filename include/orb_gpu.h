@@ -95,6 +95,27 @@ int orb_extract_color(orb_extractor* ex, const uint8_t* image, int width, int he
 int orb_cvt_color_gray(orb_context* ctx, const uint8_t* image, int width, int height, size_t stride, int channels,
                        int rgb, uint8_t* gray, size_t gray_stride);
 
+/* Rectification in front of the pyramid (Examples/Stereo/stereo_euroc.cc
+ * :124-188).  orb_rectify_create computes cv::initUndistortRectifyMap(K, D,
+ * R, P, (width, height), CV_32FC1) on the host in double (K, R, P row-major
+ * 3x3, R may be NULL for identity; D has 0, 4, 5, 8 or 12 coefficients in
+ * OpenCV's order k1 k2 p1 p2 k3 k4 k5 k6 s1 s2 s3 s4) and keeps the maps in
+ * HBM; orb_rectify_from_maps takes maps the caller already has.  orb_remap is
+ * cv::remap(INTER_LINEAR) with the default zero border; orb_extract_rectified
+ * remaps straight into the extractor's input and extracts (the output size is
+ * the map's). */
+typedef struct orb_rectify orb_rectify;
+int orb_rectify_create(orb_context* ctx, const double* K, const double* D, int nD, const double* R, const double* P,
+                       int width, int height, orb_rectify** out);
+int orb_rectify_from_maps(orb_context* ctx, const float* mapx, const float* mapy, int width, int height,
+                          orb_rectify** out);
+void orb_rectify_destroy(orb_rectify* r);
+int orb_rectify_maps(orb_rectify* r, float* mapx, float* mapy);
+int orb_remap(orb_rectify* r, const uint8_t* src, int src_width, int src_height, size_t src_stride, uint8_t* dst,
+              size_t dst_stride);
+int orb_extract_rectified(orb_extractor* ex, orb_rectify* r, const uint8_t* src, int src_width, int src_height,
+                          size_t src_stride, orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n);
+
 /* mvImagePyramid[level] (ORBextractor.h:162) of the last call, copied to a
  * host buffer (dst may be NULL to query the size). */
 int orb_extractor_pyramid_level(orb_extractor* ex, int level, uint8_t* dst, size_t dst_stride, int* width,

@@ -359,3 +359,107 @@ int oracle_cvt_gray(const uint8_t* src, int w, int h, int stride, int scn, int b
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- rectification
+// cv::initUndistortRectifyMap(K, D, R, P, size, CV_32FC1, map1, map2) and
+// cv::remap(src, dst, map1, map2, INTER_LINEAR) with the default constant
+// zero border, as Examples/Stereo/stereo_euroc.cc:124-188 calls them
+// (OpenCV 2.4/3.x, absent here, restated from its published algorithm).
+namespace {
+
+// Matx33d::inv(DECOMP_LU) for 3x3: adjugate over the determinant.
+bool inv3(const double a[9], double b[9])
+{
+    const double d = a[0] * (a[4] * a[8] - a[7] * a[5]) - a[1] * (a[3] * a[8] - a[6] * a[5]) +
+                     a[2] * (a[3] * a[7] - a[6] * a[4]);
+    if (d == 0) return false;
+    const double id = 1 / d;
+    b[0] = (a[4] * a[8] - a[5] * a[7]) * id;
+    b[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    b[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    b[3] = (a[5] * a[6] - a[3] * a[8]) * id;
+    b[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    b[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    b[6] = (a[3] * a[7] - a[4] * a[6]) * id;
+    b[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    b[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// D holds nD coefficients in OpenCV's order k1 k2 p1 p2 [k3 [k4 k5 k6 [s1 s2
+// s3 s4]]] (nD = 0, 4, 5, 8 or 12; the tilted model is not restated).  R may
+// be NULL (identity).  P is the 3x3 new camera matrix.
+int oracle_init_rectify_map(const double* K, const double* D, int nD, const double* R, const double* P, int w, int h,
+                            float* mapx, float* mapy)
+{
+    if (!(nD == 0 || nD == 4 || nD == 5 || nD == 8 || nD == 12)) return -1;
+    double c[12] = {0};
+    for (int i = 0; i < nD; i++) c[i] = D[i];
+    const double k1 = c[0], k2 = c[1], p1 = c[2], p2 = c[3], k3 = c[4], k4 = c[5], k5 = c[6], k6 = c[7];
+    const double s1 = c[8], s2 = c[9], s3 = c[10], s4 = c[11];
+    double Rm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (R)
+        for (int i = 0; i < 9; i++) Rm[i] = R[i];
+    double PR[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) PR[i * 3 + j] = P[i * 3] * Rm[j] + P[i * 3 + 1] * Rm[3 + j] + P[i * 3 + 2] * Rm[6 + j];
+    double ir[9];
+    if (!inv3(PR, ir)) return -1;
+    const double u0 = K[2], v0 = K[5], fx = K[0], fy = K[4];
+    for (int i = 0; i < h; i++) {
+        double _x = i * ir[1] + ir[2], _y = i * ir[4] + ir[5], _w = i * ir[7] + ir[8];
+        for (int j = 0; j < w; j++, _x += ir[0], _y += ir[3], _w += ir[6]) {
+            const double ww = 1. / _w, x = _x * ww, y = _y * ww;
+            const double x2 = x * x, y2 = y * y;
+            const double r2 = x2 + y2, _2xy = 2 * x * y;
+            const double kr = (1 + ((k3 * r2 + k2) * r2 + k1) * r2) / (1 + ((k6 * r2 + k5) * r2 + k4) * r2);
+            const double xd = (x * kr + p1 * _2xy + p2 * (r2 + 2 * x2) + s1 * r2 + s2 * r2 * r2);
+            const double yd = (y * kr + p1 * (r2 + 2 * y2) + p2 * _2xy + s3 * r2 + s4 * r2 * r2);
+            // the identity tilt: vecTilt = (xd, yd, 1), invProj = 1
+            const double u = fx * 1.0 * xd + u0;
+            const double v = fy * 1.0 * yd + v0;
+            mapx[(size_t)i * w + j] = (float)u;
+            mapy[(size_t)i * w + j] = (float)v;
+        }
+    }
+    return 0;
+}
+
+// remap INTER_LINEAR, 8-bit, border constant 0: X = cvRound(mapx*32),
+// Y = cvRound(mapy*32); taps at (X>>5, Y>>5), weights from the 32x32 table of
+// products (1-fx)(1-fy), fx(1-fy), (1-fx)fy, fx fy scaled by 2^15 (exact
+// integers here), out = (sum + 2^14) >> 15; taps outside the source read 0.
+int oracle_remap_linear(const uint8_t* src, int sw, int sh, int sstride, const float* mapx, const float* mapy, int w,
+                        int h, uint8_t* dst)
+{
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+            const float fxm = mapx[(size_t)i * w + j] * 32.f, fym = mapy[(size_t)i * w + j] * 32.f;
+            const double cx = std::min(std::max((double)fxm, -2147483648.0), 2147483647.0);
+            const double cy = std::min(std::max((double)fym, -2147483648.0), 2147483647.0);
+            const int X = (int)std::nearbyint(cx), Y = (int)std::nearbyint(cy);
+            const int sx = std::min(std::max(X >> 5, -32768), 32767), sy = std::min(std::max(Y >> 5, -32768), 32767);
+            const int ax = X & 31, ay = Y & 31;
+            const int w00 = (32 - ay) * (32 - ax) * 32, w01 = (32 - ay) * ax * 32;
+            const int w10 = ay * (32 - ax) * 32, w11 = ay * ax * 32;
+            uint8_t out;
+            if (sx >= sw || sx + 1 < 0 || sy >= sh || sy + 1 < 0) {
+                out = 0;
+            } else {
+                auto at = [&](int x, int y) -> int {
+                    return (x >= 0 && x < sw && y >= 0 && y < sh) ? src[(size_t)y * sstride + x] : 0;
+                };
+                const int v = at(sx, sy) * w00 + at(sx + 1, sy) * w01 + at(sx, sy + 1) * w10 + at(sx + 1, sy + 1) * w11;
+                const int r = (v + (1 << 14)) >> 15;
+                out = (uint8_t)std::min(std::max(r, 0), 255);
+            }
+            dst[(size_t)i * w + j] = out;
+        }
+    return 0;
+}
+
+}  // extern "C"

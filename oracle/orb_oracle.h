@@ -138,6 +138,13 @@ int oracle_voc_bow(const oracle_vocabulary* voc, const uint8_t* desc, int n, int
 /* cvtColor to gray, 8-bit (Tracking.cc:250-276): bidx = channel of blue. */
 int oracle_cvt_gray(const uint8_t* src, int w, int h, int stride, int scn, int bidx, uint8_t* dst);
 
+/* initUndistortRectifyMap (CV_32FC1 maps) and remap INTER_LINEAR 8-bit with a
+ * zero constant border (Examples/Stereo/stereo_euroc.cc:124-188). */
+int oracle_init_rectify_map(const double* K, const double* D, int nD, const double* R, const double* P,
+                            int w, int h, float* mapx, float* mapy);
+int oracle_remap_linear(const uint8_t* src, int sw, int sh, int sstride, const float* mapx,
+                        const float* mapy, int w, int h, uint8_t* dst);
+
 /* MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450), many
  * points at once: best[p] indexes point p's rows offsets[p]..offsets[p+1]-1
  * (-1 when it has none). */

@@ -62,6 +62,8 @@ def lib():
             "oracle_voc_bow": (I, [P, P, I, I, P, P, P, P, P, P]),
             "oracle_distinctive_descriptors": (I, [P, P, I, P]),
             "oracle_cvt_gray": (I, [P, I, I, I, I, I, P]),
+            "oracle_init_rectify_map": (I, [P, P, I, P, P, I, I, P, P]),
+            "oracle_remap_linear": (I, [P, I, I, I, P, P, I, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -334,4 +336,29 @@ def cvt_gray(img, rgb=True):
     h, w, scn = img.shape
     out = np.zeros((h, w), np.uint8)
     lib().oracle_cvt_gray(_p(img), w, h, w * scn, scn, 2 if rgb else 0, _p(out))
+    return out
+
+
+def init_rectify_map(K, D, R, P, w, h):
+    """initUndistortRectifyMap -> (mapx, mapy) float32 h x w."""
+    K = np.ascontiguousarray(K, np.float64).reshape(9)
+    D = np.ascontiguousarray(D if D is not None else [], np.float64).reshape(-1)
+    R = None if R is None else np.ascontiguousarray(R, np.float64).reshape(9)
+    P = np.ascontiguousarray(P, np.float64).reshape(3, -1)[:, :3].copy().reshape(9)
+    mx = np.zeros((h, w), np.float32)
+    my = np.zeros((h, w), np.float32)
+    rc = lib().oracle_init_rectify_map(_p(K), _p(D) if len(D) else None, len(D), _p(R), _p(P), w, h, _p(mx), _p(my))
+    if rc != 0:
+        raise ValueError("bad rectification parameters")
+    return mx, my
+
+
+def remap_linear(src, mapx, mapy):
+    """remap(src, mapx, mapy, INTER_LINEAR), border constant 0."""
+    src = np.ascontiguousarray(src, np.uint8)
+    mapx = np.ascontiguousarray(mapx, np.float32)
+    mapy = np.ascontiguousarray(mapy, np.float32)
+    h, w = mapx.shape
+    out = np.zeros((h, w), np.uint8)
+    lib().oracle_remap_linear(_p(src), src.shape[1], src.shape[0], src.shape[1], _p(mapx), _p(mapy), w, h, _p(out))
     return out

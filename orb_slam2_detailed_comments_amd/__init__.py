@@ -25,7 +25,7 @@ import os
 import numpy as np
 
 __all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "Vocabulary", "match_bf",
-           "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "cvt_color_gray", "lib", "library_path",
+           "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "cvt_color_gray", "Rectifier", "lib", "library_path",
            "OrbError"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
@@ -57,6 +57,12 @@ SIGNATURES = {
     "orb_extractor_pyramid_level": (_I, [_P, _I, _P, _S, _P, _P]),
     "orb_extract_color": (_I, [_P, _P, _I, _I, _S, _I, _I, _P, _I, _P, _P]),
     "orb_cvt_color_gray": (_I, [_P, _P, _I, _I, _S, _I, _I, _P, _S]),
+    "orb_rectify_create": (_I, [_P, _P, _P, _I, _P, _P, _I, _I, _P]),
+    "orb_rectify_from_maps": (_I, [_P, _P, _P, _I, _I, _P]),
+    "orb_rectify_destroy": (_V, [_P]),
+    "orb_rectify_maps": (_I, [_P, _P, _P]),
+    "orb_remap": (_I, [_P, _P, _I, _I, _S, _P, _S]),
+    "orb_extract_rectified": (_I, [_P, _P, _P, _I, _I, _S, _P, _I, _P, _P]),
     "orb_descriptor_distance": (_I, [_P, _P]),
     "orb_match_bf": (_I, [_P, _P, _I, _P, _I, _I, _F, _P, _P]),
     "orb_stereo_match": (_I, [_P, _P, _P, _P, _I, _P, _P, _I, _F, _F, _P, _P]),
@@ -552,6 +558,64 @@ def cvt_color_gray(image, rgb=True, context=None):
     _check("orb_cvt_color_gray", lib().orb_cvt_color_gray(ctx.h, _ptr(img), w, h, w * c, c, 1 if rgb else 0,
                                                           _ptr(out), w))
     return out
+
+
+class Rectifier:
+    """Stereo rectification maps in HBM (cv::initUndistortRectifyMap, CV_32FC1)
+    and cv::remap(INTER_LINEAR) on the device (stereo_euroc.cc:124-188).
+    Rectifier(K, D, R, P, width, height) or Rectifier(mapx=..., mapy=...)."""
+
+    def __init__(self, K=None, D=None, R=None, P=None, width=None, height=None, mapx=None, mapy=None, context=None):
+        self.ctx = context or Context.default()
+        h = ctypes.c_void_p()
+        if mapx is not None:
+            mx = np.ascontiguousarray(mapx, np.float32)
+            my = np.ascontiguousarray(mapy, np.float32)
+            self.height, self.width = mx.shape
+            _check("orb_rectify_from_maps", lib().orb_rectify_from_maps(self.ctx.h, _ptr(mx), _ptr(my), self.width,
+                                                                        self.height, ctypes.byref(h)))
+        else:
+            K = np.ascontiguousarray(K, np.float64).reshape(9)
+            D = np.ascontiguousarray(D if D is not None else [], np.float64).reshape(-1)
+            R = None if R is None else np.ascontiguousarray(R, np.float64).reshape(9)
+            P = np.ascontiguousarray(np.asarray(P, np.float64).reshape(3, -1)[:, :3]).reshape(9)
+            self.width, self.height = int(width), int(height)
+            _check("orb_rectify_create",
+                   lib().orb_rectify_create(self.ctx.h, _ptr(K), _ptr(D) if len(D) else None, len(D), _ptr(R),
+                                            _ptr(P), self.width, self.height, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orb_rectify_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def maps(self):
+        mx = np.zeros((self.height, self.width), np.float32)
+        my = np.zeros((self.height, self.width), np.float32)
+        _check("orb_rectify_maps", lib().orb_rectify_maps(self.h, _ptr(mx), _ptr(my)))
+        return mx, my
+
+    def remap(self, image):
+        img = np.ascontiguousarray(image, np.uint8)
+        out = np.zeros((self.height, self.width), np.uint8)
+        _check("orb_remap", lib().orb_remap(self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1], _ptr(out),
+                                            self.width))
+        return out
+
+    def extract(self, extractor, image):
+        """Rectify on the device, then extract: (keypoints, descriptors)."""
+        img = np.ascontiguousarray(image, np.uint8)
+        cap = extractor.capacity(self.width, self.height)
+        kps = np.zeros(cap, KEYPOINT_DTYPE)
+        desc = np.zeros((cap, 32), np.uint8)
+        n = ctypes.c_int()
+        _check("orb_extract_rectified",
+               lib().orb_extract_rectified(extractor.h, self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1],
+                                           _ptr(kps), cap, _ptr(desc), ctypes.byref(n)))
+        return kps[:n.value].copy(), desc[:n.value].copy()
 
 
 def distinctive_descriptors(points, context=None):
