@@ -281,6 +281,22 @@ int orb_vocabulary_bow(orb_vocabulary* v, int n_images, const uint8_t* desc, con
 int orb_vocabulary_bow_time(orb_vocabulary* v, int n_images, const uint8_t* desc, const int32_t* offsets,
                             int levelsup, int steps, float* ms);
 
+/* ---------------------------------------------------------------- windowed matching
+ * ORBmatcher::SearchForInitialization (src/ORBmatcher.cc:573-717) over the
+ * Frame keypoint grid (64 x 48 cells, src/Frame.cc:399-423, 590-700), for
+ * n_pairs frame pairs at once.  Pair p's F1 keypoints (mvKeysUn) and
+ * descriptors are rows off1[p] .. off1[p+1]-1, F2's rows off2[p] ..
+ * off2[p+1]-1 (at most 4096 each); bounds holds mnMinX, mnMaxX, mnMinY,
+ * mnMaxY per pair; prev_matched (x, y per F1 keypoint) is vbPrevMatched,
+ * updated in place for the matched keypoints.  The initializer calls it with
+ * window 100, ratio 0.9 and the orientation check (Tracking.cc:915-927).
+ * matches12 receives vnMatches12 per F1 keypoint, nmatches the count per
+ * pair. */
+int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypoint* kp1, const uint8_t* desc1,
+                                  const int32_t* off1, const orb_keypoint* kp2, const uint8_t* desc2,
+                                  const int32_t* off2, const float* bounds, float* prev_matched, int window_size,
+                                  float nn_ratio, int check_orientation, int32_t* matches12, int32_t* nmatches);
+
 /* ---------------------------------------------------------------- map points
  * MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450) for
  * n_points points at once.  Point p's observation descriptors are rows

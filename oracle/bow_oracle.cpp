@@ -463,3 +463,167 @@ int oracle_remap_linear(const uint8_t* src, int sw, int sh, int sstride, const f
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- windowed matching
+// ORBmatcher::SearchForInitialization (src/ORBmatcher.cc:573-717) with the
+// keypoint grid of Frame::AssignFeaturesToGrid / PosInGrid /
+// GetFeaturesInArea (src/Frame.cc:399-423, 590-700; this fork tests a circle,
+// distx*distx + disty*disty < r*r) and ORBmatcher::ComputeThreeMaxima
+// (:2035-2077).  kp arrays are mvKeysUn; bounds = mnMinX, mnMaxX, mnMinY,
+// mnMaxY; prev (2 floats per F1 keypoint) is vbPrevMatched, updated in place.
+namespace {
+
+const int kGridCols = 64, kGridRows = 48, kHisto = 30;
+
+struct Grid {
+    std::vector<std::vector<int>> cell;   // [ix * rows + iy] -> keypoint indices in index order
+    float minX, minY, invW, invH;
+};
+
+Grid build_grid(const oracle_keypoint* kp, int n, const float* b)
+{
+    Grid g;
+    g.cell.assign(kGridCols * kGridRows, {});
+    g.minX = b[0];
+    g.minY = b[2];
+    g.invW = static_cast<float>(kGridCols) / static_cast<float>(b[1] - b[0]);
+    g.invH = static_cast<float>(kGridRows) / static_cast<float>(b[3] - b[2]);
+    for (int i = 0; i < n; i++) {
+        const int px = (int)std::round((kp[i].x - g.minX) * g.invW);
+        const int py = (int)std::round((kp[i].y - g.minY) * g.invH);
+        if (px < 0 || px >= kGridCols || py < 0 || py >= kGridRows) continue;
+        g.cell[px * kGridRows + py].push_back(i);
+    }
+    return g;
+}
+
+std::vector<int> features_in_area(const Grid& g, const oracle_keypoint* kp, float x, float y, float r, int minLevel,
+                                  int maxLevel)
+{
+    std::vector<int> out;
+    const int nMinCellX = std::max(0, (int)std::floor((x - g.minX - r) * g.invW));
+    if (nMinCellX >= kGridCols) return out;
+    const int nMaxCellX = std::min(kGridCols - 1, (int)std::ceil((x - g.minX + r) * g.invW));
+    if (nMaxCellX < 0) return out;
+    const int nMinCellY = std::max(0, (int)std::floor((y - g.minY - r) * g.invH));
+    if (nMinCellY >= kGridRows) return out;
+    const int nMaxCellY = std::min(kGridRows - 1, (int)std::ceil((y - g.minY + r) * g.invH));
+    if (nMaxCellY < 0) return out;
+    const bool bCheckLevels = (minLevel > 0) || (maxLevel >= 0);
+    for (int ix = nMinCellX; ix <= nMaxCellX; ix++)
+        for (int iy = nMinCellY; iy <= nMaxCellY; iy++)
+            for (int i : g.cell[ix * kGridRows + iy]) {
+                if (bCheckLevels) {
+                    if (kp[i].octave < minLevel) continue;
+                    if (maxLevel >= 0 && kp[i].octave > maxLevel) continue;
+                }
+                const float distx = kp[i].x - x;
+                const float disty = kp[i].y - y;
+                if (distx * distx + disty * disty < r * r) out.push_back(i);
+            }
+    return out;
+}
+
+void three_maxima(const std::vector<int>* histo, int L, int& ind1, int& ind2, int& ind3)
+{
+    int max1 = 0, max2 = 0, max3 = 0;
+    for (int i = 0; i < L; i++) {
+        const int s = (int)histo[i].size();
+        if (s > max1) {
+            max3 = max2;
+            max2 = max1;
+            max1 = s;
+            ind3 = ind2;
+            ind2 = ind1;
+            ind1 = i;
+        } else if (s > max2) {
+            max3 = max2;
+            max2 = s;
+            ind3 = ind2;
+            ind2 = i;
+        } else if (s > max3) {
+            max3 = s;
+            ind3 = i;
+        }
+    }
+    if (max2 < 0.1f * (float)max1) {
+        ind2 = -1;
+        ind3 = -1;
+    } else if (max3 < 0.1f * (float)max1) {
+        ind3 = -1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_search_for_initialization(const oracle_keypoint* kp1, const uint8_t* d1, int n1,
+                                     const oracle_keypoint* kp2, const uint8_t* d2, int n2, const float* bounds,
+                                     float* prev, int windowSize, float nnratio, int checkOrientation,
+                                     int32_t* matches12)
+{
+    const int TH_LOW = 50;
+    int nmatches = 0;
+    for (int i = 0; i < n1; i++) matches12[i] = -1;
+    const Grid g = build_grid(kp2, n2, bounds);
+    std::vector<int> rotHist[kHisto];
+    const float factor = kHisto / 360.0f;
+    std::vector<int> vMatchedDistance(n2, 0x7FFFFFFF), vnMatches21(n2, -1);
+    for (int i1 = 0; i1 < n1; i1++) {
+        const int level1 = kp1[i1].octave;
+        if (level1 > 0) continue;
+        const std::vector<int> v2 =
+            features_in_area(g, kp2, prev[2 * i1], prev[2 * i1 + 1], (float)windowSize, level1, level1);
+        if (v2.empty()) continue;
+        int bestDist = 0x7FFFFFFF, bestDist2 = 0x7FFFFFFF, bestIdx2 = -1;
+        for (int i2 : v2) {
+            const int dist = forb_distance(d1 + (size_t)i1 * 32, d2 + (size_t)i2 * 32);
+            if (vMatchedDistance[i2] <= dist) continue;
+            if (dist < bestDist) {
+                bestDist2 = bestDist;
+                bestDist = dist;
+                bestIdx2 = i2;
+            } else if (dist < bestDist2) {
+                bestDist2 = dist;
+            }
+        }
+        if (bestDist <= TH_LOW && bestDist < (float)bestDist2 * nnratio) {
+            if (vnMatches21[bestIdx2] >= 0) {
+                matches12[vnMatches21[bestIdx2]] = -1;
+                nmatches--;
+            }
+            matches12[i1] = bestIdx2;
+            vnMatches21[bestIdx2] = i1;
+            vMatchedDistance[bestIdx2] = bestDist;
+            nmatches++;
+            if (checkOrientation) {
+                float rot = kp1[i1].angle - kp2[bestIdx2].angle;
+                if (rot < 0.0) rot += 360.0f;
+                int bin = (int)std::round(rot * factor);
+                if (bin == kHisto) bin = 0;
+                rotHist[bin].push_back(i1);
+            }
+        }
+    }
+    if (checkOrientation) {
+        int ind1 = -1, ind2 = -1, ind3 = -1;
+        three_maxima(rotHist, kHisto, ind1, ind2, ind3);
+        for (int i = 0; i < kHisto; i++) {
+            if (i == ind1 || i == ind2 || i == ind3) continue;
+            for (int idx1 : rotHist[i])
+                if (matches12[idx1] >= 0) {
+                    matches12[idx1] = -1;
+                    nmatches--;
+                }
+        }
+    }
+    for (int i1 = 0; i1 < n1; i1++)
+        if (matches12[i1] >= 0) {
+            prev[2 * i1] = kp2[matches12[i1]].x;
+            prev[2 * i1 + 1] = kp2[matches12[i1]].y;
+        }
+    return nmatches;
+}
+
+}  // extern "C"

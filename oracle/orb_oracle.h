@@ -145,6 +145,15 @@ int oracle_init_rectify_map(const double* K, const double* D, int nD, const doub
 int oracle_remap_linear(const uint8_t* src, int sw, int sh, int sstride, const float* mapx,
                         const float* mapy, int w, int h, uint8_t* dst);
 
+/* ORBmatcher::SearchForInitialization (src/ORBmatcher.cc:573-717) with the
+ * Frame keypoint grid (src/Frame.cc:399-423, 590-700).  bounds: mnMinX,
+ * mnMaxX, mnMinY, mnMaxY; prev: vbPrevMatched (x, y per F1 keypoint), updated
+ * in place.  Returns nmatches. */
+int oracle_search_for_initialization(const oracle_keypoint* kp1, const uint8_t* d1, int n1,
+                                     const oracle_keypoint* kp2, const uint8_t* d2, int n2,
+                                     const float* bounds, float* prev, int windowSize, float nnratio,
+                                     int checkOrientation, int32_t* matches12);
+
 /* MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450), many
  * points at once: best[p] indexes point p's rows offsets[p]..offsets[p+1]-1
  * (-1 when it has none). */

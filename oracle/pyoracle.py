@@ -64,6 +64,7 @@ def lib():
             "oracle_cvt_gray": (I, [P, I, I, I, I, I, P]),
             "oracle_init_rectify_map": (I, [P, P, I, P, P, I, I, P, P]),
             "oracle_remap_linear": (I, [P, I, I, I, P, P, I, I, P]),
+            "oracle_search_for_initialization": (I, [P, P, I, P, P, I, P, P, I, F, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -362,3 +363,17 @@ def remap_linear(src, mapx, mapy):
     out = np.zeros((h, w), np.uint8)
     lib().oracle_remap_linear(_p(src), src.shape[1], src.shape[0], src.shape[1], _p(mapx), _p(mapy), w, h, _p(out))
     return out
+
+
+def search_for_initialization(kp1, d1, kp2, d2, bounds, prev, window=100, nnratio=0.9, check_orientation=True):
+    """ORBmatcher::SearchForInitialization: (nmatches, matches12, updated prev)."""
+    kp1 = np.ascontiguousarray(kp1, KEYPOINT_DTYPE)
+    kp2 = np.ascontiguousarray(kp2, KEYPOINT_DTYPE)
+    d1 = np.ascontiguousarray(d1, np.uint8).reshape(-1, 32)
+    d2 = np.ascontiguousarray(d2, np.uint8).reshape(-1, 32)
+    b = np.ascontiguousarray(bounds, np.float32)
+    pv = np.array(prev, np.float32).reshape(-1, 2).copy()
+    m = np.zeros(max(len(kp1), 1), np.int32)
+    n = lib().oracle_search_for_initialization(_p(kp1), _p(d1), len(kp1), _p(kp2), _p(d2), len(kp2), _p(b), _p(pv),
+                                               int(window), float(nnratio), int(bool(check_orientation)), _p(m))
+    return n, m[:len(kp1)], pv

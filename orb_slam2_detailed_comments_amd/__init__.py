@@ -25,7 +25,7 @@ import os
 import numpy as np
 
 __all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "Vocabulary", "match_bf",
-           "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "cvt_color_gray", "Rectifier", "lib", "library_path",
+           "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "cvt_color_gray", "Rectifier", "search_for_initialization", "lib", "library_path",
            "OrbError"]
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
@@ -102,6 +102,7 @@ SIGNATURES = {
     "orb_vocabulary_bow": (_I, [_P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P]),
     "orb_vocabulary_bow_time": (_I, [_P, _I, _P, _P, _I, _I, _P]),
     "orb_distinctive_descriptors": (_I, [_P, _I, _P, _P, _P, _P]),
+    "orb_search_for_initialization": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _F, _I, _P, _P]),
     "orb_distinctive_descriptors_time": (_I, [_P, _I, _P, _P, _I, _P]),
     "orb_extractor_debug_image": (_I, [_P, _I, _I, _P, _S]),
     "orb_extractor_debug_candidates": (_I, [_P, _I, _P, _I, _P]),
@@ -616,6 +617,33 @@ class Rectifier:
                lib().orb_extract_rectified(extractor.h, self.h, _ptr(img), img.shape[1], img.shape[0], img.shape[1],
                                            _ptr(kps), cap, _ptr(desc), ctypes.byref(n)))
         return kps[:n.value].copy(), desc[:n.value].copy()
+
+
+def search_for_initialization(pairs, window=100, nnratio=0.9, check_orientation=True, context=None):
+    """ORBmatcher::SearchForInitialization for a list of frame pairs
+    (kp1, desc1, kp2, desc2, bounds=(minX, maxX, minY, maxY), prev (N1 x 2)).
+    Returns per pair (nmatches, matches12, updated prev)."""
+    ctx = context or Context.default()
+    k1 = [np.ascontiguousarray(p[0], KEYPOINT_DTYPE) for p in pairs]
+    d1 = [np.ascontiguousarray(p[1], np.uint8).reshape(-1, 32) for p in pairs]
+    k2 = [np.ascontiguousarray(p[2], KEYPOINT_DTYPE) for p in pairs]
+    d2 = [np.ascontiguousarray(p[3], np.uint8).reshape(-1, 32) for p in pairs]
+    o1 = np.zeros(len(pairs) + 1, np.int32)
+    o1[1:] = np.cumsum([len(k) for k in k1])
+    o2 = np.zeros(len(pairs) + 1, np.int32)
+    o2[1:] = np.cumsum([len(k) for k in k2])
+    cat = lambda xs, dt, shape: np.concatenate(xs) if sum(len(x) for x in xs) else np.zeros(shape, dt)
+    K1, D1 = cat(k1, KEYPOINT_DTYPE, 1), cat(d1, np.uint8, (1, 32))
+    K2, D2 = cat(k2, KEYPOINT_DTYPE, 1), cat(d2, np.uint8, (1, 32))
+    bounds = np.ascontiguousarray([p[4] for p in pairs], np.float32).reshape(-1)
+    prev = cat([np.asarray(p[5], np.float32).reshape(-1, 2) for p in pairs], np.float32, (1, 2)).copy()
+    m12 = np.zeros(max(int(o1[-1]), 1), np.int32)
+    nm = np.zeros(len(pairs), np.int32)
+    _check("orb_search_for_initialization",
+           lib().orb_search_for_initialization(ctx.h, len(pairs), _ptr(K1), _ptr(D1), _ptr(o1), _ptr(K2), _ptr(D2),
+                                               _ptr(o2), _ptr(bounds), _ptr(prev), int(window), float(nnratio),
+                                               int(bool(check_orientation)), _ptr(m12), _ptr(nm)))
+    return [(int(nm[i]), m12[o1[i]:o1[i + 1]].copy(), prev[o1[i]:o1[i + 1]].copy()) for i in range(len(pairs))]
 
 
 def distinctive_descriptors(points, context=None):

@@ -142,3 +142,27 @@ struct VocDev {
 struct BowJob {
     int off, n;
 };
+
+// One frame pair of ORBmatcher::SearchForInitialization: F1 / F2 keypoints
+// (mvKeysUn) and descriptors, the grid bounds (mnMinX .. mnMaxY), the search
+// positions (vbPrevMatched, updated in place), parameters, scratch and
+// outputs.
+struct WinPair {
+    const OrbKeyPoint* kp1;
+    const uint8_t* d1;
+    const OrbKeyPoint* kp2;
+    const uint8_t* d2;
+    int n1, n2;
+    float minX, maxX, minY, maxY;
+    float2* prev;
+    int window;
+    float nnratio;
+    int check_orientation;
+    int* sorted;            // n2: F2 keypoints sorted by (cell, index)
+    int* cell_start;        // 64*48 + 1
+    int* cnt;               // n1: candidates per query
+    int* off;               // n1 + 1: their offsets
+    int2* cand;             // (i2, distance) lists
+    int* m12;               // n1: vnMatches12
+    int* nmatch;            // nmatches
+};
