@@ -166,6 +166,39 @@ def mappoint_extra(orb, ctx, steps, cpu_seconds):
     return out
 
 
+def winmatch_extra(orb, ctx, cpu_seconds):
+    """SearchForInitialization (window 100, ratio 0.9, orientation check) for
+    32 frame pairs of 640x480 with the initializer's 2000 features, one call; host-to-host time (the call reads the candidate counts back
+    once), the oracle beside it on one core."""
+    from orb_slam2_detailed_comments_amd import synth
+    ex = orb.ORBextractor(2000, 1.2, 8, 20, 7, context=ctx)
+    pairs = []
+    for i in range(32):
+        a, b = synth.frame_pair(640, 480, 900 + i, angle_deg=2.0 + 0.1 * i, shift=(5.0, 3.0))
+        k1, d1 = ex(a)
+        k2, d2 = ex(b)
+        pairs.append((k1, d1, k2, d2, (0.0, 640.0, 0.0, 480.0), np.stack([k1["x"], k1["y"]], 1)))
+    orb.search_for_initialization(pairs[:2], context=ctx)           # warm-up
+    reps, t0 = 5, time.perf_counter()
+    for _ in range(reps):
+        res = orb.search_for_initialization(pairs, context=ctx)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out = {"workload": "SearchForInitialization over the 64x48 grid, 32 pairs of 640x480 (2000 features each), "
+                       "window 100, ratio 0.9, orientation check; host arrays in and out",
+           "ms_per_call": round(ms, 3), "pairs_per_s": round(32 / (ms * 1e-3), 1),
+           "mean_matches": round(float(np.mean([r[0] for r in res])), 1)}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            pyoracle.search_for_initialization(*pairs[n % 32])
+            n += 1
+        out["cpu_baseline"] = {"value": round(n / (time.perf_counter() - t0), 1), "unit": "pairs/s", "cores": 1,
+                               "kind": "port", "sample": f"{n} pairs through the oracle in ~{cpu_seconds:.0f}s"}
+    return out
+
+
 def serving_extra(orb, ctx):
     """16 camera streams (4 each of 640x480, 752x480, 1241x376, 1280x720),
     Poisson arrivals; extract + match against the stream's resident previous
@@ -341,6 +374,8 @@ def main():
         print("[bench] map-point descriptors", file=sys.stderr, flush=True)
         out["distinctive_descriptors"] = mappoint_extra(orb, ctx, max(args.steps, 5),
                                                         0.0 if args.no_cpu else min(args.cpu_seconds, 3.0))
+        print("[bench] windowed matching", file=sys.stderr, flush=True)
+        out["search_for_initialization"] = winmatch_extra(orb, ctx, 0.0 if args.no_cpu else min(args.cpu_seconds, 3.0))
     if not args.no_cpu:
         print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
         cpu = cpu_baseline(frames[:8], args.cpu_seconds)
