@@ -114,6 +114,11 @@ struct orb_context {
     // colour input staging
     uint8_t* d_color = nullptr;
     size_t color_cap = 0;
+    // windowed matcher: inputs + scratch, candidate lists
+    void* d_win = nullptr;
+    size_t win_cap = 0;
+    void* d_wcand = nullptr;
+    size_t wcand_cap = 0;
     // map-point scratch: observation descriptors, offsets, chosen rows
     uint8_t* d_mp = nullptr;
     int* d_mpo = nullptr;

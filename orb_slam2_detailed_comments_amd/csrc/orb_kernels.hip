@@ -166,17 +166,16 @@ __device__ __forceinline__ unsigned fast_maybe2(unsigned v, unsigned p0, unsigne
     const orb_us2 bc = __builtin_elementwise_sub_sat(c, hi), bd = __builtin_elementwise_sub_sat(d, hi);
     const orb_us2 ka = __builtin_elementwise_sub_sat(lo, a), kb = __builtin_elementwise_sub_sat(lo, b);
     const orb_us2 kc = __builtin_elementwise_sub_sat(lo, c), kd = __builtin_elementwise_sub_sat(lo, d);
-    orb_us2 r = __builtin_elementwise_max(__builtin_elementwise_min(ba, bb), __builtin_elementwise_min(bb, bc));
-    r = __builtin_elementwise_max(r, __builtin_elementwise_max(__builtin_elementwise_min(bc, bd),
-                                                               __builtin_elementwise_min(bd, ba)));
-    r = __builtin_elementwise_max(r, __builtin_elementwise_max(__builtin_elementwise_min(ka, kb),
-                                                               __builtin_elementwise_min(kb, kc)));
-    r = __builtin_elementwise_max(r, __builtin_elementwise_max(__builtin_elementwise_min(kc, kd),
-                                                               __builtin_elementwise_min(kd, ka)));
+    // Some circularly adjacent pair (0,4), (4,8), (8,12), (12,0) both pass
+    // <=> (0 or 8) and (4 or 12) pass: opposite points never pair up.
+    const orb_us2 br = __builtin_elementwise_min(__builtin_elementwise_max(ba, bc), __builtin_elementwise_max(bb, bd));
+    const orb_us2 dk = __builtin_elementwise_min(__builtin_elementwise_max(ka, kc), __builtin_elementwise_max(kb, kd));
+    const orb_us2 r = __builtin_elementwise_max(br, dk);
     return __builtin_bit_cast(unsigned, r);
 }
 
-// Bytes 0 and 2 (sel 0x0C0A0C08 style) of a dword pair into 16-bit lanes.
+// Bytes 0 and 2 (lo_bytes) or 1 and 3 (hi_bytes) of a dword, zero-extended
+// into the two 16-bit lanes.
 __device__ __forceinline__ unsigned lo_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C020C00u); }
 __device__ __forceinline__ unsigned hi_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C030C01u); }
 
@@ -418,6 +417,8 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const int ax1 = min(ORB_EDGE + L.nCols * L.wCell, L.maxBorderX - 3);
     const int ay1 = min(ORB_EDGE + L.nRows * L.hCell, L.maxBorderY - 3);
     const unsigned t2 = (unsigned)tmin | ((unsigned)tmin << 16);
+    // tile plus its 1-px ring wholly inside the detection area
+    const bool tile_in = tx0 - 1 >= ax0 && tx0 + LT_W < ax1 && ty0 - 1 >= ay0 && ty0 + LT_H < ay1;
     // Groups: rows tile -1..32, 4-column groups c = -1..16.  Group -1 only
     // contributes its pixel 3 (tile column -1) and group 16 its pixel 0
     // (column 64): the 1-px ring.  Their missing neighbour dwords (columns -8
@@ -442,10 +443,15 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             const unsigned m13 = fast_maybe2(hi_bytes(V), hi_bytes(Dn), hi_bytes(P4), hi_bytes(Up),
                                              hi_bytes(P12), t2);
             // pixels k in [klo, khi): detection area, and the ring rule
-            const int klo = min(max(max(ax0 - x, c < 0 ? 3 : 0), 0), 4);
-            const int khi = max(min(min(ax1 - x, c > 15 ? 1 : 4), 4), 0);
-            const bool yin = y >= ay0 && y < ay1;
-            const unsigned mask = (yin && klo < khi) ? ((1u << khi) - (1u << klo)) : 0u;
+            unsigned mask;
+            if (tile_in) {
+                mask = c < 0 ? 8u : (c > 15 ? 1u : 15u);
+            } else {
+                const int klo = min(max(max(ax0 - x, c < 0 ? 3 : 0), 0), 4);
+                const int khi = max(min(min(ax1 - x, c > 15 ? 1 : 4), 4), 0);
+                const bool yin = y >= ay0 && y < ay1;
+                mask = (yin && klo < khi) ? ((1u << khi) - (1u << klo)) : 0u;
+            }
             cand = (((m02 & 0xFFFFu) ? 1u : 0u) | ((m13 & 0xFFFFu) ? 2u : 0u) | ((m02 >> 16) ? 4u : 0u) |
                     ((m13 >> 16) ? 8u : 0u)) & mask;
         }

@@ -48,7 +48,7 @@ void orb_context_destroy(orb_context* ctx)
     if (ctx->d_kq) (void)hipFree(ctx->d_kq);
     if (ctx->d_km) (void)hipFree(ctx->d_km);
     if (ctx->d_kout) (void)hipFree(ctx->d_kout);
-    void* mp[] = {ctx->d_mp, ctx->d_mpo, ctx->d_mpb, ctx->d_mpd, ctx->d_color};
+    void* mp[] = {ctx->d_mp, ctx->d_mpo, ctx->d_mpb, ctx->d_mpd, ctx->d_color, ctx->d_win, ctx->d_wcand};
     for (void* q : mp)
         if (q) (void)hipFree(q);
     (void)hipStreamDestroy(ctx->stream);

@@ -1,7 +1,8 @@
 // orb_winmatch.cpp -- C ABI of ORBmatcher::SearchForInitialization over the
 // Frame keypoint grid, for many frame pairs per call (kernels in
 // orb_winmatch.hip).  Two launches with one host read between them: the
-// candidate counts size the (i2, distance) lists exactly.
+// candidate counts size the (i2, distance) lists exactly.  Device buffers
+// are the context's, grown on demand and kept between calls.
 #include <cstring>
 #include <vector>
 
@@ -14,14 +15,6 @@ extern "C" hipError_t orb_launch_win_scan(const WinPair* d_pairs, int npairs, in
 extern "C" hipError_t orb_launch_win_match(const WinPair* d_pairs, int npairs, int max_n1, hipStream_t st);
 
 namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-};
 
 size_t up(size_t v) { return (v + 255) / 256 * 256; }
 
@@ -58,9 +51,9 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
     const size_t s_pairs = up((size_t)n_pairs * sizeof(WinPair)), s_tot = up((size_t)n_pairs * 4);
     const size_t total = s_kp1 + s_d1 + s_kp2 + s_d2 + s_prev + s_sorted + s_cells + s_cnt + s_off + s_m12 + s_nm +
                          s_pairs + s_tot;
-    DevBuf mem;
-    if (hipMalloc(&mem.p, total) != hipSuccess) return ORB_E_OOM;
-    char* m = (char*)mem.p;
+    int rc = ctx->grow(&ctx->d_win, &ctx->win_cap, total);     // kept by the context between calls
+    if (rc) return rc;
+    char* m = (char*)ctx->d_win;
     size_t o = 0;
     auto take = [&](size_t s) {
         char* q = m + o;
@@ -126,9 +119,9 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
         base[p] = ncand;
         ncand += (size_t)tot[p];
     }
-    DevBuf cand;
-    if (hipMalloc(&cand.p, std::max<size_t>(ncand, 1) * sizeof(int2)) != hipSuccess) return ORB_E_OOM;
-    for (int p = 0; p < n_pairs; p++) pairs[p].cand = (int2*)cand.p + base[p];
+    rc = ctx->grow(&ctx->d_wcand, &ctx->wcand_cap, std::max<size_t>(ncand, 1) * sizeof(int2));
+    if (rc) return rc;
+    for (int p = 0; p < n_pairs; p++) pairs[p].cand = (int2*)ctx->d_wcand + base[p];
     HIP_TRY(hipMemcpyAsync(d_pairs, pairs.data(), n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st));
     HIP_TRY(orb_launch_win_match(d_pairs, n_pairs, max_n1, st));
     if (N1) {
