@@ -162,9 +162,9 @@ struct orb_extractor {
     float* d_depth = nullptr;
     int* d_msad = nullptr;        // per left keypoint: SAD of its stereo match
     int* d_misc = nullptr;        // [1] nl, [2] nr of caller-supplied stereo inputs
-    float4* d_srec = nullptr;     // right keypoints sorted by y (stereo scratch) ...
+    int4* d_srec = nullptr;       // right keypoints bucketed by row (stereo scratch) ...
     uint8_t* d_sdesc = nullptr;   // ... and their descriptors
-    int* d_rows = nullptr;        // per-row start into the sorted right keypoints
+    int2* d_rows = nullptr;       // per-row candidate range in the bucketed right keypoints
     OrbKeyPoint* d_kin = nullptr; // uploaded keypoints: left then right
     uint8_t* d_din = nullptr;
     int in_cap = 0;
@@ -305,7 +305,7 @@ struct orb_extractor {
         const size_t o_msad = take((size_t)p.out_cap * 4);
         const size_t o_misc = take(64);
         const size_t o_srec = take((size_t)p.out_cap * 16), o_sdesc = take((size_t)p.out_cap * 32);
-        const size_t o_rows = take(((size_t)H + 2) * 4);
+        const size_t o_rows = take(((size_t)H + 2) * 8);
         const size_t total = off;
         if (total > mem_bytes) {
             if (d_mem) (void)hipFree(d_mem);
@@ -364,9 +364,9 @@ struct orb_extractor {
         d_depth = (float*)(d_mem + o_dep);
         d_msad = (int*)(d_mem + o_msad);
         d_misc = (int*)(d_mem + o_misc);
-        d_srec = (float4*)(d_mem + o_srec);
+        d_srec = (int4*)(d_mem + o_srec);
         d_sdesc = (uint8_t*)(d_mem + o_sdesc);
-        d_rows = (int*)(d_mem + o_rows);
+        d_rows = (int2*)(d_mem + o_rows);
         if (!d_job && hipMalloc(&d_job, sizeof(ImageJob)) != hipSuccess) return ORB_E_OOM;
         if (!d_rt && hipMalloc(&d_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
         if (!h_rt && hipHostMalloc(&h_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
@@ -492,7 +492,7 @@ struct orb_extractor {
         a.match_sad = d_msad;
         a.srec = d_srec;
         a.sdesc = d_sdesc;
-        a.row_start = d_rows;
+        a.rows = d_rows;
         a.cap_r = std::min(plan.out_cap, r->plan.out_cap);
         float band = 0.f;
         for (int l = 0; l < nlevels; l++) band = std::max(band, 2.0f * scale[l]);

@@ -1155,74 +1155,140 @@ __device__ __forceinline__ int hamming32(const uint4 a0, const uint4 a1, const u
 }
 
 // ================================================================ stereo
-// Frame::ComputeStereoMatches (Frame.cc:831-1082), one wavefront per left
+// Frame::ComputeStereoMatches (Frame.cc:831-1082), a 16-lane group per left
 // keypoint.  Search: right keypoints whose row band [floor(y-2s), ceil(y+2s)]
 // holds row (size_t)vL, octave within +-1, uR in [uL-maxD, uL]; best is the
 // smallest (distance, index) below TH_HIGH.  Refinement: 11x11 SAD of
 // centre-subtracted patches over 11 shifts, parabola fit, depth = bf/disp.
 
-// Sort each pair's right keypoints by (y, index) (one workgroup per pair,
-// bitonic sort in LDS) and lay them out in that order as 16-byte records and
-// 32-byte descriptors, so a candidate is two independent loads at its sorted
-// position; plus, per level-0 row, the first entry of its band.
-#define STEREO_SORT_CAP 4096
-__global__ __launch_bounds__(1024) void stereo_prep_kernel(const StereoArgs* __restrict__ args)
+// Group each pair's right keypoints by level-0 row bucket (counting sort in
+// LDS, one workgroup per pair) and lay them out in that order as 16-byte
+// records {minr, maxr, index << 4 | octave, x} (the row band of
+// Frame.cc:867-882, with the same float statements) and 32-byte descriptors;
+// plus, per level-0 row, the entry range of the buckets that can hold the row
+// in a band.  Order inside a bucket is free: the search keeps the smallest
+// (distance, index), which does not depend on the visiting order.
+#define STEREO_NB 4096               // buckets: rows >> qs
+#define STEREO_T 1024
+#define STEREO_PER_T 4               // right keypoints per thread (<= 4096)
+__global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs* __restrict__ args)
 {
-    __shared__ unsigned long long v[STEREO_SORT_CAP];
+    __shared__ int s_cnt[STEREO_NB + 1];
+    __shared__ int s_part[STEREO_T / 64];
     const StereoArgs& A = args[blockIdx.x];
-    const int n = min(*A.nr, min(A.cap_r, STEREO_SORT_CAP));
-    int npow = 1;
-    while (npow < n) npow <<= 1;
-    for (int i = threadIdx.x; i < npow; i += 1024)
-        v[i] = i < n ? ((unsigned long long)__float_as_uint(A.kr[i].y) << 32) | (unsigned)i : ~0ull;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int n = min(min(*A.nr, A.cap_r), STEREO_T * STEREO_PER_T);
+    const int nrows = A.nrows;
+    int qs = 0;
+    while (((nrows - 1) >> qs) >= STEREO_NB) qs++;
+    const int nb = ((nrows - 1) >> qs) + 1;
+    // Bucket of a y: floor(y) clamped to the rows (NaN to row 0), >> qs.
+    auto bucket = [nrows, qs](float y) {
+        const float f = floorf(y);
+        const int r = !(f > 0.f) ? 0 : (f >= (float)(nrows - 1) ? nrows - 1 : (int)f);
+        return r >> qs;
+    };
+    for (int j = t; j <= nb; j += STEREO_T) s_cnt[j] = 0;
     __syncthreads();
-    for (int k = 2; k <= npow; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int a = threadIdx.x; a < npow; a += 1024) {
-                const int b = a ^ j;
-                if (b > a) {
-                    const unsigned long long va = v[a], vb = v[b];
-                    const bool up = (a & k) == 0;
-                    if (up ? (va > vb) : (va < vb)) {
-                        v[a] = vb;
-                        v[b] = va;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    for (int i = threadIdx.x; i < n; i += 1024) {
-        const int iR = (int)(unsigned)v[i];
-        const OrbKeyPoint& k = A.kr[iR];
-        A.srec[i] = make_float4(k.y, k.x, __int_as_float(k.octave), __int_as_float(iR));
+    int eb[STEREO_PER_T], er[STEREO_PER_T];
+#pragma unroll
+    for (int k = 0; k < STEREO_PER_T; k++) {
+        const int i = t + STEREO_T * k;
+        eb[k] = i < n ? bucket(A.kr[i].y) : 0;
+        er[k] = i < n ? atomicAdd(&s_cnt[eb[k]], 1) : 0;
     }
-    // descriptors: two 16-byte halves per entry
-    for (int i = threadIdx.x; i < 2 * n; i += 1024) {
-        const int iR = (int)(unsigned)v[i >> 1];
-        ((uint4*)A.sdesc)[i] = ((const uint4*)A.dr)[2 * iR + (i & 1)];
+    __syncthreads();
+    // Exclusive scan of the nb counts: 4 per thread, wavefront scans, then
+    // the 16 wavefront totals.
+    int loc[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int j = 4 * t + k;
+        loc[k] = sum;
+        sum += j < nb ? s_cnt[j] : 0;
     }
-    // Per level-0 row: first sorted entry that can hold the row in its band.
-    for (int row = threadIdx.x; row <= A.nrows; row += 1024) {
-        const float ylo = (float)row - 1.0f - A.band;
-        int lo = 0, hi = n;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (__uint_as_float((unsigned)(v[mid] >> 32)) <= ylo) lo = mid + 1;
-            else hi = mid;
+    int inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    if (lane == 63) s_part[wv] = inc;
+    __syncthreads();
+    int base = inc - sum;
+    for (int k = 0; k < wv; k++) base += s_part[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int j = 4 * t + k;
+        if (j < nb) s_cnt[j] = base + loc[k];
+    }
+    if (t == 0) s_cnt[nb] = n;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < STEREO_PER_T; k++) {
+        const int i = t + STEREO_T * k;
+        if (i < n) {
+            const OrbKeyPoint kp = A.kr[i];
+            const float r = 2.0f * A.scale[kp.octave];
+            const int maxr = (int)ceilf(kp.y + r);
+            const int minr = (int)floorf(kp.y - r);
+            const int pos = s_cnt[eb[k]] + er[k];
+            A.srec[pos] = make_int4(minr, maxr, (i << 4) | kp.octave, __float_as_int(kp.x));
+            const uint4* d = (const uint4*)(A.dr + (size_t)i * 32);
+            uint4* o = (uint4*)(A.sdesc + (size_t)pos * 32);
+            o[0] = d[0];
+            o[1] = d[1];
         }
-        A.row_start[row] = lo;
+    }
+    // A candidate of row `row` has y in (row - 1 - band, row + 1 + band);
+    // one more row each side absorbs the rounding of these bounds.
+    for (int row = t; row < nrows; row += STEREO_T) {
+        const int blo = bucket((float)row - 2.0f - A.band), bhi = bucket((float)row + 2.0f + A.band);
+        A.rows[row] = make_int2(s_cnt[blo], s_cnt[bhi + 1]);
     }
 }
 
-// grid (ceil(max left capacity / 4), pairs), one wavefront per left keypoint
+// One 32-bit load of a patch row, bytewise where it would pass the level's
+// right edge.
+__device__ __forceinline__ unsigned patch_dword(const uint8_t* rp, int x, int w)
+{
+    if (x + 3 < w) return *(const unsigned*)(rp + x);
+    unsigned d = 0;
+    for (int b = 0; b < 4; b++)
+        if (x + b < w) d |= (unsigned)rp[x + b] << (8 * b);
+    return d;
+}
+
+// Sum over the 16 lanes of a DPP row (all of them active); every lane of the
+// row receives the total.
+__device__ __forceinline__ unsigned row16_sum(unsigned v)
+{
+    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false);   // row_mirror
+    return v;
+}
+
+// grid (ceil(max left capacity / 16), pairs): lanes 16g .. 16g+15 of a
+// wavefront serve one left keypoint.  Every branch is uniform within a
+// group, and cross-lane steps stay inside it.
+//   search: the group's lanes stride over the row's candidate range;
+//   refinement: lane r < 11 holds patch row r of both images in registers
+//   as 16-bit pairs and accumulates its row's part of the 11 shift SADs with
+//   v_sad_u16 (two absolute differences per instruction), as
+//   |(l - cL) - (r - cR)| = |(l + cR - cL + 256) - (r + 256)|, both sides in
+//   [0, 766]; the row parts are summed across the group with DPP, two shifts
+//   per dword (a total is at most 121 * 510 < 2^16).
+#define ST_G 16
 __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restrict__ args)
 {
-    __shared__ unsigned s_patch[4][44 + 77];     // per wavefront: left 11x16 B, right 11x28 B
-    const int lane = threadIdx.x & 63;
+    const int sub = threadIdx.x & (ST_G - 1);
+    const int gbase = threadIdx.x & 63 & ~(ST_G - 1);
     const StereoArgs& A = args[blockIdx.y];
-    const int iL = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int iL = blockIdx.x * (256 / ST_G) + (int)(threadIdx.x / ST_G);
     const int nl = *A.nl, nr = *A.nr;
-    if (iL >= nl || iL >= A.cap_l) return;
+    if (iL >= nl || iL >= A.cap_l) return;             // the whole group leaves
     float outU = -1.0f, outD = -1.0f;
     int outSad = -1;
     const OrbKeyPoint kpL = A.kl[iL];
@@ -1238,7 +1304,7 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     const float maxU = uL - minD;
     bool go = row >= 0 && row < A.nrows && !(maxU < 0);
     // The level both patches are read from (octaves are validated on entry
-    // and produced in range by the extractor); fetched ahead of the search.
+    // and produced in range by the extractor).
     const LevelJob& LL = A.left->lv[levelL];
     const LevelJob& LR = A.right->lv[levelL];
     const uint8_t* pl = LL.lvl;
@@ -1247,22 +1313,17 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     int bestDist = 100, bestIdx = INT_MAX;
     float bestU = 0.f;
     if (go) {
-        // Right keypoints sorted by y: every candidate of row `row` has
-        // y in (row - 1 - band, row + 1 + band); the exact band, octave and
-        // uR tests follow.
-        const int n = min(nr, A.cap_r);
-        const float yhi = (float)row + 1.0f + A.band;
-        for (int c = A.row_start[row] + lane; c < n; c += 64) {
-            const float4 rec = A.srec[c];
-            if (rec.x >= yhi) break;
-            const int oct = __float_as_int(rec.z), iR = __float_as_int(rec.w);
-            const float r = 2.0f * A.scale[oct];
-            const int maxr = (int)ceilf(rec.x + r);
-            const int minr = (int)floorf(rec.x - r);
-            if (row < minr || row > maxr) continue;
+        (void)nr;
+        const int2 rg = A.rows[row];
+#pragma unroll 2
+        for (int c = rg.x + sub; c < rg.y; c += ST_G) {
+            const int4 rec = A.srec[c];
+            const int oct = rec.z & 15;
+            if (row < rec.x || row > rec.y) continue;
             if (oct < levelL - 1 || oct > levelL + 1) continue;
-            const float uR = rec.y;
+            const float uR = __int_as_float(rec.w);
             if (uR >= minU && uR <= maxU) {
+                const int iR = rec.z >> 4;
                 const int dist = hamming32(a0, a1, A.sdesc + (size_t)c * 32);
                 if (dist < bestDist || (dist == bestDist && iR < bestIdx)) {
                     bestDist = dist;
@@ -1271,10 +1332,11 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
                 }
             }
         }
-        // Smallest (distance, index) across the wavefront.
-        for (int o = 32; o > 0; o >>= 1) {
-            const int od = __shfl_xor(bestDist, o, 64), oi = __shfl_xor(bestIdx, o, 64);
-            const float ou = __shfl_xor(bestU, o, 64);
+        // Smallest (distance, index) across the group.
+#pragma unroll
+        for (int o = ST_G / 2; o > 0; o >>= 1) {
+            const int od = __shfl_xor(bestDist, o, ST_G), oi = __shfl_xor(bestIdx, o, ST_G);
+            const float ou = __shfl_xor(bestU, o, ST_G);
             if (od < bestDist || (od == bestDist && oi < bestIdx)) {
                 bestDist = od;
                 bestIdx = oi;
@@ -1303,98 +1365,100 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
         if (sy0 - 5 < 0 || sy0 + 5 >= hL || sy0 + 5 >= hR || sx0 - 5 < 0 || sx0 + 5 >= wL) go = false;
     }
     if (go) {
-        // Stage the 11x11 left patch and the 11x21 right strip (every shift's
-        // patch) in LDS: rows of aligned dwords, two per lane, bytewise only
-        // where a dword would pass the level's right edge.  Then 121 (shift,
-        // row) tasks over 64 lanes; per task 11 absolute differences of
-        // centre-subtracted pixels, then one wave sum per shift.
-        const int xl0 = (sx0 - 5) & ~3, nld = ((sx0 + 5) >> 2) - (xl0 >> 2) + 1;     // <= 4 dwords
-        const int xr0 = (sxR - 10) & ~3, nrd = ((sxR + 10) >> 2) - (xr0 >> 2) + 1;   // <= 7 dwords
-        unsigned* pw = s_patch[threadIdx.x >> 6];
+        // Lane r < 11: patch row r, left bytes l0..l10 (x = sx0-5 ..) and
+        // right bytes r0..r20 (x = sxR-10 ..); lanes 11..15 repeat row 10
+        // and contribute nothing.
+        const int r = sub < 11 ? sub : 10;
+        const uint8_t* rl = pl + (size_t)(sy0 - 5 + r) * pitchL;
+        const uint8_t* rr = pr + (size_t)(sy0 - 5 + r) * pitchR;
+        const int xl0 = (sx0 - 5) & ~3, offL = sx0 - 5 - xl0;
+        const int xr0 = (sxR - 10) & ~3, offR = sxR - 10 - xr0;
+        unsigned dl[4], dr[6];
 #pragma unroll
-        for (int h2 = 0; h2 < 2; h2++) {
-            const int sl = lane + 64 * h2;
-            const bool isL = sl < 11 * nld;
-            const int k = isL ? sl : sl - 11 * nld;
-            const int nd = isL ? nld : nrd;
-            const int r = k / nd, c = k - r * nd;
-            if (isL || k < 11 * nrd) {
-                const uint8_t* img = isL ? pl : pr;
-                const int x = (isL ? xl0 : xr0) + 4 * c, w = isL ? wL : wR;
-                const uint8_t* rp = img + (size_t)(sy0 - 5 + r) * (isL ? pitchL : pitchR);
-                unsigned d = 0;
-                if (x + 3 < w) {
-                    d = *(const unsigned*)(rp + x);
-                } else {
-                    for (int b2 = 0; b2 < 4; b2++)
-                        if (x + b2 < w) d |= (unsigned)rp[x + b2] << (8 * b2);
-                }
-                pw[isL ? r * 4 + c : 44 + r * 7 + c] = d;
-            }
+        for (int k = 0; k < 4; k++) dl[k] = patch_dword(rl, xl0 + 4 * k, wL);
+#pragma unroll
+        for (int k = 0; k < 6; k++) dr[k] = patch_dword(rr, xr0 + 4 * k, wR);
+        unsigned la[3], ra[6];
+#pragma unroll
+        for (int k = 0; k < 3; k++) la[k] = __builtin_amdgcn_alignbyte(dl[k + 1], dl[k], (unsigned)offL);
+#pragma unroll
+        for (int k = 0; k < 5; k++) ra[k] = __builtin_amdgcn_alignbyte(dr[k + 1], dr[k], (unsigned)offR);
+        ra[5] = dr[5] >> (8 * offR);
+        // Centres from row 5: cL = l5, cR of shift s = r(s+5), s = 0..10.
+        const int cL = __shfl((int)((la[1] >> 8) & 0xFFu), gbase + 5, 64);
+        const unsigned c0 = (unsigned)__shfl((int)__builtin_amdgcn_alignbyte(ra[2], ra[1], 1u), gbase + 5, 64);
+        const unsigned c1 = (unsigned)__shfl((int)__builtin_amdgcn_alignbyte(ra[3], ra[2], 1u), gbase + 5, 64);
+        const unsigned c2 = (unsigned)__shfl((int)__builtin_amdgcn_alignbyte(ra[4], ra[3], 1u), gbase + 5, 64);
+        // Left pairs (l2k, l2k+1), k < 5, and (l10, 0).
+        unsigned lp[6];
+#pragma unroll
+        for (int k = 0; k < 5; k++) lp[k] = __builtin_amdgcn_perm(0u, la[k >> 1], (k & 1) ? 0x0C030C02u : 0x0C010C00u);
+        lp[5] = (la[2] >> 16) & 0xFFu;
+        // Right pairs (rj, rj+1) + 256 each, j < 20, and (rj, 0) + 256, j = 10..20.
+        unsigned rp[20], rs[11];
+#pragma unroll
+        for (int j = 0; j < 20; j++) {
+            const int q = j >> 2, p = j & 3;
+            rp[j] = __builtin_amdgcn_perm(q + 1 < 6 ? ra[q + 1] : 0u, ra[q],
+                                          0x0C000C00u | ((unsigned)(p + 1) << 16) | (unsigned)p) |
+                    0x01000100u;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint8_t* lb = (const uint8_t*)pw;              // 11 rows x 16 bytes
-        const uint8_t* rb = (const uint8_t*)(pw + 44);       // 11 rows x 28 bytes
-        const int cL = lb[5 * 16 + sx0 - xl0];
-        int acc[2] = {0, 0};
 #pragma unroll
-        for (int h2 = 0; h2 < 2; h2++) {
-            const int task = lane + 64 * h2;
-            if (task < 121) {
-                const int inc = task / 11 - 5, dy = task % 11;
-                const int xc = sxR + inc - xr0;
-                const int cR = rb[5 * 28 + xc];
-                const uint8_t* rl = lb + dy * 16 + sx0 - 5 - xl0;
-                const uint8_t* rr = rb + dy * 28 + xc - 5;
-                int a = 0;
+        for (int j = 10; j <= 20; j++) rs[j - 10] = ((ra[j >> 2] >> (8 * (j & 3))) & 0xFFu) | 0x100u;
+        unsigned part[11];
 #pragma unroll
-                for (int dx = 0; dx < 11; dx++) a += abs((rl[dx] - cL) - (rr[dx] - cR));
-                acc[h2] = a;
-            }
+        for (int s = 0; s < 11; s++) {
+            const unsigned cw = s < 4 ? c0 : (s < 8 ? c1 : c2);
+            const unsigned e = ((cw >> (8 * (s & 3))) & 0xFFu) + 256u - (unsigned)cL;   // cR - cL + 256
+            const unsigned e2 = e | (e << 16);
+            unsigned acc = 0;
+#pragma unroll
+            for (int k = 0; k < 5; k++) acc = __builtin_amdgcn_sad_u16(lp[k] + e2, rp[s + 2 * k], acc);
+            part[s] = __builtin_amdgcn_sad_u16(lp[5] + e, rs[s], acc);
+        }
+        unsigned tot[6];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            const unsigned v = 2 * m + 1 < 11 ? (part[2 * m] | (part[2 * m + 1] << 16)) : part[2 * m];
+            tot[m] = row16_sum(sub < 11 ? v : 0u);
         }
         int sad[11];
 #pragma unroll
-        for (int sh = 0; sh < 11; sh++) {
-            int v = 0;
-            if (lane / 11 == sh) v += acc[0];
-            if (lane + 64 < 121 && (lane + 64) / 11 == sh) v += acc[1];
-            sad[sh] = wave_sum(v);
+        for (int s = 0; s < 11; s++) sad[s] = (int)((tot[s >> 1] >> (16 * (s & 1))) & 0xFFFFu);
+        // First minimum over inc = -5..5, then the parabola on its
+        // neighbours, evaluated identically by every lane of the group.
+        unsigned key = 0xFFFFFFFFu;
+#pragma unroll
+        for (int s = 0; s < 11; s++) key = min(key, ((unsigned)sad[s] << 4) | (unsigned)s);
+        const int best = (int)(key >> 4), bs = (int)(key & 15u);
+        const int bestinc = bs - 5;
+        int d1 = 0, d3 = 0;
+#pragma unroll
+        for (int s = 0; s < 11; s++) {
+            d1 = s == bs - 1 ? sad[s] : d1;
+            d3 = s == bs + 1 ? sad[s] : d3;
         }
-        if (lane == 0) {
-            int best = INT_MAX, bestinc = 0;
-            float vD[11];
-            for (int inc = -5; inc <= 5; inc++) {
-                const float dist = (float)sad[inc + 5];
-                if (dist < best) {
-                    best = (int)dist;
-                    bestinc = inc;
-                }
-                vD[inc + 5] = dist;
-            }
-            if (bestinc != -5 && bestinc != 5) {
-                const float dist1 = vD[5 + bestinc - 1];
-                const float dist2 = vD[5 + bestinc];
-                const float dist3 = vD[5 + bestinc + 1];
-                const float deltaR = __fdiv_rn(dist1 - dist3, 2.0f * (dist1 + dist3 - 2.0f * dist2));
-                if (!(deltaR < -1 || deltaR > 1)) {
-                    float bestuR = A.scale[levelL] * ((float)scaleduR0 + (float)bestinc + deltaR);
-                    float disparity = (uL - bestuR);
-                    if (disparity >= minD && disparity < maxD) {
-                        if (disparity <= 0) {
-                            disparity = 0.01f;
-                            bestuR = (float)((double)uL - 0.01);
-                        }
-                        outD = __fdiv_rn(A.bf, disparity);
-                        outU = bestuR;
-                        outSad = best;
+        if (bestinc != -5 && bestinc != 5) {
+            const float dist1 = (float)d1;
+            const float dist2 = (float)best;
+            const float dist3 = (float)d3;
+            const float deltaR = __fdiv_rn(dist1 - dist3, 2.0f * (dist1 + dist3 - 2.0f * dist2));
+            if (!(deltaR < -1 || deltaR > 1)) {
+                float bestuR = A.scale[levelL] * ((float)scaleduR0 + (float)bestinc + deltaR);
+                float disparity = (uL - bestuR);
+                if (disparity >= minD && disparity < maxD) {
+                    if (disparity <= 0) {
+                        disparity = 0.01f;
+                        bestuR = (float)((double)uL - 0.01);
                     }
+                    outD = __fdiv_rn(A.bf, disparity);
+                    outU = bestuR;
+                    outSad = best;
                 }
             }
         }
     }
-    if (lane == 0) {
+    if (sub == 0) {
         A.uRight[iL] = outU;
         A.depth[iL] = outD;
         A.match_sad[iL] = outSad;
@@ -1402,49 +1466,86 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
 }
 
 // Median cut (Frame.cc:1066-1081): matches whose SAD is not below
-// 1.5 * 1.4 * median are dropped.  One workgroup per pair gathers the kept
-// SADs, sorts them in LDS and clears the rejected keypoints.
-#define MEDIAN_CAP 4096
-__global__ __launch_bounds__(1024) void stereo_median_kernel(const StereoArgs* __restrict__ args)
+// 1.5 * 1.4 * median are dropped, the median being element M/2 of the M
+// kept SADs in ascending order.  One workgroup per pair selects it with two
+// 256-bin histograms (SADs are below 2^16), high byte then low byte, and
+// clears the rejected keypoints.
+#define MEDIAN_T 1024
+#define MEDIAN_PER_T 4               // left keypoints per thread (<= 4096)
+
+// Wavefront 0: the bin of s_h holding rank k (0-based) of the ascending
+// order, and k's offset inside it; bin -1 when the histogram holds <= k.
+__device__ __forceinline__ void median_pick(const int* s_h, int k, int* s_out)
 {
-    __shared__ int v[MEDIAN_CAP];
-    __shared__ int s_m;
-    const StereoArgs& A = args[blockIdx.x];
-    const int n = min(min(*A.nl, A.cap_l), MEDIAN_CAP);
-    if (threadIdx.x == 0) s_m = 0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 1024) {
-        const int sad = A.match_sad[i];
-        if (sad >= 0) v[atomicAdd(&s_m, 1)] = sad;
+    const int lane = threadIdx.x;
+    int c[4], s = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        c[j] = s_h[4 * lane + j];
+        s += c[j];
     }
+    int inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    int ex = inc - s;
+    if (lane == 63 && inc <= k) s_out[0] = -1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (ex <= k && k < ex + c[j]) {
+            s_out[0] = 4 * lane + j;
+            s_out[1] = k - ex;
+        }
+        ex += c[j];
+    }
+}
+
+__global__ __launch_bounds__(MEDIAN_T) void stereo_median_kernel(const StereoArgs* __restrict__ args)
+{
+    __shared__ int s_h[256];
+    __shared__ int s_m;
+    __shared__ int s_sel[2];
+    const StereoArgs& A = args[blockIdx.x];
+    const int t = threadIdx.x;
+    const int n = min(min(*A.nl, A.cap_l), MEDIAN_T * MEDIAN_PER_T);
+    if (t < 256) s_h[t] = 0;
+    if (t == 0) s_m = 0;
+    __syncthreads();
+    int sad[MEDIAN_PER_T], m = 0;
+#pragma unroll
+    for (int k = 0; k < MEDIAN_PER_T; k++) {
+        const int i = t + MEDIAN_T * k;
+        sad[k] = i < n ? A.match_sad[i] : -1;
+        if (sad[k] >= 0) {
+            atomicAdd(&s_h[sad[k] >> 8], 1);
+            m++;
+        }
+    }
+    if (m) atomicAdd(&s_m, m);
     __syncthreads();
     const int M = s_m;
-    if (M == 0) return;
-    int npow = 1;
-    while (npow < M) npow <<= 1;
-    for (int i = M + threadIdx.x; i < npow; i += 1024) v[i] = INT_MAX;
+    if (M == 0) return;                              // nothing to cut (uniform)
+    if (t < 64) median_pick(s_h, M / 2, s_sel);
     __syncthreads();
-    for (int k = 2; k <= npow; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int a = threadIdx.x; a < npow; a += 1024) {
-                const int b = a ^ j;
-                if (b > a) {
-                    const int va = v[a], vb = v[b];
-                    const bool up = (a & k) == 0;
-                    if (up ? (va > vb) : (va < vb)) {
-                        v[a] = vb;
-                        v[b] = va;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    const float median = (float)v[M / 2];
-    const float k = 1.5f * 1.4f;
-    const float thDist = k * median;
-    for (int i = threadIdx.x; i < n; i += 1024) {
-        const int sad = A.match_sad[i];
-        if (sad >= 0 && !((float)sad < thDist)) {
+    const int hb = s_sel[0], k2 = s_sel[1];
+    __syncthreads();
+    if (t < 256) s_h[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MEDIAN_PER_T; k++)
+        if (sad[k] >= 0 && (sad[k] >> 8) == hb) atomicAdd(&s_h[sad[k] & 255], 1);
+    __syncthreads();
+    if (t < 64) median_pick(s_h, k2, s_sel);
+    __syncthreads();
+    const float median = (float)((hb << 8) | s_sel[0]);
+    const float kf = 1.5f * 1.4f;
+    const float thDist = kf * median;
+#pragma unroll
+    for (int k = 0; k < MEDIAN_PER_T; k++) {
+        const int i = t + MEDIAN_T * k;
+        if (sad[k] >= 0 && !((float)sad[k] < thDist)) {
             A.uRight[i] = -1;
             A.depth[i] = -1;
         }
@@ -1588,7 +1689,7 @@ extern "C" hipError_t orb_launch_stereo(const StereoArgs* d_args, int npairs, in
 {
     if (npairs <= 0) return hipSuccess;
     stereo_prep_kernel<<<npairs, 1024, 0, st>>>(d_args);
-    if (max_left > 0) stereo_kernel<<<dim3((max_left + 3) / 4, npairs), 256, 0, st>>>(d_args);
+    if (max_left > 0) stereo_kernel<<<dim3((max_left + 256 / ST_G - 1) / (256 / ST_G), npairs), 256, 0, st>>>(d_args);
     if (ev) (void)hipEventRecord(ev[0], st);
     stereo_median_kernel<<<npairs, 1024, 0, st>>>(d_args);
     if (ev) (void)hipEventRecord(ev[1], st);

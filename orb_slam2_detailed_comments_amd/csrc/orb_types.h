@@ -97,10 +97,11 @@ struct StereoArgs {
     float* uRight;
     float* depth;
     int* match_sad;         // per left keypoint: SAD of its match, or -1
-    float4* srec;           // scratch: right keypoints sorted by (y, index) as
-                            // {y, x, octave bits, index bits} ...
+    int4* srec;             // scratch: right keypoints grouped by row bucket as
+                            // {minr, maxr, index << 4 | octave, x bits} ...
     uint8_t* sdesc;         // ... and their descriptors in the same order
-    int* row_start;         // first sorted entry with y > row - 1 - band (nrows + 1)
+    int2* rows;             // per level-0 row: [first, end) of the entries whose
+                            // y lies in (row - 1 - band, row + 1 + band)
     int cap_r;
     float band;             // max over levels of 2*scale: widest row band
 };
