@@ -396,7 +396,7 @@ struct orb_extractor {
         for (int j = 0; j < n; j++)
             for (int l = 0; l < maxl; l++) {
                 rt.desc_start[j * maxl + l] = acc;
-                if (l < exs[j]->nlevels) acc += (exs[j]->plan.lv[l].list_cap + 3) / 4;
+                if (l < exs[j]->nlevels) acc += (exs[j]->plan.lv[l].list_cap + ORB_DESC_KPB - 1) / ORB_DESC_KPB;
             }
         rt.desc_start[n * maxl] = acc;
     }

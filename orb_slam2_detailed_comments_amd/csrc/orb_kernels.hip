@@ -22,20 +22,54 @@
 
 #include "orb_types.h"
 
-__constant__ signed char c_pattern[1024] = {
+__constant__ __align__(16) signed char c_pattern[1024] = {   // BRIEF pairs, x0,y0,x1,y1
 #include "orb_pattern.inc"
 };
-__constant__ int c_umax[16];   // orientation disc half-widths (ctor, :542-570)
+__constant__ unsigned c_icmask[4 * 16 * 9];   // orientation disc byte masks [alignment][|v|][dword]
 __constant__ int c_gauss[7];   // 7x7 sigma-2 integer taps (sum need not be 256)
 
+// umax16: the orientation disc half-widths of the constructor
+// (ORBextractor.cc:542-570).  Byte k of dword c of a patch row whose first
+// dword starts `al` pixels left of u = -15 holds u = 4c + k - 15 - al; its
+// mask byte is 0xFF iff |u| <= umax[|v|].
 extern "C" hipError_t orb_upload_constants(const int* umax16, const int* gauss7)
 {
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_umax), umax16, 16 * sizeof(int));
+    unsigned m[4 * 16 * 9];
+    for (int al = 0; al < 4; al++)
+        for (int av = 0; av < 16; av++)
+            for (int c = 0; c < 9; c++) {
+                unsigned w = 0;
+                for (int k = 0; k < 4; k++) {
+                    const int u = 4 * c + k - 15 - al;
+                    if ((u < 0 ? -u : u) <= umax16[av]) w |= 0xFFu << (8 * k);
+                }
+                m[(al * 16 + av) * 9 + c] = w;
+            }
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_icmask), m, sizeof(m));
     if (e != hipSuccess) return e;
     return hipMemcpyToSymbol(HIP_SYMBOL(c_gauss), gauss7, 7 * sizeof(int));
 }
 
 // ------------------------------------------------------------------ helpers
+// Loads of 16, 8 and 4 bytes at 4-byte aligned global addresses, marked as
+// global so that they are issued as global (not flat) loads.
+typedef unsigned orb_v4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef unsigned orb_v2u __attribute__((ext_vector_type(2), aligned(4)));
+__device__ __forceinline__ uint4 gload16(const uint8_t* p)
+{
+    const orb_v4u v = *(const __attribute__((address_space(1))) orb_v4u*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 gload8(const uint8_t* p)
+{
+    const orb_v2u v = *(const __attribute__((address_space(1))) orb_v2u*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ unsigned gload4(const uint8_t* p)
+{
+    return *(const __attribute__((address_space(1))) unsigned*)p;
+}
+
 __device__ __forceinline__ int reflect101(int i, int n)
 {
     if (i < 0) i = -i;
@@ -54,18 +88,15 @@ __device__ __forceinline__ int find_segment(const int* starts, int n, int x)
     return lo;
 }
 
-// Sum over the 64 lanes of a wavefront (every lane must be active): DPP
-// quad swaps and row mirrors give each 16-lane row its total, then the four
-// row totals are read into scalar registers.  Returns the same value in
-// every lane.
-__device__ __forceinline__ int wave_sum(int v)
+// Sum over the 16 lanes of a DPP row (all of them active); every lane of the
+// row receives the total.
+__device__ __forceinline__ unsigned row16_sum(unsigned v)
 {
-    v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
-    v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
-    v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);   // row_half_mirror
-    v += __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false);   // row_mirror
-    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
-           __builtin_amdgcn_readlane(v, 48);
+    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false);   // row_mirror
+    return v;
 }
 
 // The dispatcher sends workgroup b to XCD b % 8, and each XCD has its own
@@ -1001,13 +1032,19 @@ __device__ __forceinline__ void sincos_canonical_dev(double x, double* s, double
     }
 }
 
-// One wavefront per kept keypoint; a workgroup serves one (image, level), so
-// the level's geometry and output offset are uniform across it.
+// A 16-lane group per kept keypoint, four per wavefront; a workgroup serves
+// 16 keypoints of one (image, level), so the level's geometry and output
+// offset are uniform across it.  Lane j of a group holds patch rows j, j+16
+// (and j+32), each loaded with constant column offsets, and BRIEF pairs
+// 16j .. 16j+15 (descriptor bytes 2j, 2j+1).
+#define DESC_G 16
+static_assert(256 / DESC_G == ORB_DESC_KPB, "describe blocks are counted on the host");
 __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restrict__ jobs,
                                                        const RunTable* __restrict__ rt)
 {
-    __shared__ unsigned s_brief[4][37 * 10];      // per wavefront: 37 rows x 40 bytes
-    const int lane = threadIdx.x & 63;
+    __shared__ uint2 s_brief[ORB_DESC_KPB][37 * 5];   // per group: 37 rows x 40 bytes
+    __shared__ uint4 s_pat[64];
+    const int sub = threadIdx.x & (DESC_G - 1), grp = threadIdx.x / DESC_G;
     const int nseg = rt->njobs * rt->max_levels;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int f = find_segment(rt->desc_start, nseg, blk);
@@ -1020,9 +1057,17 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
         base += l < level ? c : 0;
         total += c;
     }
-    const int idx = (blk - rt->desc_start[f]) * 4 + (threadIdx.x >> 6);
-    if (level == 0 && idx == 0 && lane == 0) J.counters[ORB_CNT_NOUT] = total;
-    if (level >= J.nlevels || idx >= ctr[ORB_CNT_RET + level]) return;
+    const int idx = (blk - rt->desc_start[f]) * ORB_DESC_KPB + grp;
+    if (level == 0 && idx == 0 && sub == 0) J.counters[ORB_CNT_NOUT] = total;
+    // The 256 BRIEF pairs (one dword of 4 signed bytes each) into LDS, laid
+    // out so that lane j's k-th 16-byte read holds its pairs 16j+4k .. +3
+    // and the 16 lanes of a group read consecutive 16-byte words.
+    {
+        const int t = threadIdx.x;
+        ((unsigned*)s_pat)[(((t >> 2) & 3) * 16 + (t >> 4)) * 4 + (t & 3)] = ((const unsigned*)c_pattern)[t];
+    }
+    __syncthreads();
+    if (level >= J.nlevels || idx >= ctr[ORB_CNT_RET + level]) return;   // the whole group leaves
     const LevelJob& L = J.lv[level];
     const unsigned long long key = L.ret[idx];
     int xr, yr;
@@ -1030,65 +1075,76 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     const int cx = xr + ORB_MIN_BORDER, cy = yr + ORB_MIN_BORDER;
     const int pitch = L.pitch;
 
-    // Both patches are read with aligned dword loads issued together, before
+    // Both patches are read with dword-aligned loads issued together, before
     // any use: the 31x31 orientation patch of the level and the 37x37 window
     // of the blurred level that every rotated BRIEF sample falls in (pattern
     // radius <= 18.39 px).  Keypoints lie >= 19 px inside the level (x <= w-20,
     // y <= h-20), so every dword lies in rows that have a successor and ends
     // at most 2 bytes past the row's last pixel: inside the level's buffer.
+    // Rows past a patch (31, 37..47) are clamped to its last row and unused.
     const int xa0 = (cx - 15) & ~3;               // 31 px + <= 3 of alignment: 9 dwords a row
     const int xb0 = (cx - 18) & ~3;               // 37 px + <= 3 of alignment: 10 dwords a row
-    unsigned pa[5], pb[6];
+    // A row is read as two 16-byte loads and the rest.
+    unsigned pa[2][9], pb[3][10];
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
-        const int sl = lane + 64 * t;
-        const int r = sl / 9, c = sl - r * 9;
-        pa[t] = sl < 31 * 9 ? *(const unsigned*)(L.lvl + (size_t)(cy - 15 + r) * pitch + xa0 + 4 * c) : 0u;
+    for (int h = 0; h < 2; h++) {
+        const uint8_t* rp = L.lvl + (size_t)(cy - 15 + min(sub + 16 * h, 30)) * pitch + xa0;
+        const uint4 q0 = gload16(rp), q1 = gload16(rp + 16);
+        pa[h][0] = q0.x, pa[h][1] = q0.y, pa[h][2] = q0.z, pa[h][3] = q0.w;
+        pa[h][4] = q1.x, pa[h][5] = q1.y, pa[h][6] = q1.z, pa[h][7] = q1.w;
+        pa[h][8] = gload4(rp + 32);
     }
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
-        const int sl = lane + 64 * t;
-        const int r = sl / 10, c = sl - r * 10;
-        pb[t] = sl < 37 * 10 ? *(const unsigned*)(L.blur + (size_t)(cy - 18 + r) * pitch + xb0 + 4 * c) : 0u;
+    for (int h = 0; h < 3; h++) {
+        const uint8_t* rp = L.blur + (size_t)(cy - 18 + min(sub + 16 * h, 36)) * pitch + xb0;
+        const uint4 q0 = gload16(rp), q1 = gload16(rp + 16);
+        const uint2 q2 = gload8(rp + 32);
+        pb[h][0] = q0.x, pb[h][1] = q0.y, pb[h][2] = q0.z, pb[h][3] = q0.w;
+        pb[h][4] = q1.x, pb[h][5] = q1.y, pb[h][6] = q1.z, pb[h][7] = q1.w;
+        pb[h][8] = q2.x, pb[h][9] = q2.y;
     }
-    unsigned* win = s_brief[threadIdx.x >> 6];
+    uint2* win = s_brief[grp];
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
-        const int sl = lane + 64 * t;
-        if (sl < 37 * 10) win[sl] = pb[t];
+    for (int h = 0; h < 3; h++) {
+        const int r = sub + 16 * h;
+        if (r < 37) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) win[r * 5 + c] = make_uint2(pb[h][2 * c], pb[h][2 * c + 1]);
+        }
     }
 
     // IC_Angle (ORBextractor.cc:94-141): m10 = sum u*I, m01 = sum v*I over
-    // the disc |u| <= umax[|v|].  Per dword of 4 pixels: a byte mask of the
-    // disc, then v_dot4_u32_u8 for the masked sum and for sum (u+20)*I.
-    unsigned long long umt = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) umt |= (unsigned long long)c_umax[i] << (4 * i);
+    // the disc |u| <= umax[|v|].  Per dword of 4 pixels: the disc's byte mask
+    // (a table over the row's |v| and the patch's alignment), then
+    // v_dot4_u32_u8 for the masked sum and for sum (u+20)*I.
+    const int al = (cx - 15) & 3;                 // u of byte k of dword c: 4c + k - 15 - al
+    const unsigned w0 = (unsigned)(5 - al) * 0x01010101u + 0x03020100u;   // bytes u+20 of dword 0
     int m10 = 0, m01 = 0;
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
-        const int sl = lane + 64 * t;
-        const int r = sl / 9, c = sl - r * 9;
-        const int v = r - 15, av = v < 0 ? -v : v;
-        const int um = (int)((umt >> (4 * (av & 15))) & 15u);
-        const int u0 = xa0 + 4 * c - cx;
-        const int lo = -um - u0, hi = um - u0;    // byte k is in the disc iff lo <= k <= hi
-        const int kl = min(max(lo, 0), 3), kh = max(min(hi, 3), 0);
-        const bool any = sl < 31 * 9 && lo <= 3 && hi >= 0;
-        const unsigned mask = any ? ((0xFFFFFFFFu << (8 * kl)) & (0xFFFFFFFFu >> (8 * (3 - kh)))) : 0u;
-        const unsigned md = pa[t] & mask;
-        const unsigned wts = (unsigned)(u0 + 20) * 0x01010101u + 0x03020100u;   // bytes u+20 in [2, 40]
-        const int sum = (int)__builtin_amdgcn_udot4(md, 0x01010101u, 0u, false);
-        const int sumu = (int)__builtin_amdgcn_udot4(md, wts, 0u, false);
-        m01 += v * sum;
-        m10 += sumu - 20 * sum;
+    for (int h = 0; h < 2; h++) {
+        const int r = sub + 16 * h, v = r - 15, av = v < 0 ? -v : v;
+        const unsigned* mk = c_icmask + (al * 16 + (av & 15)) * 9;
+        unsigned s1 = 0, su = 0;
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            const unsigned md = pa[h][c] & mk[c];
+            s1 = __builtin_amdgcn_udot4(md, 0x01010101u, s1, false);
+            su = __builtin_amdgcn_udot4(md, w0 + (unsigned)(4 * c) * 0x01010101u, su, false);
+        }
+        if (r < 31) {
+            m01 += v * (int)s1;
+            m10 += (int)su - 20 * (int)s1;
+        }
     }
-    m10 = wave_sum(m10);
-    m01 = wave_sum(m01);
+    m10 = (int)row16_sum((unsigned)m10);
+    m01 = (int)row16_sum((unsigned)m01);
     const float angle_deg = fast_atan2_dev((float)m01, (float)m10);
 
     // Rotated BRIEF (ORBextractor.cc:153-204) on the blurred level, sampled
     // from the staged window (row stride 40 bytes, centre at row 18).
+    // cvRound of a sample coordinate t (|t| < 19) is read from the bits of
+    // t + 1.5 * 2^23, whose low 23 bits hold 2^22 + rint(t): the sum rounds
+    // to the nearest integer, ties to even, as rint does.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1097,23 +1153,33 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     double sd, cd;
     sincos_canonical_dev((double)angle, &sd, &cd);
     const float a = (float)cd, b = (float)sd;
-    const uint8_t* center = (const uint8_t*)win + 18 * 40 + (cx - xb0);
-    int i0[4], i1[4];
+    const float kRound = 12582912.0f;
+    // sample (iy, ix) is byte (18 + iy) * 40 + (cx - xb0) + ix of the window;
+    // with both read as 2^22 + i in the low 24 bits, the byte is
+    // (2^22 + iy) * 40 + (bits of x) - kBias
+    const unsigned kBias = (1u << 22) * 40u + 0x4B400000u - 18u * 40u - (unsigned)(cx - xb0);
+    const uint8_t* wb = (const uint8_t*)win;
+    unsigned bits = 0;
 #pragma unroll
-    for (int bit = 0; bit < 4; bit++) {
-        const int p = lane * 4 + bit;   // pair index: byte p/8, bit p%8
-        const float x0 = (float)c_pattern[4 * p], y0 = (float)c_pattern[4 * p + 1];
-        const float x1 = (float)c_pattern[4 * p + 2], y1 = (float)c_pattern[4 * p + 3];
-        i0[bit] = center[(int)rintf(x0 * b + y0 * a) * 40 + (int)rintf(x0 * a - y0 * b)];
-        i1[bit] = center[(int)rintf(x1 * b + y1 * a) * 40 + (int)rintf(x1 * a - y1 * b)];
+    for (int k = 0; k < 4; k++) {
+        const uint4 q = s_pat[k * 16 + sub];
+        const unsigned qd[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int d = (int)qd[e];
+            const float px0 = (float)(signed char)d, py0 = (float)(signed char)(d >> 8);
+            const float px1 = (float)(signed char)(d >> 16), py1 = (float)(d >> 24);
+            const float y0r = px0 * b + py0 * a, x0r = px0 * a - py0 * b;
+            const float y1r = px1 * b + py1 * a, x1r = px1 * a - py1 * b;
+            const unsigned o0 = (__float_as_uint(y0r + kRound) & 0xFFFFFFu) * 40u + __float_as_uint(x0r + kRound) - kBias;
+            const unsigned o1 = (__float_as_uint(y1r + kRound) & 0xFFFFFFu) * 40u + __float_as_uint(x1r + kRound) - kBias;
+            const int i0 = wb[o0], i1 = wb[o1];
+            bits |= (unsigned)(i0 < i1) << (4 * k + e);
+        }
     }
-    int nib = 0;
-#pragma unroll
-    for (int bit = 0; bit < 4; bit++) nib |= (i0[bit] < i1[bit]) << bit;
-    const int hi = __shfl_down(nib, 1, 64);
     const int out = base + idx;
-    if ((lane & 1) == 0) J.desc[(size_t)out * 32 + (lane >> 1)] = (uint8_t)(nib | (hi << 4));
-    if (lane == 0) {
+    ((unsigned short*)(J.desc + (size_t)out * 32))[sub] = (unsigned short)bits;
+    if (sub == 0) {
         const float sc = L.scale;
         OrbKeyPoint kp;
         float fx = (float)cx, fy = (float)cy;
@@ -1259,16 +1325,6 @@ __device__ __forceinline__ unsigned patch_dword(const uint8_t* rp, int x, int w)
     return d;
 }
 
-// Sum over the 16 lanes of a DPP row (all of them active); every lane of the
-// row receives the total.
-__device__ __forceinline__ unsigned row16_sum(unsigned v)
-{
-    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
-    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
-    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);   // row_half_mirror
-    v += (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false);   // row_mirror
-    return v;
-}
 
 // grid (ceil(max left capacity / 16), pairs): lanes 16g .. 16g+15 of a
 // wavefront serve one left keypoint.  Every branch is uniform within a

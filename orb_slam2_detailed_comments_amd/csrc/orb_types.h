@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #define ORB_MAX_LEVELS 16
+#define ORB_DESC_KPB 16             // keypoints per describe workgroup (16 lanes each)
 #define ORB_MAX_ROOTS 16
 #define ORB_LIST_CAP 512       // quadtree list capacity per level (LDS)
 #define ORB_TILE_W 64          // level kernel output tile
