@@ -213,7 +213,9 @@ __device__ __forceinline__ unsigned hi_bytes(unsigned x) { return __builtin_amdg
 // Four adjacent pixels of cv::resize INTER_LINEAR 8U from source rows p0,
 // p1: horizontal 11-bit taps (a0 | a1 << 16) per column, vertical taps
 // yb = b0 | b1 << 16, rounding shift by 22.  All terms are non-negative, so
-// the shift is done unsigned.  (A signed shift with a [0, 255] clamp of lanes
+// the shift is done unsigned.  Every factor is below 2^24 (taps <= 2048,
+// S <= 255 * 4096), so the products are 24-bit multiplies (v_mul_u32_u24,
+// full rate) rather than 32-bit ones (v_mul_lo_u32, quarter rate).  (A signed shift with a [0, 255] clamp of lanes
 // 0 and 1 is lowered by the gfx950 backend to v_ashr_pk_u8_i32, whose upper
 // result half then leaked into bytes 2 and 3.)
 __device__ __forceinline__ unsigned resize4(const uint8_t* p0, const uint8_t* p1, const int* sxk, const int* nxk,
@@ -224,9 +226,9 @@ __device__ __forceinline__ unsigned resize4(const uint8_t* p0, const uint8_t* p1
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const unsigned a0 = (unsigned)tap[k] & 0xFFFFu, a1 = (unsigned)tap[k] >> 16;
-        const unsigned S0 = p0[sxk[k]] * a0 + p0[nxk[k]] * a1;
-        const unsigned S1 = p1[sxk[k]] * a0 + p1[nxk[k]] * a1;
-        const unsigned v = (S0 * b0 + S1 * b1 + (1u << 21)) >> 22;
+        const unsigned S0 = __umul24(p0[sxk[k]], a0) + __umul24(p0[nxk[k]], a1);
+        const unsigned S1 = __umul24(p1[sxk[k]], a0) + __umul24(p1[nxk[k]], a1);
+        const unsigned v = (__umul24(S0, b0) + __umul24(S1, b1) + (1u << 21)) >> 22;
         d |= min(v, 255u) << (8 * k);
     }
     return d;
@@ -359,8 +361,9 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 const int ws = nwd * 4;
                 for (int ey = r0; ey < LP_H; ey += 14) {
                     const int sy = ys0[ey];
-                    pxw[ey][c] = resize4(wb + (min(max(sy, 0), sh - 1) - wy0) * ws,
-                                         wb + (min(max(sy + 1, 0), sh - 1) - wy0) * ws, sxk, nxk, tap, yb[ey]);
+                    pxw[ey][c] = resize4(wb + __umul24(min(max(sy, 0), sh - 1) - wy0, ws),
+                                         wb + __umul24(min(max(sy + 1, 0), sh - 1) - wy0, ws), sxk, nxk, tap,
+                                         yb[ey]);
                 }
             } else {
                 for (int ey = r0; ey < LP_H; ey += 14) {
@@ -411,7 +414,9 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     }
     __syncthreads();
     {
-        const int k0 = c_gauss[3], k1 = c_gauss[4], k2 = c_gauss[5], k3 = c_gauss[6];
+        // Row sums are <= 255 * 257 and taps < 256: 24-bit multiplies.
+        const unsigned k0 = (unsigned)c_gauss[3], k1 = (unsigned)c_gauss[4], k2 = (unsigned)c_gauss[5],
+                       k3 = (unsigned)c_gauss[6];
         for (int r = tid >> 4; r < LT_H; r += 16) {
             const int y = ty0 + r, x0 = tx0 + 4 * gc;
             if (y >= h) continue;
@@ -421,8 +426,9 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             unsigned out = 0;
 #define ORB_VBLUR(comp, k)                                                                              \
             {                                                                                            \
-                const unsigned acc = q[3].comp * k0 + (q[2].comp + q[4].comp) * k1 +                    \
-                                     (q[1].comp + q[5].comp) * k2 + (q[0].comp + q[6].comp) * k3;       \
+                const unsigned acc = __umul24(q[3].comp, k0) + __umul24(q[2].comp + q[4].comp, k1) +    \
+                                     __umul24(q[1].comp + q[5].comp, k2) +                               \
+                                     __umul24(q[0].comp + q[6].comp, k3);                                \
                 const unsigned v = (acc + (1u << 15)) >> 16;                                            \
                 out |= (v > 255u ? 255u : v) << (8 * (k));                                              \
             }
