@@ -142,40 +142,47 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask)
 
 typedef unsigned short orb_us2 __attribute__((ext_vector_type(2)));
 
+typedef short orb_s2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ int fast_strength(const uint8_t* p, int stride)
 {
-    // Bresenham radius-3 circle in circular order (OpenCV makeOffsets(16)).
-    const int v = p[0];
-    int d[16];
-    d[0] = p[3 * stride] - v;       d[1] = p[3 * stride + 1] - v;
-    d[2] = p[2 * stride + 2] - v;   d[3] = p[stride + 3] - v;
-    d[4] = p[3] - v;                d[5] = p[-stride + 3] - v;
-    d[6] = p[-2 * stride + 2] - v;  d[7] = p[-3 * stride + 1] - v;
-    d[8] = p[-3 * stride] - v;      d[9] = p[-3 * stride - 1] - v;
-    d[10] = p[-2 * stride - 2] - v; d[11] = p[-stride - 3] - v;
-    d[12] = p[-3] - v;              d[13] = p[stride - 3] - v;
-    d[14] = p[2 * stride - 2] - v;  d[15] = p[3 * stride - 1] - v;
-    // Minimum and maximum over every run of 9 contiguous ring pixels.
-    int mn[16], mx[16];
+    // Bresenham radius-3 circle in circular order (OpenCV makeOffsets(16)),
+    // differences to the centre as 16-bit pairs D[j] = (d[2j], d[2j+1]),
+    // each in [-255, 255].
+    const unsigned v = p[0];
+    const unsigned r[16] = {p[3 * stride],      p[3 * stride + 1],  p[2 * stride + 2],  p[stride + 3],
+                            p[3],               p[-stride + 3],     p[-2 * stride + 2], p[-3 * stride + 1],
+                            p[-3 * stride],     p[-3 * stride - 1], p[-2 * stride - 2], p[-stride - 3],
+                            p[-3],              p[stride - 3],      p[2 * stride - 2],  p[3 * stride - 1]};
+    const orb_s2 V = __builtin_bit_cast(orb_s2, v | (v << 16));
+    orb_s2 D[8];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        mn[k] = min(d[k], d[(k + 1) & 15]);
-        mx[k] = max(d[k], d[(k + 1) & 15]);
-    }
-    int mn4[16], mx4[16];
+    for (int j = 0; j < 8; j++) D[j] = __builtin_bit_cast(orb_s2, r[2 * j] | (r[2 * j + 1] << 16)) - V;
+    // Minimum and maximum over every run of 9 contiguous ring pixels, two
+    // runs per packed op: pairs (k, k+1), then runs of 4, then 4 + 4 + 1.
+    orb_s2 MN[8], MX[8];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        mn4[k] = min(mn[k], mn[(k + 2) & 15]);
-        mx4[k] = max(mx[k], mx[(k + 2) & 15]);
+    for (int j = 0; j < 8; j++) {
+        const orb_s2 S = __builtin_bit_cast(orb_s2, __builtin_amdgcn_alignbit(__builtin_bit_cast(unsigned, D[(j + 1) & 7]),
+                                                                            __builtin_bit_cast(unsigned, D[j]), 16));
+        MN[j] = __builtin_elementwise_min(D[j], S);      // (d[2j], d[2j+1]) with (d[2j+1], d[2j+2])
+        MX[j] = __builtin_elementwise_max(D[j], S);
     }
-    int bright = -256, dark = 256;
+    orb_s2 M4[8], X4[8];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const int m9 = min(min(mn4[k], mn4[(k + 4) & 15]), d[(k + 8) & 15]);
-        const int x9 = max(max(mx4[k], mx4[(k + 4) & 15]), d[(k + 8) & 15]);
-        bright = max(bright, m9);
-        dark = min(dark, x9);
+    for (int j = 0; j < 8; j++) {
+        M4[j] = __builtin_elementwise_min(MN[j], MN[(j + 1) & 7]);
+        X4[j] = __builtin_elementwise_max(MX[j], MX[(j + 1) & 7]);
     }
+    orb_s2 B = {-256, -256}, K = {256, 256};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        B = __builtin_elementwise_max(B, __builtin_elementwise_min(__builtin_elementwise_min(M4[j], M4[(j + 2) & 7]),
+                                                                   D[(j + 4) & 7]));
+        K = __builtin_elementwise_min(K, __builtin_elementwise_max(__builtin_elementwise_max(X4[j], X4[(j + 2) & 7]),
+                                                                   D[(j + 4) & 7]));
+    }
+    const int bright = max((int)B.x, (int)B.y), dark = min((int)K.x, (int)K.y);
     // A pixel is a FAST-9 corner at threshold t iff strength > t, and then
     // OpenCV's cornerScore equals strength - 1.
     const int s = max(bright, -dark);
@@ -210,15 +217,19 @@ __device__ __forceinline__ unsigned fast_maybe2(unsigned v, unsigned p0, unsigne
 __device__ __forceinline__ unsigned lo_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C020C00u); }
 __device__ __forceinline__ unsigned hi_bytes(unsigned x) { return __builtin_amdgcn_perm(0u, x, 0x0C030C01u); }
 
-// Four adjacent pixels of cv::resize INTER_LINEAR 8U from source rows p0,
-// p1: horizontal 11-bit taps (a0 | a1 << 16) per column, vertical taps
-// yb = b0 | b1 << 16, rounding shift by 22.  All terms are non-negative, so
-// the shift is done unsigned.  Every factor is below 2^24 (taps <= 2048,
-// S <= 255 * 4096), so the products are 24-bit multiplies (v_mul_u32_u24,
-// full rate) rather than 32-bit ones (v_mul_lo_u32, quarter rate).  (A signed shift with a [0, 255] clamp of lanes
-// 0 and 1 is lowered by the gfx950 backend to v_ashr_pk_u8_i32, whose upper
-// result half then leaked into bytes 2 and 3.)
-__device__ __forceinline__ unsigned resize4(const uint8_t* p0, const uint8_t* p1, const int* sxk, const int* nxk,
+// Four adjacent pixels of cv::resize INTER_LINEAR 8U from the source rows at
+// byte offsets o0, o1 of base (the LDS window, or the level in global memory
+// with 32-bit offsets): horizontal 11-bit taps (a0 | a1 << 16) per column,
+// vertical taps yb = b0 | b1 << 16, rounding shift by 22.
+// - All terms are non-negative, so the shift is done unsigned.  (A signed
+//   shift with a [0, 255] clamp of lanes 0 and 1 is lowered by the gfx950
+//   backend to v_ashr_pk_u8_i32, whose upper result half then leaked into
+//   bytes 2 and 3.)
+// - Every factor is below 2^24 (taps <= 2048, S <= 255 * 4096), so the
+//   products are 24-bit multiplies (v_mul_u32_u24, full rate) rather than
+//   32-bit ones (v_mul_lo_u32, quarter rate).
+template <typename BytePtr>
+__device__ __forceinline__ unsigned resize4(BytePtr base, unsigned o0, unsigned o1, const int* sxk, const int* nxk,
                                             const int* tap, int yb)
 {
     const unsigned b0 = (unsigned)yb & 0xFFFFu, b1 = (unsigned)yb >> 16;
@@ -226,8 +237,8 @@ __device__ __forceinline__ unsigned resize4(const uint8_t* p0, const uint8_t* p1
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const unsigned a0 = (unsigned)tap[k] & 0xFFFFu, a1 = (unsigned)tap[k] >> 16;
-        const unsigned S0 = __umul24(p0[sxk[k]], a0) + __umul24(p0[nxk[k]], a1);
-        const unsigned S1 = __umul24(p1[sxk[k]], a0) + __umul24(p1[nxk[k]], a1);
+        const unsigned S0 = __umul24(base[o0 + sxk[k]], a0) + __umul24(base[o0 + nxk[k]], a1);
+        const unsigned S1 = __umul24(base[o1 + sxk[k]], a0) + __umul24(base[o1 + nxk[k]], a1);
         const unsigned v = (__umul24(S0, b0) + __umul24(S1, b1) + (1u << 21)) >> 22;
         d |= min(v, 255u) << (8 * k);
     }
@@ -361,15 +372,15 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 const int ws = nwd * 4;
                 for (int ey = r0; ey < LP_H; ey += 14) {
                     const int sy = ys0[ey];
-                    pxw[ey][c] = resize4(wb + __umul24(min(max(sy, 0), sh - 1) - wy0, ws),
-                                         wb + __umul24(min(max(sy + 1, 0), sh - 1) - wy0, ws), sxk, nxk, tap,
-                                         yb[ey]);
+                    pxw[ey][c] = resize4(wb, __umul24(min(max(sy, 0), sh - 1) - wy0, ws),
+                                         __umul24(min(max(sy + 1, 0), sh - 1) - wy0, ws), sxk, nxk, tap, yb[ey]);
                 }
             } else {
                 for (int ey = r0; ey < LP_H; ey += 14) {
                     const int sy = ys0[ey];
-                    pxw[ey][c] = resize4(P.lvl + (size_t)min(max(sy, 0), sh - 1) * sp,
-                                         P.lvl + (size_t)min(max(sy + 1, 0), sh - 1) * sp, sxk, nxk, tap, yb[ey]);
+                    pxw[ey][c] = resize4((const __attribute__((address_space(1))) uint8_t*)P.lvl,
+                                         __umul24(min(max(sy, 0), sh - 1), sp),
+                                         __umul24(min(max(sy + 1, 0), sh - 1), sp), sxk, nxk, tap, yb[ey]);
                 }
             }
         }
