@@ -1354,6 +1354,9 @@ __device__ __forceinline__ unsigned patch_dword(const uint8_t* rp, int x, int w)
 //   [0, 766]; the row parts are summed across the group with DPP, two shifts
 //   per dword (a total is at most 121 * 510 < 2^16).
 #define ST_G 16
+#ifndef ST_CHUNK
+#define ST_CHUNK 2
+#endif
 __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restrict__ args)
 {
     const int sub = threadIdx.x & (ST_G - 1);
@@ -1388,20 +1391,45 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     if (go) {
         (void)nr;
         const int2 rg = A.rows[row];
-#pragma unroll 2
-        for (int c = rg.x + sub; c < rg.y; c += ST_G) {
-            const int4 rec = A.srec[c];
-            const int oct = rec.z & 15;
-            if (row < rec.x || row > rec.y) continue;
-            if (oct < levelL - 1 || oct > levelL + 1) continue;
-            const float uR = __int_as_float(rec.w);
-            if (uR >= minU && uR <= maxU) {
-                const int iR = rec.z >> 4;
-                const int dist = hamming32(a0, a1, A.sdesc + (size_t)c * 32);
-                if (dist < bestDist || (dist == bestDist && iR < bestIdx)) {
-                    bestDist = dist;
-                    bestIdx = iR;
-                    bestU = uR;
+        // Chunks of ST_CHUNK candidates per lane: the records are loaded
+        // together, then the descriptors of those that pass the band, octave
+        // and uR tests, so each chunk waits on memory twice.
+        for (int c0 = rg.x + sub; c0 < rg.y; c0 += ST_CHUNK * ST_G) {
+            int4 rec[ST_CHUNK];
+#pragma unroll
+            for (int u = 0; u < ST_CHUNK; u++) {
+                const int c = c0 + u * ST_G;
+                rec[u] = c < rg.y ? A.srec[c] : make_int4(1, 0, 0, 0);   // minr > maxr: never a candidate
+            }
+            bool pass[ST_CHUNK];
+#pragma unroll
+            for (int u = 0; u < ST_CHUNK; u++) {
+                const int oct = rec[u].z & 15;
+                const float uR = __int_as_float(rec[u].w);
+                pass[u] = row >= rec[u].x && row <= rec[u].y && oct >= levelL - 1 && oct <= levelL + 1 &&
+                          uR >= minU && uR <= maxU;
+            }
+            uint4 b0[ST_CHUNK], b1[ST_CHUNK];
+#pragma unroll
+            for (int u = 0; u < ST_CHUNK; u++) {
+                if (pass[u]) {
+                    const uint8_t* d = A.sdesc + (size_t)(c0 + u * ST_G) * 32;
+                    b0[u] = gload16(d);
+                    b1[u] = gload16(d + 16);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < ST_CHUNK; u++) {
+                if (pass[u]) {
+                    const int iR = rec[u].z >> 4;
+                    const int dist = __popc(a0.x ^ b0[u].x) + __popc(a0.y ^ b0[u].y) + __popc(a0.z ^ b0[u].z) +
+                                     __popc(a0.w ^ b0[u].w) + __popc(a1.x ^ b1[u].x) + __popc(a1.y ^ b1[u].y) +
+                                     __popc(a1.z ^ b1[u].z) + __popc(a1.w ^ b1[u].w);
+                    if (dist < bestDist || (dist == bestDist && iR < bestIdx)) {
+                        bestDist = dist;
+                        bestIdx = iR;
+                        bestU = __int_as_float(rec[u].w);
+                    }
                 }
             }
         }
