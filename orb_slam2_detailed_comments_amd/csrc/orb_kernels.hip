@@ -1098,14 +1098,16 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     // radius <= 18.39 px).  Keypoints lie >= 19 px inside the level (x <= w-20,
     // y <= h-20), so every dword lies in rows that have a successor and ends
     // at most 2 bytes past the row's last pixel: inside the level's buffer.
-    // Rows past a patch (31, 37..47) are clamped to its last row and unused.
+    // Blur window rows past the patch (37..47) are clamped to its last row
+    // and unused.
     const int xa0 = (cx - 15) & ~3;               // 31 px + <= 3 of alignment: 9 dwords a row
     const int xb0 = (cx - 18) & ~3;               // 37 px + <= 3 of alignment: 10 dwords a row
     // A row is read as two 16-byte loads and the rest.
     unsigned pa[2][9], pb[3][10];
+    // Disc rows: lane j holds v = -j and v = +j, which share one disc mask.
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        const uint8_t* rp = L.lvl + (size_t)(cy - 15 + min(sub + 16 * h, 30)) * pitch + xa0;
+        const uint8_t* rp = L.lvl + (size_t)(cy + (h ? sub : -sub)) * pitch + xa0;
         const uint4 q0 = gload16(rp), q1 = gload16(rp + 16);
         pa[h][0] = q0.x, pa[h][1] = q0.y, pa[h][2] = q0.z, pa[h][3] = q0.w;
         pa[h][4] = q1.x, pa[h][5] = q1.y, pa[h][6] = q1.z, pa[h][7] = q1.w;
@@ -1137,18 +1139,21 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     const int al = (cx - 15) & 3;                 // u of byte k of dword c: 4c + k - 15 - al
     const unsigned w0 = (unsigned)(5 - al) * 0x01010101u + 0x03020100u;   // bytes u+20 of dword 0
     int m10 = 0, m01 = 0;
+    const unsigned* mk = c_icmask + (al * 16 + sub) * 9;
+    unsigned mask[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) mask[c] = mk[c];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        const int r = sub + 16 * h, v = r - 15, av = v < 0 ? -v : v;
-        const unsigned* mk = c_icmask + (al * 16 + (av & 15)) * 9;
+        const int v = h ? sub : -sub;
         unsigned s1 = 0, su = 0;
 #pragma unroll
         for (int c = 0; c < 9; c++) {
-            const unsigned md = pa[h][c] & mk[c];
+            const unsigned md = pa[h][c] & mask[c];
             s1 = __builtin_amdgcn_udot4(md, 0x01010101u, s1, false);
             su = __builtin_amdgcn_udot4(md, w0 + (unsigned)(4 * c) * 0x01010101u, su, false);
         }
-        if (r < 31) {
+        if (h == 0 || sub > 0) {                  // row v = 0 once
             m01 += v * (int)s1;
             m10 += (int)su - 20 * (int)s1;
         }
