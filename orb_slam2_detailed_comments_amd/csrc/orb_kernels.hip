@@ -629,6 +629,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
 // exactly the list's.  Ties in the phase-2 sort are broken by creation order
 // (newer = larger), the oracle's fixed stand-in for the heap address.
 #define OCT_T 1024
+#define OCT_REG 4                    // candidates per thread held in registers
 
 struct OctShared {
     int rx[2][ORB_LIST_CAP];     // x0 | x1 << 16
@@ -733,17 +734,39 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
     __syncthreads();
     const unsigned long long* keys = L.keys2;
 
+    // Each thread keeps the position and list node of its candidates
+    // k = tid + OCT_T*m, m < OCT_REG, in registers for every pass; candidates
+    // beyond OCT_REG * OCT_T (rare) keep their node in global memory.
+    int kx[OCT_REG], ky[OCT_REG], kn[OCT_REG];
+#pragma unroll
+    for (int m = 0; m < OCT_REG; m++) {
+        const int k = tid + OCT_T * m;
+        kx[m] = ky[m] = kn[m] = 0;
+        if (k < K) key_xy(keys[k], wC, hC, &kx[m], &ky[m]);
+    }
+    // f(x, y, node&) on every candidate of this thread.
+    auto for_keys = [&](auto&& f) {
+#pragma unroll
+        for (int m = 0; m < OCT_REG; m++)
+            if (tid + OCT_T * m < K) f(kx[m], ky[m], kn[m]);
+        for (int k = tid + OCT_T * OCT_REG; k < K; k += OCT_T) {
+            int x, y;
+            key_xy(keys[k], wC, hC, &x, &y);
+            int nd = node[k];
+            f(x, y, nd);
+            node[k] = nd;
+        }
+    };
+
     // ---- roots (ORBextractor.cc:695-763)
     if (tid < ORB_MAX_ROOTS) S.cc[tid][0] = 0;
     __syncthreads();
-    for (int k = tid; k < K; k += OCT_T) {
-        int x, y;
-        key_xy(keys[k], wC, hC, &x, &y);
+    for_keys([&](int x, int, int& nd) {
         int r = (int)__fdiv_rn((float)x, L.hX);
         r = min(r, L.nIni - 1);
-        node[k] = r;
+        nd = r;
         atomicAdd(&S.cc[r][0], 1);
-    }
+    });
     __syncthreads();
     {
         const int c = tid < L.nIni ? S.cc[tid][0] : 0;
@@ -757,7 +780,7 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         if (tid < L.nIni) S.keep[tid] = (short)pos;
     }
     __syncthreads();
-    for (int k = tid; k < K; k += OCT_T) node[k] = S.keep[node[k]];
+    for_keys([&](int, int, int& nd) { nd = S.keep[nd]; });
     int cur = 0;
     int size = total;
     bool phase2 = false;
@@ -768,14 +791,10 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         const int prev = size;
         for (int i = tid; i < size; i += OCT_T) S.cc[i][0] = S.cc[i][1] = S.cc[i][2] = S.cc[i][3] = 0;
         __syncthreads();
-        for (int k = tid; k < K; k += OCT_T) {
-            const int i = node[k];
-            if (S.cnt[cur][i] >= 2) {
-                int x, y;
-                key_xy(keys[k], wC, hC, &x, &y);
-                atomicAdd(&S.cc[i][quad_of(S.rx[cur][i], S.ry[cur][i], x, y)], 1);
-            }
-        }
+        for_keys([&](int x, int y, int& nd) {
+            const int i = nd;
+            if (S.cnt[cur][i] >= 2) atomicAdd(&S.cc[i][quad_of(S.rx[cur][i], S.ry[cur][i], x, y)], 1);
+        });
         __syncthreads();
         const int i = tid;
         const bool split = i < size && S.cnt[cur][i] >= 2;
@@ -824,16 +843,10 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
             size = 0;
             break;
         }
-        for (int k = tid; k < K; k += OCT_T) {
-            const int n0 = node[k];
-            if (S.cnt[cur][n0] >= 2) {
-                int x, y;
-                key_xy(keys[k], wC, hC, &x, &y);
-                node[k] = S.map[n0][quad_of(S.rx[cur][n0], S.ry[cur][n0], x, y)];
-            } else {
-                node[k] = S.keep[n0];
-            }
-        }
+        for_keys([&](int x, int y, int& nd) {
+            const int n0 = nd;
+            nd = S.cnt[cur][n0] >= 2 ? S.map[n0][quad_of(S.rx[cur][n0], S.ry[cur][n0], x, y)] : S.keep[n0];
+        });
         cur ^= 1;
         size = nsize;
         __syncthreads();
@@ -849,14 +862,10 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         const int prev = size;
         for (int i = tid; i < size; i += OCT_T) S.cc[i][0] = S.cc[i][1] = S.cc[i][2] = S.cc[i][3] = 0;
         __syncthreads();
-        for (int k = tid; k < K; k += OCT_T) {
-            const int i = node[k];
-            if (S.cnt[cur][i] >= 2) {
-                int x, y;
-                key_xy(keys[k], wC, hC, &x, &y);
-                atomicAdd(&S.cc[i][quad_of(S.rx[cur][i], S.ry[cur][i], x, y)], 1);
-            }
-        }
+        for_keys([&](int x, int y, int& nd) {
+            const int i = nd;
+            if (S.cnt[cur][i] >= 2) atomicAdd(&S.cc[i][quad_of(S.rx[cur][i], S.ry[cur][i], x, y)], 1);
+        });
         // Expandable nodes, sorted by (size, creation order), largest first.
         const bool e = tid < size && S.cnt[cur][tid] >= 2;
         int nE;
@@ -948,17 +957,11 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         __syncthreads();
         if (kp) S.keep[tid] = (short)keeppos;
         __syncthreads();
-        for (int k = tid; k < K; k += OCT_T) {
-            const int n0 = node[k];
+        for_keys([&](int x, int y, int& nd) {
+            const int n0 = nd;
             const int kv = S.keep[n0];
-            if (kv < 0) {
-                int x, y;
-                key_xy(keys[k], wC, hC, &x, &y);
-                node[k] = S.map[n0][quad_of(S.rx[cur][n0], S.ry[cur][n0], x, y)];
-            } else {
-                node[k] = kv;
-            }
-        }
+            nd = kv < 0 ? S.map[n0][quad_of(S.rx[cur][n0], S.ry[cur][n0], x, y)] : kv;
+        });
         cur ^= 1;
         size = nsize;
         __syncthreads();
@@ -968,11 +971,14 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
     // ---- retain the best response per node; first in key order on ties
     for (int i = tid; i < size; i += OCT_T) S.best[i] = 0ull;
     __syncthreads();
-    for (int k = tid; k < K; k += OCT_T) {
-        const unsigned long long key = keys[k];
+    auto retain = [&](unsigned long long key, int nd) {
         const unsigned long long packed = (key & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFull - (key & 0xFFFFFFFFull));
-        atomicMax(&S.best[node[k]], packed);
-    }
+        atomicMax(&S.best[nd], packed);
+    };
+#pragma unroll
+    for (int m = 0; m < OCT_REG; m++)
+        if (tid + OCT_T * m < K) retain(keys[tid + OCT_T * m], kn[m]);
+    for (int k = tid + OCT_T * OCT_REG; k < K; k += OCT_T) retain(keys[k], node[k]);
     __syncthreads();
     const int cap = min(L.list_cap, ORB_LIST_CAP);
     if (size > cap) {
