@@ -870,34 +870,25 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         const bool e = tid < size && S.cnt[cur][tid] >= 2;
         int nE;
         const int ex_e = block_scan(e ? 1 : 0, S.scan, &nE);
-        int npow = 1;
-        while (npow < nE) npow <<= 1;
+        // The keys are distinct (they carry the node index), so a key's
+        // place in descending order is the number of larger keys; every
+        // thread reads the same key at a time (an LDS broadcast).
         if (e)
             S.sortk[ex_e] = ((unsigned long long)S.cnt[cur][tid] << 32) |
                             ((unsigned long long)(unsigned)S.seq[cur][tid] << 16) | (unsigned)tid;
-        for (int j = nE + tid; j < npow; j += OCT_T) S.sortk[j] = 0ull;
         __syncthreads();
-        for (int k2 = 2; k2 <= npow; k2 <<= 1) {
-            for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
-                for (int a = tid; a < npow; a += OCT_T) {
-                    const int b = a ^ j2;
-                    if (b > a) {
-                        const unsigned long long va = S.sortk[a], vb = S.sortk[b];
-                        const bool desc = (a & k2) == 0;
-                        if (desc ? (va < vb) : (va > vb)) {
-                            S.sortk[a] = vb;
-                            S.sortk[b] = va;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
+        if (tid < nE) {
+            const unsigned long long mine = S.sortk[tid];
+            int rank = 0;
+            for (int j2 = 0; j2 < nE; j2++) rank += S.sortk[j2] > mine;
+            S.best[rank] = mine;                  // S.best is free until the retain step
         }
+        __syncthreads();
         // Gains in processing order and the first split reaching the quota.
         const int j = tid;
         int node_j = -1, nc = 0;
         if (j < nE) {
-            node_j = (int)(S.sortk[j] & 0xFFFF);
+            node_j = (int)(S.best[j] & 0xFFFF);
             nc = (S.cc[node_j][0] > 0) + (S.cc[node_j][1] > 0) + (S.cc[node_j][2] > 0) + (S.cc[node_j][3] > 0);
         }
         int gsum;
