@@ -786,8 +786,15 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
     bool phase2 = false;
     __syncthreads();
 
-    // ---- phase 1: full passes (ORBextractor.cc:779-893)
-    while (true) {
+    // ---- phase 1: full passes (ORBextractor.cc:779-893).  Every pass
+    // that does not end the loop grows the list, which holds at most
+    // ORB_LIST_CAP nodes, so a pass count beyond that is an internal error.
+    for (int pass = 0;; pass++) {
+        if (pass > ORB_LIST_CAP) {
+            if (tid == 0) atomicOr(&ctr[ORB_CNT_ERR], ORB_ERR_LIST);
+            size = 0;
+            break;
+        }
         const int prev = size;
         for (int i = tid; i < size; i += OCT_T) S.cc[i][0] = S.cc[i][1] = S.cc[i][2] = S.cc[i][3] = 0;
         __syncthreads();
@@ -857,8 +864,14 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         }
     }
 
-    // ---- phase 2: split the largest nodes until the quota (:908-991)
-    while (phase2) {
+    // ---- phase 2: split the largest nodes until the quota (:908-991); as
+    // in phase 1, every round that does not end the loop grows the list.
+    for (int round = 0; phase2; round++) {
+        if (round > ORB_LIST_CAP) {
+            if (tid == 0) atomicOr(&ctr[ORB_CNT_ERR], ORB_ERR_LIST);
+            size = 0;
+            break;
+        }
         const int prev = size;
         for (int i = tid; i < size; i += OCT_T) S.cc[i][0] = S.cc[i][1] = S.cc[i][2] = S.cc[i][3] = 0;
         __syncthreads();
@@ -872,16 +885,19 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         const int ex_e = block_scan(e ? 1 : 0, S.scan, &nE);
         // The keys are distinct (they carry the node index), so a key's
         // place in descending order is the number of larger keys; every
-        // thread reads the same key at a time (an LDS broadcast).
+        // thread reads the same key at a time (an LDS broadcast).  S.keep
+        // holds each node's place (32767: not expandable).
         if (e)
             S.sortk[ex_e] = ((unsigned long long)S.cnt[cur][tid] << 32) |
                             ((unsigned long long)(unsigned)S.seq[cur][tid] << 16) | (unsigned)tid;
+        if (tid < size) S.keep[tid] = 32767;
         __syncthreads();
         if (tid < nE) {
             const unsigned long long mine = S.sortk[tid];
             int rank = 0;
             for (int j2 = 0; j2 < nE; j2++) rank += S.sortk[j2] > mine;
             S.best[rank] = mine;                  // S.best is free until the retain step
+            S.keep[mine & 0xFFFF] = (short)rank;
         }
         __syncthreads();
         // Gains in processing order and the first split reaching the quota.
@@ -899,16 +915,12 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         __syncthreads();
         const int jstar = S.vars[0];
         const bool proc = j <= jstar && j < nE;
-        int Cp;
-        const int ex_nc = block_scan(proc ? nc : 0, S.scan, &Cp);
-        // Mark processed list nodes.
-        if (tid < size) S.keep[tid] = 0;
-        __syncthreads();
-        if (proc) S.keep[node_j] = -1;
-        __syncthreads();
-        const bool kp = tid < size && S.keep[tid] == 0;
-        int nkeep;
-        const int ex_kp = block_scan(kp ? 1 : 0, S.scan, &nkeep);
+        // Kept list nodes: all but the processed ones (place <= jstar).  One
+        // scan for the processed nodes' children (< 2^12) and the kept nodes.
+        const bool kp = tid < size && S.keep[tid] > jstar;
+        int tot;
+        const int ex = block_scan((proc ? nc : 0) | ((kp ? 1 : 0) << 12), S.scan, &tot);
+        const int ex_nc = ex & 0xFFF, ex_kp = ex >> 12, Cp = tot & 0xFFF, nkeep = tot >> 12;
         const int nsize = Cp + nkeep;
         if (nsize > ORB_LIST_CAP) {
             if (tid == 0) atomicOr(&ctr[ORB_CNT_ERR], ORB_ERR_LIST);
@@ -946,7 +958,7 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
             S.seq[nb][keeppos] = S.seq[cur][i];
         }
         __syncthreads();
-        if (kp) S.keep[tid] = (short)keeppos;
+        if (tid < size) S.keep[tid] = (short)(kp ? keeppos : -1);
         __syncthreads();
         for_keys([&](int x, int y, int& nd) {
             const int n0 = nd;
