@@ -293,7 +293,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the dense k=2 search timing")
-    ap.add_argument("--branches", type=int, default=2, help="independent hipGraph branches (mode graph)")
+    ap.add_argument("--branches", type=int, default=0,
+                    help="independent hipGraph branches (mode graph); 0: pick by the batch's autotuner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (--gpus 1): frames and streams shard across GPUs with no collective, "
@@ -312,7 +313,14 @@ def main():
         batch.upload(2 * i + 1, r)
     b = float(np.float32(BF) / np.float32(FX))
     batch.set_stereo(list(range(0, 2 * B, 2)), list(range(1, 2 * B, 2)), BF, b)
-    batch.set_branches(args.branches)
+    tuned = None
+    if args.branches > 0:
+        batch.set_branches(args.branches)
+    elif mode == orb.Batch.GRAPH:
+        best, tms = batch.autotune(steps=10)
+        tuned = {"chosen": best, "ms_per_replay": {str(k): round(v, 4) for k, v in tms.items()}}
+        args.branches = best
+        print(f"[bench] autotuned graph branches: {best} ({tuned['ms_per_replay']})", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         batch.run(mode)
@@ -354,7 +362,7 @@ def main():
            "data": "synthetic (seeded shapes+noise frames; right view = left through a 12-plane disparity map)",
            "config": {"workload": f"stereo extract+match, {W}x{H}, nFeatures {NF}, 8 levels, scale 1.2, FAST 20/7, "
                                   f"bf {BF}, fx {FX}", "frames_per_step": B, "images_per_step": 2 * B,
-                      "run_mode": args.mode, "graph_branches": args.branches,
+                      "run_mode": args.mode, "graph_branches": args.branches, "branch_autotune": tuned,
                       "mean_keypoints_per_image": round(float(np.mean(kps)), 1),
                       "mean_stereo_matches": round(float(np.mean(matched)), 1)},
            "roofline": roofline}

@@ -197,6 +197,12 @@ int orb_batch_synchronize(orb_batch* b);
  * contiguous groups of near-equal size that keep every stereo pair whole; if
  * no such cut exists the graph has fewer branches.  Results are identical. */
 int orb_batch_set_branches(orb_batch* b, int branches);
+/* Autotuner of the graph's branch count: times `steps` mode-1 replays at 1, 2,
+ * 3, 4, 6 and 8 branches (whatever the slots' current images), keeps the
+ * fastest (the smaller count within 1 %), builds its graph and returns it in
+ * *best.  ms (8 floats, may be NULL) receives the mean replay time in ms at k
+ * branches in ms[k-1], or -1 where k was not timed. */
+int orb_batch_autotune(orb_batch* b, int steps, int* best, float* ms);
 /* Stage timing with HIP events on the batch's stream, averaged over `steps`
  * mode-0 runs.  ms receives orb_batch_stage_count() floats: pyramid levels
  * 0..L-1 (pyramid, blur, FAST with per-cell suppression), quadtree, describe,

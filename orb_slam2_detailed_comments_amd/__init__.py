@@ -80,6 +80,7 @@ SIGNATURES = {
     "orb_batch_set_stereo": (_I, [_P, _I, _P, _P, _F, _F]),
     "orb_batch_run": (_I, [_P, _I]),
     "orb_batch_set_branches": (_I, [_P, _I]),
+    "orb_batch_autotune": (_I, [_P, _I, _P, _P]),
     "orb_batch_synchronize": (_I, [_P]),
     "orb_batch_stage_count": (_I, [_P]),
     "orb_batch_stage_times": (_I, [_P, _I, _P]),
@@ -431,6 +432,14 @@ class Batch:
     def set_branches(self, branches):
         """Split the mode-1 graph into this many independent branches."""
         _check("orb_batch_set_branches", lib().orb_batch_set_branches(self.h, int(branches)))
+
+    def autotune(self, steps=10):
+        """Pick the fastest graph branch count (1, 2, 3, 4, 6 or 8) by timing
+        `steps` replays of each; returns (best, {branches: ms per replay})."""
+        best = ctypes.c_int(0)
+        ms = np.zeros(8, np.float32)
+        _check("orb_batch_autotune", lib().orb_batch_autotune(self.h, int(steps), ctypes.byref(best), _ptr(ms)))
+        return best.value, {k + 1: float(v) for k, v in enumerate(ms) if v >= 0}
 
     def run(self, mode=GRAPH):
         _check("orb_batch_run", lib().orb_batch_run(self.h, int(mode)))

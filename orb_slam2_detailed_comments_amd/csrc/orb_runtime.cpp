@@ -712,6 +712,51 @@ int orb_batch_set_branches(orb_batch* b, int branches)
     return ORB_OK;
 }
 
+// The graph-branch autotuner: time mode-1 replays of the batch at each branch
+// count in {1, 2, 3, 4, 6, 8}, keep the fastest (the smaller count on a tie
+// within 1 %) and leave its graph built.  ms[k-1] receives the mean replay
+// time at k branches, or -1 where k was not timed.
+int orb_batch_autotune(orb_batch* b, int steps, int* best, float* ms)
+{
+    if (!b || steps <= 0 || !best) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    if (ms)
+        for (int k = 0; k < 8; k++) ms[k] = -1.0f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return ORB_E_DEVICE;
+    }
+    int rc = ORB_OK, pick = 1;
+    float pick_ms = 0.f;
+    for (const int k : {1, 2, 3, 4, 6, 8}) {
+        rc = orb_batch_set_branches(b, k);
+        for (int w = 0; w < 2 && !rc; w++) rc = orb_batch_run(b, 1);
+        if (rc) break;
+        if (hipEventRecord(e0, b->ctx->stream) != hipSuccess) rc = ORB_E_DEVICE;
+        for (int s = 0; s < steps && !rc; s++) rc = orb_batch_run(b, 1);
+        float t = 0.f;
+        if (!rc && (hipEventRecord(e1, b->ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            rc = ORB_E_DEVICE;
+        if (rc) break;
+        t /= (float)steps;
+        if (ms) ms[k - 1] = t;
+        if (k == 1 || t < 0.99f * pick_ms) {
+            pick = k;
+            pick_ms = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    rc = orb_batch_set_branches(b, pick);
+    if (!rc) rc = orb_batch_run(b, 1);              // build the chosen graph
+    if (!rc) *best = pick;
+    return rc;
+}
+
 int orb_batch_run(orb_batch* b, int mode)
 {
     if (!b || mode < 0 || mode > 2) return ORB_E_ARG;

@@ -83,6 +83,21 @@ def test_batch_graph_branches_match_oracle(mixed):
             np.testing.assert_array_equal(gd, od, err_msg=f"branches {k} slot {s} descriptors")
 
 
+def test_batch_autotune_keeps_results(mixed):
+    """The branch autotuner times every candidate count, picks one of them and
+    leaves a graph that still gives the oracle's keypoints and descriptors."""
+    batch, ref = mixed
+    best, ms = batch.autotune(steps=3)
+    assert sorted(ms) == [1, 2, 3, 4, 6, 8] and all(v > 0 for v in ms.values())
+    assert best in ms and ms[best] <= min(ms.values()) / 0.99 + 1e-6
+    batch.run(1)
+    batch.synchronize()
+    for s, (ok, od) in enumerate(ref):
+        gk, gd = batch.results(s)
+        np.testing.assert_array_equal(gk.view(np.uint8), ok.view(np.uint8), err_msg=f"autotuned slot {s}")
+        np.testing.assert_array_equal(gd, od, err_msg=f"autotuned slot {s} descriptors")
+
+
 @pytest.mark.parametrize("left,right", [([0, 2, 4], [1, 3, 5]), ([0, 1, 2], [5, 4, 3])])
 def test_batch_stereo_branches(orb, oracle, left, right):
     """Pairs stay whole across branches; a nesting with no valid cut keeps
