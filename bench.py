@@ -199,6 +199,33 @@ def winmatch_extra(orb, ctx, cpu_seconds):
     return out
 
 
+def batch_extra(orb, ctx, steps):
+    """Mixed-size batch: 64 frames, 16 each of 640x480, 752x480 (nFeatures
+    1000), 1241x376 and 1280x720 (nFeatures 2000), extracted by one
+    graph-replayed launch per stage against per-frame launches of the same
+    kernels."""
+    from orb_slam2_detailed_comments_amd import synth
+    sizes = [(640, 480, 1000), (752, 480, 1000), (1241, 376, 2000), (1280, 720, 2000)]
+    slots = [sizes[i // 16] for i in range(64)]
+    imgs = [synth.shapes_frame(w, h, 3000 + i) for i, (w, h, _) in enumerate(slots)]
+    batch = orb.Batch([(w, h) for w, h, _ in slots], [nf for _, _, nf in slots], context=ctx)
+    for i, im in enumerate(imgs):
+        batch.upload(i, im)
+    best, tms = batch.autotune(steps=5)
+    out = {"frames": 64, "graph_branches": best}
+    for name, mode in (("graph", orb.Batch.GRAPH), ("per_frame_launches", orb.Batch.PER_IMAGE)):
+        for _ in range(3):
+            batch.run(mode)
+        batch.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.run(mode)
+        batch.synchronize()
+        out[f"{name}_frames_per_s"] = round(64 * steps / (time.perf_counter() - t0), 1)
+    out["graph_over_per_frame"] = round(out["graph_frames_per_s"] / out["per_frame_launches_frames_per_s"], 2)
+    return out
+
+
 def serving_extra(orb, ctx):
     """16 camera streams (4 each of 640x480, 752x480, 1241x376, 1280x720),
     Poisson arrivals; extract + match against the stream's resident previous
@@ -375,6 +402,9 @@ def main():
                                           "GB_per_s": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
         print("[bench] dense k=2 search", file=sys.stderr, flush=True)
         out["knn2"] = knn_extra(orb, ctx, max(3, args.steps // 2))
+        print("[bench] mixed-size batch", file=sys.stderr, flush=True)
+        out["mixed_batch"] = batch_extra(orb, ctx, max(5, args.steps // 2))
+        print(f"[bench] mixed-size batch: {out['mixed_batch']}", file=sys.stderr, flush=True)
         print("[bench] serving", file=sys.stderr, flush=True)
         out["serving"] = serving_extra(orb, ctx)
         print("[bench] vocabulary transform", file=sys.stderr, flush=True)
