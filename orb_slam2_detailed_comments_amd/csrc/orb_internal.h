@@ -305,7 +305,7 @@ struct orb_extractor {
         const size_t o_msad = take((size_t)p.out_cap * 4);
         const size_t o_misc = take(64);
         const size_t o_srec = take((size_t)p.out_cap * 16), o_sdesc = take((size_t)p.out_cap * 32);
-        const size_t o_rows = take(((size_t)H + 2) * 8);
+        const size_t o_rows = take((size_t)nlevels * ((size_t)H + 2) * 8);
         const size_t total = off;
         if (total > mem_bytes) {
             if (d_mem) (void)hipFree(d_mem);
@@ -494,9 +494,7 @@ struct orb_extractor {
         a.sdesc = d_sdesc;
         a.rows = d_rows;
         a.cap_r = std::min(plan.out_cap, r->plan.out_cap);
-        float band = 0.f;
-        for (int l = 0; l < nlevels; l++) band = std::max(band, 2.0f * scale[l]);
-        a.band = band;
+        a.nlev = nlevels;
     }
 
     ~orb_extractor()

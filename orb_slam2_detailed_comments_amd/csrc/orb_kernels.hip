@@ -1258,12 +1258,12 @@ __device__ __forceinline__ int hamming32(const uint4 a0, const uint4 a1, const u
 // smallest (distance, index) below TH_HIGH.  Refinement: 11x11 SAD of
 // centre-subtracted patches over 11 shifts, parabola fit, depth = bf/disp.
 
-// Group each pair's right keypoints by level-0 row bucket (counting sort in
-// LDS, one workgroup per pair) and lay them out in that order as 16-byte
-// records {minr, maxr, index << 4 | octave, x} (the row band of
-// Frame.cc:867-882, with the same float statements) and 32-byte descriptors;
-// plus, per level-0 row, the entry range of the buckets that can hold the row
-// in a band.  Order inside a bucket is free: the search keeps the smallest
+// Group each pair's right keypoints by (octave, level-0 row bucket)
+// (counting sort in LDS, one workgroup per pair) and lay them out in that
+// order as 16-byte records {minr, maxr, index << 4 | octave, x} (the row band
+// of Frame.cc:867-882, with the same float statements) and 32-byte
+// descriptors; plus, per octave and level-0 row, the entry range of that
+// octave's buckets that can hold the row in a band.  Order inside a bucket is free: the search keeps the smallest
 // (distance, index), which does not depend on the visiting order.
 #define STEREO_NB 4096               // buckets: rows >> qs
 #define STEREO_T 1024
@@ -1275,34 +1275,35 @@ __global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs*
     const StereoArgs& A = args[blockIdx.x];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int n = min(min(*A.nr, A.cap_r), STEREO_T * STEREO_PER_T);
-    const int nrows = A.nrows;
+    const int nrows = A.nrows, nlev = A.nlev;
     int qs = 0;
-    while (((nrows - 1) >> qs) >= STEREO_NB) qs++;
-    const int nb = ((nrows - 1) >> qs) + 1;
+    while (nlev * (((nrows - 1) >> qs) + 1) > STEREO_NB) qs++;
+    const int nb = ((nrows - 1) >> qs) + 1;      // row buckets per octave
+    const int nk = nlev * nb;                    // keys: octave * nb + row bucket
     // Bucket of a y: floor(y) clamped to the rows (NaN to row 0), >> qs.
     auto bucket = [nrows, qs](float y) {
         const float f = floorf(y);
         const int r = !(f > 0.f) ? 0 : (f >= (float)(nrows - 1) ? nrows - 1 : (int)f);
         return r >> qs;
     };
-    for (int j = t; j <= nb; j += STEREO_T) s_cnt[j] = 0;
+    for (int j = t; j <= nk; j += STEREO_T) s_cnt[j] = 0;
     __syncthreads();
     int eb[STEREO_PER_T], er[STEREO_PER_T];
 #pragma unroll
     for (int k = 0; k < STEREO_PER_T; k++) {
         const int i = t + STEREO_T * k;
-        eb[k] = i < n ? bucket(A.kr[i].y) : 0;
+        eb[k] = i < n ? min(A.kr[i].octave, nlev - 1) * nb + bucket(A.kr[i].y) : 0;
         er[k] = i < n ? atomicAdd(&s_cnt[eb[k]], 1) : 0;
     }
     __syncthreads();
-    // Exclusive scan of the nb counts: 4 per thread, wavefront scans, then
+    // Exclusive scan of the nk counts: 4 per thread, wavefront scans, then
     // the 16 wavefront totals.
     int loc[4], sum = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int j = 4 * t + k;
         loc[k] = sum;
-        sum += j < nb ? s_cnt[j] : 0;
+        sum += j < nk ? s_cnt[j] : 0;
     }
     int inc = sum;
 #pragma unroll
@@ -1317,9 +1318,9 @@ __global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs*
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int j = 4 * t + k;
-        if (j < nb) s_cnt[j] = base + loc[k];
+        if (j < nk) s_cnt[j] = base + loc[k];
     }
-    if (t == 0) s_cnt[nb] = n;
+    if (t == 0) s_cnt[nk] = n;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < STEREO_PER_T; k++) {
@@ -1337,11 +1338,14 @@ __global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs*
             o[1] = d[1];
         }
     }
-    // A candidate of row `row` has y in (row - 1 - band, row + 1 + band);
-    // one more row each side absorbs the rounding of these bounds.
-    for (int row = t; row < nrows; row += STEREO_T) {
-        const int blo = bucket((float)row - 2.0f - A.band), bhi = bucket((float)row + 2.0f + A.band);
-        A.rows[row] = make_int2(s_cnt[blo], s_cnt[bhi + 1]);
+    // A candidate of octave o for row `row` has y in (row - 1 - r, row + 1 + r),
+    // r = 2 * scale[o]; one more row each side absorbs the rounding of these
+    // bounds.  rows[o * nrows + row]: the entries of those buckets of octave o.
+    for (int idx = t; idx < nlev * nrows; idx += STEREO_T) {
+        const int o = idx / nrows, row = idx - o * nrows;
+        const float r = 2.0f * A.scale[o];
+        const int blo = bucket((float)row - 2.0f - r), bhi = bucket((float)row + 2.0f + r);
+        A.rows[idx] = make_int2(s_cnt[o * nb + blo], s_cnt[o * nb + bhi + 1]);
     }
 }
 
@@ -1404,45 +1408,49 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     float bestU = 0.f;
     if (go) {
         (void)nr;
-        const int2 rg = A.rows[row];
-        // Chunks of ST_CHUNK candidates per lane: the records are loaded
-        // together, then the descriptors of those that pass the band, octave
-        // and uR tests, so each chunk waits on memory twice.
-        for (int c0 = rg.x + sub; c0 < rg.y; c0 += ST_CHUNK * ST_G) {
-            int4 rec[ST_CHUNK];
+        // Octaves levelL-1 .. levelL+1 only (Frame.cc:929), each over the
+        // entries of its own row band.
+        for (int oc = max(levelL - 1, 0); oc <= min(levelL + 1, A.nlev - 1); oc++) {
+            const int2 rg = A.rows[oc * A.nrows + row];
+            // Chunks of ST_CHUNK candidates per lane: the records are loaded
+            // together, then the descriptors of those that pass the band, octave
+            // and uR tests, so each chunk waits on memory twice.
+            for (int c0 = rg.x + sub; c0 < rg.y; c0 += ST_CHUNK * ST_G) {
+                int4 rec[ST_CHUNK];
 #pragma unroll
-            for (int u = 0; u < ST_CHUNK; u++) {
-                const int c = c0 + u * ST_G;
-                rec[u] = c < rg.y ? A.srec[c] : make_int4(1, 0, 0, 0);   // minr > maxr: never a candidate
-            }
-            bool pass[ST_CHUNK];
-#pragma unroll
-            for (int u = 0; u < ST_CHUNK; u++) {
-                const int oct = rec[u].z & 15;
-                const float uR = __int_as_float(rec[u].w);
-                pass[u] = row >= rec[u].x && row <= rec[u].y && oct >= levelL - 1 && oct <= levelL + 1 &&
-                          uR >= minU && uR <= maxU;
-            }
-            uint4 b0[ST_CHUNK], b1[ST_CHUNK];
-#pragma unroll
-            for (int u = 0; u < ST_CHUNK; u++) {
-                if (pass[u]) {
-                    const uint8_t* d = A.sdesc + (size_t)(c0 + u * ST_G) * 32;
-                    b0[u] = gload16(d);
-                    b1[u] = gload16(d + 16);
+                for (int u = 0; u < ST_CHUNK; u++) {
+                    const int c = c0 + u * ST_G;
+                    rec[u] = c < rg.y ? A.srec[c] : make_int4(1, 0, 0, 0);   // minr > maxr: never a candidate
                 }
-            }
+                bool pass[ST_CHUNK];
 #pragma unroll
-            for (int u = 0; u < ST_CHUNK; u++) {
-                if (pass[u]) {
-                    const int iR = rec[u].z >> 4;
-                    const int dist = __popc(a0.x ^ b0[u].x) + __popc(a0.y ^ b0[u].y) + __popc(a0.z ^ b0[u].z) +
-                                     __popc(a0.w ^ b0[u].w) + __popc(a1.x ^ b1[u].x) + __popc(a1.y ^ b1[u].y) +
-                                     __popc(a1.z ^ b1[u].z) + __popc(a1.w ^ b1[u].w);
-                    if (dist < bestDist || (dist == bestDist && iR < bestIdx)) {
-                        bestDist = dist;
-                        bestIdx = iR;
-                        bestU = __int_as_float(rec[u].w);
+                for (int u = 0; u < ST_CHUNK; u++) {
+                    const int oct = rec[u].z & 15;
+                    const float uR = __int_as_float(rec[u].w);
+                    pass[u] = row >= rec[u].x && row <= rec[u].y && oct >= levelL - 1 && oct <= levelL + 1 &&
+                              uR >= minU && uR <= maxU;
+                }
+                uint4 b0[ST_CHUNK], b1[ST_CHUNK];
+#pragma unroll
+                for (int u = 0; u < ST_CHUNK; u++) {
+                    if (pass[u]) {
+                        const uint8_t* d = A.sdesc + (size_t)(c0 + u * ST_G) * 32;
+                        b0[u] = gload16(d);
+                        b1[u] = gload16(d + 16);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < ST_CHUNK; u++) {
+                    if (pass[u]) {
+                        const int iR = rec[u].z >> 4;
+                        const int dist = __popc(a0.x ^ b0[u].x) + __popc(a0.y ^ b0[u].y) + __popc(a0.z ^ b0[u].z) +
+                                         __popc(a0.w ^ b0[u].w) + __popc(a1.x ^ b1[u].x) + __popc(a1.y ^ b1[u].y) +
+                                         __popc(a1.z ^ b1[u].z) + __popc(a1.w ^ b1[u].w);
+                        if (dist < bestDist || (dist == bestDist && iR < bestIdx)) {
+                            bestDist = dist;
+                            bestIdx = iR;
+                            bestU = __int_as_float(rec[u].w);
+                        }
                     }
                 }
             }

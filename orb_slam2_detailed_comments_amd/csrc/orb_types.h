@@ -101,10 +101,10 @@ struct StereoArgs {
     int4* srec;             // scratch: right keypoints grouped by row bucket as
                             // {minr, maxr, index << 4 | octave, x bits} ...
     uint8_t* sdesc;         // ... and their descriptors in the same order
-    int2* rows;             // per level-0 row: [first, end) of the entries whose
-                            // y lies in (row - 1 - band, row + 1 + band)
+    int2* rows;             // per (octave o, level-0 row): [first, end) of the entries
+                            // of octave o whose row band can hold the row (nlev x nrows)
     int cap_r;
-    float band;             // max over levels of 2*scale: widest row band
+    int nlev;               // pyramid levels of both extractors
 };
 
 // Arguments of one brute-force match of a query set against a train set
