@@ -1385,11 +1385,11 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     if (iL >= nl || iL >= A.cap_l) return;             // the whole group leaves
     float outU = -1.0f, outD = -1.0f;
     int outSad = -1;
-    const OrbKeyPoint kpL = A.kl[iL];
-    const uint4 a0 = *(const uint4*)(A.dl + (size_t)iL * 32);
-    const uint4 a1 = *(const uint4*)(A.dl + (size_t)iL * 32 + 16);
-    const int levelL = kpL.octave;
-    const float vL = kpL.y, uL = kpL.x;
+    const uint8_t* kpb = (const uint8_t*)(A.kl + iL);
+    const uint2 kxy = gload8(kpb + offsetof(OrbKeyPoint, x));
+    const uint4 a0 = gload16(A.dl + (size_t)iL * 32), a1 = gload16(A.dl + (size_t)iL * 32 + 16);
+    const int levelL = (int)gload4(kpb + offsetof(OrbKeyPoint, octave));
+    const float vL = __uint_as_float(kxy.y), uL = __uint_as_float(kxy.x);
     const int row = (int)vL;
     const float minZ = A.mb;
     const float minD = 0;
@@ -1420,7 +1420,12 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
 #pragma unroll
                 for (int u = 0; u < ST_CHUNK; u++) {
                     const int c = c0 + u * ST_G;
-                    rec[u] = c < rg.y ? A.srec[c] : make_int4(1, 0, 0, 0);   // minr > maxr: never a candidate
+                    if (c < rg.y) {
+                        const uint4 q = gload16((const uint8_t*)(A.srec + c));
+                        rec[u] = make_int4((int)q.x, (int)q.y, (int)q.z, (int)q.w);
+                    } else {
+                        rec[u] = make_int4(1, 0, 0, 0);   // minr > maxr: never a candidate
+                    }
                 }
                 bool pass[ST_CHUNK];
 #pragma unroll
@@ -1473,8 +1478,8 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     if (go) {
         const float uR0 = bestU;
         const float scaleFactor = A.inv_scale[levelL];
-        const float scaleduL = roundf(kpL.x * scaleFactor);
-        const float scaledvL = roundf(kpL.y * scaleFactor);
+        const float scaleduL = roundf(uL * scaleFactor);
+        const float scaledvL = roundf(vL * scaleFactor);
         scaleduR0 = roundf(uR0 * scaleFactor);
         const float iniu = scaleduR0 - 5 - 5;
         const float endu = scaleduR0 + 5 + 5 + 1;
@@ -1497,10 +1502,21 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
         const int xl0 = (sx0 - 5) & ~3, offL = sx0 - 5 - xl0;
         const int xr0 = (sxR - 10) & ~3, offR = sxR - 10 - xr0;
         unsigned dl[4], dr[6];
+        if (xl0 + 15 < wL) {                     // the whole 16 bytes lie in the row
+            const uint4 q = gload16(rl + xl0);
+            dl[0] = q.x, dl[1] = q.y, dl[2] = q.z, dl[3] = q.w;
+        } else {
 #pragma unroll
-        for (int k = 0; k < 4; k++) dl[k] = patch_dword(rl, xl0 + 4 * k, wL);
+            for (int k = 0; k < 4; k++) dl[k] = patch_dword(rl, xl0 + 4 * k, wL);
+        }
+        if (xr0 + 23 < wR) {
+            const uint4 q = gload16(rr + xr0);
+            const uint2 q2 = gload8(rr + xr0 + 16);
+            dr[0] = q.x, dr[1] = q.y, dr[2] = q.z, dr[3] = q.w, dr[4] = q2.x, dr[5] = q2.y;
+        } else {
 #pragma unroll
-        for (int k = 0; k < 6; k++) dr[k] = patch_dword(rr, xr0 + 4 * k, wR);
+            for (int k = 0; k < 6; k++) dr[k] = patch_dword(rr, xr0 + 4 * k, wR);
+        }
         unsigned la[3], ra[6];
 #pragma unroll
         for (int k = 0; k < 3; k++) la[k] = __builtin_amdgcn_alignbyte(dl[k + 1], dl[k], (unsigned)offL);
