@@ -65,6 +65,18 @@ __device__ __forceinline__ uint2 gload8(const uint8_t* p)
     const orb_v2u v = *(const __attribute__((address_space(1))) orb_v2u*)p;
     return make_uint2(v.x, v.y);
 }
+typedef unsigned orb_u32a1 __attribute__((aligned(1)));
+__device__ __forceinline__ unsigned gload4u(const uint8_t* p)   // any alignment
+{
+    return *(const __attribute__((address_space(1))) orb_u32a1*)p;
+}
+// Stores through global (not flat) addresses.
+__device__ __forceinline__ void gstore4(uint8_t* p, unsigned v) { *(__attribute__((address_space(1))) unsigned*)p = v; }
+__device__ __forceinline__ void gstore1(uint8_t* p, uint8_t v) { *(__attribute__((address_space(1))) uint8_t*)p = v; }
+__device__ __forceinline__ void gstore8(unsigned long long* p, unsigned long long v)
+{
+    *(__attribute__((address_space(1))) unsigned long long*)p = v;
+}
 __device__ __forceinline__ unsigned gload4(const uint8_t* p)
 {
     return *(const __attribute__((address_space(1))) unsigned*)p;
@@ -299,7 +311,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             unsigned d = 0;
             if (interior) {
                 // unaligned dword load (rows of the input are W bytes apart)
-                __builtin_memcpy(&d, src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex), 4);
+                d = gload4u(src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex));
             } else {
                 const int y = reflect101(ty0 - LT_HALO + ey, h);
                 const uint8_t* r = src + (size_t)y * sp;
@@ -349,11 +361,11 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // registers) and rows r0, r0 + 14, r0 + 28 of the tile.
         const bool staged = nwd <= 64 && nwd * nwr <= LW_MAX;
         if (staged) {
-            const unsigned* srow = (const unsigned*)(P.lvl + (size_t)wy0 * sp + wx0);
+            const uint8_t* srow = P.lvl + (size_t)wy0 * sp + wx0;
             const int lg = nwd <= 16 ? 4 : (nwd <= 32 ? 5 : 6);     // threads per row: 2^lg >= nwd
             const int tc = tid & ((1 << lg) - 1);
             for (int r = tid >> lg; r < nwr; r += 256 >> lg)
-                if (tc < nwd) u.win[r * nwd + tc] = srow[(size_t)r * (sp >> 2) + tc];
+                if (tc < nwd) u.win[r * nwd + tc] = gload4(srow + (size_t)r * sp + 4 * tc);
             __syncthreads();
         }
         if (tid < 14 * LP_WD) {
@@ -398,7 +410,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             const unsigned d = pxw[r + LT_HALO][gc + 1];
             uint8_t* dst = L.lvl + (size_t)y * pitch + x0;
             if (x0 + 3 < w) {
-                *(unsigned*)dst = d;
+                gstore4(dst, d);
             } else {
                 for (int k = 0; k < 4; k++)
                     if (x0 + k < w) dst[k] = (uint8_t)(d >> (8 * k));
@@ -447,7 +459,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
 #undef ORB_VBLUR
             uint8_t* db = L.blur + (size_t)y * pitch + x0;
             if (x0 + 3 < w) {
-                *(unsigned*)db = out;
+                gstore4(db, out);
             } else {
                 for (int k = 0; k < 4; k++)
                     if (x0 + k < w) db[k] = (uint8_t)(out >> (8 * k));
@@ -579,7 +591,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             if (!keepA && !keepB) continue;
             const unsigned loc = ((unsigned)ly << 6) | (unsigned)lx | ((unsigned)mc << 11);
             if (keepA) {
-                L.cellflag[ci * L.nCols + cj] = 1;
+                gstore1(L.cellflag + ci * L.nCols + cj, 1);
                 const int i = atomicAdd(&s_kn, 1);
                 if (i < LK_CAP) u.f.kloc[i] = loc;
             }
@@ -612,7 +624,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | (yl << 6) | xl;
         unsigned long long key = ((unsigned long long)(unsigned)(mc - 1) << 32) | ok;
         if (loc & (1u << 19)) key |= ORB_KEY_RETRY;
-        L.keys[base + i] = key;
+        gstore8(L.keys + base + i, key);
     }
 }
 
