@@ -77,6 +77,19 @@ __device__ __forceinline__ void gstore8(unsigned long long* p, unsigned long lon
 {
     *(__attribute__((address_space(1))) unsigned long long*)p = v;
 }
+__device__ __forceinline__ unsigned long long gload8u64(const unsigned long long* p)
+{
+    return *(const __attribute__((address_space(1))) unsigned long long*)p;
+}
+__device__ __forceinline__ uint8_t gload1(const uint8_t* p) { return *(const __attribute__((address_space(1))) uint8_t*)p; }
+__device__ __forceinline__ void gstore2(uint8_t* p, unsigned short v)
+{
+    *(__attribute__((address_space(1))) unsigned short*)p = v;
+}
+__device__ __forceinline__ void gstore16(uint8_t* p, uint4 v)
+{
+    *(__attribute__((address_space(1))) orb_v4u*)p = orb_v4u{v.x, v.y, v.z, v.w};
+}
 __device__ __forceinline__ unsigned gload4(const uint8_t* p)
 {
     return *(const __attribute__((address_space(1))) unsigned*)p;
@@ -734,13 +747,13 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         bool v = false;
         unsigned long long key = 0;
         if (k < K0) {
-            key = L.keys[k];
+            key = gload8u64(L.keys + k);
             const unsigned ok = (unsigned)key;
-            const bool flagged = L.cellflag[((ok >> 20) & 255) * L.nCols + ((ok >> 12) & 255)] != 0;
+            const bool flagged = gload1(L.cellflag + ((ok >> 20) & 255) * L.nCols + ((ok >> 12) & 255)) != 0;
             v = (key & ORB_KEY_RETRY) ? !flagged : flagged;
         }
         const int pos = block_scan(v ? 1 : 0, S.scan, &total);
-        if (v) L.keys2[K + pos] = key & ~ORB_KEY_RETRY;
+        if (v) gstore8(L.keys2 + K + pos, key & ~ORB_KEY_RETRY);
         K += total;
     }
     __syncthreads();
@@ -754,7 +767,7 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
     for (int m = 0; m < OCT_REG; m++) {
         const int k = tid + OCT_T * m;
         kx[m] = ky[m] = kn[m] = 0;
-        if (k < K) key_xy(keys[k], wC, hC, &kx[m], &ky[m]);
+        if (k < K) key_xy(gload8u64(keys + k), wC, hC, &kx[m], &ky[m]);
     }
     // f(x, y, node&) on every candidate of this thread.
     auto for_keys = [&](auto&& f) {
@@ -992,7 +1005,7 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
     };
 #pragma unroll
     for (int m = 0; m < OCT_REG; m++)
-        if (tid + OCT_T * m < K) retain(keys[tid + OCT_T * m], kn[m]);
+        if (tid + OCT_T * m < K) retain(gload8u64(keys + tid + OCT_T * m), kn[m]);
     for (int k = tid + OCT_T * OCT_REG; k < K; k += OCT_T) retain(keys[k], node[k]);
     __syncthreads();
     const int cap = min(L.list_cap, ORB_LIST_CAP);
@@ -1002,7 +1015,7 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
     }
     for (int i = tid; i < size; i += OCT_T) {
         const unsigned long long p = S.best[i];
-        L.ret[i] = (p & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFull - (p & 0xFFFFFFFFull));
+        gstore8(L.ret + i, (p & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFull - (p & 0xFFFFFFFFull)));
     }
     if (tid == 0) ctr[ORB_CNT_RET + level] = size;
 }
@@ -1091,7 +1104,7 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     const int* ctr = J.counters;
     int base = 0, total = 0;
     for (int l = 0; l < J.nlevels; l++) {
-        const int c = ctr[ORB_CNT_RET + l];
+        const int c = (int)gload4((const uint8_t*)(ctr + ORB_CNT_RET + l));
         base += l < level ? c : 0;
         total += c;
     }
@@ -1105,9 +1118,9 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
         ((unsigned*)s_pat)[(((t >> 2) & 3) * 16 + (t >> 4)) * 4 + (t & 3)] = ((const unsigned*)c_pattern)[t];
     }
     __syncthreads();
-    if (level >= J.nlevels || idx >= ctr[ORB_CNT_RET + level]) return;   // the whole group leaves
+    if (level >= J.nlevels || idx >= (int)gload4((const uint8_t*)(ctr + ORB_CNT_RET + level))) return;   // the whole group leaves
     const LevelJob& L = J.lv[level];
-    const unsigned long long key = L.ret[idx];
+    const unsigned long long key = gload8u64(L.ret + idx);
     int xr, yr;
     key_xy(key, L.wCell, L.hCell, &xr, &yr);
     const int cx = xr + ORB_MIN_BORDER, cy = yr + ORB_MIN_BORDER;
@@ -1221,7 +1234,7 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
         }
     }
     const int out = base + idx;
-    ((unsigned short*)(J.desc + (size_t)out * 32))[sub] = (unsigned short)bits;
+    gstore2(J.desc + (size_t)out * 32 + 2 * sub, (unsigned short)bits);
     if (sub == 0) {
         const float sc = L.scale;
         OrbKeyPoint kp;
@@ -1237,7 +1250,12 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
         kp.response = (float)(unsigned)(key >> 32);
         kp.octave = level;
         kp.class_id = -1;
-        J.kps[out] = kp;
+        uint8_t* kd = (uint8_t*)(J.kps + out);     // 28 bytes: 16 + 8 + 4
+        gstore16(kd, make_uint4(__float_as_uint(kp.x), __float_as_uint(kp.y), __float_as_uint(kp.size),
+                                __float_as_uint(kp.angle)));
+        *(__attribute__((address_space(1))) orb_v2u*)(kd + 16) =
+            orb_v2u{__float_as_uint(kp.response), (unsigned)kp.octave};
+        gstore4(kd + 24, (unsigned)kp.class_id);
     }
 }
 
