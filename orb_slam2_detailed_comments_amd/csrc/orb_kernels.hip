@@ -1322,7 +1322,12 @@ __global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs*
 #pragma unroll
     for (int k = 0; k < STEREO_PER_T; k++) {
         const int i = t + STEREO_T * k;
-        eb[k] = i < n ? min(A.kr[i].octave, nlev - 1) * nb + bucket(A.kr[i].y) : 0;
+        eb[k] = 0;
+        if (i < n) {
+            const uint8_t* kb = (const uint8_t*)(A.kr + i);
+            eb[k] = min((int)gload4(kb + offsetof(OrbKeyPoint, octave)), nlev - 1) * nb +
+                    bucket(__uint_as_float(gload4(kb + offsetof(OrbKeyPoint, y))));
+        }
         er[k] = i < n ? atomicAdd(&s_cnt[eb[k]], 1) : 0;
     }
     __syncthreads();
@@ -1356,16 +1361,19 @@ __global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs*
     for (int k = 0; k < STEREO_PER_T; k++) {
         const int i = t + STEREO_T * k;
         if (i < n) {
-            const OrbKeyPoint kp = A.kr[i];
-            const float r = 2.0f * A.scale[kp.octave];
-            const int maxr = (int)ceilf(kp.y + r);
-            const int minr = (int)floorf(kp.y - r);
+            const uint8_t* kb = (const uint8_t*)(A.kr + i);
+            const uint2 xy = gload8(kb + offsetof(OrbKeyPoint, x));
+            const int oct = (int)gload4(kb + offsetof(OrbKeyPoint, octave));
+            const float ky = __uint_as_float(xy.y);
+            const float r = 2.0f * A.scale[oct];
+            const int maxr = (int)ceilf(ky + r);
+            const int minr = (int)floorf(ky - r);
             const int pos = s_cnt[eb[k]] + er[k];
-            A.srec[pos] = make_int4(minr, maxr, (i << 4) | kp.octave, __float_as_int(kp.x));
-            const uint4* d = (const uint4*)(A.dr + (size_t)i * 32);
-            uint4* o = (uint4*)(A.sdesc + (size_t)pos * 32);
-            o[0] = d[0];
-            o[1] = d[1];
+            gstore16((uint8_t*)(A.srec + pos), make_uint4((unsigned)minr, (unsigned)maxr, (unsigned)((i << 4) | oct), xy.x));
+            const uint8_t* d = A.dr + (size_t)i * 32;
+            uint8_t* o = A.sdesc + (size_t)pos * 32;
+            gstore16(o, gload16(d));
+            gstore16(o + 16, gload16(d + 16));
         }
     }
     // A candidate of octave o for row `row` has y in (row - 1 - r, row + 1 + r),
@@ -1375,7 +1383,8 @@ __global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs*
         const int o = idx / nrows, row = idx - o * nrows;
         const float r = 2.0f * A.scale[o];
         const int blo = bucket((float)row - 2.0f - r), bhi = bucket((float)row + 2.0f + r);
-        A.rows[idx] = make_int2(s_cnt[o * nb + blo], s_cnt[o * nb + bhi + 1]);
+        *(__attribute__((address_space(1))) orb_v2u*)(A.rows + idx) =
+            orb_v2u{(unsigned)s_cnt[o * nb + blo], (unsigned)s_cnt[o * nb + bhi + 1]};
     }
 }
 
@@ -1686,7 +1695,7 @@ __global__ __launch_bounds__(MEDIAN_T) void stereo_median_kernel(const StereoArg
 #pragma unroll
     for (int k = 0; k < MEDIAN_PER_T; k++) {
         const int i = t + MEDIAN_T * k;
-        sad[k] = i < n ? A.match_sad[i] : -1;
+        sad[k] = i < n ? (int)gload4((const uint8_t*)(A.match_sad + i)) : -1;
         if (sad[k] >= 0) {
             atomicAdd(&s_h[sad[k] >> 8], 1);
             m++;
@@ -1715,8 +1724,8 @@ __global__ __launch_bounds__(MEDIAN_T) void stereo_median_kernel(const StereoArg
     for (int k = 0; k < MEDIAN_PER_T; k++) {
         const int i = t + MEDIAN_T * k;
         if (sad[k] >= 0 && !((float)sad[k] < thDist)) {
-            A.uRight[i] = -1;
-            A.depth[i] = -1;
+            gstore4((uint8_t*)(A.uRight + i), __float_as_uint(-1.0f));
+            gstore4((uint8_t*)(A.depth + i), __float_as_uint(-1.0f));
         }
     }
 }
