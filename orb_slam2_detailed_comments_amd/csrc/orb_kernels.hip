@@ -253,6 +253,8 @@ __device__ __forceinline__ unsigned hi_bytes(unsigned x) { return __builtin_amdg
 // - Every factor is below 2^24 (taps <= 2048, S <= 255 * 4096), so the
 //   products are 24-bit multiplies (v_mul_u32_u24, full rate) rather than
 //   32-bit ones (v_mul_lo_u32, quarter rate).
+typedef unsigned short orb_u16x2 __attribute__((ext_vector_type(2)));
+
 template <typename BytePtr>
 __device__ __forceinline__ unsigned resize4(BytePtr base, unsigned o0, unsigned o1, const int* sxk, const int* nxk,
                                             const int* tap, int yb)
@@ -392,7 +394,49 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 nxk[k] = (sx + 1 < sw ? sx + 1 : sx) - xoff;
                 tap[k] = xa[4 * c + k];
             }
-            if (staged) {
+            // The four pixels' source bytes usually lie within 8 bytes (scale
+            // factors up to ~2.3; reflected halo columns may run backwards, so
+            // the span is taken over all of them): then a source row is three
+            // aligned dword reads realigned by two v_alignbyte, each pixel's
+            // tap pair (p[sx], p[nx]) one v_perm into 16-bit lanes, and S one
+            // v_dot2_u32_u16 with the packed taps (a0 | a1 << 16).
+            int b0 = sxk[0], b1 = nxk[0];
+#pragma unroll
+            for (int k = 1; k < 4; k++) {
+                b0 = min(b0, sxk[k]);
+                b1 = max(b1, nxk[k]);
+            }
+            if (staged && b1 - b0 <= 7) {
+                const unsigned* wd = u.win + (b0 >> 2);
+                const unsigned shb = (unsigned)(b0 & 3);
+                const int ws = nwd;                        // window row stride in dwords
+                unsigned sel[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    sel[k] = 0x0C000C00u | ((unsigned)(nxk[k] - b0) << 16) | (unsigned)(sxk[k] - b0);
+                for (int ey = r0; ey < LP_H; ey += 14) {
+                    const int sy = ys0[ey];
+                    const unsigned* q0 = wd + __umul24(min(max(sy, 0), sh - 1) - wy0, ws);
+                    const unsigned* q1 = wd + __umul24(min(max(sy + 1, 0), sh - 1) - wy0, ws);
+                    const unsigned w00 = __builtin_amdgcn_alignbyte(q0[1], q0[0], shb);
+                    const unsigned w01 = __builtin_amdgcn_alignbyte(q0[2], q0[1], shb);
+                    const unsigned w10 = __builtin_amdgcn_alignbyte(q1[1], q1[0], shb);
+                    const unsigned w11 = __builtin_amdgcn_alignbyte(q1[2], q1[1], shb);
+                    const unsigned bv0 = (unsigned)yb[ey] & 0xFFFFu, bv1 = (unsigned)yb[ey] >> 16;
+                    unsigned d = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const orb_u16x2 tk = __builtin_bit_cast(orb_u16x2, (unsigned)tap[k]);
+                        const unsigned S0 = __builtin_amdgcn_udot2(
+                            __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w01, w00, sel[k])), tk, 0u, false);
+                        const unsigned S1 = __builtin_amdgcn_udot2(
+                            __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w11, w10, sel[k])), tk, 0u, false);
+                        const unsigned v = (__umul24(S0, bv0) + __umul24(S1, bv1) + (1u << 21)) >> 22;
+                        d |= min(v, 255u) << (8 * k);
+                    }
+                    pxw[ey][c] = d;
+                }
+            } else if (staged) {
                 const uint8_t* wb = (const uint8_t*)u.win;
                 const int ws = nwd * 4;
                 for (int ey = r0; ey < LP_H; ey += 14) {
