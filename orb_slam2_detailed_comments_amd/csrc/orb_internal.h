@@ -306,6 +306,9 @@ struct orb_extractor {
         const size_t o_misc = take(64);
         const size_t o_srec = take((size_t)p.out_cap * 16), o_sdesc = take((size_t)p.out_cap * 32);
         const size_t o_rows = take((size_t)nlevels * ((size_t)H + 2) * 8);
+        size_t ntab = 0;
+        for (int l = 1; l < nlevels; l++) ntab += (size_t)p.lv[l].w + p.lv[l].h;
+        const size_t o_tab = take(ntab * 8);
         const size_t total = off;
         if (total > mem_bytes) {
             if (d_mem) (void)hipFree(d_mem);
@@ -359,6 +362,44 @@ struct orb_extractor {
             L.keys2 = (unsigned long long*)(d_mem + o_keys2) + P.key_off;
             L.node = (int*)(d_mem + o_node) + P.key_off;
             L.ret = (unsigned long long*)(d_mem + o_ret) + P.ret_off;
+        }
+        // cv::resize INTER_LINEAR taps of every level from the level above
+        // (ComputePyramid, ORBextractor.cc:1677): per destination column and
+        // row, the source index and the two 11-bit weights, with the float
+        // statements of OpenCV's generic path.
+        {
+            std::vector<int2> tab(ntab);
+            size_t t = 0;
+            for (int l = 1; l < nlevels; l++) {
+                const int w = p.lv[l].w, h = p.lv[l].h, sw = p.lv[l - 1].w, sh = p.lv[l - 1].h;
+                J.lv[l].xtab = (const int2*)(d_mem + o_tab) + t;
+                const double scale_x = 1. / ((double)w / sw);
+                for (int x = 0; x < w; x++, t++) {
+                    float fx = (float)((x + 0.5) * scale_x - 0.5);
+                    int sx = (int)std::floor(fx);
+                    fx -= sx;
+                    if (sx < 0) {
+                        fx = 0.f;
+                        sx = 0;
+                    }
+                    if (sx >= sw - 1) {
+                        fx = 0.f;
+                        sx = sw - 1;
+                    }
+                    tab[t].x = sx;
+                    tab[t].y = (int)std::nearbyint((1.f - fx) * 2048.f) | ((int)std::nearbyint(fx * 2048.f) << 16);
+                }
+                J.lv[l].ytab = (const int2*)(d_mem + o_tab) + t;
+                const double scale_y = 1. / ((double)h / sh);
+                for (int y = 0; y < h; y++, t++) {
+                    float fy = (float)((y + 0.5) * scale_y - 0.5);
+                    const int sy = (int)std::floor(fy);
+                    fy -= sy;
+                    tab[t].x = sy;
+                    tab[t].y = (int)std::nearbyint((1.f - fy) * 2048.f) | ((int)std::nearbyint(fy * 2048.f) << 16);
+                }
+            }
+            if (ntab) HIP_TRY(hipMemcpy(d_mem + o_tab, tab.data(), ntab * 8, hipMemcpyHostToDevice));
         }
         d_uR = (float*)(d_mem + o_uR);
         d_depth = (float*)(d_mem + o_dep);

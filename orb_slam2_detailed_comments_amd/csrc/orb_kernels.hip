@@ -336,32 +336,24 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             pxw[ey][ex >> 2] = d;
         }
     } else {
-        // cv::resize INTER_LINEAR 8U fixed point from level l-1: double source
-        // coordinate, float fraction, 11-bit taps (half-even), computed once
-        // per tile column and row.
+        // cv::resize INTER_LINEAR 8U fixed point from level l-1: the source
+        // index and 11-bit taps of each column and row come from the level's
+        // tables (computed on the host with OpenCV's double/float statements).
         const LevelJob& P = J.lv[level - 1];
         const int sw = P.w, sh = P.h, sp = P.pitch;
         if (tid < LP_W) {
-            const double scale_x = 1. / ((double)w / sw);
-            const int x = reflect101(tx0 - LT_HALO + tid, w);
-            float fx = (float)((x + 0.5) * scale_x - 0.5);
-            int sx = (int)floorf(fx);
-            fx -= sx;
-            if (sx < 0) { fx = 0.f; sx = 0; }
-            if (sx >= sw - 1) { fx = 0.f; sx = sw - 1; }
+            const uint2 tx = gload8((const uint8_t*)(L.xtab + reflect101(tx0 - LT_HALO + tid, w)));
+            const int sx = (int)tx.x;
             xs0[tid] = sx;
-            xa[tid] = (int)rintf((1.f - fx) * 2048.f) | ((int)rintf(fx * 2048.f) << 16);
+            xa[tid] = (int)tx.y;
             atomicMin(&s_wx0, sx);
             atomicMax(&s_wx1, sx + 1 < sw ? sx + 1 : sx);
         } else if (tid >= 128 && tid < 128 + LP_H) {
             const int r = tid - 128;
-            const double scale_y = 1. / ((double)h / sh);
-            const int y = reflect101(ty0 - LT_HALO + r, h);
-            float fy = (float)((y + 0.5) * scale_y - 0.5);
-            const int sy = (int)floorf(fy);
-            fy -= sy;
+            const uint2 ty = gload8((const uint8_t*)(L.ytab + reflect101(ty0 - LT_HALO + r, h)));
+            const int sy = (int)ty.x;
             ys0[r] = sy;
-            yb[r] = (int)rintf((1.f - fy) * 2048.f) | ((int)rintf(fy * 2048.f) << 16);
+            yb[r] = (int)ty.y;
             atomicMin(&s_wy0, min(max(sy, 0), sh - 1));
             atomicMax(&s_wy1, min(max(sy + 1, 0), sh - 1));
         }

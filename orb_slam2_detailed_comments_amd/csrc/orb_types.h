@@ -58,6 +58,8 @@ struct LevelJob {
     unsigned long long* keys2;       // candidates after the per-cell retry
     int* node;                       // per-candidate quadtree node (scratch)
     unsigned long long* ret;         // kept keypoints in output order
+    const int2* xtab;                // levels >= 1: per column {sx, a0 | a1 << 16} of the
+    const int2* ytab;                // resize from level-1; per row {sy, b0 | b1 << 16}
 };
 
 struct ImageJob {
