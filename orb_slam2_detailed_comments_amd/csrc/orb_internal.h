@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -72,6 +73,14 @@ inline void disc_umax(int umax[16])
         umax[v] = v0;
         ++v0;
     }
+}
+
+// Border index of REFLECT_101 (the level kernel's halo rule).
+inline int reflect101(int i, int n)
+{
+    if (i < 0) i = -i;
+    if (i >= n) i = 2 * n - 2 - i;
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
 struct LevelPlan {
@@ -307,7 +316,8 @@ struct orb_extractor {
         const size_t o_srec = take((size_t)p.out_cap * 16), o_sdesc = take((size_t)p.out_cap * 32);
         const size_t o_rows = take((size_t)nlevels * ((size_t)H + 2) * 8);
         size_t ntab = 0;
-        for (int l = 1; l < nlevels; l++) ntab += (size_t)p.lv[l].w + p.lv[l].h;
+        for (int l = 1; l < nlevels; l++)
+            ntab += (size_t)p.lv[l].w + p.lv[l].h + p.lv[l].tiles_x + p.lv[l].tiles_y;
         const size_t o_tab = take(ntab * 8);
         const size_t total = off;
         if (total > mem_bytes) {
@@ -389,6 +399,7 @@ struct orb_extractor {
                     tab[t].x = sx;
                     tab[t].y = (int)std::nearbyint((1.f - fx) * 2048.f) | ((int)std::nearbyint(fx * 2048.f) << 16);
                 }
+                const size_t tx = t - w;
                 J.lv[l].ytab = (const int2*)(d_mem + o_tab) + t;
                 const double scale_y = 1. / ((double)h / sh);
                 for (int y = 0; y < h; y++, t++) {
@@ -397,6 +408,31 @@ struct orb_extractor {
                     fy -= sy;
                     tab[t].x = sy;
                     tab[t].y = (int)std::nearbyint((1.f - fy) * 2048.f) | ((int)std::nearbyint(fy * 2048.f) << 16);
+                }
+                const size_t ty = t - h;
+                // Per tile column and row: the source range its taps read,
+                // halo (REFLECT_101) included, as the level kernel stages it.
+                J.lv[l].xwin = (const int2*)(d_mem + o_tab) + t;
+                for (int c = 0; c < p.lv[l].tiles_x; c++, t++) {
+                    int lo = INT_MAX, hi = -1;
+                    for (int i = 0; i < ORB_TILE_W + 8; i++) {
+                        const int sx = tab[tx + reflect101(c * ORB_TILE_W - 4 + i, w)].x;
+                        lo = std::min(lo, sx);
+                        hi = std::max(hi, sx + 1 < sw ? sx + 1 : sx);
+                    }
+                    tab[t].x = lo;
+                    tab[t].y = hi;
+                }
+                J.lv[l].ywin = (const int2*)(d_mem + o_tab) + t;
+                for (int r = 0; r < p.lv[l].tiles_y; r++, t++) {
+                    int lo = INT_MAX, hi = -1;
+                    for (int i = 0; i < ORB_TILE_H + 8; i++) {
+                        const int sy = tab[ty + reflect101(r * ORB_TILE_H - 4 + i, h)].x;
+                        lo = std::min(lo, std::min(std::max(sy, 0), sh - 1));
+                        hi = std::max(hi, std::min(std::max(sy + 1, 0), sh - 1));
+                    }
+                    tab[t].x = lo;
+                    tab[t].y = hi;
                 }
             }
             if (ntab) HIP_TRY(hipMemcpy(d_mem + o_tab, tab.data(), ntab * 8, hipMemcpyHostToDevice));

@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     __shared__ unsigned mmw[LM_H][LP_WD];         // FAST strength, tile + 1-px ring
     __shared__ LevelScratch u;
     __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
-    __shared__ int s_job, s_blk, s_qn, s_nq, s_kn, s_base, s_wx0, s_wx1, s_wy0, s_wy1;
+    __shared__ int s_job, s_blk, s_qn, s_nq, s_kn, s_base;
     const uint8_t* px = (const uint8_t*)pxw;
     uint8_t* mm = (uint8_t*)mmw;
     const int tid = threadIdx.x;
@@ -303,8 +303,6 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         s_qn = 0;
         s_nq = 0;
         s_kn = 0;
-        s_wx0 = s_wy0 = 0x7FFFFFFF;
-        s_wx1 = s_wy1 = -1;
     }
     for (int e = tid; e < LM_H * LP_WD; e += 256) mmw[e / LP_WD][e % LP_WD] = 0u;
     __syncthreads();
@@ -341,40 +339,35 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // tables (computed on the host with OpenCV's double/float statements).
         const LevelJob& P = J.lv[level - 1];
         const int sw = P.w, sh = P.h, sp = P.pitch;
+        // The source rectangle the tile reads (the level's per-tile-column and
+        // per-tile-row tables), in whole aligned dwords (level rows start
+        // 64-byte aligned, so no read leaves its row's pitch).  It is staged
+        // in LDS when it fits, in the same memory round trip as the taps;
+        // otherwise (scale factors near 2 and above) the taps are read from
+        // global memory directly.
+        const int2 xw = L.xwin[t % L.tiles_x], yw = L.ywin[t / L.tiles_x];
+        const int wx0 = xw.x & ~3, wy0 = yw.x;
+        const int nwd = (xw.y >> 2) - (wx0 >> 2) + 1, nwr = yw.y - wy0 + 1;
+        const bool staged = nwd <= 64 && nwd * nwr <= LW_MAX;
         if (tid < LP_W) {
             const uint2 tx = gload8((const uint8_t*)(L.xtab + reflect101(tx0 - LT_HALO + tid, w)));
-            const int sx = (int)tx.x;
-            xs0[tid] = sx;
+            xs0[tid] = (int)tx.x;
             xa[tid] = (int)tx.y;
-            atomicMin(&s_wx0, sx);
-            atomicMax(&s_wx1, sx + 1 < sw ? sx + 1 : sx);
         } else if (tid >= 128 && tid < 128 + LP_H) {
-            const int r = tid - 128;
-            const uint2 ty = gload8((const uint8_t*)(L.ytab + reflect101(ty0 - LT_HALO + r, h)));
-            const int sy = (int)ty.x;
-            ys0[r] = sy;
-            yb[r] = (int)ty.y;
-            atomicMin(&s_wy0, min(max(sy, 0), sh - 1));
-            atomicMax(&s_wy1, min(max(sy + 1, 0), sh - 1));
+            const uint2 ty = gload8((const uint8_t*)(L.ytab + reflect101(ty0 - LT_HALO + (tid - 128), h)));
+            ys0[tid - 128] = (int)ty.x;
+            yb[tid - 128] = (int)ty.y;
         }
-        __syncthreads();
-        // The source rectangle the tile reads, in whole aligned dwords
-        // (level rows start 64-byte aligned, so no read leaves its row's
-        // pitch).  It is staged in LDS when it fits; otherwise (scale factors
-        // near 2 and above) the taps are read from global memory directly.
-        const int wx0 = s_wx0 & ~3, wy0 = s_wy0;
-        const int nwd = (s_wx1 >> 2) - (wx0 >> 2) + 1, nwr = s_wy1 - wy0 + 1;
-        // Each of 252 threads owns one 4-column group (taps kept in
-        // registers) and rows r0, r0 + 14, r0 + 28 of the tile.
-        const bool staged = nwd <= 64 && nwd * nwr <= LW_MAX;
         if (staged) {
             const uint8_t* srow = P.lvl + (size_t)wy0 * sp + wx0;
             const int lg = nwd <= 16 ? 4 : (nwd <= 32 ? 5 : 6);     // threads per row: 2^lg >= nwd
             const int tc = tid & ((1 << lg) - 1);
             for (int r = tid >> lg; r < nwr; r += 256 >> lg)
                 if (tc < nwd) u.win[r * nwd + tc] = gload4(srow + (size_t)r * sp + 4 * tc);
-            __syncthreads();
         }
+        __syncthreads();
+        // Each of 252 threads owns one 4-column group (taps kept in
+        // registers) and rows r0, r0 + 14, r0 + 28 of the tile.
         if (tid < 14 * LP_WD) {
             const int c = tid % LP_WD, r0 = tid / LP_WD;
             const int xoff = staged ? wx0 : 0;

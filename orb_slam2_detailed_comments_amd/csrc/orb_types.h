@@ -60,6 +60,8 @@ struct LevelJob {
     unsigned long long* ret;         // kept keypoints in output order
     const int2* xtab;                // levels >= 1: per column {sx, a0 | a1 << 16} of the
     const int2* ytab;                // resize from level-1; per row {sy, b0 | b1 << 16}
+    const int2* xwin;                // levels >= 1: per tile column / row, the source
+    const int2* ywin;                // columns / rows {first, last} the tile's taps read
 };
 
 struct ImageJob {
