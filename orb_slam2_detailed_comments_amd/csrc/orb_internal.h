@@ -400,6 +400,7 @@ struct orb_extractor {
                     tab[t].y = (int)std::nearbyint((1.f - fx) * 2048.f) | ((int)std::nearbyint(fx * 2048.f) << 16);
                 }
                 const size_t tx = t - w;
+                // ytab == xtab + w: the level kernel reads both through xtab.
                 J.lv[l].ytab = (const int2*)(d_mem + o_tab) + t;
                 const double scale_y = 1. / ((double)h / sh);
                 for (int y = 0; y < h; y++, t++) {

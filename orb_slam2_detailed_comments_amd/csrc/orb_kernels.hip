@@ -65,6 +65,13 @@ __device__ __forceinline__ uint2 gload8(const uint8_t* p)
     const orb_v2u v = *(const __attribute__((address_space(1))) orb_v2u*)p;
     return make_uint2(v.x, v.y);
 }
+// 8 bytes that no thread writes during the kernel, at a wave-uniform address:
+// constant address space, so that it is a scalar load.
+__device__ __forceinline__ uint2 cload8(const void* p)
+{
+    const orb_v2u v = *(const __attribute__((address_space(4))) orb_v2u*)p;
+    return make_uint2(v.x, v.y);
+}
 typedef unsigned orb_u32a1 __attribute__((aligned(1)));
 __device__ __forceinline__ unsigned gload4u(const uint8_t* p)   // any alignment
 {
@@ -276,6 +283,7 @@ __device__ __forceinline__ unsigned resize4(BytePtr base, unsigned o0, unsigned 
 // (phase 1), the horizontal blur sums (phase 2), then the FAST queue and the
 // tile's candidate list (phases 3-5).
 #define LW_MAX 2656                  // source window dwords: scale <= ~1.9
+#define LS_K 6                       // window dwords per thread read before the first wait
 union LevelScratch {
     unsigned win[LW_MAX];
     uint4 hs[LT_H + 6][LT_W / 4];
@@ -292,23 +300,25 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     __shared__ unsigned mmw[LM_H][LP_WD];         // FAST strength, tile + 1-px ring
     __shared__ LevelScratch u;
     __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
-    __shared__ int s_job, s_blk, s_qn, s_nq, s_kn, s_base;
+    __shared__ int s_qn, s_nq, s_kn, s_base;
     const uint8_t* px = (const uint8_t*)pxw;
     uint8_t* mm = (uint8_t*)mmw;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    // Every wave finds its tile itself (uniform scalar loads), so the first
+    // barrier comes only after the tile's memory reads have been issued; the
+    // counters and mmw cleared here are first used after that barrier.
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int job = find_segment(rt->tile_start[level], rt->njobs, blk);
     if (tid == 0) {
-        s_blk = xcd_block(blockIdx.x, gridDim.x);
-        s_job = find_segment(rt->tile_start[level], rt->njobs, s_blk);
         s_qn = 0;
         s_nq = 0;
         s_kn = 0;
     }
     for (int e = tid; e < LM_H * LP_WD; e += 256) mmw[e / LP_WD][e % LP_WD] = 0u;
-    __syncthreads();
-    const ImageJob& J = jobs[s_job];
+    const ImageJob& J = jobs[job];
     const LevelJob& L = J.lv[level];
-    const int t = s_blk - rt->tile_start[level][s_job];
+    const int t = blk - rt->tile_start[level][job];
     const int tx0 = (t % L.tiles_x) * LT_W;
     const int ty0 = (t / L.tiles_x) * LT_H;
     const int w = L.w, h = L.h, pitch = L.pitch;
@@ -345,25 +355,45 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // in LDS when it fits, in the same memory round trip as the taps;
         // otherwise (scale factors near 2 and above) the taps are read from
         // global memory directly.
-        const int2 xw = L.xwin[t % L.tiles_x], yw = L.ywin[t / L.tiles_x];
-        const int wx0 = xw.x & ~3, wy0 = yw.x;
-        const int nwd = (xw.y >> 2) - (wx0 >> 2) + 1, nwr = yw.y - wy0 + 1;
+        const uint2 xw = cload8(L.xwin + t % L.tiles_x), yw = cload8(L.ywin + t / L.tiles_x);
+        const int wx0 = (int)xw.x & ~3, wy0 = (int)yw.x;
+        const int nwd = ((int)xw.y >> 2) - (wx0 >> 2) + 1, nwr = (int)yw.y - wy0 + 1;
         const bool staged = nwd <= 64 && nwd * nwr <= LW_MAX;
-        if (tid < LP_W) {
-            const uint2 tx = gload8((const uint8_t*)(L.xtab + reflect101(tx0 - LT_HALO + tid, w)));
-            xs0[tid] = (int)tx.x;
-            xa[tid] = (int)tx.y;
-        } else if (tid >= 128 && tid < 128 + LP_H) {
-            const uint2 ty = gload8((const uint8_t*)(L.ytab + reflect101(ty0 - LT_HALO + (tid - 128), h)));
-            ys0[tid - 128] = (int)ty.x;
-            yb[tid - 128] = (int)ty.y;
+        // All reads are issued before the first wait: one tap table entry per
+        // thread (columns: threads 0..71, rows: 128..167) and the first
+        // LS_K * 256 window dwords, in row-major order (row = e / nwd by a
+        // 20-bit reciprocal, exact for e * nwd < 2^20).
+        // (The row table follows the column table: ytab == xtab + w.)
+        const int ti = tid < 128 ? reflect101(tx0 - LT_HALO + min(tid, LP_W - 1), w)
+                                 : w + reflect101(ty0 - LT_HALO + min(tid - 128, LP_H - 1), h);
+        const uint2 tv = gload8((const uint8_t*)(L.xtab + ti));
+        const int nst = staged ? nwd * nwr : 0;
+        const unsigned rcp = ((1u << 20) + (unsigned)nwd - 1u) / (unsigned)nwd;
+        const uint8_t* srow = P.lvl + (size_t)wy0 * sp + wx0;
+        unsigned wv[LS_K];
+        if (nst > 0) {
+            // unconditional reads (indexes past the window repeat its last
+            // dword), so that none of them waits for another
+#pragma unroll
+            for (int k = 0; k < LS_K; k++) {
+                const int e = min(tid + 256 * k, nst - 1);
+                const int r = (int)(__umul24((unsigned)e, rcp) >> 20);
+                wv[k] = gload4(srow + (size_t)r * sp + 4 * (e - r * nwd));
+            }
         }
-        if (staged) {
-            const uint8_t* srow = P.lvl + (size_t)wy0 * sp + wx0;
-            const int lg = nwd <= 16 ? 4 : (nwd <= 32 ? 5 : 6);     // threads per row: 2^lg >= nwd
-            const int tc = tid & ((1 << lg) - 1);
-            for (int r = tid >> lg; r < nwr; r += 256 >> lg)
-                if (tc < nwd) u.win[r * nwd + tc] = gload4(srow + (size_t)r * sp + 4 * tc);
+        if (tid < LP_W) {
+            xs0[tid] = (int)tv.x;
+            xa[tid] = (int)tv.y;
+        } else if (tid >= 128 && tid < 128 + LP_H) {
+            ys0[tid - 128] = (int)tv.x;
+            yb[tid - 128] = (int)tv.y;
+        }
+#pragma unroll
+        for (int k = 0; k < LS_K; k++)
+            if (tid + 256 * k < nst) u.win[tid + 256 * k] = wv[k];
+        for (int e = tid + 256 * LS_K; e < nst; e += 256) {
+            const int r = (int)(__umul24((unsigned)e, rcp) >> 20);
+            u.win[e] = gload4(srow + (size_t)r * sp + 4 * (e - r * nwd));
         }
         __syncthreads();
         // Each of 252 threads owns one 4-column group (taps kept in
