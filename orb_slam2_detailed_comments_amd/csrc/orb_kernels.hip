@@ -296,7 +296,7 @@ union LevelScratch {
 __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__ jobs,
                                                     const RunTable* __restrict__ rt, int level)
 {
-    __shared__ unsigned pxw[LP_H][LP_WD];         // level pixels, tile + 4-px halo
+    __shared__ __align__(16) unsigned pxw[LP_H][LP_WD];   // level pixels, tile + 4-px halo
     __shared__ unsigned mmw[LM_H][LP_WD];         // FAST strength, tile + 1-px ring
     __shared__ LevelScratch u;
     __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     __syncthreads();                              // hs is reused below
 
     // ---- 3. FAST strength on the tile plus a 1-px ring, detection areas
-    // only.  The cardinal pre-test runs on 4-pixel groups (two pixels per
+    // only.  The cardinal pre-test runs on 8-pixel groups (two pixels per
     // 16-bit lane pair); survivors are compacted, then scored exactly.
     const int tA = min(max(J.iniTh, 0), 255), tB = min(max(J.minTh, 0), 255);
     const int tmin = min(tA, tB);
@@ -551,56 +551,67 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const unsigned t2 = (unsigned)tmin | ((unsigned)tmin << 16);
     // tile plus its 1-px ring wholly inside the detection area
     const bool tile_in = tx0 - 1 >= ax0 && tx0 + LT_W < ax1 && ty0 - 1 >= ay0 && ty0 + LT_H < ay1;
-    // Groups: rows tile -1..32, 4-column groups c = -1..16.  Group -1 only
-    // contributes its pixel 3 (tile column -1) and group 16 its pixel 0
-    // (column 64): the 1-px ring.  Their missing neighbour dwords (columns -8
-    // and 68 onwards) feed only the bytes that are masked off.
-    for (int e0 = 0; e0 < LM_H * 18; e0 += 256) {
+    // Lanes: rows tile -1..32 and 8-column groups g = 0..8 (px dwords 2g and
+    // 2g + 1, tile columns 8g - 4 .. 8g + 3).  Group 0 only contributes its
+    // pixel 3 (tile column -1) and group 8 its pixel 4 (column 64): the 1-px
+    // ring.  Their missing neighbour dwords (columns -8 and 68 onwards) feed
+    // only the bytes that are masked off.
+    for (int e0 = 0; e0 < LM_H * 9; e0 += 256) {
         const int e = e0 + tid;
         unsigned cand = 0u;                       // bit k: pixel k of the group
-        int my = 0, c = 0;
-        if (e < LM_H * 18) {
-            my = e / 18;
-            c = e - my * 18 - 1;
+        int my = 0, g = 0;
+        if (e < LM_H * 9) {
+            my = e / 9;
+            g = e - my * 9;
             const int y = ty0 - 1 + my;
             const int py = my + 3;                // px row of tile row y
-            const int x = tx0 + 4 * c;
-            const unsigned V = pxw[py][c + 1], Up = pxw[py - 3][c + 1], Dn = pxw[py + 3][c + 1];
-            const unsigned Lw = c >= 0 ? pxw[py][c] : 0u;
-            const unsigned Rw = c <= 15 ? pxw[py][c + 2] : 0u;
-            const unsigned P4 = __builtin_amdgcn_alignbyte(Rw, V, 3);
-            const unsigned P12 = __builtin_amdgcn_alignbyte(V, Lw, 1);
-            const unsigned m02 = fast_maybe2(lo_bytes(V), lo_bytes(Dn), lo_bytes(P4), lo_bytes(Up),
-                                             lo_bytes(P12), t2);
-            const unsigned m13 = fast_maybe2(hi_bytes(V), hi_bytes(Dn), hi_bytes(P4), hi_bytes(Up),
-                                             hi_bytes(P12), t2);
+            const int x = tx0 + 8 * g - 4;
+            const uint2 V = *(const uint2*)&pxw[py][2 * g];
+            const uint2 Up = *(const uint2*)&pxw[py - 3][2 * g];
+            const uint2 Dn = *(const uint2*)&pxw[py + 3][2 * g];
+            const unsigned Lw = g > 0 ? pxw[py][2 * g - 1] : 0u;
+            const unsigned Rw = g < 8 ? pxw[py][2 * g + 2] : 0u;
+            const unsigned P4a = __builtin_amdgcn_alignbyte(V.y, V.x, 3);
+            const unsigned P4b = __builtin_amdgcn_alignbyte(Rw, V.y, 3);
+            const unsigned P12a = __builtin_amdgcn_alignbyte(V.x, Lw, 1);
+            const unsigned P12b = __builtin_amdgcn_alignbyte(V.y, V.x, 1);
+            const unsigned m02a = fast_maybe2(lo_bytes(V.x), lo_bytes(Dn.x), lo_bytes(P4a), lo_bytes(Up.x),
+                                              lo_bytes(P12a), t2);
+            const unsigned m13a = fast_maybe2(hi_bytes(V.x), hi_bytes(Dn.x), hi_bytes(P4a), hi_bytes(Up.x),
+                                              hi_bytes(P12a), t2);
+            const unsigned m02b = fast_maybe2(lo_bytes(V.y), lo_bytes(Dn.y), lo_bytes(P4b), lo_bytes(Up.y),
+                                              lo_bytes(P12b), t2);
+            const unsigned m13b = fast_maybe2(hi_bytes(V.y), hi_bytes(Dn.y), hi_bytes(P4b), hi_bytes(Up.y),
+                                              hi_bytes(P12b), t2);
             // pixels k in [klo, khi): detection area, and the ring rule
             unsigned mask;
             if (tile_in) {
-                mask = c < 0 ? 8u : (c > 15 ? 1u : 15u);
+                mask = g == 0 ? 0xF8u : (g == 8 ? 0x1Fu : 0xFFu);
             } else {
-                const int klo = min(max(max(ax0 - x, c < 0 ? 3 : 0), 0), 4);
-                const int khi = max(min(min(ax1 - x, c > 15 ? 1 : 4), 4), 0);
+                const int klo = min(max(max(ax0 - x, g == 0 ? 3 : 0), 0), 8);
+                const int khi = max(min(min(ax1 - x, g == 8 ? 5 : 8), 8), 0);
                 const bool yin = y >= ay0 && y < ay1;
                 mask = (yin && klo < khi) ? ((1u << khi) - (1u << klo)) : 0u;
             }
-            cand = (((m02 & 0xFFFFu) ? 1u : 0u) | ((m13 & 0xFFFFu) ? 2u : 0u) | ((m02 >> 16) ? 4u : 0u) |
-                    ((m13 >> 16) ? 8u : 0u)) & mask;
+            cand = (((m02a & 0xFFFFu) ? 1u : 0u) | ((m13a & 0xFFFFu) ? 2u : 0u) | ((m02a >> 16) ? 4u : 0u) |
+                    ((m13a >> 16) ? 8u : 0u) | ((m02b & 0xFFFFu) ? 16u : 0u) | ((m13b & 0xFFFFu) ? 32u : 0u) |
+                    ((m02b >> 16) ? 64u : 0u) | ((m13b >> 16) ? 128u : 0u)) & mask;
         }
-        // Append: the wave's exclusive prefix of the per-lane counts (0..4)
-        // from three ballots of the count's bits, one LDS atomic per wave.
+        // Append: the wave's exclusive prefix of the per-lane counts (0..8)
+        // from four ballots of the count's bits, one LDS atomic per wave.
         const int cnt = __popc(cand);
-        const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
-        const int tot = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+        const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4),
+                                 b3 = __ballot(cnt & 8);
+        const int tot = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3);
         if (tot) {
             int base = 0;
             if (lane == 0) base = atomicAdd(&s_qn, tot);
             base = __shfl(base, 0, 64);
-            int pos = base + lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2);
+            int pos = base + lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2) + 8 * lanes_below(b3);
             // queue entry: m-row << 8 | byte column in mmw (tile col + 4)
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if ((cand >> k) & 1u) u.f.queue[pos++] = (unsigned short)((my << 8) | (4 + 4 * c + k));
+            const unsigned ebase = ((unsigned)my << 8) | (unsigned)(8 * g);
+            for (unsigned m = cand; m; m &= m - 1)
+                u.f.queue[pos++] = (unsigned short)(ebase + (unsigned)__builtin_ctz(m));
         }
     }
     __syncthreads();
