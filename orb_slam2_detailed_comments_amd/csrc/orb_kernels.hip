@@ -232,15 +232,13 @@ __device__ __forceinline__ unsigned fast_maybe2(unsigned v, unsigned p0, unsigne
     const orb_us2 lo = __builtin_elementwise_sub_sat(V, T);     // max(v - t, 0)
     const orb_us2 a = __builtin_bit_cast(orb_us2, p0), b = __builtin_bit_cast(orb_us2, p4);
     const orb_us2 c = __builtin_bit_cast(orb_us2, p8), d = __builtin_bit_cast(orb_us2, p12);
-    const orb_us2 ba = __builtin_elementwise_sub_sat(a, hi), bb = __builtin_elementwise_sub_sat(b, hi);
-    const orb_us2 bc = __builtin_elementwise_sub_sat(c, hi), bd = __builtin_elementwise_sub_sat(d, hi);
-    const orb_us2 ka = __builtin_elementwise_sub_sat(lo, a), kb = __builtin_elementwise_sub_sat(lo, b);
-    const orb_us2 kc = __builtin_elementwise_sub_sat(lo, c), kd = __builtin_elementwise_sub_sat(lo, d);
-    // Some circularly adjacent pair (0,4), (4,8), (8,12), (12,0) both pass
-    // <=> (0 or 8) and (4 or 12) pass: opposite points never pair up.
-    const orb_us2 br = __builtin_elementwise_min(__builtin_elementwise_max(ba, bc), __builtin_elementwise_max(bb, bd));
-    const orb_us2 dk = __builtin_elementwise_min(__builtin_elementwise_max(ka, kc), __builtin_elementwise_max(kb, kd));
-    const orb_us2 r = __builtin_elementwise_max(br, dk);
+    // Some circularly adjacent pair (0,4), (4,8), (8,12), (12,0) is brighter
+    // than hi <=> (0 or 8) and (4 or 12) are <=> min(max(p0, p8),
+    // max(p4, p12)) > hi: opposite points never pair up.  Darker likewise
+    // with max(min, min) < lo.  Nonzero lane <=> the pixel may pass.
+    const orb_us2 M = __builtin_elementwise_min(__builtin_elementwise_max(a, c), __builtin_elementwise_max(b, d));
+    const orb_us2 m = __builtin_elementwise_max(__builtin_elementwise_min(a, c), __builtin_elementwise_min(b, d));
+    const orb_us2 r = __builtin_elementwise_max(__builtin_elementwise_sub_sat(M, hi), __builtin_elementwise_sub_sat(lo, m));
     return __builtin_bit_cast(unsigned, r);
 }
 
@@ -571,18 +569,18 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             const uint2 Dn = *(const uint2*)&pxw[py + 3][2 * g];
             const unsigned Lw = g > 0 ? pxw[py][2 * g - 1] : 0u;
             const unsigned Rw = g < 8 ? pxw[py][2 * g + 2] : 0u;
-            const unsigned P4a = __builtin_amdgcn_alignbyte(V.y, V.x, 3);
-            const unsigned P4b = __builtin_amdgcn_alignbyte(Rw, V.y, 3);
-            const unsigned P12a = __builtin_amdgcn_alignbyte(V.x, Lw, 1);
-            const unsigned P12b = __builtin_amdgcn_alignbyte(V.y, V.x, 1);
-            const unsigned m02a = fast_maybe2(lo_bytes(V.x), lo_bytes(Dn.x), lo_bytes(P4a), lo_bytes(Up.x),
-                                              lo_bytes(P12a), t2);
-            const unsigned m13a = fast_maybe2(hi_bytes(V.x), hi_bytes(Dn.x), hi_bytes(P4a), hi_bytes(Up.x),
-                                              hi_bytes(P12a), t2);
-            const unsigned m02b = fast_maybe2(lo_bytes(V.y), lo_bytes(Dn.y), lo_bytes(P4b), lo_bytes(Up.y),
-                                              lo_bytes(P12b), t2);
-            const unsigned m13b = fast_maybe2(hi_bytes(V.y), hi_bytes(Dn.y), hi_bytes(P4b), hi_bytes(Up.y),
-                                              hi_bytes(P12b), t2);
+            // Even / odd pixels of the lane in 16-bit lanes; the neighbours
+            // 3 columns right (P4) and left (P12) are byte selections of
+            // V, Lw and Rw: (lo / hi = pixels 0, 2 / 1, 3 of a half)
+            const unsigned Vxl = lo_bytes(V.x), Vxh = hi_bytes(V.x), Vyl = lo_bytes(V.y), Vyh = hi_bytes(V.y);
+            const unsigned P4al = __builtin_amdgcn_perm(V.y, V.x, 0x0C050C03u);     // columns 3, 5
+            const unsigned P4bl = __builtin_amdgcn_perm(Rw, V.y, 0x0C050C03u);      // 7, 9
+            const unsigned P12ah = __builtin_amdgcn_perm(V.x, Lw, 0x0C040C02u);     // -2, 0
+            const unsigned P12bh = __builtin_amdgcn_perm(V.y, V.x, 0x0C040C02u);    // 2, 4
+            const unsigned m02a = fast_maybe2(Vxl, lo_bytes(Dn.x), P4al, lo_bytes(Up.x), hi_bytes(Lw), t2);
+            const unsigned m13a = fast_maybe2(Vxh, hi_bytes(Dn.x), Vyl, hi_bytes(Up.x), P12ah, t2);
+            const unsigned m02b = fast_maybe2(Vyl, lo_bytes(Dn.y), P4bl, lo_bytes(Up.y), Vxh, t2);
+            const unsigned m13b = fast_maybe2(Vyh, hi_bytes(Dn.y), lo_bytes(Rw), hi_bytes(Up.y), P12bh, t2);
             // pixels k in [klo, khi): detection area, and the ring rule
             unsigned mask;
             if (tile_in) {
@@ -593,9 +591,18 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 const bool yin = y >= ay0 && y < ay1;
                 mask = (yin && klo < khi) ? ((1u << khi) - (1u << klo)) : 0u;
             }
-            cand = (((m02a & 0xFFFFu) ? 1u : 0u) | ((m13a & 0xFFFFu) ? 2u : 0u) | ((m02a >> 16) ? 4u : 0u) |
-                    ((m13a >> 16) ? 8u : 0u) | ((m02b & 0xFFFFu) ? 16u : 0u) | ((m13b & 0xFFFFu) ? 32u : 0u) |
-                    ((m02b >> 16) ? 64u : 0u) | ((m13b >> 16) ? 128u : 0u)) & mask;
+            // each 16-bit lane clamped to 0 / 1: pixel k's bit at k (k = 0, 1,
+            // 4, 5) or k + 14 (k = 2, 3, 6, 7)
+            // (1 | 1 << 16 hidden from the optimiser, which otherwise turns
+            // the clamp into per-lane compares and selects)
+            unsigned one2 = 0x00010001u;
+            asm volatile("" : "+s"(one2));
+            const orb_us2 one = __builtin_bit_cast(orb_us2, one2);
+            const unsigned x01 = __builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m02a), one)) |
+                                 (__builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m13a), one)) << 1) |
+                                 (__builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m02b), one)) << 4) |
+                                 (__builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m13b), one)) << 5);
+            cand = (x01 | (x01 >> 14)) & mask;
         }
         // Append: the wave's exclusive prefix of the per-lane counts (0..8)
         // from four ballots of the count's bits, one LDS atomic per wave.
@@ -606,7 +613,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         if (tot) {
             int base = 0;
             if (lane == 0) base = atomicAdd(&s_qn, tot);
-            base = __shfl(base, 0, 64);
+            base = __builtin_amdgcn_readfirstlane(base);     // every lane is active here
             int pos = base + lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2) + 8 * lanes_below(b3);
             // queue entry: m-row << 8 | byte column in mmw (tile col + 4)
             const unsigned ebase = ((unsigned)my << 8) | (unsigned)(8 * g);
