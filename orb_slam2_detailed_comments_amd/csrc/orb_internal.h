@@ -407,7 +407,9 @@ struct orb_extractor {
                     float fy = (float)((y + 0.5) * scale_y - 0.5);
                     const int sy = (int)std::floor(fy);
                     fy -= sy;
-                    tab[t].x = sy;
+                    // the two source rows, clamped (the vertical taps are not
+                    // zeroed at the border): sy | sy + 1 << 16
+                    tab[t].x = std::min(std::max(sy, 0), sh - 1) | (std::min(std::max(sy + 1, 0), sh - 1) << 16);
                     tab[t].y = (int)std::nearbyint((1.f - fy) * 2048.f) | ((int)std::nearbyint(fy * 2048.f) << 16);
                 }
                 const size_t ty = t - h;
@@ -428,9 +430,9 @@ struct orb_extractor {
                 for (int r = 0; r < p.lv[l].tiles_y; r++, t++) {
                     int lo = INT_MAX, hi = -1;
                     for (int i = 0; i < ORB_TILE_H + 8; i++) {
-                        const int sy = tab[ty + reflect101(r * ORB_TILE_H - 4 + i, h)].x;
-                        lo = std::min(lo, std::min(std::max(sy, 0), sh - 1));
-                        hi = std::max(hi, std::min(std::max(sy + 1, 0), sh - 1));
+                        const int sy01 = tab[ty + reflect101(r * ORB_TILE_H - 4 + i, h)].x;
+                        lo = std::min(lo, sy01 & 0xFFFF);
+                        hi = std::max(hi, sy01 >> 16);
                     }
                     tab[t].x = lo;
                     tab[t].y = hi;

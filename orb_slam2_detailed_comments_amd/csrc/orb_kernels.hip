@@ -271,8 +271,10 @@ __device__ __forceinline__ unsigned resize4(BytePtr base, unsigned o0, unsigned 
         const unsigned a0 = (unsigned)tap[k] & 0xFFFFu, a1 = (unsigned)tap[k] >> 16;
         const unsigned S0 = __umul24(base[o0 + sxk[k]], a0) + __umul24(base[o0 + nxk[k]], a1);
         const unsigned S1 = __umul24(base[o1 + sxk[k]], a0) + __umul24(base[o1 + nxk[k]], a1);
+        // Each tap pair sums to 2047..2049 (two roundings of values summing
+        // to 2048), so v <= (255 * 2049 * 2049 + 2^21) >> 22 = 255: no clamp.
         const unsigned v = (__umul24(S0, b0) + __umul24(S1, b1) + (1u << 21)) >> 22;
-        d |= min(v, 255u) << (8 * k);
+        d |= v << (8 * k);
     }
     return d;
 }
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // index and 11-bit taps of each column and row come from the level's
         // tables (computed on the host with OpenCV's double/float statements).
         const LevelJob& P = J.lv[level - 1];
-        const int sw = P.w, sh = P.h, sp = P.pitch;
+        const int sw = P.w, sp = P.pitch;
         // The source rectangle the tile reads (the level's per-tile-column and
         // per-tile-row tables), in whole aligned dwords (level rows start
         // 64-byte aligned, so no read leaves its row's pitch).  It is staged
@@ -428,9 +430,9 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 for (int k = 0; k < 4; k++)
                     sel[k] = 0x0C000C00u | ((unsigned)(nxk[k] - b0) << 16) | (unsigned)(sxk[k] - b0);
                 for (int ey = r0; ey < LP_H; ey += 14) {
-                    const int sy = ys0[ey];
-                    const unsigned* q0 = wd + __umul24(min(max(sy, 0), sh - 1) - wy0, ws);
-                    const unsigned* q1 = wd + __umul24(min(max(sy + 1, 0), sh - 1) - wy0, ws);
+                    const unsigned sy01 = (unsigned)ys0[ey];
+                    const unsigned* q0 = wd + __umul24((sy01 & 0xFFFFu) - wy0, ws);
+                    const unsigned* q1 = wd + __umul24((sy01 >> 16) - wy0, ws);
                     const unsigned w00 = __builtin_amdgcn_alignbyte(q0[1], q0[0], shb);
                     const unsigned w01 = __builtin_amdgcn_alignbyte(q0[2], q0[1], shb);
                     const unsigned w10 = __builtin_amdgcn_alignbyte(q1[1], q1[0], shb);
@@ -444,8 +446,9 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                             __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w01, w00, sel[k])), tk, 0u, false);
                         const unsigned S1 = __builtin_amdgcn_udot2(
                             __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w11, w10, sel[k])), tk, 0u, false);
+                        // <= 255: see resize4
                         const unsigned v = (__umul24(S0, bv0) + __umul24(S1, bv1) + (1u << 21)) >> 22;
-                        d |= min(v, 255u) << (8 * k);
+                        d |= v << (8 * k);
                     }
                     pxw[ey][c] = d;
                 }
@@ -453,16 +456,15 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 const uint8_t* wb = (const uint8_t*)u.win;
                 const int ws = nwd * 4;
                 for (int ey = r0; ey < LP_H; ey += 14) {
-                    const int sy = ys0[ey];
-                    pxw[ey][c] = resize4(wb, __umul24(min(max(sy, 0), sh - 1) - wy0, ws),
-                                         __umul24(min(max(sy + 1, 0), sh - 1) - wy0, ws), sxk, nxk, tap, yb[ey]);
+                    const unsigned sy01 = (unsigned)ys0[ey];
+                    pxw[ey][c] = resize4(wb, __umul24((sy01 & 0xFFFFu) - wy0, ws), __umul24((sy01 >> 16) - wy0, ws),
+                                         sxk, nxk, tap, yb[ey]);
                 }
             } else {
                 for (int ey = r0; ey < LP_H; ey += 14) {
-                    const int sy = ys0[ey];
+                    const unsigned sy01 = (unsigned)ys0[ey];
                     pxw[ey][c] = resize4((const __attribute__((address_space(1))) uint8_t*)P.lvl,
-                                         __umul24(min(max(sy, 0), sh - 1), sp),
-                                         __umul24(min(max(sy + 1, 0), sh - 1), sp), sxk, nxk, tap, yb[ey]);
+                                         __umul24(sy01 & 0xFFFFu, sp), __umul24(sy01 >> 16, sp), sxk, nxk, tap, yb[ey]);
                 }
             }
         }
