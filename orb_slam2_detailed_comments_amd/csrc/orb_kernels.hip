@@ -301,6 +301,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     __shared__ LevelScratch u;
     __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
     __shared__ int s_qn, s_nq, s_kn, s_base;
+    __shared__ int s_col[LT_W], s_row[LT_H];      // FAST cell of each tile column / row, edge flags
     const uint8_t* px = (const uint8_t*)pxw;
     uint8_t* mm = (uint8_t*)mmw;
     const int tid = threadIdx.x;
@@ -551,6 +552,19 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const unsigned t2 = (unsigned)tmin | ((unsigned)tmin << 16);
     // tile plus its 1-px ring wholly inside the detection area
     const bool tile_in = tx0 - 1 >= ax0 && tx0 + LT_W < ax1 && ty0 - 1 >= ay0 && ty0 + LT_H < ay1;
+    // Per tile column: its FAST cell column cj, and whether the pixels left
+    // and right of it fall outside that cell's window (bits 16, 17); rows
+    // likewise.  Read after the barrier that ends this phase.
+    if (tid < LT_W + LT_H) {
+        const bool isc = tid < LT_W;
+        const int z = isc ? tx0 + tid : ty0 + (tid - LT_W);
+        const int cs = isc ? L.wCell : L.hCell, a0 = isc ? ax0 : ay0, a1 = isc ? ax1 : ay1;
+        const int cz = max(z - ORB_EDGE, 0) / cs;
+        const int c0 = max(ORB_EDGE + cz * cs, a0), c1 = min(ORB_EDGE + (cz + 1) * cs, a1);
+        const int info = cz | ((z - 1 < c0) << 16) | ((z + 1 >= c1) << 17);
+        if (isc) s_col[tid] = info;
+        else s_row[tid - LT_W] = info;
+    }
     // Lanes: rows tile -1..32 and 8-column groups g = 0..8 (px dwords 2g and
     // 2g + 1, tile columns 8g - 4 .. 8g + 3).  Group 0 only contributes its
     // pixel 3 (tile column -1) and group 8 its pixel 4 (column 64): the 1-px
@@ -659,9 +673,8 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         const int x = tx0 + lx, y = ty0 + ly;
         const int mc = mm[(ly + 1) * LP_W + lx + 4];
         {
-            const int cj = (x - ORB_EDGE) / L.wCell, ci = (y - ORB_EDGE) / L.hCell;
-            const int cx0 = max(ORB_EDGE + cj * L.wCell, ax0), cx1 = min(ORB_EDGE + (cj + 1) * L.wCell, ax1);
-            const int cy0 = max(ORB_EDGE + ci * L.hCell, ay0), cy1 = min(ORB_EDGE + (ci + 1) * L.hCell, ay1);
+            const int colv = s_col[lx], rowv = s_row[ly];
+            const int cj = colv & 0xFFFF, ci = rowv & 0xFFFF;
             bool keepA = mc > tA, keepB = mc > tB;
             const int sA = mc - 1, sB = mc - 1;
             const uint8_t* mrow = mm + (ly + 1) * LP_W + (lx + 4);
@@ -669,8 +682,9 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             for (int n = 0; n < 9; n++) {
                 if (n == 4) continue;
                 const int dx = n % 3 - 1, dy = n / 3 - 1;
-                const int qx = x + dx, qy = y + dy;
-                if (qx < cx0 || qx >= cx1 || qy < cy0 || qy >= cy1) {
+                const bool out = (dx < 0 && (colv & 0x10000)) || (dx > 0 && (colv & 0x20000)) ||
+                                 (dy < 0 && (rowv & 0x10000)) || (dy > 0 && (rowv & 0x20000));
+                if (out) {
                     // outside this cell's window: score 0
                     keepA = keepA && sA > 0;
                     keepB = keepB && sB > 0;
@@ -710,7 +724,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         const unsigned loc = u.f.kloc[i];
         const int x = tx0 + (int)(loc & 63u), y = ty0 + (int)((loc >> 6) & 31u);
         const int mc = (int)((loc >> 11) & 255u);
-        const int cj = (x - ORB_EDGE) / L.wCell, ci = (y - ORB_EDGE) / L.hCell;
+        const int cj = s_col[loc & 63u] & 0xFFFF, ci = s_row[(loc >> 6) & 31u] & 0xFFFF;
         const unsigned xl = (unsigned)(x - (ORB_MIN_BORDER + cj * L.wCell));
         const unsigned yl = (unsigned)(y - (ORB_MIN_BORDER + ci * L.hCell));
         const unsigned ok = ((unsigned)ci << 20) | ((unsigned)cj << 12) | (yl << 6) | xl;
