@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     __syncthreads();                              // hs is reused below
 
     // ---- 3. FAST strength on the tile plus a 1-px ring, detection areas
-    // only.  The cardinal pre-test runs on 8-pixel groups (two pixels per
+    // only.  The cardinal pre-test runs on 12-pixel groups (two pixels per
     // 16-bit lane pair); survivors are compacted, then scored exactly.
     const int tA = min(max(J.iniTh, 0), 255), tB = min(max(J.minTh, 0), 255);
     const int tmin = min(tA, tB);
@@ -565,62 +565,68 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         if (isc) s_col[tid] = info;
         else s_row[tid - LT_W] = info;
     }
-    // Lanes: rows tile -1..32 and 8-column groups g = 0..8 (px dwords 2g and
-    // 2g + 1, tile columns 8g - 4 .. 8g + 3).  Group 0 only contributes its
-    // pixel 3 (tile column -1) and group 8 its pixel 4 (column 64): the 1-px
-    // ring.  Their missing neighbour dwords (columns -8 and 68 onwards) feed
-    // only the bytes that are masked off.
-    for (int e0 = 0; e0 < LM_H * 9; e0 += 256) {
-        const int e = e0 + tid;
+    // Lanes: rows tile -1..32 and 12-column groups g = 0..5 (px dwords 3g ..
+    // 3g + 2, tile columns 12g - 4 .. 12g + 7): 204 lanes, one pass.  Group 0
+    // only contributes its pixels 3.. (tile columns -1..) and group 5 its
+    // pixels ..8 (columns ..64): the 1-px ring.  Their missing neighbour
+    // dwords (columns -8 and 68 onwards) feed only the bytes masked off.
+    static_assert(LM_H * 6 <= 256, "one pre-test pass");
+    {
         unsigned cand = 0u;                       // bit k: pixel k of the group
         int my = 0, g = 0;
-        if (e < LM_H * 9) {
-            my = e / 9;
-            g = e - my * 9;
+        if (tid < LM_H * 6) {
+            my = tid / 6;
+            g = tid - my * 6;
             const int y = ty0 - 1 + my;
             const int py = my + 3;                // px row of tile row y
-            const int x = tx0 + 8 * g - 4;
-            const uint2 V = *(const uint2*)&pxw[py][2 * g];
-            const uint2 Up = *(const uint2*)&pxw[py - 3][2 * g];
-            const uint2 Dn = *(const uint2*)&pxw[py + 3][2 * g];
-            const unsigned Lw = g > 0 ? pxw[py][2 * g - 1] : 0u;
-            const unsigned Rw = g < 8 ? pxw[py][2 * g + 2] : 0u;
-            // Even / odd pixels of the lane in 16-bit lanes; the neighbours
-            // 3 columns right (P4) and left (P12) are byte selections of
-            // V, Lw and Rw: (lo / hi = pixels 0, 2 / 1, 3 of a half)
-            const unsigned Vxl = lo_bytes(V.x), Vxh = hi_bytes(V.x), Vyl = lo_bytes(V.y), Vyh = hi_bytes(V.y);
-            const unsigned P4al = __builtin_amdgcn_perm(V.y, V.x, 0x0C050C03u);     // columns 3, 5
-            const unsigned P4bl = __builtin_amdgcn_perm(Rw, V.y, 0x0C050C03u);      // 7, 9
-            const unsigned P12ah = __builtin_amdgcn_perm(V.x, Lw, 0x0C040C02u);     // -2, 0
-            const unsigned P12bh = __builtin_amdgcn_perm(V.y, V.x, 0x0C040C02u);    // 2, 4
-            const unsigned m02a = fast_maybe2(Vxl, lo_bytes(Dn.x), P4al, lo_bytes(Up.x), hi_bytes(Lw), t2);
-            const unsigned m13a = fast_maybe2(Vxh, hi_bytes(Dn.x), Vyl, hi_bytes(Up.x), P12ah, t2);
-            const unsigned m02b = fast_maybe2(Vyl, lo_bytes(Dn.y), P4bl, lo_bytes(Up.y), Vxh, t2);
-            const unsigned m13b = fast_maybe2(Vyh, hi_bytes(Dn.y), lo_bytes(Rw), hi_bytes(Up.y), P12bh, t2);
+            const int x = tx0 + 12 * g - 4;
+            const unsigned* vr = &pxw[py][3 * g];
+            const unsigned* ur = &pxw[py - 3][3 * g];
+            const unsigned* dr = &pxw[py + 3][3 * g];
+            const unsigned Vx = vr[0], Vy = vr[1], Vz = vr[2];
+            const unsigned Lw = g > 0 ? vr[-1] : 0u;
+            const unsigned Rw = g < 5 ? vr[3] : 0u;
+            // Even / odd pixels of each dword in 16-bit lanes (lo / hi =
+            // pixels 0, 2 / 1, 3 of the dword); the neighbours 3 columns
+            // right (p4) and left (p12) are byte selections of the row.
+            const unsigned Vxl = lo_bytes(Vx), Vxh = hi_bytes(Vx), Vyl = lo_bytes(Vy), Vyh = hi_bytes(Vy);
+            const unsigned Vzl = lo_bytes(Vz), Vzh = hi_bytes(Vz);
+            const unsigned m02a = fast_maybe2(Vxl, lo_bytes(dr[0]), __builtin_amdgcn_perm(Vy, Vx, 0x0C050C03u),
+                                              lo_bytes(ur[0]), hi_bytes(Lw), t2);        // columns 0, 2
+            const unsigned m13a = fast_maybe2(Vxh, hi_bytes(dr[0]), Vyl, hi_bytes(ur[0]),
+                                              __builtin_amdgcn_perm(Vx, Lw, 0x0C040C02u), t2);   // 1, 3
+            const unsigned m02b = fast_maybe2(Vyl, lo_bytes(dr[1]), __builtin_amdgcn_perm(Vz, Vy, 0x0C050C03u),
+                                              lo_bytes(ur[1]), Vxh, t2);                 // 4, 6
+            const unsigned m13b = fast_maybe2(Vyh, hi_bytes(dr[1]), Vzl, hi_bytes(ur[1]),
+                                              __builtin_amdgcn_perm(Vy, Vx, 0x0C040C02u), t2);   // 5, 7
+            const unsigned m02c = fast_maybe2(Vzl, lo_bytes(dr[2]), __builtin_amdgcn_perm(Rw, Vz, 0x0C050C03u),
+                                              lo_bytes(ur[2]), Vyh, t2);                 // 8, 10
+            const unsigned m13c = fast_maybe2(Vzh, hi_bytes(dr[2]), lo_bytes(Rw), hi_bytes(ur[2]),
+                                              __builtin_amdgcn_perm(Vz, Vy, 0x0C040C02u), t2);   // 9, 11
             // pixels k in [klo, khi): detection area, and the ring rule
             unsigned mask;
             if (tile_in) {
-                mask = g == 0 ? 0xF8u : (g == 8 ? 0x1Fu : 0xFFu);
+                mask = g == 0 ? 0xFF8u : (g == 5 ? 0x1FFu : 0xFFFu);
             } else {
-                const int klo = min(max(max(ax0 - x, g == 0 ? 3 : 0), 0), 8);
-                const int khi = max(min(min(ax1 - x, g == 8 ? 5 : 8), 8), 0);
+                const int klo = min(max(max(ax0 - x, g == 0 ? 3 : 0), 0), 12);
+                const int khi = max(min(min(ax1 - x, g == 5 ? 9 : 12), 12), 0);
                 const bool yin = y >= ay0 && y < ay1;
                 mask = (yin && klo < khi) ? ((1u << khi) - (1u << klo)) : 0u;
             }
             // each 16-bit lane clamped to 0 / 1: pixel k's bit at k (k = 0, 1,
-            // 4, 5) or k + 14 (k = 2, 3, 6, 7)
+            // 4, 5, 8, 9) or k + 14 (k = 2, 3, 6, 7, 10, 11)
             // (1 | 1 << 16 hidden from the optimiser, which otherwise turns
             // the clamp into per-lane compares and selects)
             unsigned one2 = 0x00010001u;
             asm volatile("" : "+s"(one2));
             const orb_us2 one = __builtin_bit_cast(orb_us2, one2);
-            const unsigned x01 = __builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m02a), one)) |
-                                 (__builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m13a), one)) << 1) |
-                                 (__builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m02b), one)) << 4) |
-                                 (__builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m13b), one)) << 5);
+#define ORB_BIT01(m) __builtin_bit_cast(unsigned, __builtin_elementwise_min(__builtin_bit_cast(orb_us2, m), one))
+            const unsigned x01 = ORB_BIT01(m02a) | (ORB_BIT01(m13a) << 1) | (ORB_BIT01(m02b) << 4) |
+                                 (ORB_BIT01(m13b) << 5) | (ORB_BIT01(m02c) << 8) | (ORB_BIT01(m13c) << 9);
+#undef ORB_BIT01
             cand = (x01 | (x01 >> 14)) & mask;
         }
-        // Append: the wave's exclusive prefix of the per-lane counts (0..8)
+        // Append: the wave's exclusive prefix of the per-lane counts (0..12)
         // from four ballots of the count's bits, one LDS atomic per wave.
         const int cnt = __popc(cand);
         const unsigned long long b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4),
@@ -632,7 +638,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             base = __builtin_amdgcn_readfirstlane(base);     // every lane is active here
             int pos = base + lanes_below(b0) + 2 * lanes_below(b1) + 4 * lanes_below(b2) + 8 * lanes_below(b3);
             // queue entry: m-row << 8 | byte column in mmw (tile col + 4)
-            const unsigned ebase = ((unsigned)my << 8) | (unsigned)(8 * g);
+            const unsigned ebase = ((unsigned)my << 8) | (unsigned)(12 * g);
             for (unsigned m = cand; m; m &= m - 1)
                 u.f.queue[pos++] = (unsigned short)(ebase + (unsigned)__builtin_ctz(m));
         }
