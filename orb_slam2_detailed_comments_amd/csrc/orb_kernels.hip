@@ -671,46 +671,52 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     }
     __syncthreads();
 
-    // ---- 4. per-cell non-max suppression at both thresholds
+    // ---- 4. per-cell non-max suppression at both thresholds.  A neighbour's
+    // score at threshold t is (m > t ? m - 1 : 0) for its strength m, 0 outside
+    // the cell's window; it grows with m, so the largest neighbour score is
+    // the score of the largest in-window neighbour strength M.  The pixel is
+    // kept at t iff mc > t and mc - 1 > score(M) (which is >= 0, so this also
+    // beats the out-of-window zeros).
     const int nq = s_nq;
-    for (int q = tid; q < nq; q += 256) {
-        const int e = u.f.queue[q];
-        const int ly = (e >> 8) - 1, lx = (e & 255) - 4;
-        const int x = tx0 + lx, y = ty0 + ly;
-        const int mc = mm[(ly + 1) * LP_W + lx + 4];
-        {
-            const int colv = s_col[lx], rowv = s_row[ly];
-            const int cj = colv & 0xFFFF, ci = rowv & 0xFFFF;
-            bool keepA = mc > tA, keepB = mc > tB;
-            const int sA = mc - 1, sB = mc - 1;
+    for (int q0 = 0; q0 < nq; q0 += 256) {
+        const int q = q0 + tid;
+        bool keepA = false, keepB = false;
+        unsigned loc = 0u;
+        if (q < nq) {
+            const int e = u.f.queue[q];
+            const int ly = (e >> 8) - 1, lx = (e & 255) - 4;
             const uint8_t* mrow = mm + (ly + 1) * LP_W + (lx + 4);
+            const int mc = mrow[0];
+            const int colv = s_col[lx], rowv = s_row[ly];
+            int M = 0;
 #pragma unroll
             for (int n = 0; n < 9; n++) {
                 if (n == 4) continue;
                 const int dx = n % 3 - 1, dy = n / 3 - 1;
                 const bool out = (dx < 0 && (colv & 0x10000)) || (dx > 0 && (colv & 0x20000)) ||
                                  (dy < 0 && (rowv & 0x10000)) || (dy > 0 && (rowv & 0x20000));
-                if (out) {
-                    // outside this cell's window: score 0
-                    keepA = keepA && sA > 0;
-                    keepB = keepB && sB > 0;
-                    continue;
-                }
                 const int mq = mrow[dy * LP_W + dx];
-                keepA = keepA && sA > (mq > tA ? mq - 1 : 0);
-                keepB = keepB && sB > (mq > tB ? mq - 1 : 0);
+                M = max(M, out ? 0 : mq);
             }
-            if (!keepA && !keepB) continue;
-            const unsigned loc = ((unsigned)ly << 6) | (unsigned)lx | ((unsigned)mc << 11);
+            keepA = mc > tA && mc - 1 > (M > tA ? M - 1 : 0);
+            keepB = mc > tB && mc - 1 > (M > tB ? M - 1 : 0);
+            loc = ((unsigned)ly << 6) | (unsigned)lx | ((unsigned)mc << 11);
+            if (keepA) gstore1(L.cellflag + (rowv & 0xFFFF) * L.nCols + (colv & 0xFFFF), 1);
+        }
+        // Append (A entry, then B entry, per lane): the wave's exclusive
+        // prefix from two ballots, one LDS atomic per wave.
+        const unsigned long long ba = __ballot(keepA), bb = __ballot(keepB);
+        const int tot = __popcll(ba) + __popcll(bb);
+        if (tot) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&s_kn, tot);
+            base = __builtin_amdgcn_readfirstlane(base);     // every lane is active here
+            int pos = base + lanes_below(ba) + lanes_below(bb);
             if (keepA) {
-                gstore1(L.cellflag + ci * L.nCols + cj, 1);
-                const int i = atomicAdd(&s_kn, 1);
-                if (i < LK_CAP) u.f.kloc[i] = loc;
+                if (pos < LK_CAP) u.f.kloc[pos] = loc;
+                pos++;
             }
-            if (keepB) {
-                const int i = atomicAdd(&s_kn, 1);
-                if (i < LK_CAP) u.f.kloc[i] = loc | (1u << 19);
-            }
+            if (keepB && pos < LK_CAP) u.f.kloc[pos] = loc | (1u << 19);
         }
     }
     __syncthreads();
