@@ -303,7 +303,6 @@ struct orb_extractor {
             off = align_up(off + bytes, 256);
             return o;
         };
-        const size_t o_src = take((size_t)W * H);
         const size_t o_lvl = take(p.img_bytes), o_blur = take(p.img_bytes), o_cell = take(p.ncells);
         const size_t o_keys = take(p.nkeys * 8), o_keys2 = take(p.nkeys * 8), o_node = take(p.nkeys * 4);
         const size_t o_ret = take(p.nret * 8);
@@ -334,8 +333,10 @@ struct orb_extractor {
         gen++;
         ImageJob& J = hjob;
         std::memset(&J, 0, sizeof(J));
-        J.src = (const uint8_t*)(d_mem + o_src);
-        J.src_pitch = W;
+        // The input frame is written straight into level 0 (64-byte aligned
+        // rows), which the level-0 kernel then reads in place.
+        J.src = (const uint8_t*)(d_mem + o_lvl + p.lv[0].img_off);
+        J.src_pitch = p.lv[0].pitch;
         J.nlevels = nlevels;
         J.iniTh = iniTh;
         J.minTh = minTh;
@@ -481,17 +482,17 @@ struct orb_extractor {
         rt.desc_start[n * maxl] = acc;
     }
 
-    // Input frame into the contiguous staging buffer hjob.src (one linear
-    // copy when the rows are packed); the level-0 kernel moves it into the
-    // pitched pyramid.
+    // Input frame into level 0 of the pyramid (hjob.src, src_pitch bytes a
+    // row), where the level-0 kernel reads it in place.
     int upload_image(const uint8_t* img, size_t stride, hipMemcpyKind kind, hipStream_t st = nullptr)
     {
         if (!st) st = ctx->stream;
         uint8_t* dst = const_cast<uint8_t*>(hjob.src);
-        if (stride == (size_t)plan.W)
-            HIP_TRY(hipMemcpyAsync(dst, img, (size_t)plan.W * plan.H, kind, st));
+        const size_t dp = (size_t)hjob.src_pitch;
+        if (stride == dp)
+            HIP_TRY(hipMemcpyAsync(dst, img, dp * plan.H, kind, st));
         else
-            HIP_TRY(hipMemcpy2DAsync(dst, plan.W, img, stride, plan.W, plan.H, kind, st));
+            HIP_TRY(hipMemcpy2DAsync(dst, dp, img, stride, plan.W, plan.H, kind, st));
         return ORB_OK;
     }
 

@@ -333,8 +333,10 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         for (int e = tid; e < LP_H * LP_WD; e += 256) {
             const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
             unsigned d = 0;
-            if (interior) {
-                // unaligned dword load (rows of the input are W bytes apart)
+            if (interior && (sp & 3) == 0) {
+                // aligned dword load (the input sits in level 0, whose rows are 64-byte aligned)
+                d = gload4(src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex));
+            } else if (interior) {
                 d = gload4u(src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex));
             } else {
                 const int y = reflect101(ty0 - LT_HALO + ey, h);

@@ -65,8 +65,8 @@ struct LevelJob {
 };
 
 struct ImageJob {
-    const uint8_t* src;              // input image (level 0), src_pitch bytes per row;
-    int src_pitch;                   // the level-0 kernel copies it into lv[0].lvl
+    const uint8_t* src;              // input image (level 0), src_pitch bytes per row; the
+    int src_pitch;                   // level-0 kernel copies it into lv[0].lvl unless it is lv[0].lvl
     int nlevels, iniTh, minTh;
     int out_cap;                     // keypoint output capacity
     int* counters;                   // ORB_CNT_SLOTS ints
