@@ -1263,15 +1263,6 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
         pb[h][4] = q1.x, pb[h][5] = q1.y, pb[h][6] = q1.z, pb[h][7] = q1.w;
         pb[h][8] = q2.x, pb[h][9] = q2.y;
     }
-    uint2* win = s_brief[grp];
-#pragma unroll
-    for (int h = 0; h < 3; h++) {
-        const int r = sub + 16 * h;
-        if (r < 37) {
-#pragma unroll
-            for (int c = 0; c < 5; c++) win[r * 5 + c] = make_uint2(pb[h][2 * c], pb[h][2 * c + 1]);
-        }
-    }
 
     // IC_Angle (ORBextractor.cc:94-141): m10 = sum u*I, m01 = sum v*I over
     // the disc |u| <= umax[|v|].  Per dword of 4 pixels: the disc's byte mask
@@ -1303,6 +1294,24 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     m01 = (int)row16_sum((unsigned)m01);
     const float angle_deg = fast_atan2_dev((float)m01, (float)m10);
 
+    const float factorPI = (float)(3.1415926535897932384626433832795 / 180.f);
+    const float angle = angle_deg * factorPI;
+    double sd, cd;
+    sincos_canonical_dev((double)angle, &sd, &cd);
+    const float a = (float)cd, b = (float)sd;
+
+    // The blur window goes to LDS only now, so that its reads stay in flight
+    // under the orientation and rotation arithmetic above.
+    uint2* win = s_brief[grp];
+#pragma unroll
+    for (int h = 0; h < 3; h++) {
+        const int r = sub + 16 * h;
+        if (r < 37) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) win[r * 5 + c] = make_uint2(pb[h][2 * c], pb[h][2 * c + 1]);
+        }
+    }
+
     // Rotated BRIEF (ORBextractor.cc:153-204) on the blurred level, sampled
     // from the staged window (row stride 40 bytes, centre at row 18).
     // cvRound of a sample coordinate t (|t| < 19) is read from the bits of
@@ -1311,11 +1320,6 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const float factorPI = (float)(3.1415926535897932384626433832795 / 180.f);
-    const float angle = angle_deg * factorPI;
-    double sd, cd;
-    sincos_canonical_dev((double)angle, &sd, &cd);
-    const float a = (float)cd, b = (float)sd;
     const float kRound = 12582912.0f;
     // sample (iy, ix) is byte (18 + iy) * 40 + (cx - xb0) + ix of the window;
     // with both read as 2^22 + i in the low 24 bits, the byte is
