@@ -162,6 +162,7 @@ struct orb_extractor {
     unsigned gen = 0;             // bumped whenever buffers or plan change
     char* d_mem = nullptr;
     size_t mem_bytes = 0;
+    uint8_t* d_stage = nullptr;   // contiguous landing buffer for host frames
     ImageJob hjob{};
     ImageJob* d_job = nullptr;
     RunTable* h_rt = nullptr;     // pinned
@@ -303,6 +304,7 @@ struct orb_extractor {
             off = align_up(off + bytes, 256);
             return o;
         };
+        const size_t o_stage = take((size_t)W * H);   // host frames land here (one linear copy)
         const size_t o_lvl = take(p.img_bytes), o_blur = take(p.img_bytes), o_cell = take(p.ncells);
         const size_t o_keys = take(p.nkeys * 8), o_keys2 = take(p.nkeys * 8), o_node = take(p.nkeys * 4);
         const size_t o_ret = take(p.nret * 8);
@@ -337,6 +339,7 @@ struct orb_extractor {
         // rows), which the level-0 kernel then reads in place.
         J.src = (const uint8_t*)(d_mem + o_lvl + p.lv[0].img_off);
         J.src_pitch = p.lv[0].pitch;
+        d_stage = (uint8_t*)(d_mem + o_stage);
         J.nlevels = nlevels;
         J.iniTh = iniTh;
         J.minTh = minTh;
@@ -483,16 +486,24 @@ struct orb_extractor {
     }
 
     // Input frame into level 0 of the pyramid (hjob.src, src_pitch bytes a
-    // row), where the level-0 kernel reads it in place.
+    // row), where the level-0 kernel reads it in place.  A packed host frame
+    // lands in the staging buffer with one linear copy (a pitched host copy
+    // is issued row by row and is far slower) and is then laid out on the
+    // device.
     int upload_image(const uint8_t* img, size_t stride, hipMemcpyKind kind, hipStream_t st = nullptr)
     {
         if (!st) st = ctx->stream;
         uint8_t* dst = const_cast<uint8_t*>(hjob.src);
-        const size_t dp = (size_t)hjob.src_pitch;
+        const size_t dp = (size_t)hjob.src_pitch, W = (size_t)plan.W;
+        if (kind == hipMemcpyHostToDevice && stride == W) {
+            HIP_TRY(hipMemcpyAsync(d_stage, img, W * plan.H, kind, st));
+            img = d_stage;
+            kind = hipMemcpyDeviceToDevice;
+        }
         if (stride == dp)
             HIP_TRY(hipMemcpyAsync(dst, img, dp * plan.H, kind, st));
         else
-            HIP_TRY(hipMemcpy2DAsync(dst, dp, img, stride, plan.W, plan.H, kind, st));
+            HIP_TRY(hipMemcpy2DAsync(dst, dp, img, stride, W, plan.H, kind, st));
         return ORB_OK;
     }
 
