@@ -1377,10 +1377,12 @@ __global__ void zero_counters_kernel(const ImageJob* __restrict__ jobs, int njob
     const int j = blockIdx.x;
     if (j >= njobs) return;
     const ImageJob& J = jobs[j];
-    if (threadIdx.x < ORB_CNT_SLOTS) J.counters[threadIdx.x] = 0;
-    for (int l = 0; l < J.nlevels; l++) {
+    if (threadIdx.x < ORB_CNT_SLOTS) gstore4((uint8_t*)(J.counters + threadIdx.x), 0u);
+    // 32 threads per level, the levels side by side
+    for (int l = threadIdx.x >> 5; l < J.nlevels; l += blockDim.x >> 5) {
         uint8_t* f = J.lv[l].cellflag;
-        for (int i = threadIdx.x; i < J.lv[l].ncells; i += blockDim.x) f[i] = 0;
+        const int n = J.lv[l].ncells;
+        for (int i = threadIdx.x & 31; i < n; i += 32) gstore1(f + i, 0);
     }
 }
 
