@@ -359,6 +359,9 @@ struct orb_extractor {
             L.nRows = P.nRows;
             L.wCell = P.wCell;
             L.hCell = P.hCell;
+            // exact for x * cell < 2^32 (x < 2^16, cell <= 58)
+            L.wrcp = (unsigned)(((1ull << 32) + (unsigned)P.wCell - 1) / (unsigned)P.wCell);
+            L.hrcp = (unsigned)(((1ull << 32) + (unsigned)P.hCell - 1) / (unsigned)P.hCell);
             L.maxBorderX = P.maxBorderX;
             L.maxBorderY = P.maxBorderY;
             L.quota = quota[l];

@@ -561,7 +561,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         const bool isc = tid < LT_W;
         const int z = isc ? tx0 + tid : ty0 + (tid - LT_W);
         const int cs = isc ? L.wCell : L.hCell, a0 = isc ? ax0 : ay0, a1 = isc ? ax1 : ay1;
-        const int cz = max(z - ORB_EDGE, 0) / cs;
+        const int cz = (int)__umulhi((unsigned)max(z - ORB_EDGE, 0), isc ? L.wrcp : L.hrcp);   // / cs
         const int c0 = max(ORB_EDGE + cz * cs, a0), c1 = min(ORB_EDGE + (cz + 1) * cs, a1);
         const int info = cz | ((z - 1 < c0) << 16) | ((z + 1 >= c1) << 17);
         if (isc) s_col[tid] = info;
@@ -646,29 +646,35 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         }
     }
     __syncthreads();
-    // Exact strengths of the queued pixels.  Pixels of the tile proper that
-    // beat the lower threshold are compacted again, in place (a slot is only
-    // rewritten after the barrier that ends its iteration's reads), so that
-    // suppression runs on dense lanes.
+    // Exact strengths of the queued pixels, two per thread before one barrier.
+    // Pixels of the tile proper that beat the lower threshold are compacted
+    // again, in place (a slot is only rewritten after the barrier that ends
+    // the reads of its chunk of 512), so that suppression runs on dense lanes.
     const int qn = s_qn;
-    for (int q0 = 0; q0 < qn; q0 += 256) {
-        const int q = q0 + tid;
-        int e = 0;
-        bool keep = false;
-        if (q < qn) {
-            e = u.f.queue[q];
-            const int my = e >> 8, col = e & 255;
-            const int sv = fast_strength(px + (my + 3) * LP_W + col, LP_W);
-            mm[my * LP_W + col] = (uint8_t)sv;
-            keep = sv > tmin && my >= 1 && my <= LT_H && col >= 4 && col < 4 + LT_W;
+    for (int q0 = 0; q0 < qn; q0 += 512) {
+        int e[2] = {0, 0};
+        bool keep[2] = {false, false};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int q = q0 + 256 * h + tid;
+            if (q < qn) {
+                e[h] = u.f.queue[q];
+                const int my = e[h] >> 8, col = e[h] & 255;
+                const int sv = fast_strength(px + (my + 3) * LP_W + col, LP_W);
+                mm[my * LP_W + col] = (uint8_t)sv;
+                keep[h] = sv > tmin && my >= 1 && my <= LT_H && col >= 4 && col < 4 + LT_W;
+            }
         }
         __syncthreads();
-        const unsigned long long bal = __ballot(keep);
-        if (bal) {
-            int base = 0;
-            if (lane == 0) base = atomicAdd(&s_nq, __popcll(bal));
-            base = __shfl(base, 0, 64);
-            if (keep) u.f.queue[base + lanes_below(bal)] = (unsigned short)e;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const unsigned long long bal = __ballot(keep[h]);
+            if (bal) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&s_nq, __popcll(bal));
+                base = __builtin_amdgcn_readfirstlane(base);     // every lane is active here
+                if (keep[h]) u.f.queue[base + lanes_below(bal)] = (unsigned short)e[h];
+            }
         }
     }
     __syncthreads();

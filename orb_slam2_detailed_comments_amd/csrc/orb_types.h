@@ -42,6 +42,7 @@ struct LevelJob {
     int w, h, pitch;                 // level size and row pitch (bytes)
     int tiles_x, tiles_y;            // level kernel tiles
     int nCols, nRows, wCell, hCell;  // FAST cell grid (ORBextractor.cc:1065-1071)
+    unsigned wrcp, hrcp;             // ceil(2^32 / wCell), ceil(2^32 / hCell): x / cell = umulhi(x, rcp)
     int maxBorderX, maxBorderY;
     int quota;                       // mnFeaturesPerLevel[level]
     int nIni;                        // quadtree roots
