@@ -515,18 +515,23 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // Row sums are <= 255 * 257 and taps < 256: 24-bit multiplies.
         const unsigned k0 = (unsigned)c_gauss[3], k1 = (unsigned)c_gauss[4], k2 = (unsigned)c_gauss[5],
                        k3 = (unsigned)c_gauss[6];
-        for (int r = tid >> 4; r < LT_H; r += 16) {
-            const int y = ty0 + r, x0 = tx0 + 4 * gc;
-            if (y >= h) continue;
-            uint4 q[7];
+        // rows r and r + 1 of the tile from 8 row-sum vectors
+        static_assert((LT_H / 2) * (LT_W / 4) == 256, "one row pair and column group per thread");
+        const int r = (tid >> 4) * 2, x0 = tx0 + 4 * gc;
+        uint4 q[8];
 #pragma unroll
-            for (int i = 0; i < 7; i++) q[i] = u.hs[r + i][gc];
+        for (int i = 0; i < 8; i++) q[i] = u.hs[r + i][gc];
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+            const int y = ty0 + r + hh;
+            if (y >= h) continue;
             unsigned out = 0;
 #define ORB_VBLUR(comp, k)                                                                              \
             {                                                                                            \
-                const unsigned acc = __umul24(q[3].comp, k0) + __umul24(q[2].comp + q[4].comp, k1) +    \
-                                     __umul24(q[1].comp + q[5].comp, k2) +                               \
-                                     __umul24(q[0].comp + q[6].comp, k3);                                \
+                const unsigned acc = __umul24(q[hh + 3].comp, k0) +                                      \
+                                     __umul24(q[hh + 2].comp + q[hh + 4].comp, k1) +                     \
+                                     __umul24(q[hh + 1].comp + q[hh + 5].comp, k2) +                     \
+                                     __umul24(q[hh].comp + q[hh + 6].comp, k3);                          \
                 const unsigned v = (acc + (1u << 15)) >> 16;                                            \
                 out |= (v > 255u ? 255u : v) << (8 * (k));                                              \
             }
