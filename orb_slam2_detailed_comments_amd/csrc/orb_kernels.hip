@@ -494,21 +494,27 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     }
     const unsigned T1 = (unsigned)c_gauss[0] | (c_gauss[1] << 8) | (c_gauss[2] << 16) | ((unsigned)c_gauss[3] << 24);
     const unsigned T2 = (unsigned)c_gauss[4] | (c_gauss[5] << 8) | (c_gauss[6] << 16);
-    for (int e = tid; e < (LT_H + 6) * (LT_W / 4); e += 256) {
-        const int rr = e >> 4, c = e & 15;       // blur row rr <-> px row rr+1
-        const unsigned D0 = pxw[rr + 1][c], D1 = pxw[rr + 1][c + 1], D2 = pxw[rr + 1][c + 2];
-        uint4 o;
-        o.x = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 1), T1,
-                                     __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 1), T2, 0u, false),
-                                     false);
-        o.y = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 2), T1,
-                                     __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 2), T2, 0u, false),
-                                     false);
-        o.z = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 3), T1,
-                                     __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 3), T2, 0u, false),
-                                     false);
-        o.w = __builtin_amdgcn_udot4(D1, T1, __builtin_amdgcn_udot4(D2, T2, 0u, false), false);
-        u.hs[rr][c] = o;
+    // Two adjacent 4-column groups per task (they share two source dwords).
+    for (int e = tid; e < (LT_H + 6) * (LT_W / 8); e += 256) {
+        const int rr = e >> 3, c = 2 * (e & 7);  // blur row rr <-> px row rr+1
+        const unsigned* pr = &pxw[rr + 1][c];
+        const unsigned D[4] = {pr[0], pr[1], pr[2], pr[3]};
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            const unsigned D0 = D[g], D1 = D[g + 1], D2 = D[g + 2];
+            uint4 o;
+            o.x = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 1), T1,
+                                         __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 1), T2, 0u, false),
+                                         false);
+            o.y = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 2), T1,
+                                         __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 2), T2, 0u, false),
+                                         false);
+            o.z = __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D1, D0, 3), T1,
+                                         __builtin_amdgcn_udot4(__builtin_amdgcn_alignbyte(D2, D1, 3), T2, 0u, false),
+                                         false);
+            o.w = __builtin_amdgcn_udot4(D1, T1, __builtin_amdgcn_udot4(D2, T2, 0u, false), false);
+            u.hs[rr][c + g] = o;
+        }
     }
     __syncthreads();
     {
