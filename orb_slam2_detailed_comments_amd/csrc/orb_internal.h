@@ -478,6 +478,11 @@ struct orb_extractor {
                 if (l < exs[j]->nlevels) acc += exs[j]->plan.lv[l].tiles_x * exs[j]->plan.lv[l].tiles_y;
             }
             rt.tile_start[l][n] = acc;
+            // equal tile counts: the kernels find a block's job by one division
+            bool eq = n > 0;
+            for (int j = 0; j < n; j++)
+                eq = eq && rt.tile_start[l][j + 1] - rt.tile_start[l][j] == rt.tile_start[l][1] - rt.tile_start[l][0];
+            rt.tiles_per_job[l] = eq ? rt.tile_start[l][1] - rt.tile_start[l][0] : 0;
         }
         int acc = 0;
         for (int j = 0; j < n; j++)
@@ -486,6 +491,10 @@ struct orb_extractor {
                 if (l < exs[j]->nlevels) acc += (exs[j]->plan.lv[l].list_cap + ORB_DESC_KPB - 1) / ORB_DESC_KPB;
             }
         rt.desc_start[n * maxl] = acc;
+        bool eq = n > 0;
+        for (int j = 0; j < n; j++)
+            eq = eq && rt.desc_start[(j + 1) * maxl] - rt.desc_start[j * maxl] == rt.desc_start[maxl];
+        rt.desc_per_job = eq ? rt.desc_start[maxl] : 0;
     }
 
     // Input frame into level 0 of the pyramid (hjob.src, src_pitch bytes a

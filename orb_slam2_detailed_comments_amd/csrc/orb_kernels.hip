@@ -310,7 +310,8 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     // barrier comes only after the tile's memory reads have been issued; the
     // counters and mmw cleared here are first used after that barrier.
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int job = find_segment(rt->tile_start[level], rt->njobs, blk);
+    const int tpj = rt->tiles_per_job[level];
+    const int job = tpj > 0 ? blk / tpj : find_segment(rt->tile_start[level], rt->njobs, blk);
     if (tid == 0) {
         s_qn = 0;
         s_nq = 0;
@@ -1222,7 +1223,10 @@ __global__ __launch_bounds__(256) void describe_kernel(const ImageJob* __restric
     const int sub = threadIdx.x & (DESC_G - 1), grp = threadIdx.x / DESC_G;
     const int nseg = rt->njobs * rt->max_levels;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int f = find_segment(rt->desc_start, nseg, blk);
+    const int dpj = rt->desc_per_job;
+    const int f = dpj > 0 ? (blk / dpj) * rt->max_levels +
+                                find_segment(rt->desc_start + (blk / dpj) * rt->max_levels, rt->max_levels, blk)
+                          : find_segment(rt->desc_start, nseg, blk);
     const int jb = f / rt->max_levels, level = f % rt->max_levels;
     const ImageJob& J = jobs[jb];
     const int* ctr = J.counters;

@@ -84,6 +84,8 @@ struct RunTable {
     int max_levels;
     int tile_start[ORB_MAX_LEVELS][ORB_MAX_JOBS + 1];   // per level: first tile block of job j
     int desc_start[(ORB_MAX_JOBS + 1) * ORB_MAX_LEVELS];  // describe kernel: first block of (job, level)
+    int tiles_per_job[ORB_MAX_LEVELS];  // > 0: every job has this many tiles at the level
+    int desc_per_job;                   // > 0: every job has this many describe blocks
 };
 
 // Arguments of one stereo match (Frame::ComputeStereoMatches).
