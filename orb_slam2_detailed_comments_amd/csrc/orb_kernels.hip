@@ -1452,15 +1452,26 @@ __global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs*
     };
     for (int j = t; j <= nk; j += STEREO_T) s_cnt[j] = 0;
     __syncthreads();
-    int eb[STEREO_PER_T], er[STEREO_PER_T];
+    // Each thread's keypoints and descriptors are read once, here; the
+    // scatter after the scan uses the registers.
+    int eb[STEREO_PER_T], er[STEREO_PER_T], ko[STEREO_PER_T];
+    uint2 kxy[STEREO_PER_T];
+    uint4 kd0[STEREO_PER_T], kd1[STEREO_PER_T];
 #pragma unroll
     for (int k = 0; k < STEREO_PER_T; k++) {
         const int i = t + STEREO_T * k;
         eb[k] = 0;
+        ko[k] = 0;
+        kxy[k] = make_uint2(0u, 0u);
+        kd0[k] = kd1[k] = make_uint4(0u, 0u, 0u, 0u);
         if (i < n) {
             const uint8_t* kb = (const uint8_t*)(A.kr + i);
-            eb[k] = min((int)gload4(kb + offsetof(OrbKeyPoint, octave)), nlev - 1) * nb +
-                    bucket(__uint_as_float(gload4(kb + offsetof(OrbKeyPoint, y))));
+            kxy[k] = gload8(kb + offsetof(OrbKeyPoint, x));
+            ko[k] = (int)gload4(kb + offsetof(OrbKeyPoint, octave));
+            const uint8_t* d = A.dr + (size_t)i * 32;
+            kd0[k] = gload16(d);
+            kd1[k] = gload16(d + 16);
+            eb[k] = min(ko[k], nlev - 1) * nb + bucket(__uint_as_float(kxy[k].y));
         }
         er[k] = i < n ? atomicAdd(&s_cnt[eb[k]], 1) : 0;
     }
@@ -1495,19 +1506,17 @@ __global__ __launch_bounds__(STEREO_T) void stereo_prep_kernel(const StereoArgs*
     for (int k = 0; k < STEREO_PER_T; k++) {
         const int i = t + STEREO_T * k;
         if (i < n) {
-            const uint8_t* kb = (const uint8_t*)(A.kr + i);
-            const uint2 xy = gload8(kb + offsetof(OrbKeyPoint, x));
-            const int oct = (int)gload4(kb + offsetof(OrbKeyPoint, octave));
+            const uint2 xy = kxy[k];
+            const int oct = ko[k];
             const float ky = __uint_as_float(xy.y);
             const float r = 2.0f * A.scale[oct];
             const int maxr = (int)ceilf(ky + r);
             const int minr = (int)floorf(ky - r);
             const int pos = s_cnt[eb[k]] + er[k];
             gstore16((uint8_t*)(A.srec + pos), make_uint4((unsigned)minr, (unsigned)maxr, (unsigned)((i << 4) | oct), xy.x));
-            const uint8_t* d = A.dr + (size_t)i * 32;
             uint8_t* o = A.sdesc + (size_t)pos * 32;
-            gstore16(o, gload16(d));
-            gstore16(o + 16, gload16(d + 16));
+            gstore16(o, kd0[k]);
+            gstore16(o + 16, kd1[k]);
         }
     }
     // A candidate of octave o for row `row` has y in (row - 1 - r, row + 1 + r),
