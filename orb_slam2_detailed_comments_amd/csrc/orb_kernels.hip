@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             unsigned d = 0;
             if (interior && (sp & 3) == 0) {
                 // aligned dword load (the input sits in level 0, whose rows are 64-byte aligned)
-                d = gload4(src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex));
+                d = gload4(src + (unsigned)((ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex)));
             } else if (interior) {
                 d = gload4u(src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex));
             } else {
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             for (int k = 0; k < LS_K; k++) {
                 const int e = min(tid + 256 * k, nst - 1);
                 const int r = (int)(__umul24((unsigned)e, rcp) >> 20);
-                wv[k] = gload4(srow + (size_t)r * sp + 4 * (e - r * nwd));
+                wv[k] = gload4(srow + (unsigned)(r * sp + 4 * (e - r * nwd)));
             }
         }
         if (tid < LP_W) {
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             if (tid + 256 * k < nst) u.win[tid + 256 * k] = wv[k];
         for (int e = tid + 256 * LS_K; e < nst; e += 256) {
             const int r = (int)(__umul24((unsigned)e, rcp) >> 20);
-            u.win[e] = gload4(srow + (size_t)r * sp + 4 * (e - r * nwd));
+            u.win[e] = gload4(srow + (unsigned)(r * sp + 4 * (e - r * nwd)));
         }
         __syncthreads();
         // Each of 252 threads owns one 4-column group (taps kept in
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             const int y = ty0 + r, x0 = tx0 + 4 * gc;
             if (y >= h) continue;
             const unsigned d = pxw[r + LT_HALO][gc + 1];
-            uint8_t* dst = L.lvl + (size_t)y * pitch + x0;
+            uint8_t* dst = L.lvl + (unsigned)(y * pitch + x0);
             if (x0 + 3 < w) {
                 gstore4(dst, d);
             } else {
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             }
             ORB_VBLUR(x, 0) ORB_VBLUR(y, 1) ORB_VBLUR(z, 2) ORB_VBLUR(w, 3)
 #undef ORB_VBLUR
-            uint8_t* db = L.blur + (size_t)y * pitch + x0;
+            uint8_t* db = L.blur + (unsigned)(y * pitch + x0);
             if (x0 + 3 < w) {
                 gstore4(db, out);
             } else {
