@@ -1569,7 +1569,7 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
     int outSad = -1;
     const uint8_t* kpb = (const uint8_t*)(A.kl + iL);
     const uint2 kxy = gload8(kpb + offsetof(OrbKeyPoint, x));
-    const uint4 a0 = gload16(A.dl + (size_t)iL * 32), a1 = gload16(A.dl + (size_t)iL * 32 + 16);
+    const uint4 a0 = gload16(A.dl + (unsigned)iL * 32u), a1 = gload16(A.dl + (unsigned)iL * 32u + 16u);
     const int levelL = (int)gload4(kpb + offsetof(OrbKeyPoint, octave));
     const float vL = __uint_as_float(kxy.y), uL = __uint_as_float(kxy.x);
     const int row = (int)vL;
@@ -1603,7 +1603,7 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
                 for (int u = 0; u < ST_CHUNK; u++) {
                     const int c = c0 + u * ST_G;
                     if (c < rg.y) {
-                        const uint4 q = gload16((const uint8_t*)(A.srec + c));
+                        const uint4 q = gload16((const uint8_t*)A.srec + (unsigned)c * 16u);
                         rec[u] = make_int4((int)q.x, (int)q.y, (int)q.z, (int)q.w);
                     } else {
                         rec[u] = make_int4(1, 0, 0, 0);   // minr > maxr: never a candidate
@@ -1621,7 +1621,7 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
 #pragma unroll
                 for (int u = 0; u < ST_CHUNK; u++) {
                     if (pass[u]) {
-                        const uint8_t* d = A.sdesc + (size_t)(c0 + u * ST_G) * 32;
+                        const uint8_t* d = A.sdesc + (unsigned)(c0 + u * ST_G) * 32u;
                         b0[u] = gload16(d);
                         b1[u] = gload16(d + 16);
                     }
@@ -1679,8 +1679,8 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
         // right bytes r0..r20 (x = sxR-10 ..); lanes 11..15 repeat row 10
         // and contribute nothing.
         const int r = sub < 11 ? sub : 10;
-        const uint8_t* rl = pl + (size_t)(sy0 - 5 + r) * pitchL;
-        const uint8_t* rr = pr + (size_t)(sy0 - 5 + r) * pitchR;
+        const uint8_t* rl = pl + (unsigned)((sy0 - 5 + r) * pitchL);
+        const uint8_t* rr = pr + (unsigned)((sy0 - 5 + r) * pitchR);
         const int xl0 = (sx0 - 5) & ~3, offL = sx0 - 5 - xl0;
         const int xr0 = (sxR - 10) & ~3, offR = sxR - 10 - xr0;
         unsigned dl[4], dr[6];
