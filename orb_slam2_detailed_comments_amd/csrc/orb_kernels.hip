@@ -317,7 +317,6 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         s_nq = 0;
         s_kn = 0;
     }
-    for (int e = tid; e < LM_H * LP_WD; e += 256) mmw[e / LP_WD][e % LP_WD] = 0u;
     const ImageJob& J = jobs[job];
     const LevelJob& L = J.lv[level];
     const int t = blk - rt->tile_start[level][job];
@@ -585,6 +584,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     // pixels ..8 (columns ..64): the 1-px ring.  Their missing neighbour
     // dwords (columns -8 and 68 onwards) feed only the bytes masked off.
     static_assert(LM_H * 6 <= 256, "one pre-test pass");
+    static_assert(LP_WD == 18, "6 lanes of 3 dwords cover a strength-map row");
     {
         unsigned cand = 0u;                       // bit k: pixel k of the group
         int my = 0, g = 0;
@@ -597,6 +597,11 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             const unsigned* vr = &pxw[py][3 * g];
             const unsigned* ur = &pxw[py - 3][3 * g];
             const unsigned* dr = &pxw[py + 3][3 * g];
+            // The lanes tile the strength map exactly (34 rows x 18 dwords):
+            // each clears its 3 dwords; candidates get their strength below.
+            mmw[my][3 * g] = 0u;
+            mmw[my][3 * g + 1] = 0u;
+            mmw[my][3 * g + 2] = 0u;
             const unsigned Vx = vr[0], Vy = vr[1], Vz = vr[2];
             const unsigned Lw = g > 0 ? vr[-1] : 0u;
             const unsigned Rw = g < 5 ? vr[3] : 0u;
