@@ -326,25 +326,60 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const bool interior = tx0 >= LT_HALO && tx0 + LT_W + LT_HALO <= w && ty0 >= LT_HALO &&
                           ty0 + LT_H + LT_HALO <= h;
 
+    // FAST detection area of the level.
+    const int ax0 = ORB_EDGE, ay0 = ORB_EDGE;
+    const int ax1 = min(ORB_EDGE + L.nCols * L.wCell, L.maxBorderX - 3);
+    const int ay1 = min(ORB_EDGE + L.nRows * L.hCell, L.maxBorderY - 3);
+    // Per tile column: its FAST cell column cj, and whether the pixels left
+    // and right of it fall outside that cell's window (bits 16, 17); rows
+    // likewise.  Built while the tile's first reads are in flight; read
+    // after later barriers.
+    auto cell_tables = [&]() {
+        if (tid < LT_W + LT_H) {
+            const bool isc = tid < LT_W;
+            const int z = isc ? tx0 + tid : ty0 + (tid - LT_W);
+            const int cs = isc ? L.wCell : L.hCell, a0 = isc ? ax0 : ay0, a1 = isc ? ax1 : ay1;
+            const int cz = (int)__umulhi((unsigned)max(z - ORB_EDGE, 0), isc ? L.wrcp : L.hrcp);   // / cs
+            const int c0 = max(ORB_EDGE + cz * cs, a0), c1 = min(ORB_EDGE + (cz + 1) * cs, a1);
+            const int info = cz | ((z - 1 < c0) << 16) | ((z + 1 >= c1) << 17);
+            if (isc) s_col[tid] = info;
+            else s_row[tid - LT_W] = info;
+        }
+    };
+
     // ---- 1. pixels with halo, one dword (4 columns) per slot
     if (level == 0) {
         const uint8_t* src = J.src;
         const int sp = J.src_pitch;
-        for (int e = tid; e < LP_H * LP_WD; e += 256) {
-            const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
-            unsigned d = 0;
-            if (interior && (sp & 3) == 0) {
-                // aligned dword load (the input sits in level 0, whose rows are 64-byte aligned)
-                d = gload4(src + (unsigned)((ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex)));
-            } else if (interior) {
-                d = gload4u(src + (size_t)(ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex));
-            } else {
+        if (interior) {
+            // The 720 dwords, 3 per thread, read before any wait (indexes past
+            // the end repeat the last dword); aligned when the input sits in
+            // level 0, whose rows are 64-byte aligned.
+            unsigned d[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int e = min(tid + 256 * k, LP_H * LP_WD - 1);
+                const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
+                const unsigned off = (unsigned)((ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex));
+                d[k] = (sp & 3) == 0 ? gload4(src + off) : gload4u(src + off);
+            }
+            cell_tables();
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int e = tid + 256 * k;
+                if (e < LP_H * LP_WD) pxw[e / LP_WD][e % LP_WD] = d[k];
+            }
+        } else {
+            cell_tables();
+            for (int e = tid; e < LP_H * LP_WD; e += 256) {
+                const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
                 const int y = reflect101(ty0 - LT_HALO + ey, h);
                 const uint8_t* r = src + (size_t)y * sp;
+                unsigned d = 0;
 #pragma unroll
                 for (int k = 0; k < 4; k++) d |= (unsigned)r[reflect101(tx0 - LT_HALO + ex + k, w)] << (8 * k);
+                pxw[ey][ex >> 2] = d;
             }
-            pxw[ey][ex >> 2] = d;
         }
     } else {
         // cv::resize INTER_LINEAR 8U fixed point from level l-1: the source
@@ -384,6 +419,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 wv[k] = gload4(srow + (unsigned)(r * sp + 4 * (e - r * nwd)));
             }
         }
+        cell_tables();
         if (tid < LP_W) {
             xs0[tid] = (int)tv.x;
             xa[tid] = (int)tv.y;
@@ -559,25 +595,9 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     // 16-bit lane pair); survivors are compacted, then scored exactly.
     const int tA = min(max(J.iniTh, 0), 255), tB = min(max(J.minTh, 0), 255);
     const int tmin = min(tA, tB);
-    const int ax0 = ORB_EDGE, ay0 = ORB_EDGE;
-    const int ax1 = min(ORB_EDGE + L.nCols * L.wCell, L.maxBorderX - 3);
-    const int ay1 = min(ORB_EDGE + L.nRows * L.hCell, L.maxBorderY - 3);
     const unsigned t2 = (unsigned)tmin | ((unsigned)tmin << 16);
     // tile plus its 1-px ring wholly inside the detection area
     const bool tile_in = tx0 - 1 >= ax0 && tx0 + LT_W < ax1 && ty0 - 1 >= ay0 && ty0 + LT_H < ay1;
-    // Per tile column: its FAST cell column cj, and whether the pixels left
-    // and right of it fall outside that cell's window (bits 16, 17); rows
-    // likewise.  Read after the barrier that ends this phase.
-    if (tid < LT_W + LT_H) {
-        const bool isc = tid < LT_W;
-        const int z = isc ? tx0 + tid : ty0 + (tid - LT_W);
-        const int cs = isc ? L.wCell : L.hCell, a0 = isc ? ax0 : ay0, a1 = isc ? ax1 : ay1;
-        const int cz = (int)__umulhi((unsigned)max(z - ORB_EDGE, 0), isc ? L.wrcp : L.hrcp);   // / cs
-        const int c0 = max(ORB_EDGE + cz * cs, a0), c1 = min(ORB_EDGE + (cz + 1) * cs, a1);
-        const int info = cz | ((z - 1 < c0) << 16) | ((z + 1 >= c1) << 17);
-        if (isc) s_col[tid] = info;
-        else s_row[tid - LT_W] = info;
-    }
     // Lanes: rows tile -1..32 and 12-column groups g = 0..5 (px dwords 3g ..
     // 3g + 2, tile columns 12g - 4 .. 12g + 7): 204 lanes, one pass.  Group 0
     // only contributes its pixels 3.. (tile columns -1..) and group 5 its
