@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const int lane = tid & 63;
     // Every wave finds its tile itself (uniform scalar loads), so the first
     // barrier comes only after the tile's memory reads have been issued; the
-    // counters and mmw cleared here are first used after that barrier.
+    // counters cleared here are first used after that barrier.
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int tpj = rt->tiles_per_job[level];
     const int job = tpj > 0 ? blk / tpj : find_segment(rt->tile_start[level], rt->njobs, blk);
