@@ -347,6 +347,21 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         }
     };
 
+    // A resized dword of the tile proper (px row ey, dword c) also goes
+    // straight to the level image.
+    auto level_out = [&](int ey, int c, unsigned d) {
+        const int y = ty0 + ey - LT_HALO, x0 = tx0 + 4 * (c - 1);
+        if (ey >= LT_HALO && ey < LT_HALO + LT_H && c >= 1 && c <= LT_W / 4 && y < h) {
+            uint8_t* dst = L.lvl + (unsigned)(y * pitch + x0);
+            if (x0 + 3 < w) {
+                gstore4(dst, d);
+            } else {
+                for (int k = 0; k < 4; k++)
+                    if (x0 + k < w) dst[k] = (uint8_t)(d >> (8 * k));
+            }
+        }
+    };
+
     // ---- 1. pixels with halo, one dword (4 columns) per slot
     if (level == 0) {
         const uint8_t* src = J.src;
@@ -490,20 +505,26 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                         d |= v << (8 * k);
                     }
                     pxw[ey][c] = d;
+                    level_out(ey, c, d);
                 }
             } else if (staged) {
                 const uint8_t* wb = (const uint8_t*)u.win;
                 const int ws = nwd * 4;
                 for (int ey = r0; ey < LP_H; ey += 14) {
                     const unsigned sy01 = (unsigned)ys0[ey];
-                    pxw[ey][c] = resize4(wb, __umul24((sy01 & 0xFFFFu) - wy0, ws), __umul24((sy01 >> 16) - wy0, ws),
-                                         sxk, nxk, tap, yb[ey]);
+                    const unsigned d = resize4(wb, __umul24((sy01 & 0xFFFFu) - wy0, ws),
+                                               __umul24((sy01 >> 16) - wy0, ws), sxk, nxk, tap, yb[ey]);
+                    pxw[ey][c] = d;
+                    level_out(ey, c, d);
                 }
             } else {
                 for (int ey = r0; ey < LP_H; ey += 14) {
                     const unsigned sy01 = (unsigned)ys0[ey];
-                    pxw[ey][c] = resize4((const __attribute__((address_space(1))) uint8_t*)P.lvl,
-                                         __umul24(sy01 & 0xFFFFu, sp), __umul24(sy01 >> 16, sp), sxk, nxk, tap, yb[ey]);
+                    const unsigned d = resize4((const __attribute__((address_space(1))) uint8_t*)P.lvl,
+                                               __umul24(sy01 & 0xFFFFu, sp), __umul24(sy01 >> 16, sp), sxk, nxk, tap,
+                                               yb[ey]);
+                    pxw[ey][c] = d;
+                    level_out(ey, c, d);
                 }
             }
         }
@@ -514,7 +535,9 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     // two v_dot4_u32_u8 per output on byte quads (v_alignbyte); the vertical
     // pass on the row sums, symmetric taps.
     const int gc = tid & 15;                      // 4-column group of the tile
-    if (level > 0 || J.src != L.lvl) {
+    // Level 0 copied out when the input is not level 0 itself (levels >= 1
+    // are written by the resize).
+    if (level == 0 && J.src != L.lvl) {
         for (int r = tid >> 4; r < LT_H; r += 16) {
             const int y = ty0 + r, x0 = tx0 + 4 * gc;
             if (y >= h) continue;
