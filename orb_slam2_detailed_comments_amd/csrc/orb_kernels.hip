@@ -914,19 +914,32 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
 
     // ---- per-cell retry (ORBextractor.cc:1111-1124): a cell keeps its
     // iniThFAST candidates when it has any, else its minThFAST candidates.
+    // OCT_REG candidates per thread per round: their keys, then their cell
+    // flags, are read together, and one scan places the survivors.  (The
+    // candidates' order does not matter: the level kernels append them in
+    // no fixed order, and every later step depends only on the set.)
     int K = 0;
-    for (int c0 = 0; c0 < K0; c0 += OCT_T) {
-        const int k = c0 + tid;
-        bool v = false;
-        unsigned long long key = 0;
-        if (k < K0) {
-            key = gload8u64(L.keys + k);
-            const unsigned ok = (unsigned)key;
-            const bool flagged = gload1(L.cellflag + ((ok >> 20) & 255) * L.nCols + ((ok >> 12) & 255)) != 0;
-            v = (key & ORB_KEY_RETRY) ? !flagged : flagged;
+    for (int c0 = 0; c0 < K0; c0 += OCT_T * OCT_REG) {
+        unsigned long long key[OCT_REG];
+        unsigned char flag[OCT_REG];
+#pragma unroll
+        for (int m = 0; m < OCT_REG; m++) key[m] = gload8u64(L.keys + min(c0 + tid + OCT_T * m, K0 - 1));
+#pragma unroll
+        for (int m = 0; m < OCT_REG; m++) {
+            const unsigned ok = (unsigned)key[m];
+            flag[m] = gload1(L.cellflag + ((ok >> 20) & 255) * L.nCols + ((ok >> 12) & 255));
         }
-        const int pos = block_scan(v ? 1 : 0, S.scan, &total);
-        if (v) gstore8(L.keys2 + K + pos, key & ~ORB_KEY_RETRY);
+        int cnt = 0;
+        bool v[OCT_REG];
+#pragma unroll
+        for (int m = 0; m < OCT_REG; m++) {
+            v[m] = c0 + tid + OCT_T * m < K0 && ((key[m] & ORB_KEY_RETRY) ? flag[m] == 0 : flag[m] != 0);
+            cnt += v[m] ? 1 : 0;
+        }
+        int pos = block_scan(cnt, S.scan, &total);
+#pragma unroll
+        for (int m = 0; m < OCT_REG; m++)
+            if (v[m]) gstore8(L.keys2 + K + pos++, key[m] & ~ORB_KEY_RETRY);
         K += total;
     }
     __syncthreads();
@@ -936,11 +949,13 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
     // k = tid + OCT_T*m, m < OCT_REG, in registers for every pass; candidates
     // beyond OCT_REG * OCT_T (rare) keep their node in global memory.
     int kx[OCT_REG], ky[OCT_REG], kn[OCT_REG];
+    unsigned long long rk[OCT_REG];
+#pragma unroll
+    for (int m = 0; m < OCT_REG; m++) rk[m] = K > 0 ? gload8u64(keys + min(tid + OCT_T * m, K - 1)) : 0ull;
 #pragma unroll
     for (int m = 0; m < OCT_REG; m++) {
-        const int k = tid + OCT_T * m;
         kx[m] = ky[m] = kn[m] = 0;
-        if (k < K) key_xy(gload8u64(keys + k), wC, hC, &kx[m], &ky[m]);
+        if (tid + OCT_T * m < K) key_xy(rk[m], wC, hC, &kx[m], &ky[m]);
     }
     // f(x, y, node&) on every candidate of this thread.
     auto for_keys = [&](auto&& f) {
