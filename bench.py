@@ -270,18 +270,74 @@ def serving_extra(orb, ctx):
     return out
 
 
-def cpu_baseline(frames, seconds):
+# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz; a SIMD issues one wave64 VALU
+# instruction every 4 cycles (16 lanes wide), so the VALU issue ceiling is
+# 256 * 4 * 2.4e9 / 4 wave-instructions per second.
+VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def committed_counters(stage, stage_ms, hbm_frac):
+    """Counter-derived fields for the dominant stage, from the summaries the
+    rocprofv3 runs left in profiles/ (counters are not collected while
+    timing): L2-to-fabric traffic (FETCH_SIZE + WRITE_SIZE, which also counts
+    Infinity Cache hits, so it bounds HBM bytes from above), and the VALU
+    issue fraction = SQ_INSTS_VALU per launch x 4 cycles over this run's
+    launch time on 1024 SIMDs at 2.4 GHz, beside the profiler's VALUBusy."""
+    out = {}
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{stage}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            out["traffic"] = json.load(f).get("bytes_per_launch")
+        out["traffic_kind"] = f"L2-fabric bytes (FETCH_SIZE+WRITE_SIZE), committed profiles/traffic_{stage}.json"
+    vfile = os.path.join(ROOT, "profiles", f"valu_{stage}.json")
+    if os.path.exists(vfile):
+        with open(vfile) as f:
+            v = json.load(f)
+        insts = v["valu_insts_per_launch"]
+        frac = insts / (stage_ms * 1e-3) / VALU_WAVE_INSTS_PER_S
+        out["valu"] = {"issue_frac": round(frac, 4), "valu_insts_per_launch": insts,
+                       "cycles_per_inst": 4, "profiled_valu_busy": v.get("valu_busy"),
+                       "source": f"profiles/valu_{stage}.json"}
+        if hbm_frac is not None and frac > hbm_frac:
+            out["bound"] = "valu"
+    return out
+
+
+def parity_sample(pyoracle, frames, gpu):
+    """The oracle as checker on a sample of the timed batch: for each sampled
+    pair p, gpu[p] = (kl, dl, kr, dr, uRight, depth) read back after the
+    timed loop; keypoints, descriptors, uRight and depth must be bit-exact."""
+    b = float(np.float32(BF) / np.float32(FX))
+    L = pyoracle.OracleExtractor(NF, 1.2, NLEV, 20, 7)
+    R = pyoracle.OracleExtractor(NF, 1.2, NLEV, 20, 7)
+    bad, nkp = [], 0
+    for p, (kl, dl, kr, dr, uR, dep) in sorted(gpu.items()):
+        okl, odl, okr, odr, ouR, odep = pyoracle.stereo_frame(L, R, frames[p][0], frames[p][1], BF, b)
+        same = (len(kl) == len(okl) and len(kr) == len(okr) and
+                kl.tobytes() == okl.tobytes() and kr.tobytes() == okr.tobytes() and
+                np.array_equal(dl, odl) and np.array_equal(dr, odr) and
+                uR.tobytes() == ouR.tobytes() and dep.tobytes() == odep.tobytes())
+        nkp += len(okl) + len(okr)
+        if not same:
+            bad.append(p)
+    return {"pairs": sorted(gpu), "images": 2 * len(gpu), "keypoints": nkp,
+            "fields": "keypoints (all 28 bytes), descriptors, uRight, depth", "bit_exact": not bad,
+            "mismatched_pairs": bad}
+
+
+def cpu_baseline(frames, seconds, gpu_sample=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     cores = len(os.sched_getaffinity(0))
     b = np.float32(BF) / np.float32(FX)
+    parity = parity_sample(pyoracle, frames, gpu_sample) if gpu_sample else None
 
     def pipeline(stop, counter, idx0):
         L = pyoracle.OracleExtractor(NF, 1.2, NLEV, 20, 7)
         R = pyoracle.OracleExtractor(NF, 1.2, NLEV, 20, 7)
         i = idx0
         while not stop.is_set():
-            imL, imR = frames[i % len(frames)]
+            imL, imR = frames[i % min(len(frames), 8)]
             pyoracle.stereo_frame(L, R, imL, imR, BF, float(b))
             counter.append(1)
             i += 1
@@ -304,10 +360,10 @@ def cpu_baseline(frames, seconds):
     npipe = max(1, cores // 2)
     allc = run(npipe, seconds * 0.7)
     return {"value": round(allc, 2), "unit": "frames/s", "cores": 2 * npipe, "kind": "port",
-            "sample": f"{len(frames)} synthetic 1241x376 stereo frames cycled for ~{seconds:.0f}s: {npipe} "
-                      f"independent Frame pipelines (L/R extraction on 2 threads + stereo match) on "
+            "sample": f"{min(len(frames), 8)} synthetic 1241x376 stereo frames cycled for ~{seconds:.0f}s: "
+                      f"{npipe} independent Frame pipelines (L/R extraction on 2 threads + stereo match) on "
                       f"{cores} host cores",
-            "reference_structure_frames_per_s": round(single, 2), "host_cores": cores}
+            "reference_structure_frames_per_s": round(single, 2), "host_cores": cores}, parity
 
 
 def main():
@@ -363,26 +419,25 @@ def main():
     value = B * args.steps / (t1 - t0)
     print(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
 
-    # Result sizes of the last step (keypoints and FAST candidates per image).
+    # Result sizes of the last step (keypoints and FAST candidates per image),
+    # and the outputs of six pairs spread over the batch (and so over the
+    # graph's branches), kept for the oracle's check after the CPU timing.
     kps = [len(batch.results(s)[0]) for s in range(2 * B)]
     matched = [int((batch.stereo_results(p)[0] >= 0).sum()) for p in range(B)]
+    sample = sorted({round(i * (B - 1) / 5) for i in range(6)})
+    gpu_sample = {p: batch.results(2 * p) + batch.results(2 * p + 1) + batch.stereo_results(p) for p in sample}
 
     stages = batch.stage_times(args.steps)
     dom = max((k for k in stages if k != "stereo_median"), key=lambda k: stages[k])
     cands = None
     bytes_dom = stage_bytes(dom, 2 * B, float(np.mean(kps)), 10 * NF if cands is None else cands)
     achieved = bytes_dom / (stages[dom] * 1e-3) / 1e9 if bytes_dom else None
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2) if achieved else None,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
-                "traffic": traffic, "avg_launch_ms": round(stages[dom], 4),
+                "traffic": None, "avg_launch_ms": round(stages[dom], 4),
                 "algorithmic_bytes_per_launch": bytes_dom,
                 "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
-
+    roofline.update(committed_counters(dom, stages[dom], roofline["frac"]))
     out = {"metric": METRIC, "value": round(value, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "u8",
@@ -414,11 +469,14 @@ def main():
                                                         0.0 if args.no_cpu else min(args.cpu_seconds, 3.0))
         print("[bench] windowed matching", file=sys.stderr, flush=True)
         out["search_for_initialization"] = winmatch_extra(orb, ctx, 0.0 if args.no_cpu else min(args.cpu_seconds, 3.0))
+    out["parity_checked"] = None
     if not args.no_cpu:
-        print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(frames[:8], args.cpu_seconds)
+        print("[bench] checking sampled pairs against the oracle, then timing it", file=sys.stderr, flush=True)
+        cpu, parity = cpu_baseline(frames, args.cpu_seconds, gpu_sample)
         out["cpu_baseline"] = cpu
         out["gpu_over_cpu"] = round(value / cpu["value"], 1) if cpu["value"] else None
+        out["parity_checked"] = parity
+        print(f"[bench] parity: {parity}", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -154,7 +154,8 @@ int orb_knn2_time(orb_context* ctx, const uint8_t* query, int nq, const uint8_t*
 /* Frame::ComputeStereoMatches() (include/Frame.h:249, src/Frame.cc:831-1082)
  * on keypoints/descriptors of a left and a right image, reading the two
  * extractors' pyramids from their last calls.  bf = Camera.bf, b = bf/fx.
- * uRight/depth receive nl floats each (-1 where unmatched). */
+ * uRight/depth receive nl floats each (-1 where unmatched).  Both extractors
+ * must belong to the same context (ORB_E_ARG otherwise). */
 int orb_stereo_match(orb_extractor* left, orb_extractor* right, const orb_keypoint* kl, const uint8_t* dl,
                      int nl, const orb_keypoint* kr, const uint8_t* dr, int nr, float bf, float b, float* uRight,
                      float* depth);

@@ -58,7 +58,14 @@ struct Image {
 // FAST score, and the position in vToDistributeKeys (its insertion order).
 struct Cand {
     float x, y, response;
+    int id = -1;  // position in the caller's candidate list (leaf export only)
 };
+
+// Planted mistakes, one family per oracle function, so that a test can show
+// its checks reject a plausible wrong statement.  All zero = the faithful
+// statement; only oracle_plant changes them.
+enum { PLANT_ATAN = 1, PLANT_RETAIN = 2, PLANT_STEREO = 3, PLANT_FAMILIES = 4 };
+int g_plant[PLANT_FAMILIES] = {0, 0, 0, 0};
 
 // ------------------------------------------------------------ cv::resize
 // cv::resize(src, dst, dsize, 0, 0, INTER_LINEAR) for CV_8UC1, OpenCV
@@ -263,16 +270,27 @@ const float kAtanP7 = -0.04432655554792128f * (float)(180 / kCvPi);
 
 float fast_atan2(float y, float x)
 {
+    if (g_plant[PLANT_ATAN] == 1) {  // planted: libm atan2
+        const float d = (float)(std::atan2((double)y, (double)x) * (180 / kCvPi));
+        return d < 0 ? d + 360.f : d;
+    }
+    // planted 3: two digits of p3's decimal expansion transposed
+    const float p3 = g_plant[PLANT_ATAN] == 3 ? -0.3258038974640975f * (float)(180 / kCvPi) : kAtanP3;
     const float ax = std::fabs(x), ay = std::fabs(y);
     float a, c, c2;
     if (ax >= ay) {
         c = ay / (ax + (float)DBL_EPSILON);
         c2 = c * c;
-        a = (((kAtanP7 * c2 + kAtanP5) * c2 + kAtanP3) * c2 + kAtanP1) * c;
+        a = (((kAtanP7 * c2 + kAtanP5) * c2 + p3) * c2 + kAtanP1) * c;
     } else {
         c = ax / (ay + (float)DBL_EPSILON);
         c2 = c * c;
-        a = 90.f - (((kAtanP7 * c2 + kAtanP5) * c2 + kAtanP3) * c2 + kAtanP1) * c;
+        a = 90.f - (((kAtanP7 * c2 + kAtanP5) * c2 + p3) * c2 + kAtanP1) * c;
+    }
+    if (g_plant[PLANT_ATAN] == 2) {  // planted: quadrant fix-ups in the other order
+        if (y < 0) a = 360.f - a;
+        if (x < 0) a = 180.f - a;
+        return a;
     }
     if (x < 0) a = 180.f - a;
     if (y < 0) a = 360.f - a;
@@ -442,8 +460,11 @@ struct Node {
 };
 
 // ORBextractor::DistributeOctTree (ORBextractor.cc:688-1033).
+// leafRect/leafOf (optional) receive each kept node's rectangle (ulx, uly,
+// urx, bry) in list order and, per input candidate, the node it ended in.
 std::vector<Cand> distribute_octree(const std::vector<Cand>& toDistribute, int minX, int maxX, int minY,
-                                    int maxY, int N)
+                                    int maxY, int N, std::vector<int>* leafRect = nullptr,
+                                    std::vector<int>* leafOf = nullptr)
 {
     const int nIni = (int)std::round((float)(maxX - minX) / (maxY - minY));
     if (nIni <= 0) return {};  // the reference divides by zero here
@@ -536,14 +557,22 @@ std::vector<Cand> distribute_octree(const std::vector<Cand>& toDistribute, int m
     // ties (strict >, ORBextractor.cc:1004-1029).
     std::vector<Cand> result;
     result.reserve(lNodes.size());
+    if (leafOf) leafOf->assign(toDistribute.size(), -1);
     for (const Node& n : lNodes) {
         const Cand* best = &n.keys[0];
         float maxResponse = best->response;
-        for (size_t k = 1; k < n.keys.size(); k++)
-            if (n.keys[k].response > maxResponse) {
+        for (size_t k = 1; k < n.keys.size(); k++) {
+            const bool better = g_plant[PLANT_RETAIN] == 1 ? n.keys[k].response >= maxResponse  // planted
+                                                           : n.keys[k].response > maxResponse;
+            if (better) {
                 best = &n.keys[k];
                 maxResponse = n.keys[k].response;
             }
+        }
+        if (leafRect) leafRect->insert(leafRect->end(), {n.ulx, n.uly, n.urx, n.bry});
+        if (leafOf)
+            for (const Cand& k : n.keys)
+                if (k.id >= 0 && k.id < (int)leafOf->size()) (*leafOf)[k.id] = (int)result.size();
         result.push_back(*best);
     }
     return result;
@@ -778,6 +807,7 @@ static int stereo_match(oracle_extractor* L, oracle_extractor* R, const oracle_k
     if (nl == 0) return 0;
     const int TH_HIGH = 100, TH_LOW = 50;
     const int thOrbDist = (TH_HIGH + TH_LOW) / 2;
+    const int plant = g_plant[PLANT_STEREO];
     const int nRows = L->pyramid[0].h;
     std::vector<std::vector<size_t>> rowIdx(nRows);
     for (int iR = 0; iR < nr; iR++) {
@@ -796,7 +826,7 @@ static int stereo_match(oracle_extractor* L, oracle_extractor* R, const oracle_k
         const oracle_keypoint& kpL = kl[iL];
         const int levelL = kpL.octave;
         const float vL = kpL.y, uL = kpL.x;
-        const size_t row = (size_t)vL;
+        const size_t row = plant == 7 ? (size_t)std::round(vL) : (size_t)vL;  // planted 7: rounded row
         if (row >= (size_t)nRows) continue;
         const std::vector<size_t>& cands = rowIdx[row];
         if (cands.empty()) continue;
@@ -809,15 +839,15 @@ static int stereo_match(oracle_extractor* L, oracle_extractor* R, const oracle_k
             const oracle_keypoint& kpR = kr[iR];
             if (kpR.octave < levelL - 1 || kpR.octave > levelL + 1) continue;
             const float uR = kpR.x;
-            if (uR >= minU && uR <= maxU) {
+            if (uR >= minU && (plant == 8 ? uR < maxU : uR <= maxU)) {  // planted 8: open at uL
                 const int dist = descriptor_distance(dl + 32 * (size_t)iL, dr + 32 * iR);
-                if (dist < bestDist) {
+                if (plant == 11 ? dist <= bestDist : dist < bestDist) {  // planted 11: last tie wins
                     bestDist = dist;
                     bestIdxR = iR;
                 }
             }
         }
-        if (bestDist >= thOrbDist) continue;
+        if (plant == 10 ? bestDist > thOrbDist : bestDist >= thOrbDist) continue;  // planted 10: <= 75 kept
         const float uR0 = kr[bestIdxR].x;
         const float scaleFactor = L->invScale[kpL.octave];
         const float scaleduL = std::round(kpL.x * scaleFactor);
@@ -834,7 +864,8 @@ static int stereo_match(oracle_extractor* L, oracle_extractor* R, const oracle_k
         float vDists[2 * Lr + 1];
         const float iniu = scaleduR0 - Lr - w;
         const float endu = scaleduR0 + Lr + w + 1;
-        if (iniu < 0 || endu >= imR.w) continue;
+        if ((plant == 1 ? iniu <= 0 : iniu < 0) || (plant == 9 ? endu > imR.w : endu >= imR.w))
+            continue;  // planted 1 / 9: off-by-one bounds
         for (int incR = -Lr; incR <= Lr; incR++) {
             const int xc = xR0 + incR;
             const int cR = imR.row(yL)[xc];
@@ -853,15 +884,17 @@ static int stereo_match(oracle_extractor* L, oracle_extractor* R, const oracle_k
             vDists[Lr + incR] = dist;
         }
         if (bestincR == -Lr || bestincR == Lr) continue;
+        if (plant == 2 && (bestincR == -Lr + 1 || bestincR == Lr - 1)) continue;  // planted 2: +-(L-1) too
         const float dist1 = vDists[Lr + bestincR - 1];
         const float dist2 = vDists[Lr + bestincR];
         const float dist3 = vDists[Lr + bestincR + 1];
-        const float deltaR = (dist1 - dist3) / (2.0f * (dist1 + dist3 - 2.0f * dist2));
+        const float deltaR = plant == 3 ? (dist3 - dist1) / (2.0f * (dist1 + dist3 - 2.0f * dist2))  // planted
+                                        : (dist1 - dist3) / (2.0f * (dist1 + dist3 - 2.0f * dist2));
         if (deltaR < -1 || deltaR > 1) continue;
         float bestuR = L->scale[kpL.octave] * ((float)scaleduR0 + (float)bestincR + deltaR);
         float disparity = (uL - bestuR);
         if (disparity >= minD && disparity < maxD) {
-            if (disparity <= 0) {
+            if (disparity <= 0 && plant != 4) {  // planted 4: no 0.01 step
                 disparity = 0.01;
                 bestuR = uL - 0.01;
             }
@@ -874,10 +907,11 @@ static int stereo_match(oracle_extractor* L, oracle_extractor* R, const oracle_k
     // (Frame.cc:1070); with no matches there is nothing to cut.
     if (distIdx.empty()) return 0;
     std::sort(distIdx.begin(), distIdx.end());
-    const float median = (float)distIdx[distIdx.size() / 2].first;
+    const size_t mid = plant == 6 ? (distIdx.size() - 1) / 2 : distIdx.size() / 2;  // planted 6: lower median
+    const float median = (float)distIdx[mid].first;
     const float thDist = 1.5f * 1.4f * median;
     for (int i = (int)distIdx.size() - 1; i >= 0; i--) {
-        if (distIdx[i].first < thDist) break;
+        if (plant == 5 ? distIdx[i].first <= thDist : distIdx[i].first < thDist) break;  // planted 5
         uRight[distIdx[i].second] = -1;
         depth[distIdx[i].second] = -1;
     }
@@ -955,6 +989,27 @@ int oracle_level_candidates(const oracle_extractor* ex, int level, int* xyr, int
         xyr[3 * i + 1] = (int)c[i].y;
         xyr[3 * i + 2] = (int)c[i].response;
     }
+    return 0;
+}
+
+int oracle_distribute_octree_leaves(const int* xyr, int n, int minX, int maxX, int minY, int maxY, int N,
+                                    int* rect4, int cap, int* leaf_of, int* nleaves)
+{
+    std::vector<Cand> in(n);
+    for (int i = 0; i < n; i++) in[i] = Cand{(float)xyr[3 * i], (float)xyr[3 * i + 1], (float)xyr[3 * i + 2], i};
+    std::vector<int> rects, of;
+    const std::vector<Cand> r = distribute_octree(in, minX, maxX, minY, maxY, N, &rects, &of);
+    *nleaves = (int)r.size();
+    if ((int)r.size() > cap) return -2;
+    std::copy(rects.begin(), rects.end(), rect4);
+    std::copy(of.begin(), of.end(), leaf_of);
+    return 0;
+}
+
+int oracle_plant(int family, int variant)
+{
+    if (family <= 0 || family >= PLANT_FAMILIES || variant < 0 || variant > 15) return -1;
+    g_plant[family] = variant;
     return 0;
 }
 

@@ -71,6 +71,17 @@ int oracle_level_candidates(const oracle_extractor* ex, int level, int* xyr, int
 int oracle_distribute_octree(const int* xyr, int n, int minX, int maxX, int minY, int maxY, int N, int* out,
                              int cap, int* nout);
 
+/* The same, exporting the kept nodes: rect4 gets (ulx, uly, urx, bry) per
+ * node in list order (node i produced output i), leaf_of[j] the node holding
+ * input candidate j. */
+int oracle_distribute_octree_leaves(const int* xyr, int n, int minX, int maxX, int minY, int maxY, int N,
+                                    int* rect4, int cap, int* leaf_of, int* nleaves);
+
+/* Test hook: plant a deliberate mistake in one oracle function (family 1:
+ * fastAtan2, 2: DistributeOctTree retain, 3: ComputeStereoMatches), so a
+ * test can show that its checks reject it.  variant 0 = faithful. */
+int oracle_plant(int family, int variant);
+
 /* Stand-alone primitives. */
 int oracle_resize_linear(const uint8_t* src, int sw, int sh, int sstride,
                          uint8_t* dst, int dw, int dh);

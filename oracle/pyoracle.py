@@ -41,6 +41,8 @@ def lib():
             "oracle_level_candidates": (I, [P, I, P, I, P]),
             "oracle_resize_linear": (I, [P, I, I, I, P, I, I]),
             "oracle_distribute_octree": (I, [P, I, I, I, I, I, I, P, I, P]),
+            "oracle_distribute_octree_leaves": (I, [P, I, I, I, I, I, I, P, I, P, P]),
+            "oracle_plant": (I, [I, I]),
             "oracle_gaussian7": (I, [P, I, I, I, P]),
             "oracle_gaussian7_kernel": (I, [P]),
             "oracle_fast": (I, [P, I, I, I, I, P, I, P]),
@@ -149,6 +151,41 @@ def distribute_octree(xyr, minX, maxX, minY, maxY, N):
     if rc != 0:
         raise RuntimeError(rc)
     return out[:n.value].copy()
+
+
+def distribute_octree_leaves(xyr, minX, maxX, minY, maxY, N):
+    """Kept nodes of DistributeOctTree: (n_leaves x 4) rects (ulx, uly, urx,
+    bry) in list order, and the node index holding each input candidate."""
+    xyr = np.ascontiguousarray(xyr, np.int32).reshape(-1, 3)
+    cap = len(xyr) + 64
+    rects = np.zeros((cap, 4), np.int32)
+    leaf_of = np.full(len(xyr), -1, np.int32)
+    n = ctypes.c_int()
+    rc = lib().oracle_distribute_octree_leaves(_p(xyr), len(xyr), minX, maxX, minY, maxY, N, _p(rects), cap,
+                                               _p(leaf_of), ctypes.byref(n))
+    if rc != 0:
+        raise RuntimeError(rc)
+    return rects[:n.value].copy(), leaf_of
+
+
+PLANT_ATAN, PLANT_RETAIN, PLANT_STEREO = 1, 2, 3
+
+
+class planted:
+    """Context manager: run the oracle with a deliberate mistake planted in
+    one function (see oracle_plant), restoring the faithful statement after."""
+
+    def __init__(self, family, variant):
+        self.family, self.variant = family, variant
+
+    def __enter__(self):
+        if lib().oracle_plant(self.family, self.variant) != 0:
+            raise ValueError((self.family, self.variant))
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_plant(self.family, 0)
+        return False
 
 
 def resize_linear(src, dw, dh):

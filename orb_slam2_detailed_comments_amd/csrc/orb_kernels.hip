@@ -897,7 +897,7 @@ __device__ __forceinline__ void child_rect(int rx, int ry, int q, int* crx, int*
     *cry = cy0 | (cy1 << 16);
 }
 
-__global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restrict__ jobs, int max_levels)
+__global__ __launch_bounds__(OCT_T, 8) void octree_kernel(const ImageJob* __restrict__ jobs, int max_levels)
 {
     __shared__ OctShared S;
     const int tid = threadIdx.x;
@@ -1129,11 +1129,13 @@ __global__ __launch_bounds__(OCT_T) void octree_kernel(const ImageJob* __restric
         const int jstar = S.vars[0];
         const bool proc = j <= jstar && j < nE;
         // Kept list nodes: all but the processed ones (place <= jstar).  One
-        // scan for the processed nodes' children (< 2^12) and the kept nodes.
+        // scan for the processed nodes' children (<= 4 * ORB_LIST_CAP < 2^16)
+        // and the kept nodes (<= ORB_LIST_CAP).
+        static_assert(4 * ORB_LIST_CAP < (1 << 16) && ORB_LIST_CAP <= OCT_T, "octree list packing");
         const bool kp = tid < size && S.keep[tid] > jstar;
         int tot;
-        const int ex = block_scan((proc ? nc : 0) | ((kp ? 1 : 0) << 12), S.scan, &tot);
-        const int ex_nc = ex & 0xFFF, ex_kp = ex >> 12, Cp = tot & 0xFFF, nkeep = tot >> 12;
+        const int ex = block_scan((proc ? nc : 0) | ((kp ? 1 : 0) << 16), S.scan, &tot);
+        const int ex_nc = ex & 0xFFFF, ex_kp = ex >> 16, Cp = tot & 0xFFFF, nkeep = tot >> 16;
         const int nsize = Cp + nkeep;
         if (nsize > ORB_LIST_CAP) {
             if (tid == 0) atomicOr(&ctr[ORB_CNT_ERR], ORB_ERR_LIST);

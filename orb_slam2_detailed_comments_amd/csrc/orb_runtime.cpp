@@ -280,8 +280,10 @@ int orb_stereo_match(orb_extractor* left, orb_extractor* right, const orb_keypoi
     if (!left || !right || nl < 0 || nr < 0 || (nl && (!kl || !dl || !uRight || !depth)) || (nr && (!kr || !dr)))
         return ORB_E_ARG;
     if (nl == 0) return ORB_OK;
+    // both pyramids are read on the left context's stream, so the right one
+    // must have been written on that stream too
     if (!left->d_mem || !right->d_mem || left->plan.W != right->plan.W || left->plan.H != right->plan.H ||
-        left->nlevels != right->nlevels)
+        left->nlevels != right->nlevels || left->ctx != right->ctx)
         return ORB_E_ARG;
     for (int i = 0; i < nl; i++)
         if (kl[i].octave < 0 || kl[i].octave >= left->nlevels) return ORB_E_ARG;

@@ -12,7 +12,7 @@
 #define ORB_MAX_LEVELS 16
 #define ORB_DESC_KPB 16             // keypoints per describe workgroup (16 lanes each)
 #define ORB_MAX_ROOTS 16
-#define ORB_LIST_CAP 512       // quadtree list capacity per level (LDS)
+#define ORB_LIST_CAP 1024      // quadtree list capacity per level (LDS; one node per octree thread)
 #define ORB_TILE_W 64          // level kernel output tile
 #define ORB_TILE_H 32
 #define ORB_EDGE 19            // EDGE_THRESHOLD, ORBextractor.cc:81

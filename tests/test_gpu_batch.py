@@ -126,3 +126,54 @@ def test_batch_stereo_branches(orb, oracle, left, right):
     orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
     okl, odl, okr, odr, ouR, odep = oracle.stereo_frame(ol, orr, l, r, bf, b)
     np.testing.assert_array_equal(want[0][0].view(np.uint32), ouR.view(np.uint32))
+
+
+def test_mixed_batch_64_frames_graph_autotuned(orb, oracle):
+    """The mixed-size batch at its full size: 64 frames, 16 each of 640x480,
+    752x480 (1000 features), 1241x376 and 1280x720 (2000 features), replayed
+    from the graph with the autotuner's branch count; every slot bit-exact."""
+    slots = [SIZES[i // 16] for i in range(64)]
+    imgs = [synth.shapes_frame(w, h, 3000 + i) for i, (w, h, _) in enumerate(slots)]
+    batch = orb.Batch([(w, h) for w, h, _ in slots], [nf for _, _, nf in slots])
+    for i, im in enumerate(imgs):
+        batch.upload(i, im)
+    best, _ = batch.autotune(steps=3)
+    for _ in range(3):
+        batch.run(1)
+    batch.synchronize()
+    oracles = {nf: oracle.OracleExtractor(nf, 1.2, 8, 20, 7) for nf in (1000, 2000)}
+    for s, ((w, h, nf), im) in enumerate(zip(slots, imgs)):
+        ok, od = oracles[nf](im)
+        gk, gd = batch.results(s)
+        assert len(gk) == len(ok), (best, s, len(gk), len(ok))
+        np.testing.assert_array_equal(gk.view(np.uint8), ok.view(np.uint8), err_msg=f"branches {best} slot {s}")
+        np.testing.assert_array_equal(gd, od, err_msg=f"branches {best} slot {s} descriptors")
+
+
+def test_stereo_batch_32_pairs_graph_autotuned(orb, oracle):
+    """The timed stereo batch at its full size: 32 pairs of 1241x376 (the
+    frames the benchmark generates), graph replay with the autotuned branch
+    count; every pair's keypoints, descriptors, uRight and depth bit-exact."""
+    bf, fx = 386.1, 718.856
+    b = float(np.float32(bf) / np.float32(fx))
+    frames = [synth.stereo_pair(1241, 376, 1000 + i)[:2] for i in range(32)]
+    batch = orb.Batch([(1241, 376)] * 64, 2000)
+    for i, (l, r) in enumerate(frames):
+        batch.upload(2 * i, l)
+        batch.upload(2 * i + 1, r)
+    batch.set_stereo(list(range(0, 64, 2)), list(range(1, 64, 2)), bf, b)
+    best, _ = batch.autotune(steps=3)
+    for _ in range(3):
+        batch.run(1)
+    batch.synchronize()
+    ol = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    for p, (l, r) in enumerate(frames):
+        okl, odl, okr, odr, ouR, odep = oracle.stereo_frame(ol, orr, l, r, bf, b)
+        gkl, gdl = batch.results(2 * p)
+        gkr, gdr = batch.results(2 * p + 1)
+        uR, dep = batch.stereo_results(p)
+        assert gkl.tobytes() == okl.tobytes() and gkr.tobytes() == okr.tobytes(), (best, p)
+        assert np.array_equal(gdl, odl) and np.array_equal(gdr, odr), (best, p)
+        np.testing.assert_array_equal(uR.view(np.uint32), ouR.view(np.uint32), err_msg=f"pair {p}")
+        np.testing.assert_array_equal(dep.view(np.uint32), odep.view(np.uint32), err_msg=f"pair {p}")

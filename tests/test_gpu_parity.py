@@ -146,3 +146,56 @@ def test_stereo_no_matches(orb):
     kl, dl, kr, dr, uR, dep = orb.stereo_frame(gl, gr, L, R, 386.1, 386.1 / 718.856)
     assert len(kr) == 0 and len(kl) > 0
     assert np.all(uR == -1) and np.all(dep == -1)
+
+
+# Parameter sets beyond the KITTI defaults.  Scale factors 1.5, 2.0 and 2.5
+# drive the level kernel's three resize forms (narrow staged window, the
+# global-tap form above ~1.9, the wide staged window), at sizes whose every
+# level still has a FAST cell row (the reference divides by zero otherwise,
+# ORBextractor.cc:1067-1071) and whose largest level quota fits the quadtree
+# list (quota + 3 <= 1024).
+PARAMS = [
+    (640, 480, 1000, 1.5, 4, 20, 7), (1241, 376, 2000, 1.5, 4, 20, 7),
+    (640, 480, 1000, 2.0, 3, 20, 7), (1241, 376, 1500, 2.0, 3, 20, 7),
+    (640, 480, 1000, 2.5, 3, 20, 7), (1241, 376, 1200, 2.5, 2, 20, 7),
+    (640, 480, 1000, 1.2, 1, 20, 7), (1241, 376, 1000, 1.2, 1, 20, 7),
+    (640, 480, 1500, 1.2, 12, 20, 7), (1241, 376, 2000, 1.2, 10, 20, 7),
+    (1241, 376, 2000, 1.2, 8, 30, 10), (640, 480, 1000, 1.2, 8, 12, 5),
+]
+
+
+@pytest.mark.parametrize("w,h,nf,sf,nl,ini,mn", PARAMS)
+def test_extract_parameter_sets(orb, oracle, w, h, nf, sf, nl, ini, mn):
+    img = synth.shapes_frame(w, h, 40 + nl)
+    g = orb.ORBextractor(nf, sf, nl, ini, mn)
+    o = oracle.OracleExtractor(nf, sf, nl, ini, mn)
+    gk, gd = g(img)
+    ok, od = o(img)
+    for level in range(nl):
+        np.testing.assert_array_equal(g.pyramid_level(level), o.level_image(level), err_msg=f"level {level}")
+        np.testing.assert_array_equal(g.debug_image(level, 1), o.level_image(level, blurred=True),
+                                      err_msg=f"blur {level}")
+    _assert_same_keypoints(gk, gd, ok, od)
+    assert len(gk) > 0
+
+
+def test_extract_monocular_init_4000(orb, oracle):
+    """Tracking's monocular initialiser runs ORBextractor(2*nFeatures, ...)
+    (Tracking.cc:187-188): 4000 features on the KITTI yaml's 2000, a level-0
+    quota of 869."""
+    img = synth.shapes_frame(1241, 376, 17)
+    g = orb.ORBextractor(4000, 1.2, 8, 20, 7)
+    assert g.quotas()[0] == 869
+    o = oracle.OracleExtractor(4000, 1.2, 8, 20, 7)
+    gk, gd = g(img)
+    ok, od = o(img)
+    _assert_same_keypoints(gk, gd, ok, od)
+    assert len(gk) > 3000
+
+
+def test_nfeatures_bound(orb):
+    """The quadtree list holds 1024 nodes per level: quota + 3 <= 1024, i.e.
+    nfeatures up to 4703 at scale 1.2 over 8 levels; above, ORB_E_ARG."""
+    assert orb.ORBextractor(4703, 1.2, 8, 20, 7).capacity(1241, 376) > 0
+    with pytest.raises(orb.OrbError):
+        orb.ORBextractor(4704, 1.2, 8, 20, 7).capacity(1241, 376)

@@ -200,22 +200,75 @@ def test_fast_known_answers(oracle):
 
 
 # -------------------------------------------------- fastAtan2, cos/sin
-def test_fast_atan2_accuracy_and_quadrants(oracle):
-    assert oracle.fast_atan2(0.0, 1.0) == 0.0
-    assert abs(oracle.fast_atan2(1.0, 0.0) - 90.0) < 1e-4
-    assert abs(oracle.fast_atan2(0.0, -1.0) - 180.0) < 1e-4
-    assert abs(oracle.fast_atan2(-1.0, 0.0) - 270.0) < 1e-4
+def _fast_atan2_numpy(y, x):
+    """OpenCV 2.4/3.x fastAtan2 (core/src/mathfuncs.cpp; hal fastAtan32f in
+    3.x), vectorised in numpy float32: every operation is one IEEE float op,
+    as in the unfused C statement the reference links against
+    (ORBextractor.cc:140)."""
+    f = np.float32
+    k = f(180 / math.pi)                        # (float)(180/CV_PI)
+    p1, p3 = f(0.9997878412794807) * k, f(-0.3258083974640975) * k
+    p5, p7 = f(0.1555786518463281) * k, f(-0.04432655554792128) * k
+    y, x = np.asarray(y, f), np.asarray(x, f)
+    ax, ay = np.abs(x), np.abs(y)
+    eps = f(np.finfo(np.float64).eps)           # (float)DBL_EPSILON
+    first = ax >= ay
+    c = np.where(first, ay / (ax + eps), ax / (ay + eps)).astype(f)
+    c2 = c * c
+    poly = (((p7 * c2 + p5) * c2 + p3) * c2 + p1) * c
+    a = np.where(first, poly, f(90) - poly).astype(f)
+    a = np.where(x < 0, f(180) - a, a).astype(f)
+    a = np.where(y < 0, f(360) - a, a).astype(f)
+    return a
+
+
+def _atan2_moment_pairs():
+    """>= 10^5 integer moment pairs (m01, m10) of the sizes IC_Angle produces
+    (|m| < 15 * 255 * 721), plus the axes, diagonals and near-diagonals."""
     rng = np.random.default_rng(2)
-    worst = 0.0
-    for y, x in rng.integers(-20000, 20000, (5000, 2)):
-        if x == 0 and y == 0:
-            continue
-        ref = math.degrees(math.atan2(y, x)) % 360.0
-        got = oracle.fast_atan2(float(y), float(x))
-        assert 0.0 <= got <= 360.0
-        err = min(abs(got - ref), 360.0 - abs(got - ref))
-        worst = max(worst, err)
-    assert worst < 0.02, worst   # the degree-7 polynomial is good to ~0.01 deg
+    big = rng.integers(-2_800_000, 2_800_001, (60_000, 2))
+    small = rng.integers(-300, 301, (30_000, 2))
+    mid = rng.integers(-40_000, 40_001, (20_000, 2))
+    v = np.arange(1, 2001)
+    special = [np.stack([v, v], 1), np.stack([-v, v], 1), np.stack([v, -v], 1), np.stack([-v, -v], 1),
+               np.stack([v, v + 1], 1), np.stack([v + 1, -v], 1), np.stack([v, 0 * v], 1),
+               np.stack([0 * v, -v], 1), np.stack([-v, 0 * v], 1), np.stack([0 * v, v], 1)]
+    return np.concatenate([big, small, mid] + special).astype(np.float32)
+
+
+def _fast_atan2_mismatches(oracle, pairs, want):
+    got = np.array([oracle.fast_atan2(float(y), float(x)) for y, x in pairs], np.float32)
+    return int(np.sum(got.view(np.uint32) != want.view(np.uint32)))
+
+
+def test_fast_atan2_bit_exact_against_independent_statement(oracle):
+    pairs = _atan2_moment_pairs()
+    assert len(pairs) >= 100_000
+    want = _fast_atan2_numpy(pairs[:, 0], pairs[:, 1])
+    assert _fast_atan2_mismatches(oracle, pairs, want) == 0
+    # known values of OpenCV's polynomial: 45 deg comes out as 44.990456 (the
+    # minimax fit's error at c = 1); exact 0 / 90 / 180 / 270 on the axes
+    assert abs(oracle.fast_atan2(1.0, 1.0) - 44.990456) < 2e-6
+    assert abs(oracle.fast_atan2(-1.0, -1.0) - (180 + 44.990456)) < 2e-5
+    assert oracle.fast_atan2(0.0, 1.0) == 0.0 and oracle.fast_atan2(0.0, 0.0) == 0.0
+    assert oracle.fast_atan2(1.0, 0.0) == 90.0 and oracle.fast_atan2(0.0, -1.0) == 180.0
+    assert oracle.fast_atan2(-1.0, 0.0) == 270.0
+    # the polynomial's own error against atan2 is ~1e-2 deg, ten times the
+    # 1e-3 deg bar: a libm substitute is not within tolerance
+    ref = np.degrees(np.arctan2(pairs[:, 0].astype(np.float64), pairs[:, 1].astype(np.float64))) % 360
+    err = np.abs(want - ref)
+    err = np.minimum(err, 360 - err)
+    assert 5e-3 < err.max() < 1.2e-2
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_fast_atan2_check_rejects_planted_mistakes(oracle, variant):
+    """1: libm atan2; 2: quadrant fix-ups swapped; 3: two digits of p3 transposed."""
+    pairs = _atan2_moment_pairs()[::7]
+    want = _fast_atan2_numpy(pairs[:, 0], pairs[:, 1])
+    with oracle.planted(oracle.PLANT_ATAN, variant):
+        assert _fast_atan2_mismatches(oracle, pairs, want) > 0
+    assert _fast_atan2_mismatches(oracle, pairs, want) == 0
 
 
 def test_canonical_cos_sin_is_correctly_rounded(oracle):
@@ -338,6 +391,77 @@ def test_distribute_octree_matches_python(oracle, n, N, seed, w, h):
     if n:
         assert min(n, N) <= len(got) <= max(N + 3, 4 * round(w / h))
         assert len({tuple(k) for k in got}) == len(got)
+
+
+def _leaf_violations(keys, rects, leaf_of, out, N, w, h):
+    """Checks of the kept node list that any correct DistributeOctTree
+    satisfies (ORBextractor.cc:688-1033):
+      * every candidate lies in exactly one kept node, and every node holds
+        at least one (empty children are never pushed, :852-875);
+      * nodes are pairwise disjoint rectangles, and a node's candidates lie in
+        its rectangle (x may reach past a root's right edge by < 1 px: roots
+        take keys by x / hX, their rects end at (int)(hX * (i + 1)), :717-740);
+      * output i is node i's maximum-response candidate, the first one in
+        candidate order on ties (strict >, :1004-1029);
+      * at most N + 3 nodes (each phase-2 split adds at most 3, :982)."""
+    bad = []
+    n = len(keys)
+    if n == 0:
+        return [] if len(out) == 0 else ["output without candidates"]
+    if np.any(leaf_of < 0):
+        bad.append("candidate in no node")
+    counts = np.bincount(leaf_of[leaf_of >= 0], minlength=len(rects))
+    if len(out) != len(rects) or np.any(counts == 0):
+        bad.append("empty node or node/output count mismatch")
+    for i in range(len(rects)):
+        for j in range(i + 1, len(rects)):
+            a, b = rects[i], rects[j]
+            if a[0] < b[2] and b[0] < a[2] and a[1] < b[3] and b[1] < a[3]:
+                bad.append(f"nodes {i} and {j} overlap")
+    for k in range(n):
+        r = rects[leaf_of[k]]
+        x, y = keys[k, 0], keys[k, 1]
+        if not (r[0] <= x < r[2] + 1 and r[1] <= y <= r[3]):
+            bad.append(f"candidate {k} outside its node")
+    for i in range(len(rects)):
+        members = np.nonzero(leaf_of == i)[0]
+        if len(members) == 0:
+            continue
+        first_max = members[np.argmax(keys[members, 2])]   # argmax returns the first maximum
+        if not np.array_equal(out[i], keys[first_max]):
+            bad.append(f"output {i} is not its node's first maximum")
+    n_ini = max(1, int(math.floor(float(np.float32(np.float32(w) / np.float32(h))) + 0.5)))
+    if len(out) > max(N + 3, 4 * n_ini):      # the first pass alone may split every root
+        bad.append("more than N + 3 nodes")
+    return bad
+
+
+def _tie_heavy_keys(n, w, h, seed):
+    rng = np.random.default_rng(seed)
+    pts = set()
+    while len(pts) < n:
+        pts.add((int(rng.integers(3, w - 3)), int(rng.integers(3, h - 3))))
+    pts = sorted(pts, key=lambda p: (p[1] // 30, p[0] // 30, p[1], p[0]))
+    return np.array([(x, y, int(rng.integers(7, 11))) for x, y in pts], np.int64).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("n,N,seed,w,h", [(1, 10, 1, 300, 100), (40, 10, 2, 300, 100), (200, 60, 3, 147, 102),
+                                          (300, 217, 4, 608, 448), (120, 122, 5, 314, 73),
+                                          (500, 87, 6, 225, 161), (900, 434, 7, 1209, 344)])
+def test_distribute_octree_leaf_invariants(oracle, n, N, seed, w, h):
+    keys = _tie_heavy_keys(n, w, h, seed)
+    rects, leaf_of = oracle.distribute_octree_leaves(keys, 16, 16 + w, 16, 16 + h, N)
+    out = oracle.distribute_octree(keys, 16, 16 + w, 16, 16 + h, N)
+    assert _leaf_violations(keys, rects, leaf_of, out, N, w, h) == []
+
+
+def test_distribute_octree_leaf_check_rejects_planted_retain(oracle):
+    """Planted: the last maximum of a node wins ties (>= instead of >)."""
+    keys = _tie_heavy_keys(500, 225, 161, 6)
+    rects, leaf_of = oracle.distribute_octree_leaves(keys, 16, 241, 16, 177, 87)
+    with oracle.planted(oracle.PLANT_RETAIN, 1):
+        out = oracle.distribute_octree(keys, 16, 241, 16, 177, 87)
+    assert any("first maximum" in v for v in _leaf_violations(keys, rects, leaf_of, out, 87, 225, 161))
 
 
 # ---------------------------------------------------------------- extraction
@@ -478,3 +602,102 @@ def test_stereo_no_right_keypoints(oracle):
     orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
     kl, dl, kr, dr, uR, dep = oracle.stereo_frame(ol, orr, L, np.full_like(L, 50), 386.1, 0.537)
     assert len(kr) == 0 and np.all(uR == -1) and np.all(dep == -1)
+
+
+# -------------------------------------------- stereo: hand-built known answers
+def _scene_run(oracle, scene, nlevels=3):
+    """Oracle pyramids of the scene's images, then ComputeStereoMatches by the
+    oracle and by the independent statement."""
+    import stereo_scenes as ss
+    ol = oracle.OracleExtractor(500, 1.2, nlevels, 20, 7)
+    orr = oracle.OracleExtractor(500, 1.2, nlevels, 20, 7)
+    ol(scene.left)
+    orr(scene.right)
+    levL = [ol.level_image(l) for l in range(nlevels)]
+    levR = [orr.level_image(l) for l in range(nlevels)]
+    assert np.array_equal(levL[0], scene.left) and np.array_equal(levR[0], scene.right)
+    t = ol.tables()
+    kl, dl, kr, dr = scene.keypoints(oracle.KEYPOINT_DTYPE)
+    got = oracle.stereo_match(ol, orr, kl, dl, kr, dr, scene.bf, scene.b)
+    want = ss.compute_stereo_matches(levL, levR, t["scale"], t["inv_scale"], kl, dl, kr, dr, scene.bf, scene.b)
+    return got, want, (ol, orr, kl, dl, kr, dr)
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+def test_stereo_scenes_match_independent_statement(oracle):
+    import stereo_scenes as ss
+    for name, build in ss.SCENES:
+        (uR, dep), (wu, wd), _ = _scene_run(oracle, build())
+        assert _bits_equal(uR, wu) and _bits_equal(dep, wd), name
+
+
+def test_stereo_scene_known_outcomes(oracle):
+    """What each scene's rules decide, stated per keypoint."""
+    import stereo_scenes as ss
+    res = {name: _scene_run(oracle, build())[0] for name, build in ss.SCENES}
+    uR, _ = res["bounds"]
+    assert list(uR >= 0) == [True, False, True, False, True]          # iniu = 0 / -1, endu = W-1 / W
+    uR, _ = res["shift_limits"]
+    assert list(uR >= 0) == [False, True, True, False, True]           # best at -5, -4, +4, +5
+    assert abs(uR[1] - 220.0) < 0.5 and abs(uR[2] - 320.0) < 0.5
+    uR, dep = res["zero_disparity"]
+    f = np.float32
+    assert uR[0] == f(150.0 - 0.01) and dep[0] == f(f(10.0) / f(0.01))    # disparity 0 -> 0.01
+    assert uR[1] == f(300.0 - 0.01)
+    assert uR[2] == -1                                                  # disparity == maxD
+    assert uR[3] >= 0 and uR[4] >= 0
+    uR, _ = res["median_tie"]                                           # SADs 3 10 10 10 10 20 21 40
+    assert list(uR >= 0) == [True] * 6 + [False, False]                 # th = 21.0: 21 is cut, 20 kept
+    uR, _ = res["median_index"]                                         # SADs 3 4 10 12 25 26 30 60
+    assert list(uR >= 0) == [True] * 7 + [False]                        # median = element 4 = 25
+    uR, _ = res["median_zero"]
+    assert np.all(uR == -1)                                             # median 0: th = 0 cuts all
+    uR, _ = res["hamming"]
+    assert list(uR[:3] >= 0) == [True, False, False]                    # 74 / 75 / 100 bits
+    assert not (abs(uR[3] - 188.0) < 0.5)                               # tie -> the lower index
+    assert uR[4] == -1 and uR[5] >= 0                                   # octave 2 ignored, 1 counts
+    assert uR[6] == -1                                                  # uR > uL
+    assert not (abs(uR[7] - 388.0) < 0.5) and abs(uR[8] - 428.0) < 0.5   # uL - maxD in, 0.5 px past out
+    uR, _ = res["rows"]
+    assert list(uR[:5] >= 0) == [True, False, True, False, True]
+    assert uR[5] >= 0
+    uR, _ = res["octave1"]
+    assert np.sum(uR >= 0) >= 6
+
+
+PLANTED_STEREO = {
+    1: "iniu <= 0", 2: "best at +-(L-1) rejected too", 3: "parabola sign", 4: "no 0.01 step",
+    5: "SAD equal to the threshold kept", 6: "lower median", 7: "rounded left row",
+    8: "uR window open at uL", 9: "endu > cols", 10: "Hamming <= 75 kept", 11: "last tie wins",
+}
+
+
+@pytest.mark.parametrize("variant", sorted(PLANTED_STEREO))
+def test_stereo_scenes_reject_planted_mistakes(oracle, variant):
+    import stereo_scenes as ss
+    differs = []
+    with oracle.planted(oracle.PLANT_STEREO, variant):
+        for name, build in ss.SCENES:
+            (uR, dep), (wu, wd), _ = _scene_run(oracle, build())
+            if not (_bits_equal(uR, wu) and _bits_equal(dep, wd)):
+                differs.append(name)
+    assert differs, PLANTED_STEREO[variant]
+
+
+def test_stereo_independent_statement_on_extracted_pair(oracle):
+    """The two statements agree on a synthetic driving-rig pair (640x480)."""
+    import stereo_scenes as ss
+    L, R, _ = synth.stereo_pair(640, 480, 3)
+    ol = oracle.OracleExtractor(1000, 1.2, 8, 20, 7)
+    orr = oracle.OracleExtractor(1000, 1.2, 8, 20, 7)
+    bf, b = 386.1, float(np.float32(386.1) / np.float32(718.856))
+    kl, dl, kr, dr, uR, dep = oracle.stereo_frame(ol, orr, L, R, bf, b)
+    t = ol.tables()
+    wu, wd = ss.compute_stereo_matches([ol.level_image(l) for l in range(8)], [orr.level_image(l) for l in range(8)],
+                                       t["scale"], t["inv_scale"], kl, dl, kr, dr, bf, b)
+    assert (uR >= 0).sum() > 200
+    assert _bits_equal(uR, wu) and _bits_equal(dep, wd)
+
