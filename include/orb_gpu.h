@@ -304,6 +304,62 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
                                   const int32_t* off2, const float* bounds, float* prev_matched, int window_size,
                                   float nn_ratio, int check_orientation, int32_t* matches12, int32_t* nmatches);
 
+/* ORBmatcher::SearchByProjection, both per-frame overloads, for n_frames
+ * frames at once.  Frame f's keypoints (mvKeysUn, at most 4096),
+ * descriptors, mvuRight and occupied[i] = (mvpMapPoints[i] &&
+ * mvpMapPoints[i]->Observations() > 0) are rows key_off[f] .. key_off[f+1]-1;
+ * its points are rows point_off[f] .. point_off[f+1]-1 of points /
+ * point_desc (MapPoint::GetDescriptor), in the order the reference visits
+ * them; bounds holds mnMinX, mnMaxX, mnMinY, mnMaxY per frame and
+ * scale_factors the frame's mvScaleFactors.  Outputs: point_match[p] = the
+ * keypoint point p was assigned (-1 none); key_point[i] = -1 untouched, >= 0
+ * the point (index within its frame) now in mvpMapPoints[i], -2 set to NULL;
+ * nmatches per frame.  The caller applies key_point to mvpMapPoints.  A later
+ * point skips keypoints an earlier point with observations took, exactly as
+ * the reference's in-order pass. */
+
+/* One entry of vpMapPoints as Frame::isInFrustum (src/Frame.cc:491-578) left
+ * it.  flags: 1 mbTrackInView, 2 isBad(), 4 Observations() > 0. */
+typedef struct orb_proj_point {
+    float proj_x, proj_y, proj_xr;   /* mTrackProjX, mTrackProjY, mTrackProjXR */
+    float view_cos;                  /* mTrackViewCos */
+    int level;                       /* mnTrackScaleLevel */
+    int flags;
+} orb_proj_point;
+
+/* ORBmatcher::SearchByProjection(Frame &F, const vector<MapPoint*>
+ * &vpMapPoints, const float th) (include/ORBmatcher.h:82, src/ORBmatcher.cc:72-183,
+ * RadiusByViewingCos :186-202); nn_ratio is the matcher's mfNNratio (0.8 in
+ * Tracking::SearchLocalPoints). */
+int orb_search_by_projection(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                             const float* uright, const uint8_t* occupied, const int32_t* key_off,
+                             const float* bounds, const float* scale_factors, int nlevels,
+                             const orb_proj_point* points, const uint8_t* point_desc, const int32_t* point_off,
+                             float th, float nn_ratio, int32_t* point_match, int32_t* key_point, int32_t* nmatches);
+
+/* LastFrame.mvpMapPoints[i] in index order: its position in the current
+ * camera frame (Rcw * x3Dw + tcw, computed by the caller), the last frame's
+ * mvKeysUn[i].angle and mvKeys[i].octave.  flags: 1 mvpMapPoints[i] &&
+ * !mvbOutlier[i], 4 Observations() > 0. */
+typedef struct orb_last_point {
+    float xc, yc, zc;
+    float angle;
+    int octave;
+    int flags;
+} orb_last_point;
+
+/* ORBmatcher::SearchByProjection(Frame &CurrentFrame, const Frame &LastFrame,
+ * const float th, const bool bMono) (include/ORBmatcher.h:102,
+ * src/ORBmatcher.cc:1710-1889).  camera holds per frame fx, fy, cx, cy, mbf,
+ * mb and tlc.z (the z of the last-to-current translation Rlw * twc + tlw,
+ * computed by the caller); check_orientation is mbCheckOrientation. */
+int orb_search_by_projection_last(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                                  const float* uright, const uint8_t* occupied, const int32_t* key_off,
+                                  const float* bounds, const float* scale_factors, int nlevels, const float* camera,
+                                  int mono, const orb_last_point* points, const uint8_t* point_desc,
+                                  const int32_t* point_off, float th, int check_orientation, int32_t* point_match,
+                                  int32_t* key_point, int32_t* nmatches);
+
 /* ---------------------------------------------------------------- map points
  * MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450) for
  * n_points points at once.  Point p's observation descriptors are rows

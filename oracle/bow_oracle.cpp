@@ -519,6 +519,10 @@ std::vector<int> features_in_area(const Grid& g, const oracle_keypoint* kp, floa
                 }
                 const float distx = kp[i].x - x;
                 const float disty = kp[i].y - y;
+                if (oracle_planted(4) == 2) {     // planted: the original square window
+                    if (std::fabs(distx) < r && std::fabs(disty) < r) out.push_back(i);
+                    continue;
+                }
                 if (distx * distx + disty * disty < r * r) out.push_back(i);
             }
     return out;
@@ -627,3 +631,177 @@ int oracle_search_for_initialization(const oracle_keypoint* kp1, const uint8_t* 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------- SearchByProjection
+// Both per-frame overloads of ORBmatcher::SearchByProjection, statement by
+// statement, with the same grid and GetFeaturesInArea as above.  The
+// MapPoint / Frame members they read arrive as plain arrays; the pose algebra
+// (isInFrustum for the first, Rcw * x3Dw + tcw and tlc for the second) stays
+// with the caller.  Planted mistakes (family 4) are marked where they act.
+namespace {
+
+const int TH_HIGH = 100;
+
+// ORBmatcher::RadiusByViewingCos (:186-202): the float viewCos is compared
+// with the double 0.998.
+float radius_by_viewing_cos(float viewCos)
+{
+    if (oracle_planted(4) == 1) return viewCos > 0.998f ? 2.5f : 4.0f;   // planted: float compare
+    return viewCos > 0.998 ? 2.5f : 4.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_search_by_projection(const oracle_keypoint* kp, const uint8_t* desc, const float* uright,
+                                const uint8_t* occupied, int n, const float* bounds, const float* scale,
+                                const oracle_proj_point* pts, const uint8_t* pdesc, int npts, float th,
+                                float nnratio, int32_t* pt_match, int32_t* key_point)
+{
+    const int plant = oracle_planted(4);
+    int nmatches = 0;
+    const bool bFactor = th != 1.0;
+    const Grid g = build_grid(kp, n, bounds);
+    // mvpMapPoints[i]: -1 none, -2 a pre-existing point with observations
+    // (occupied), else the index of the point that claimed it here
+    std::vector<int> mp(n, -1);
+    for (int i = 0; i < n; i++)
+        if (occupied[i]) mp[i] = -2;
+    auto has_obs = [&](int i) { return mp[i] == -2 || (mp[i] >= 0 && (pts[mp[i]].flags & 4)); };
+    for (int i = 0; i < n; i++) key_point[i] = -1;
+    for (int p = 0; p < npts; p++) {
+        pt_match[p] = -1;
+        const oracle_proj_point& P = pts[p];
+        if (!(P.flags & 1)) continue;
+        if (P.flags & 2) continue;
+        const int nPredictedLevel = P.level;
+        float r = radius_by_viewing_cos(P.view_cos);
+        if (bFactor) r *= th;
+        const std::vector<int> vIndices =
+            features_in_area(g, kp, P.proj_x, P.proj_y, r * scale[nPredictedLevel], nPredictedLevel - 1,
+                             nPredictedLevel);
+        if (vIndices.empty()) continue;
+        const uint8_t* MPdescriptor = pdesc + (size_t)p * 32;
+        int bestDist = 256, bestLevel = -1, bestDist2 = 256, bestLevel2 = -1, bestIdx = -1;
+        for (int idx : vIndices) {
+            if (mp[idx] != -1 && plant != 4)              // planted 4: claims here never block
+                if (has_obs(idx)) continue;
+            if (plant == 4 && mp[idx] == -2) continue;
+            if (uright[idx] > 0) {
+                const float er = std::fabs(P.proj_xr - uright[idx]);
+                if (plant == 5 ? er >= r * scale[nPredictedLevel] : er > r * scale[nPredictedLevel]) continue;
+            }
+            const int dist = forb_distance(MPdescriptor, desc + (size_t)idx * 32);
+            if (dist < bestDist) {
+                bestDist2 = bestDist;
+                bestDist = dist;
+                bestLevel2 = bestLevel;
+                bestLevel = kp[idx].octave;
+                bestIdx = idx;
+            } else if (dist < bestDist2) {
+                bestLevel2 = kp[idx].octave;
+                bestDist2 = dist;
+            }
+        }
+        if (plant == 6 ? bestDist < TH_HIGH : bestDist <= TH_HIGH) {
+            if ((bestLevel == bestLevel2 || plant == 3) && bestDist > nnratio * bestDist2) continue;
+            mp[bestIdx] = p;
+            key_point[bestIdx] = p;
+            pt_match[p] = bestIdx;
+            nmatches++;
+        }
+    }
+    return nmatches;
+}
+
+int oracle_search_by_projection_last(const oracle_keypoint* kp, const uint8_t* desc, const float* uright,
+                                     const uint8_t* occupied, int n, const float* bounds, const float* scale,
+                                     const float* cam, int mono, const oracle_last_point* pts,
+                                     const uint8_t* pdesc, int npts, float th, int check_orientation,
+                                     int32_t* pt_match, int32_t* key_point)
+{
+    const int plant = oracle_planted(4);
+    int nmatches = 0;
+    std::vector<int> rotHist[kHisto];
+    const float factor = kHisto / 360.0f;
+    const float fx = cam[0], fy = cam[1], cx = cam[2], cy = cam[3], mbf = cam[4], mb = cam[5], tlc_z = cam[6];
+    const bool bForward = tlc_z > mb && !mono;
+    const bool bBackward = -tlc_z > mb && !mono;
+    const Grid g = build_grid(kp, n, bounds);
+    std::vector<int> mp(n, -1);
+    for (int i = 0; i < n; i++)
+        if (occupied[i]) mp[i] = -2;
+    auto has_obs = [&](int i) {
+        return mp[i] == -2 || (mp[i] >= 0 && ((pts[mp[i]].flags & 4) || plant == 9));   // planted 9
+    };
+    for (int i = 0; i < n; i++) key_point[i] = -1;
+    for (int p = 0; p < npts; p++) {
+        pt_match[p] = -1;
+        const oracle_last_point& P = pts[p];
+        if (!(P.flags & 1)) continue;
+        const float xc = P.xc, yc = P.yc;
+        const float invzc = 1.0 / P.zc;
+        if (invzc < 0) continue;
+        const float u = fx * xc * invzc + cx;
+        const float v = fy * yc * invzc + cy;
+        if (u < bounds[0] || u > bounds[1]) continue;
+        if (v < bounds[2] || v > bounds[3]) continue;
+        const int nLastOctave = P.octave;
+        const float radius = th * scale[nLastOctave];
+        std::vector<int> vIndices2;
+        if (bForward)
+            vIndices2 = features_in_area(g, kp, u, v, radius, nLastOctave, -1);
+        else if (bBackward)
+            vIndices2 = plant == 7 ? features_in_area(g, kp, u, v, radius, nLastOctave - 1, nLastOctave)
+                                   : features_in_area(g, kp, u, v, radius, 0, nLastOctave);
+        else
+            vIndices2 = features_in_area(g, kp, u, v, radius, nLastOctave - 1, nLastOctave + 1);
+        if (vIndices2.empty()) continue;
+        const uint8_t* dMP = pdesc + (size_t)p * 32;
+        int bestDist = 256, bestIdx2 = -1;
+        for (int i2 : vIndices2) {
+            if (mp[i2] != -1 && plant != 4)
+                if (has_obs(i2)) continue;
+            if (plant == 4 && mp[i2] == -2) continue;
+            if (uright[i2] > 0) {
+                const float ur = u - mbf * invzc;
+                const float er = std::fabs(ur - uright[i2]);
+                if (plant == 5 ? er >= radius : er > radius) continue;
+            }
+            const int dist = forb_distance(dMP, desc + (size_t)i2 * 32);
+            if (dist < bestDist) {
+                bestDist = dist;
+                bestIdx2 = i2;
+            }
+        }
+        if (plant == 6 ? bestDist < TH_HIGH : bestDist <= TH_HIGH) {
+            mp[bestIdx2] = p;
+            key_point[bestIdx2] = p;
+            pt_match[p] = bestIdx2;
+            nmatches++;
+            if (check_orientation) {
+                float rot = P.angle - kp[bestIdx2].angle;
+                if (rot < 0.0) rot += 360.0f;
+                int bin = plant == 8 ? (int)std::nearbyint(rot * factor) : (int)std::round(rot * factor);
+                if (bin == kHisto) bin = 0;
+                rotHist[bin].push_back(bestIdx2);
+            }
+        }
+    }
+    if (check_orientation) {
+        int ind1 = -1, ind2 = -1, ind3 = -1;
+        three_maxima(rotHist, kHisto, ind1, ind2, ind3);
+        for (int i = 0; i < kHisto; i++) {
+            if (i == ind1 || i == ind2 || i == ind3) continue;
+            for (int i2 : rotHist[i]) {
+                key_point[i2] = -2;
+                nmatches--;
+            }
+        }
+    }
+    return nmatches;
+}
+
+}  // extern "C"
+

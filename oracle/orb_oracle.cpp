@@ -64,8 +64,8 @@ struct Cand {
 // Planted mistakes, one family per oracle function, so that a test can show
 // its checks reject a plausible wrong statement.  All zero = the faithful
 // statement; only oracle_plant changes them.
-enum { PLANT_ATAN = 1, PLANT_RETAIN = 2, PLANT_STEREO = 3, PLANT_FAMILIES = 4 };
-int g_plant[PLANT_FAMILIES] = {0, 0, 0, 0};
+enum { PLANT_ATAN = 1, PLANT_RETAIN = 2, PLANT_STEREO = 3, PLANT_PROJECTION = 4, PLANT_FAMILIES = 5 };
+int g_plant[PLANT_FAMILIES] = {0, 0, 0, 0, 0};
 
 // ------------------------------------------------------------ cv::resize
 // cv::resize(src, dst, dsize, 0, 0, INTER_LINEAR) for CV_8UC1, OpenCV
@@ -1012,6 +1012,8 @@ int oracle_plant(int family, int variant)
     g_plant[family] = variant;
     return 0;
 }
+
+int oracle_planted(int family) { return family > 0 && family < PLANT_FAMILIES ? g_plant[family] : 0; }
 
 int oracle_distribute_octree(const int* xyr, int n, int minX, int maxX, int minY, int maxY, int N, int* out,
                              int cap, int* nout)

@@ -78,9 +78,11 @@ int oracle_distribute_octree_leaves(const int* xyr, int n, int minX, int maxX, i
                                     int* rect4, int cap, int* leaf_of, int* nleaves);
 
 /* Test hook: plant a deliberate mistake in one oracle function (family 1:
- * fastAtan2, 2: DistributeOctTree retain, 3: ComputeStereoMatches), so a
- * test can show that its checks reject it.  variant 0 = faithful. */
+ * fastAtan2, 2: DistributeOctTree retain, 3: ComputeStereoMatches,
+ * 4: SearchByProjection), so a test can show that its checks reject it.
+ * variant 0 = faithful. */
 int oracle_plant(int family, int variant);
+int oracle_planted(int family);
 
 /* Stand-alone primitives. */
 int oracle_resize_linear(const uint8_t* src, int sw, int sh, int sstride,
@@ -164,6 +166,45 @@ int oracle_search_for_initialization(const oracle_keypoint* kp1, const uint8_t* 
                                      const oracle_keypoint* kp2, const uint8_t* d2, int n2,
                                      const float* bounds, float* prev, int windowSize, float nnratio,
                                      int checkOrientation, int32_t* matches12);
+
+/* ORBmatcher::SearchByProjection(Frame&, const vector<MapPoint*>&, float th)
+ * (src/ORBmatcher.cc:72-183, RadiusByViewingCos :186-202) on frame keypoints
+ * kp (mvKeysUn), descriptors, mvuRight and occupied[i] = (mvpMapPoints[i] &&
+ * Observations() > 0), over the grid of bounds (mnMinX, mnMaxX, mnMinY,
+ * mnMaxY) and the frame's scale factors.  Points carry what
+ * Frame::isInFrustum left in the MapPoint.  pt_match[p] = keypoint claimed
+ * by point p or -1; key_point[i] = the point last assigned to keypoint i, or
+ * -1 when untouched.  Returns nmatches. */
+typedef struct oracle_proj_point {
+    float proj_x, proj_y, proj_xr;   /* mTrackProjX, mTrackProjY, mTrackProjXR */
+    float view_cos;                  /* mTrackViewCos */
+    int level;                       /* mnTrackScaleLevel */
+    int flags;                       /* 1: mbTrackInView, 2: isBad(), 4: Observations() > 0 */
+} oracle_proj_point;
+int oracle_search_by_projection(const oracle_keypoint* kp, const uint8_t* desc, const float* uright,
+                                const uint8_t* occupied, int n, const float* bounds, const float* scale,
+                                const oracle_proj_point* pts, const uint8_t* pdesc, int npts, float th,
+                                float nnratio, int32_t* pt_match, int32_t* key_point);
+
+/* ORBmatcher::SearchByProjection(Frame& Current, const Frame& Last, float th,
+ * bool bMono) (src/ORBmatcher.cc:1710-1889).  Points are LastFrame's map
+ * points in index order with their camera coordinates in the current frame
+ * (Rcw * x3Dw + tcw); cam = fx, fy, cx, cy, mbf, mb and the z of tlc (the
+ * last-to-current translation in the last frame).  key_point[i] = -1
+ * untouched, -2 set to NULL by the rotation check, else the last point
+ * assigned.  pt_match[p] = the keypoint point p claimed (before the rotation
+ * check) or -1.  Returns nmatches. */
+typedef struct oracle_last_point {
+    float xc, yc, zc;   /* x3Dc */
+    float angle;        /* LastFrame.mvKeysUn[i].angle */
+    int octave;         /* LastFrame.mvKeys[i].octave */
+    int flags;          /* 1: mvpMapPoints[i] && !mvbOutlier[i], 4: Observations() > 0 */
+} oracle_last_point;
+int oracle_search_by_projection_last(const oracle_keypoint* kp, const uint8_t* desc, const float* uright,
+                                     const uint8_t* occupied, int n, const float* bounds, const float* scale,
+                                     const float* cam, int mono, const oracle_last_point* pts,
+                                     const uint8_t* pdesc, int npts, float th, int check_orientation,
+                                     int32_t* pt_match, int32_t* key_point);
 
 /* MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450), many
  * points at once: best[p] indexes point p's rows offsets[p]..offsets[p+1]-1

@@ -67,6 +67,9 @@ def lib():
             "oracle_init_rectify_map": (I, [P, P, I, P, P, I, I, P, P]),
             "oracle_remap_linear": (I, [P, I, I, I, P, P, I, I, P]),
             "oracle_search_for_initialization": (I, [P, P, I, P, P, I, P, P, I, F, I, P]),
+            "oracle_search_by_projection": (I, [P, P, P, P, I, P, P, P, P, I, F, F, P, P]),
+            "oracle_search_by_projection_last": (I, [P, P, P, P, I, P, P, P, I, P, P, I, F, I, P, P]),
+            "oracle_planted": (I, [I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -168,7 +171,52 @@ def distribute_octree_leaves(xyr, minX, maxX, minY, maxY, N):
     return rects[:n.value].copy(), leaf_of
 
 
-PLANT_ATAN, PLANT_RETAIN, PLANT_STEREO = 1, 2, 3
+PLANT_ATAN, PLANT_RETAIN, PLANT_STEREO, PLANT_PROJECTION = 1, 2, 3, 4
+
+PROJ_POINT_DTYPE = np.dtype([("proj_x", "<f4"), ("proj_y", "<f4"), ("proj_xr", "<f4"), ("view_cos", "<f4"),
+                             ("level", "<i4"), ("flags", "<i4")])
+LAST_POINT_DTYPE = np.dtype([("xc", "<f4"), ("yc", "<f4"), ("zc", "<f4"), ("angle", "<f4"), ("octave", "<i4"),
+                             ("flags", "<i4")])
+
+
+def _frame_args(kp, desc, uright, occupied, bounds, scale):
+    kp = np.ascontiguousarray(kp, KEYPOINT_DTYPE)
+    desc = np.ascontiguousarray(desc, np.uint8).reshape(-1, 32)
+    uright = np.ascontiguousarray(uright, np.float32)
+    occupied = np.ascontiguousarray(occupied, np.uint8)
+    bounds = np.ascontiguousarray(bounds, np.float32)
+    scale = np.ascontiguousarray(scale, np.float32)
+    assert len(desc) == len(kp) == len(uright) == len(occupied)
+    return kp, desc, uright, occupied, bounds, scale
+
+
+def search_by_projection(kp, desc, uright, occupied, bounds, scale, pts, pdesc, th=1.0, nnratio=0.8):
+    """ORBmatcher::SearchByProjection(F, vpMapPoints, th) -> (nmatches,
+    pt_match, key_point); see orb_oracle.h."""
+    kp, desc, uright, occupied, bounds, scale = _frame_args(kp, desc, uright, occupied, bounds, scale)
+    pts = np.ascontiguousarray(pts, PROJ_POINT_DTYPE)
+    pdesc = np.ascontiguousarray(pdesc, np.uint8).reshape(-1, 32)
+    pm = np.zeros(len(pts), np.int32)
+    kpm = np.zeros(len(kp), np.int32)
+    n = lib().oracle_search_by_projection(_p(kp), _p(desc), _p(uright), _p(occupied), len(kp), _p(bounds),
+                                          _p(scale), _p(pts), _p(pdesc), len(pts), th, nnratio, _p(pm), _p(kpm))
+    return n, pm, kpm
+
+
+def search_by_projection_last(kp, desc, uright, occupied, bounds, scale, cam, mono, pts, pdesc, th,
+                              check_orientation=True):
+    """ORBmatcher::SearchByProjection(Current, Last, th, bMono) -> (nmatches,
+    pt_match, key_point); cam = fx, fy, cx, cy, mbf, mb, tlc_z."""
+    kp, desc, uright, occupied, bounds, scale = _frame_args(kp, desc, uright, occupied, bounds, scale)
+    cam = np.ascontiguousarray(cam, np.float32)
+    pts = np.ascontiguousarray(pts, LAST_POINT_DTYPE)
+    pdesc = np.ascontiguousarray(pdesc, np.uint8).reshape(-1, 32)
+    pm = np.zeros(len(pts), np.int32)
+    kpm = np.zeros(len(kp), np.int32)
+    n = lib().oracle_search_by_projection_last(_p(kp), _p(desc), _p(uright), _p(occupied), len(kp), _p(bounds),
+                                               _p(scale), _p(cam), 1 if mono else 0, _p(pts), _p(pdesc), len(pts),
+                                               th, 1 if check_orientation else 0, _p(pm), _p(kpm))
+    return n, pm, kpm
 
 
 class planted:

@@ -104,6 +104,9 @@ SIGNATURES = {
     "orb_vocabulary_bow_time": (_I, [_P, _I, _P, _P, _I, _I, _P]),
     "orb_distinctive_descriptors": (_I, [_P, _I, _P, _P, _P, _P]),
     "orb_search_for_initialization": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _F, _I, _P, _P]),
+    "orb_search_by_projection": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _F, _F, _P, _P, _P]),
+    "orb_search_by_projection_last": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _F, _I, _P,
+                                           _P, _P]),
     "orb_distinctive_descriptors_time": (_I, [_P, _I, _P, _P, _I, _P]),
     "orb_extractor_debug_image": (_I, [_P, _I, _I, _P, _S]),
     "orb_extractor_debug_candidates": (_I, [_P, _I, _P, _I, _P]),
@@ -653,6 +656,75 @@ def search_for_initialization(pairs, window=100, nnratio=0.9, check_orientation=
                                                _ptr(o2), _ptr(bounds), _ptr(prev), int(window), float(nnratio),
                                                int(bool(check_orientation)), _ptr(m12), _ptr(nm)))
     return [(int(nm[i]), m12[o1[i]:o1[i + 1]].copy(), prev[o1[i]:o1[i + 1]].copy()) for i in range(len(pairs))]
+
+
+PROJ_POINT_DTYPE = np.dtype([("proj_x", "<f4"), ("proj_y", "<f4"), ("proj_xr", "<f4"), ("view_cos", "<f4"),
+                             ("level", "<i4"), ("flags", "<i4")])
+LAST_POINT_DTYPE = np.dtype([("xc", "<f4"), ("yc", "<f4"), ("zc", "<f4"), ("angle", "<f4"), ("octave", "<i4"),
+                             ("flags", "<i4")])
+
+
+def _projection_frames(frames, pdtype):
+    """Concatenates per-frame (kp, desc, uright, occupied, bounds, points,
+    point_desc, ...) tuples into the flat arrays of the C ABI."""
+    cat = lambda xs, dt, shape: np.concatenate(xs) if sum(len(x) for x in xs) else np.zeros(shape, dt)
+    kp = [np.ascontiguousarray(f[0], KEYPOINT_DTYPE) for f in frames]
+    ko = np.zeros(len(frames) + 1, np.int32)
+    ko[1:] = np.cumsum([len(k) for k in kp])
+    pts = [np.ascontiguousarray(f[5], pdtype) for f in frames]
+    po = np.zeros(len(frames) + 1, np.int32)
+    po[1:] = np.cumsum([len(p) for p in pts])
+    arrays = dict(
+        kp=cat(kp, KEYPOINT_DTYPE, 1),
+        desc=cat([np.ascontiguousarray(f[1], np.uint8).reshape(-1, 32) for f in frames], np.uint8, (1, 32)),
+        uright=cat([np.ascontiguousarray(f[2], np.float32) for f in frames], np.float32, 1),
+        occupied=cat([np.ascontiguousarray(f[3], np.uint8) for f in frames], np.uint8, 1),
+        bounds=np.ascontiguousarray([f[4] for f in frames], np.float32).reshape(-1),
+        pts=cat(pts, pdtype, 1),
+        pdesc=cat([np.ascontiguousarray(f[6], np.uint8).reshape(-1, 32) for f in frames], np.uint8, (1, 32)),
+        ko=ko, po=po)
+    arrays["pm"] = np.zeros(max(int(po[-1]), 1), np.int32)
+    arrays["kpm"] = np.zeros(max(int(ko[-1]), 1), np.int32)
+    arrays["nm"] = np.zeros(len(frames), np.int32)
+    return arrays
+
+
+def _projection_results(a, n):
+    return [(int(a["nm"][i]), a["pm"][a["po"][i]:a["po"][i + 1]].copy(), a["kpm"][a["ko"][i]:a["ko"][i + 1]].copy())
+            for i in range(n)]
+
+
+def search_by_projection(frames, scale_factors, th=3.0, nn_ratio=0.8, context=None):
+    """ORBmatcher::SearchByProjection(F, vpMapPoints, th) for a list of frames
+    (kp, desc, uright, occupied, bounds, points PROJ_POINT_DTYPE, point_desc).
+    Returns per frame (nmatches, point_match, key_point)."""
+    ctx = context or Context.default()
+    a = _projection_frames(frames, PROJ_POINT_DTYPE)
+    sc = np.ascontiguousarray(scale_factors, np.float32)
+    _check("orb_search_by_projection",
+           lib().orb_search_by_projection(ctx.h, len(frames), _ptr(a["kp"]), _ptr(a["desc"]), _ptr(a["uright"]),
+                                          _ptr(a["occupied"]), _ptr(a["ko"]), _ptr(a["bounds"]), _ptr(sc), len(sc),
+                                          _ptr(a["pts"]), _ptr(a["pdesc"]), _ptr(a["po"]), float(th),
+                                          float(nn_ratio), _ptr(a["pm"]), _ptr(a["kpm"]), _ptr(a["nm"])))
+    return _projection_results(a, len(frames))
+
+
+def search_by_projection_last(frames, scale_factors, th, mono=False, check_orientation=True, context=None):
+    """ORBmatcher::SearchByProjection(Current, Last, th, bMono) for a list of
+    frames (kp, desc, uright, occupied, bounds, points LAST_POINT_DTYPE,
+    point_desc, camera = (fx, fy, cx, cy, mbf, mb, tlc_z)).  Returns per
+    frame (nmatches, point_match, key_point)."""
+    ctx = context or Context.default()
+    a = _projection_frames(frames, LAST_POINT_DTYPE)
+    sc = np.ascontiguousarray(scale_factors, np.float32)
+    cam = np.ascontiguousarray([f[7] for f in frames], np.float32).reshape(-1)
+    _check("orb_search_by_projection_last",
+           lib().orb_search_by_projection_last(ctx.h, len(frames), _ptr(a["kp"]), _ptr(a["desc"]), _ptr(a["uright"]),
+                                               _ptr(a["occupied"]), _ptr(a["ko"]), _ptr(a["bounds"]), _ptr(sc),
+                                               len(sc), _ptr(cam), int(bool(mono)), _ptr(a["pts"]), _ptr(a["pdesc"]),
+                                               _ptr(a["po"]), float(th), int(bool(check_orientation)),
+                                               _ptr(a["pm"]), _ptr(a["kpm"]), _ptr(a["nm"])))
+    return _projection_results(a, len(frames))
 
 
 def distinctive_descriptors(points, context=None):

@@ -175,3 +175,45 @@ struct WinPair {
     int* m12;               // n1: vnMatches12
     int* nmatch;            // nmatches
 };
+
+// ORBmatcher::SearchByProjection: one query (a map point to place in the
+// frame) after the per-point arithmetic of either overload.
+struct ProjQuery {
+    float x, y, r;          // window centre and radius (GetFeaturesInArea)
+    float su;               // right-image u compared with mvuRight (radius r)
+    int minL, maxL;         // GetFeaturesInArea level arguments
+    int flags;              // ORB_PQ_VALID | ORB_PQ_OBS
+    float angle;            // last-frame keypoint angle (rotation check)
+};
+#define ORB_PQ_VALID 1
+#define ORB_PQ_OBS 4
+#define ORB_PROJ_MAXIT 48   // claim-resolution rounds before the sequential pass
+
+// One frame of a SearchByProjection call: its keypoints (mvKeysUn),
+// descriptors, mvuRight, occupancy and grid, the queries, their candidate
+// lists, and the claim-resolution state.
+struct ProjFrame {
+    const OrbKeyPoint* kp;
+    const uint8_t* desc;
+    const float* uright;
+    const uint8_t* occupied;    // mvpMapPoints[i] && Observations() > 0
+    int n;
+    float minX, maxX, minY, maxY;
+    const int* sorted;          // grid, as WinPair's
+    const int* cell_start;
+    ProjQuery* q;
+    const uint8_t* qdesc;
+    int nq;
+    int* cnt;                   // candidates per query
+    const int* off;             // their offsets (nq + 1)
+    unsigned* cand;             // i2 | distance << 12 | octave << 21, GetFeaturesInArea order
+    int* out[2];                // outcome per query by round parity: keypoint or -1
+    int* claim[3];              // per keypoint: first claiming query (with observations), by round mod 3
+    int* changed;               // [ORB_PROJ_MAXIT]: round k changed some outcome
+    int rule;                   // 1: local map (best + second, level ratio); 2: last frame (best, rotation)
+    float nnratio;
+    int check_orientation;
+    int* pt_match;              // outputs
+    int* key_point;
+    int* nmatch;
+};

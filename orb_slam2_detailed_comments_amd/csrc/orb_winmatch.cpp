@@ -134,3 +134,228 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------- SearchByProjection
+extern "C" hipError_t orb_launch_proj_prep(ProjFrame* d_frames, int nframes, int max_nq, int rule, const float* d_pts,
+                                           const int* d_pt_off, const float* d_scale, int nlevels, const float* d_cam,
+                                           int mono, float th, hipStream_t st);
+extern "C" hipError_t orb_launch_proj_count(const WinPair* d_grid, ProjFrame* d_frames, int nframes, int max_nq,
+                                            int* d_totals, hipStream_t st);
+extern "C" hipError_t orb_launch_proj_fill(ProjFrame* d_frames, int nframes, int max_nq, hipStream_t st);
+extern "C" hipError_t orb_launch_proj_rounds(ProjFrame* d_frames, int nframes, int max_nq, int k0, int k1,
+                                             hipStream_t st);
+extern "C" hipError_t orb_launch_proj_finish(ProjFrame* d_frames, int nframes, int sequential, hipStream_t st);
+
+namespace {
+
+// Both overloads: rule 1 = SearchByProjection(F, vpMapPoints, th), rule 2 =
+// SearchByProjection(Current, Last, th, bMono).  Point records are 24 bytes
+// (orb_proj_point / orb_last_point).  Rounds run in chunks of 4 between
+// reads of the per-round change flags; frames still changing after
+// ORB_PROJ_MAXIT rounds get the sequential pass.
+int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                const float* uright, const uint8_t* occupied, const int32_t* key_off, const float* bounds,
+                const float* scale, int nlevels, const void* pts, const uint8_t* pdesc, const int32_t* pt_off,
+                const float* cam, int mono, float th, float nnratio, int check_orientation, int32_t* pt_match,
+                int32_t* key_point, int32_t* nmatches, int* rounds_used)
+{
+    static_assert(sizeof(orb_proj_point) == 24 && sizeof(orb_last_point) == 24, "point records");
+    if (!ctx || n_frames < 0 || nlevels < 1 || nlevels > ORB_MAX_LEVELS ||
+        (n_frames && (!key_off || !pt_off || !bounds || !scale || !nmatches || (rule == 2 && !cam))))
+        return ORB_E_ARG;
+    if (n_frames == 0) return ORB_OK;
+    const int N = key_off[n_frames], Q = pt_off[n_frames];
+    if ((N && (!kp || !desc || !uright || !occupied || !key_point)) || (Q && (!pts || !pdesc || !pt_match)))
+        return ORB_E_ARG;
+    int max_nq = 0;
+    for (int f = 0; f < n_frames; f++) {
+        const int n = key_off[f + 1] - key_off[f], nq = pt_off[f + 1] - pt_off[f];
+        if (n < 0 || nq < 0 || key_off[f] < 0 || pt_off[f] < 0) return ORB_E_ARG;
+        if (n > 4096 || nq > (1 << 20)) return ORB_E_CAPACITY;
+        if (!(bounds[4 * f + 1] > bounds[4 * f]) || !(bounds[4 * f + 3] > bounds[4 * f + 2])) return ORB_E_ARG;
+        max_nq = std::max(max_nq, nq);
+    }
+    for (int i = 0; i < N; i++)
+        if (kp[i].octave < 0 || kp[i].octave >= ORB_MAX_LEVELS) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int F = n_frames;
+    const size_t s_kp = up((size_t)N * sizeof(OrbKeyPoint)), s_desc = up((size_t)N * 32), s_ur = up((size_t)N * 4),
+                 s_occ = up((size_t)N), s_pts = up((size_t)Q * 24), s_pdesc = up((size_t)Q * 32),
+                 s_ptoff = up((size_t)(F + 1) * 4), s_scale = up(ORB_MAX_LEVELS * 4), s_cam = up((size_t)F * 7 * 4),
+                 s_sorted = up((size_t)N * 4), s_cells = up((size_t)F * (64 * 48 + 1) * 4),
+                 s_q = up((size_t)Q * sizeof(ProjQuery)), s_cnt = up((size_t)Q * 4), s_off = up((size_t)(Q + F) * 4),
+                 s_out = up((size_t)Q * 4), s_claim = up((size_t)N * 4), s_chg = up((size_t)F * ORB_PROJ_MAXIT * 4),
+                 s_nm = up((size_t)F * 4), s_grid = up((size_t)F * sizeof(WinPair)),
+                 s_frames = up((size_t)F * sizeof(ProjFrame)), s_tot = up((size_t)F * 4);
+    const size_t total = s_kp + s_desc + s_ur + s_occ + s_pts + s_pdesc + s_ptoff + s_scale + s_cam + s_sorted +
+                         s_cells + s_q + s_cnt + s_off + 3 * s_out + 4 * s_claim + s_chg + s_nm + s_grid + s_frames +
+                         s_tot;
+    int rc = ctx->grow(&ctx->d_win, &ctx->win_cap, total);
+    if (rc) return rc;
+    char* m = (char*)ctx->d_win;
+    size_t o = 0;
+    auto take = [&](size_t s) {
+        char* q = m + o;
+        o += s;
+        return q;
+    };
+    OrbKeyPoint* d_kp = (OrbKeyPoint*)take(s_kp);
+    uint8_t* d_desc = (uint8_t*)take(s_desc);
+    float* d_ur = (float*)take(s_ur);
+    uint8_t* d_occ = (uint8_t*)take(s_occ);
+    float* d_pts = (float*)take(s_pts);
+    uint8_t* d_pdesc = (uint8_t*)take(s_pdesc);
+    int* d_ptoff = (int*)take(s_ptoff);
+    float* d_scale = (float*)take(s_scale);
+    float* d_cam = (float*)take(s_cam);
+    int* d_sorted = (int*)take(s_sorted);
+    int* d_cells = (int*)take(s_cells);
+    ProjQuery* d_q = (ProjQuery*)take(s_q);
+    int* d_cnt = (int*)take(s_cnt);
+    int* d_off = (int*)take(s_off);
+    int* d_out0 = (int*)take(s_out);
+    int* d_out1 = (int*)take(s_out);
+    int* d_ptm = (int*)take(s_out);
+    int* d_claim0 = (int*)take(s_claim);
+    int* d_claim1 = (int*)take(s_claim);
+    int* d_claim2 = (int*)take(s_claim);
+    int* d_keyp = (int*)take(s_claim);
+    int* d_chg = (int*)take(s_chg);
+    int* d_nm = (int*)take(s_nm);
+    WinPair* d_grid = (WinPair*)take(s_grid);
+    ProjFrame* d_frames = (ProjFrame*)take(s_frames);
+    int* d_tot = (int*)take(s_tot);
+    static_assert(sizeof(orb_keypoint) == sizeof(OrbKeyPoint), "keypoint layout");
+    if (N) {
+        HIP_TRY(hipMemcpyAsync(d_kp, kp, (size_t)N * sizeof(OrbKeyPoint), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_desc, desc, (size_t)N * 32, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_ur, uright, (size_t)N * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_occ, occupied, (size_t)N, hipMemcpyHostToDevice, st));
+    }
+    if (Q) {
+        HIP_TRY(hipMemcpyAsync(d_pts, pts, (size_t)Q * 24, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_pdesc, pdesc, (size_t)Q * 32, hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(hipMemcpyAsync(d_ptoff, pt_off, (size_t)(F + 1) * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_scale, scale, (size_t)nlevels * 4, hipMemcpyHostToDevice, st));
+    if (rule == 2) HIP_TRY(hipMemcpyAsync(d_cam, cam, (size_t)F * 7 * 4, hipMemcpyHostToDevice, st));
+    std::vector<WinPair> grid(F);
+    std::vector<ProjFrame> frames(F);
+    for (int f = 0; f < F; f++) {
+        const int k0 = key_off[f], q0 = pt_off[f];
+        WinPair& w = grid[f];
+        std::memset(&w, 0, sizeof(w));
+        w.kp2 = d_kp + k0;
+        w.n2 = key_off[f + 1] - k0;
+        w.minX = bounds[4 * f];
+        w.maxX = bounds[4 * f + 1];
+        w.minY = bounds[4 * f + 2];
+        w.maxY = bounds[4 * f + 3];
+        w.sorted = d_sorted + k0;
+        w.cell_start = d_cells + (size_t)f * (64 * 48 + 1);
+        w.n1 = pt_off[f + 1] - q0;
+        w.cnt = d_cnt + q0;
+        w.off = d_off + q0 + f;
+        ProjFrame& P = frames[f];
+        std::memset(&P, 0, sizeof(P));
+        P.kp = d_kp + k0;
+        P.desc = d_desc + (size_t)k0 * 32;
+        P.uright = d_ur + k0;
+        P.occupied = d_occ + k0;
+        P.n = w.n2;
+        P.minX = w.minX;
+        P.maxX = w.maxX;
+        P.minY = w.minY;
+        P.maxY = w.maxY;
+        P.sorted = w.sorted;
+        P.cell_start = w.cell_start;
+        P.q = d_q + q0;
+        P.qdesc = d_pdesc + (size_t)q0 * 32;
+        P.nq = w.n1;
+        P.cnt = w.cnt;
+        P.off = w.off;
+        P.out[0] = d_out0 + q0;
+        P.out[1] = d_out1 + q0;
+        P.claim[0] = d_claim0 + k0;
+        P.claim[1] = d_claim1 + k0;
+        P.claim[2] = d_claim2 + k0;
+        P.changed = d_chg + (size_t)f * ORB_PROJ_MAXIT;
+        P.rule = rule;
+        P.nnratio = nnratio;
+        P.check_orientation = check_orientation ? 1 : 0;
+        P.pt_match = d_ptm + q0;
+        P.key_point = d_keyp + k0;
+        P.nmatch = d_nm + f;
+    }
+    HIP_TRY(hipMemcpyAsync(d_grid, grid.data(), (size_t)F * sizeof(WinPair), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_frames, frames.data(), (size_t)F * sizeof(ProjFrame), hipMemcpyHostToDevice, st));
+    HIP_TRY(orb_launch_proj_prep(d_frames, F, max_nq, rule, d_pts, d_ptoff, d_scale, nlevels, d_cam, mono, th, st));
+    HIP_TRY(orb_launch_proj_count(d_grid, d_frames, F, max_nq, d_tot, st));
+    std::vector<int> tot(F);
+    HIP_TRY(hipMemcpyAsync(tot.data(), d_tot, (size_t)F * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    size_t ncand = 0;
+    for (int f = 0; f < F; f++) ncand += (size_t)tot[f];
+    rc = ctx->grow(&ctx->d_wcand, &ctx->wcand_cap, std::max<size_t>(ncand, 1) * sizeof(unsigned));
+    if (rc) return rc;
+    size_t base = 0;
+    for (int f = 0; f < F; f++) {
+        frames[f].cand = (unsigned*)ctx->d_wcand + base;
+        base += (size_t)tot[f];
+    }
+    HIP_TRY(hipMemcpyAsync(d_frames, frames.data(), (size_t)F * sizeof(ProjFrame), hipMemcpyHostToDevice, st));
+    HIP_TRY(orb_launch_proj_fill(d_frames, F, max_nq, st));
+    // claim-resolution rounds until every frame has a round that changed nothing
+    std::vector<int> chg((size_t)F * ORB_PROJ_MAXIT);
+    int k = 0;
+    bool settled = false;
+    while (!settled && k < ORB_PROJ_MAXIT) {
+        const int k1 = std::min(ORB_PROJ_MAXIT, k + 4);
+        HIP_TRY(orb_launch_proj_rounds(d_frames, F, max_nq, k, k1, st));
+        k = k1;
+        HIP_TRY(hipMemcpyAsync(chg.data(), d_chg, chg.size() * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        settled = true;
+        for (int f = 0; f < F && settled; f++) {
+            bool s = false;
+            for (int r = 1; r < k && !s; r++) s = chg[(size_t)f * ORB_PROJ_MAXIT + r] == 0;
+            settled = s;
+        }
+    }
+    if (rounds_used) *rounds_used = k;
+    HIP_TRY(orb_launch_proj_finish(d_frames, F, settled ? 0 : 1, st));
+    if (Q) HIP_TRY(hipMemcpyAsync(pt_match, d_ptm, (size_t)Q * 4, hipMemcpyDeviceToHost, st));
+    if (N) HIP_TRY(hipMemcpyAsync(key_point, d_keyp, (size_t)N * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(nmatches, d_nm, (size_t)F * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return ORB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orb_search_by_projection(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                             const float* uright, const uint8_t* occupied, const int32_t* key_off,
+                             const float* bounds, const float* scale_factors, int nlevels,
+                             const orb_proj_point* points, const uint8_t* point_desc, const int32_t* point_off,
+                             float th, float nn_ratio, int32_t* point_match, int32_t* key_point, int32_t* nmatches)
+{
+    return proj_search(ctx, 1, n_frames, kp, desc, uright, occupied, key_off, bounds, scale_factors, nlevels, points,
+                       point_desc, point_off, nullptr, 0, th, nn_ratio, 0, point_match, key_point, nmatches, nullptr);
+}
+
+int orb_search_by_projection_last(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                                  const float* uright, const uint8_t* occupied, const int32_t* key_off,
+                                  const float* bounds, const float* scale_factors, int nlevels, const float* camera,
+                                  int mono, const orb_last_point* points, const uint8_t* point_desc,
+                                  const int32_t* point_off, float th, int check_orientation, int32_t* point_match,
+                                  int32_t* key_point, int32_t* nmatches)
+{
+    return proj_search(ctx, 2, n_frames, kp, desc, uright, occupied, key_off, bounds, scale_factors, nlevels, points,
+                       point_desc, point_off, camera, mono, th, 0.f, check_orientation, point_match, key_point,
+                       nmatches, nullptr);
+}
+
+}  // extern "C"
