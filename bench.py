@@ -199,6 +199,42 @@ def winmatch_extra(orb, ctx, cpu_seconds):
     return out
 
 
+def projection_extra(orb, ctx, cpu_seconds):
+    """ORBmatcher::SearchByProjection, both per-frame overloads, for 32
+    frames of 2000 keypoints and 1800 projected points each in one call
+    (host arrays in and out); the oracle on one core beside."""
+    from orb_slam2_detailed_comments_amd import synth
+    scale = np.float32(1.2) ** np.arange(8, dtype=np.float32)
+    out = {"workload": "SearchByProjection, 32 frames x (2000 keypoints, 1800 points) per call; local map (th 3, "
+                       "ratio 0.8) and last frame (th 15, rotation check); host arrays in and out"}
+    for name, last in (("local_map", False), ("last_frame", True)):
+        frames = [synth.projection_frame(500 + i, last=last) for i in range(32)]
+        fn = (lambda: orb.search_by_projection_last(frames, scale, 15.0, False, True, context=ctx)) if last else \
+             (lambda: orb.search_by_projection(frames, scale, 3.0, 0.8, context=ctx))
+        fn()
+        reps, t0 = 5, time.perf_counter()
+        for _ in range(reps):
+            res = fn()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        r = {"ms_per_call": round(ms, 3), "frames_per_s": round(32 / (ms * 1e-3), 1),
+             "mean_matches": round(float(np.mean([x[0] for x in res])), 1)}
+        if cpu_seconds > 0:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle
+            n, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < cpu_seconds / 2:
+                kp, d, ur, occ, b, pts, pd, cam = frames[n % 32]
+                if last:
+                    pyoracle.search_by_projection_last(kp, d, ur, occ, b, scale, cam, False, pts, pd, 15.0, True)
+                else:
+                    pyoracle.search_by_projection(kp, d, ur, occ, b, scale, pts, pd, 3.0, 0.8)
+                n += 1
+            r["cpu_baseline"] = {"value": round(n / (time.perf_counter() - t0), 1), "unit": "frames/s", "cores": 1,
+                                 "kind": "port", "sample": f"{n} frames through the oracle"}
+        out[name] = r
+    return out
+
+
 def batch_extra(orb, ctx, steps):
     """Mixed-size batch: 64 frames, 16 each of 640x480, 752x480 (nFeatures
     1000), 1241x376 and 1280x720 (nFeatures 2000), extracted by one
@@ -375,7 +411,9 @@ def main():
     ap.add_argument("--mode", choices=["graph", "stages", "per-image"], default="graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the dense k=2 search timing")
+    ap.add_argument("--no-extras", action="store_true", help="skip every extra workload")
+    ap.add_argument("--extras", default="all",
+                    help="comma-separated extras to run: knn, batch, serving, bow, mappoint, winmatch, projection")
     ap.add_argument("--branches", type=int, default=0,
                     help="independent hipGraph branches (mode graph); 0: pick by the batch's autotuner")
     args = ap.parse_args()
@@ -427,6 +465,29 @@ def main():
     sample = sorted({round(i * (B - 1) / 5) for i in range(6)})
     gpu_sample = {p: batch.results(2 * p) + batch.results(2 * p + 1) + batch.stereo_results(p) for p in sample}
 
+    # Host to host: the same step with the 64 images copied in from host
+    # memory and every result read back (one step at a time, no overlap of
+    # copies with compute), outside the headline figure.
+    imgs = [im for pair in frames for im in pair]
+    res = None
+    for _ in range(2):
+        batch.upload_all(imgs)
+        batch.run(mode)
+        res = batch.read_all(out=res)
+    h2h_steps = max(5, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(h2h_steps):
+        batch.upload_all(imgs)
+        batch.run(mode)
+        res = batch.read_all(out=res)
+    h2h_s = time.perf_counter() - t0
+    host_to_host = {"frames_per_s": round(B * h2h_steps / h2h_s, 1), "ms_per_step": round(h2h_s * 1e3 / h2h_steps, 3),
+                    "steps": h2h_steps,
+                    "includes": "copies of the 64 host images (pageable memory), graph replay, read-back of every "
+                                "slot's keypoints + descriptors and every pair's uRight + depth; no copy/compute "
+                                "overlap"}
+    print(f"[bench] host to host: {host_to_host}", file=sys.stderr, flush=True)
+
     stages = batch.stage_times(args.steps)
     dom = max((k for k in stages if k != "stereo_median"), key=lambda k: stages[k])
     cands = None
@@ -447,28 +508,41 @@ def main():
                       "run_mode": args.mode, "graph_branches": args.branches, "branch_autotune": tuned,
                       "mean_keypoints_per_image": round(float(np.mean(kps)), 1),
                       "mean_stereo_matches": round(float(np.mean(matched)), 1)},
-           "roofline": roofline}
-    if not args.no_extras:
+           "roofline": roofline, "host_to_host": host_to_host}
+    want = set() if args.no_extras else (
+        {"knn", "batch", "serving", "bow", "mappoint", "winmatch", "projection", "hbm"} if args.extras == "all"
+        else set(args.extras.split(",")))
+    cpu_s = 0.0 if args.no_cpu else min(args.cpu_seconds, 3.0)
+    if "hbm" in want:
         import ctypes
         ms = ctypes.c_float()
         nbytes = 1 << 30
         if orb.lib().orb_hbm_copy_time(ctx.h, nbytes, 10, ctypes.byref(ms)) == 0:
             out["hbm_copy_microbench"] = {"bytes_moved": 2 * nbytes, "ms": round(ms.value, 4),
                                           "GB_per_s": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
+    if "knn" in want:
         print("[bench] dense k=2 search", file=sys.stderr, flush=True)
         out["knn2"] = knn_extra(orb, ctx, max(3, args.steps // 2))
+    if "batch" in want:
         print("[bench] mixed-size batch", file=sys.stderr, flush=True)
         out["mixed_batch"] = batch_extra(orb, ctx, max(5, args.steps // 2))
         print(f"[bench] mixed-size batch: {out['mixed_batch']}", file=sys.stderr, flush=True)
+    if "serving" in want:
         print("[bench] serving", file=sys.stderr, flush=True)
         out["serving"] = serving_extra(orb, ctx)
+    if "bow" in want:
         print("[bench] vocabulary transform", file=sys.stderr, flush=True)
         out["bow"] = bow_extra(orb, ctx, max(args.steps, 5), 0.0 if args.no_cpu else min(args.cpu_seconds, 5.0))
+    if "mappoint" in want:
         print("[bench] map-point descriptors", file=sys.stderr, flush=True)
-        out["distinctive_descriptors"] = mappoint_extra(orb, ctx, max(args.steps, 5),
-                                                        0.0 if args.no_cpu else min(args.cpu_seconds, 3.0))
+        out["distinctive_descriptors"] = mappoint_extra(orb, ctx, max(args.steps, 5), cpu_s)
+    if "winmatch" in want:
         print("[bench] windowed matching", file=sys.stderr, flush=True)
-        out["search_for_initialization"] = winmatch_extra(orb, ctx, 0.0 if args.no_cpu else min(args.cpu_seconds, 3.0))
+        out["search_for_initialization"] = winmatch_extra(orb, ctx, cpu_s)
+    if "projection" in want:
+        print("[bench] search by projection", file=sys.stderr, flush=True)
+        out["search_by_projection"] = projection_extra(orb, ctx, cpu_s)
+        print(f"[bench] search by projection: {out['search_by_projection']}", file=sys.stderr, flush=True)
     out["parity_checked"] = None
     if not args.no_cpu:
         print("[bench] checking sampled pairs against the oracle, then timing it", file=sys.stderr, flush=True)

@@ -54,6 +54,13 @@ int orb_extractor_create(orb_context* ctx, int nfeatures, float scaleFactor, int
                          int minThFAST, orb_extractor** out);
 void orb_extractor_destroy(orb_extractor* ex);
 
+/* How resize and the 8U Gaussian round, i.e. which OpenCV build the results
+ * match.  No counterpart in the reference, whose rounding is its OpenCV
+ * library's: mode 1 (the default) = the SSE2 kernels every x86 build of
+ * OpenCV 2.4/3.x runs (resize's VResizeLinearVec_32s8u, the Gaussian's
+ * SymmColumnVec_32s8u), mode 0 = OpenCV's generic scalar code (builds
+ * without SSE2).  Takes effect at the next extraction. */
+int orb_extractor_set_opencv_rounding(orb_extractor* ex, int mode);
 /* GetLevels / GetScaleFactor (ORBextractor.h:119-124). */
 int orb_extractor_levels(const orb_extractor* ex);
 float orb_extractor_scale_factor(const orb_extractor* ex);
@@ -212,6 +219,19 @@ int orb_batch_stage_count(const orb_batch* b);
 int orb_batch_stage_times(orb_batch* b, int steps, float* ms);
 int orb_batch_results(orb_batch* b, int slot, orb_keypoint* keypoints, int capacity, uint8_t* descriptors, int* n);
 int orb_batch_stereo_results(orb_batch* b, int pair, float* uRight, float* depth, int capacity, int* n);
+/* Whole-batch host transfers (no reference counterpart; the per-slot calls
+ * above wait once per slot).  orb_batch_upload_images enqueues the copies of
+ * n host images into the given slots on the batch's stream and returns: the
+ * next orb_batch_run is ordered after them, and the host buffers may be
+ * reused on return.  orb_batch_read_all waits for the batch, then copies
+ * every slot's keypoints and descriptors (slot s at keypoints + s *
+ * capacity, descriptors + s * capacity * 32, counts[s] of them) and, for
+ * stereo pairs, every pair's uRight / depth (pair p at uRight + p * capacity;
+ * either may be NULL), with one wait for the counts and one for the data. */
+int orb_batch_upload_images(orb_batch* b, int n, const int* slots, const uint8_t* const* images,
+                            const size_t* strides);
+int orb_batch_read_all(orb_batch* b, orb_keypoint* keypoints, uint8_t* descriptors, int capacity, int* counts,
+                       float* uRight, float* depth);
 
 /* ---------------------------------------------------------------- serving
  * Continuous-batching request loop for many camera streams.  No reference

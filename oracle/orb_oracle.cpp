@@ -67,6 +67,15 @@ struct Cand {
 enum { PLANT_ATAN = 1, PLANT_RETAIN = 2, PLANT_STEREO = 3, PLANT_PROJECTION = 4, PLANT_FAMILIES = 5 };
 int g_plant[PLANT_FAMILIES] = {0, 0, 0, 0, 0};
 
+// OpenCV 2.4/3.x kernels whose SSE2 form rounds differently from the
+// generic scalar one.  Every x86 build of OpenCV runs the SSE2 form (it is
+// compiled in and checkHardwareSupport(CV_CPU_SSE2) holds on every x86-64
+// CPU), so that is the default here; oracle_set_opencv_simd selects the
+// scalar form per kernel (builds without SIMD, other architectures):
+// bit 0: VResizeLinearVec_32s8u, the vertical pass of resize INTER_LINEAR 8U;
+// bit 1: SymmColumnVec_32s8u, the float column pass of the 8U Gaussian.
+int g_simd = 3;
+
 // ------------------------------------------------------------ cv::resize
 // cv::resize(src, dst, dsize, 0, 0, INTER_LINEAR) for CV_8UC1, OpenCV
 // 2.4/3.x generic path (resizeGeneric_ with HResizeLinear / VResizeLinear):
@@ -104,6 +113,14 @@ void resize_linear(const uint8_t* src, int sw, int sh, int sstride, uint8_t* dst
             out[dx] = s[sx] * ia[2 * dx] + s[nx] * ia[2 * dx + 1];
         }
     };
+    // VResizeLinearVec_32s8u covers dx < simd_end: 16 pixels at a time while
+    // dx <= dw - 16, then 4 at a time while dx < dw - 4; the scalar pass the rest.
+    int simd_end = 0;
+    if (g_simd & 1) {
+        while (simd_end <= dw - 16) simd_end += 16;
+        while (simd_end < dw - 4) simd_end += 4;
+    }
+    auto sat16 = [](int v) { return v < -32768 ? -32768 : (v > 32767 ? 32767 : v); };
     for (int dy = 0; dy < dh; dy++) {
         float fy = (float)((dy + 0.5) * scale_y - 0.5);
         int sy = cv_floor(fy);
@@ -115,7 +132,15 @@ void resize_linear(const uint8_t* src, int sw, int sh, int sstride, uint8_t* dst
         hpass(r0, h0);
         hpass(r1, h1);
         uint8_t* d = dst + (size_t)dy * dw;
-        for (int dx = 0; dx < dw; dx++)
+        for (int dx = 0; dx < simd_end; dx++) {
+            // (S >> 4) packed to int16, _mm_mulhi_epi16 by the 16-bit taps,
+            // saturating adds of the two products and of 2, arithmetic >> 2,
+            // unsigned saturation to 8 bits
+            const int x0 = sat16(h0[dx] >> 4), y0 = sat16(h1[dx] >> 4);
+            const int v = sat16(sat16(((x0 * b0) >> 16) + ((y0 * b1) >> 16)) + 2) >> 2;
+            d[dx] = sat_u8(v);
+        }
+        for (int dx = simd_end; dx < dw; dx++)
             d[dx] = sat_u8((h0[dx] * b0 + h1[dx] * b1 + (1 << 21)) >> 22);
     }
 }
@@ -166,11 +191,30 @@ void gaussian7(const uint8_t* src, int w, int h, int stride, uint8_t* dst)
             r[x] = acc;
         }
     }
+    // SymmColumnVec_32s8u covers x < simd_end: 16 columns at a time while
+    // x <= w - 16, then 4 at a time while x <= w - 4.  It evaluates the
+    // column taps as floats k / 2^16: s = S0 * ky0 + 0, s += (S-t + S+t) *
+    // kyt, then _mm_cvtps_epi32 (round half to even) and saturating packs.
+    int simd_end = 0;
+    if (g_simd & 2) {
+        while (simd_end <= w - 16) simd_end += 16;
+        while (simd_end <= w - 4) simd_end += 4;
+    }
+    float ky[4];
+    for (int t = 0; t < 4; t++) ky[t] = (float)(k[3 + t] * (1. / 65536));
     for (int y = 0; y < h; y++) {
         uint8_t* d = dst + (size_t)y * w;
-        for (int x = 0; x < w; x++) {
+        const int* rm[7];
+        for (int t = -3; t <= 3; t++) rm[t + 3] = rows.data() + (size_t)reflect101(y + t, h) * w;
+        for (int x = 0; x < simd_end; x++) {
+            float sf = (float)rm[3][x] * ky[0] + 0.0f;
+            for (int t = 1; t <= 3; t++) sf = sf + (float)(rm[3 + t][x] + rm[3 - t][x]) * ky[t];
+            const int v = (int)std::nearbyint(sf);
+            d[x] = sat_u8(v < -32768 ? -32768 : (v > 32767 ? 32767 : v));
+        }
+        for (int x = simd_end; x < w; x++) {
             int acc = 0;
-            for (int t = -3; t <= 3; t++) acc += k[t + 3] * rows[(size_t)reflect101(y + t, h) * w + x];
+            for (int t = -3; t <= 3; t++) acc += k[t + 3] * rm[t + 3][x];
             d[x] = sat_u8((acc + (1 << 15)) >> 16);
         }
     }
@@ -1014,6 +1058,13 @@ int oracle_plant(int family, int variant)
 }
 
 int oracle_planted(int family) { return family > 0 && family < PLANT_FAMILIES ? g_plant[family] : 0; }
+
+int oracle_set_opencv_simd(int flags)
+{
+    if (flags < 0 || flags > 3) return -1;
+    g_simd = flags;
+    return 0;
+}
 
 int oracle_distribute_octree(const int* xyr, int n, int minX, int maxX, int minY, int maxY, int N, int* out,
                              int cap, int* nout)

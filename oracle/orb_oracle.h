@@ -84,6 +84,12 @@ int oracle_distribute_octree_leaves(const int* xyr, int n, int minX, int maxX, i
 int oracle_plant(int family, int variant);
 int oracle_planted(int family);
 
+/* Which form of OpenCV's 8U kernels the oracle follows: bit 0 set = the
+ * SSE2 resize vertical pass (VResizeLinearVec_32s8u), bit 1 set = the SSE2
+ * Gaussian column pass (SymmColumnVec_32s8u); clear = the generic scalar
+ * pass.  Default 3, what every x86 build of OpenCV 2.4/3.x runs. */
+int oracle_set_opencv_simd(int flags);
+
 /* Stand-alone primitives. */
 int oracle_resize_linear(const uint8_t* src, int sw, int sh, int sstride,
                          uint8_t* dst, int dw, int dh);

@@ -70,6 +70,7 @@ def lib():
             "oracle_search_by_projection": (I, [P, P, P, P, I, P, P, P, P, I, F, F, P, P]),
             "oracle_search_by_projection_last": (I, [P, P, P, P, I, P, P, P, I, P, P, I, F, I, P, P]),
             "oracle_planted": (I, [I]),
+            "oracle_set_opencv_simd": (I, [I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -233,6 +234,28 @@ class planted:
 
     def __exit__(self, *exc):
         lib().oracle_plant(self.family, 0)
+        return False
+
+
+OPENCV_SSE2 = 3          # the default: both kernels as x86 OpenCV builds run them
+
+
+class opencv_simd:
+    """Context manager: which form of OpenCV's 8U kernels the oracle follows
+    (flags bit 0: SSE2 resize vertical pass, bit 1: SSE2 Gaussian column
+    pass; a clear bit selects the generic scalar pass).  Restores the
+    default (both SSE2) on exit."""
+
+    def __init__(self, flags):
+        self.flags = flags
+
+    def __enter__(self):
+        if lib().oracle_set_opencv_simd(self.flags) != 0:
+            raise ValueError(self.flags)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_opencv_simd(OPENCV_SSE2)
         return False
 
 

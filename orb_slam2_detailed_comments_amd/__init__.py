@@ -42,6 +42,9 @@ _ERRORS = {-1: "bad argument", -2: "output capacity too small", -3: "HIP device 
 _P, _I, _F, _S, _V = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t, None
 SIGNATURES = {
     "orb_context_create": (_I, [_I, _P]),
+    "orb_batch_upload_images": (_I, [_P, _I, _P, _P, _P]),
+    "orb_batch_read_all": (_I, [_P, _P, _P, _I, _P, _P, _P]),
+    "orb_extractor_set_opencv_rounding": (_I, [_P, _I]),
     "orb_context_destroy": (_V, [_P]),
     "orb_context_synchronize": (_I, [_P]),
     "orb_extractor_create": (_I, [_P, _I, _F, _I, _I, _I, _P]),
@@ -180,7 +183,10 @@ class ORBextractor:
     Mirrors include/ORBextractor.h:93-162; the pixel work runs on the GPU.
     """
 
-    def __init__(self, nfeatures, scaleFactor, nlevels, iniThFAST, minThFAST, context=None):
+    def __init__(self, nfeatures, scaleFactor, nlevels, iniThFAST, minThFAST, context=None, opencv_sse2=True):
+        """opencv_sse2: round resize and the Gaussian as x86 builds of OpenCV
+        2.4/3.x do (their SSE2 kernels; the default), else as OpenCV's generic
+        scalar code."""
         self.ctx = context or Context.default()
         h = ctypes.c_void_p()
         _check("orb_extractor_create", lib().orb_extractor_create(self.ctx.h, int(nfeatures), float(scaleFactor),
@@ -189,6 +195,8 @@ class ORBextractor:
         self.h = h
         self.nfeatures, self.nlevels = int(nfeatures), int(nlevels)
         self._size = None
+        if not opencv_sse2:
+            _check("orb_extractor_set_opencv_rounding", lib().orb_extractor_set_opencv_rounding(self.h, 0))
 
     def close(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -424,6 +432,33 @@ class Batch:
         if img.shape != (self.sizes[slot][1], self.sizes[slot][0]):
             raise ValueError(f"slot {slot} expects {self.sizes[slot]}, got {img.shape[::-1]}")
         _check("orb_batch_upload", lib().orb_batch_upload(self.h, slot, _ptr(img), img.strides[0]))
+
+    def upload_all(self, images, slots=None):
+        """Enqueue host images into slots (default 0..n-1) without waiting."""
+        slots = list(range(len(images))) if slots is None else list(slots)
+        imgs = [np.ascontiguousarray(im, np.uint8) for im in images]
+        for s, im in zip(slots, imgs):
+            if im.shape != (self.sizes[s][1], self.sizes[s][0]):
+                raise ValueError(f"slot {s} expects {self.sizes[s]}, got {im.shape[::-1]}")
+        sl = np.asarray(slots, np.int32)
+        ptrs = (ctypes.c_void_p * len(imgs))(*[im.ctypes.data for im in imgs])
+        strides = np.asarray([im.strides[0] for im in imgs], np.uint64)
+        _check("orb_batch_upload_images", lib().orb_batch_upload_images(self.h, len(imgs), _ptr(sl), ptrs,
+                                                                        _ptr(strides)))
+
+    def read_all(self, capacity=None, out=None):
+        """Every slot's (keypoints, descriptors) and every pair's (uRight,
+        depth) after one wait; `out` (from a previous call) is refilled."""
+        cap = int(capacity or (int(self.nfeatures.max()) + 64 * self.nlevels))
+        n, npairs = len(self.sizes), self.npairs
+        if out is None:
+            out = {"kps": np.zeros((n, cap), KEYPOINT_DTYPE), "desc": np.zeros((n, cap, 32), np.uint8),
+                   "counts": np.zeros(n, np.int32), "uR": np.zeros((max(npairs, 1), cap), np.float32),
+                   "depth": np.zeros((max(npairs, 1), cap), np.float32)}
+        _check("orb_batch_read_all", lib().orb_batch_read_all(self.h, _ptr(out["kps"]), _ptr(out["desc"]), cap,
+                                                              _ptr(out["counts"]), _ptr(out["uR"]),
+                                                              _ptr(out["depth"])))
+        return out
 
     def set_stereo(self, left_slots, right_slots, bf, baseline):
         l = np.ascontiguousarray(left_slots, np.int32)

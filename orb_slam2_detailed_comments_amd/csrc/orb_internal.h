@@ -76,6 +76,17 @@ inline void disc_umax(int umax[16])
 }
 
 // Border index of REFLECT_101 (the level kernel's halo rule).
+// First level column that OpenCV's SSE2 VResizeLinearVec_32s8u leaves to
+// the scalar pass: it covers 16 columns at a time while x <= w - 16, then 4
+// at a time while x < w - 4 (imgwarp.cpp, OpenCV 2.4/3.x).
+inline int sse2_resize_end(int w)
+{
+    int x = 0;
+    while (x <= w - 16) x += 16;
+    while (x < w - 4) x += 4;
+    return x;
+}
+
 inline int reflect101(int i, int n)
 {
     if (i < 0) i = -i;
@@ -155,6 +166,9 @@ struct orb_extractor {
     int nfeatures = 0, nlevels = 0, iniTh = 0, minTh = 0;
     float scaleFactorF = 1.2f;
     double scaleFactor = 1.2;
+    // 1: resize and Gaussian 8U rounded as OpenCV's SSE2 kernels do (every
+    // x86 build of OpenCV 2.4/3.x); 0: as its generic scalar code
+    int rounding = 1;
     std::vector<float> scale, invScale, sigma2, invSigma2;
     std::vector<int> quota;
 
@@ -372,6 +386,8 @@ struct orb_extractor {
             L.key_cap = P.key_cap;
             L.ncells = P.ncells;
             L.list_cap = P.list_cap;
+            L.rs_end = rounding ? sse2_resize_end(P.w) : 0;
+            L.bs_end = rounding ? (P.w & ~3) : 0;
             L.lvl = (uint8_t*)(d_mem + o_lvl + P.img_off);
             L.blur = (uint8_t*)(d_mem + o_blur + P.img_off);
             L.cellflag = (uint8_t*)(d_mem + o_cell + P.cell_off);
@@ -391,6 +407,7 @@ struct orb_extractor {
                 const int w = p.lv[l].w, h = p.lv[l].h, sw = p.lv[l - 1].w, sh = p.lv[l - 1].h;
                 J.lv[l].xtab = (const int2*)(d_mem + o_tab) + t;
                 const double scale_x = 1. / ((double)w / sw);
+                const int rs_end = J.lv[l].rs_end;
                 for (int x = 0; x < w; x++, t++) {
                     float fx = (float)((x + 0.5) * scale_x - 0.5);
                     int sx = (int)std::floor(fx);
@@ -403,7 +420,7 @@ struct orb_extractor {
                         fx = 0.f;
                         sx = sw - 1;
                     }
-                    tab[t].x = sx;
+                    tab[t].x = sx | ((x < rs_end ? 1 : 0) << 30);
                     tab[t].y = (int)std::nearbyint((1.f - fx) * 2048.f) | ((int)std::nearbyint(fx * 2048.f) << 16);
                 }
                 const size_t tx = t - w;
@@ -426,7 +443,7 @@ struct orb_extractor {
                 for (int c = 0; c < p.lv[l].tiles_x; c++, t++) {
                     int lo = INT_MAX, hi = -1;
                     for (int i = 0; i < ORB_TILE_W + 8; i++) {
-                        const int sx = tab[tx + reflect101(c * ORB_TILE_W - 4 + i, w)].x;
+                        const int sx = tab[tx + reflect101(c * ORB_TILE_W - 4 + i, w)].x & 0x3FFFFFFF;
                         lo = std::min(lo, sx);
                         hi = std::max(hi, sx + 1 < sw ? sx + 1 : sx);
                     }

@@ -260,9 +260,31 @@ __device__ __forceinline__ unsigned hi_bytes(unsigned x) { return __builtin_amdg
 //   32-bit ones (v_mul_lo_u32, quarter rate).
 typedef unsigned short orb_u16x2 __attribute__((ext_vector_type(2)));
 
+// The vertical pass on the horizontal sums S0, S1 (each <= 255 * 2049) with
+// the row taps b0, b1, rounded as OpenCV's generic scalar code does
+// ((S0*b0 + S1*b1 + 2^21) >> 22) or as its SSE2 kernel VResizeLinearVec_32s8u
+// does (S >> 4 as int16, _mm_mulhi_epi16 by the taps, + 2, >> 2).  Neither
+// needs a clamp or a saturation: every tap pair sums to 2047..2049 (two
+// roundings of values summing to 2048), so the scalar form is at most
+// (255 * 2049 * 2049 + 2^21) >> 22 = 255, the SSE2 one at most
+// (2 * ((32655 * 2049) >> 17) + 2) >> 2 = 255, and S >> 4 <= 32655 fits int16.
+// MODE 0: scalar, 1: SSE2, 2: SSE2 where sse != 0.
+template <int MODE>
+__device__ __forceinline__ unsigned vresize(unsigned S0, unsigned S1, unsigned b0, unsigned b1, unsigned sse)
+{
+    const unsigned vc = (__umul24(S0, b0) + __umul24(S1, b1) + (1u << 21)) >> 22;
+    const unsigned vs = ((__umul24(S0 >> 4, b0) >> 16) + (__umul24(S1 >> 4, b1) >> 16) + 2u) >> 2;
+    return MODE == 0 ? vc : (MODE == 1 ? vs : (sse ? vs : vc));
+}
+
+template <int M>
+struct ModeTag {
+    static constexpr int value = M;
+};
+
 template <typename BytePtr>
 __device__ __forceinline__ unsigned resize4(BytePtr base, unsigned o0, unsigned o1, const int* sxk, const int* nxk,
-                                            const int* tap, int yb)
+                                            const int* tap, int yb, const unsigned* sse)
 {
     const unsigned b0 = (unsigned)yb & 0xFFFFu, b1 = (unsigned)yb >> 16;
     unsigned d = 0;
@@ -271,10 +293,7 @@ __device__ __forceinline__ unsigned resize4(BytePtr base, unsigned o0, unsigned 
         const unsigned a0 = (unsigned)tap[k] & 0xFFFFu, a1 = (unsigned)tap[k] >> 16;
         const unsigned S0 = __umul24(base[o0 + sxk[k]], a0) + __umul24(base[o0 + nxk[k]], a1);
         const unsigned S1 = __umul24(base[o1 + sxk[k]], a0) + __umul24(base[o1 + nxk[k]], a1);
-        // Each tap pair sums to 2047..2049 (two roundings of values summing
-        // to 2048), so v <= (255 * 2049 * 2049 + 2^21) >> 22 = 255: no clamp.
-        const unsigned v = (__umul24(S0, b0) + __umul24(S1, b1) + (1u << 21)) >> 22;
-        d |= v << (8 * k);
+        d |= vresize<2>(S0, S1, b0, b1, sse[k]) << (8 * k);
     }
     return d;
 }
@@ -456,13 +475,21 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             const int c = tid % LP_WD, r0 = tid / LP_WD;
             const int xoff = staged ? wx0 : 0;
             int sxk[4], nxk[4], tap[4];
+            unsigned sse[4];                       // SSE2 rounding for the column (xtab bit 30)
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const int sx = xs0[4 * c + k];
+                const int e = xs0[4 * c + k];
+                const int sx = e & 0x3FFFFFFF;
+                sse[k] = (unsigned)e >> 30;
                 sxk[k] = sx - xoff;
                 nxk[k] = (sx + 1 < sw ? sx + 1 : sx) - xoff;
                 tap[k] = xa[4 * c + k];
             }
+            // Rounding of the tile's columns (reflected halo included): all
+            // scalar, all SSE2, or per column where the tile reaches the last
+            // columns the SSE2 kernel leaves to the scalar pass.
+            const int maxcol = max(min(tx0 + LT_W + LT_HALO - 1, w - 1), LT_HALO);
+            const int rmode = L.rs_end == 0 ? 0 : (maxcol < L.rs_end ? 1 : 2);
             // The four pixels' source bytes usually lie within 8 bytes (scale
             // factors up to ~2.3; reflected halo columns may run backwards, so
             // the span is taken over all of them): then a source row is three
@@ -483,37 +510,41 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
 #pragma unroll
                 for (int k = 0; k < 4; k++)
                     sel[k] = 0x0C000C00u | ((unsigned)(nxk[k] - b0) << 16) | (unsigned)(sxk[k] - b0);
-                for (int ey = r0; ey < LP_H; ey += 14) {
-                    const unsigned sy01 = (unsigned)ys0[ey];
-                    const unsigned* q0 = wd + __umul24((sy01 & 0xFFFFu) - wy0, ws);
-                    const unsigned* q1 = wd + __umul24((sy01 >> 16) - wy0, ws);
-                    const unsigned w00 = __builtin_amdgcn_alignbyte(q0[1], q0[0], shb);
-                    const unsigned w01 = __builtin_amdgcn_alignbyte(q0[2], q0[1], shb);
-                    const unsigned w10 = __builtin_amdgcn_alignbyte(q1[1], q1[0], shb);
-                    const unsigned w11 = __builtin_amdgcn_alignbyte(q1[2], q1[1], shb);
-                    const unsigned bv0 = (unsigned)yb[ey] & 0xFFFFu, bv1 = (unsigned)yb[ey] >> 16;
-                    unsigned d = 0;
+                auto rows = [&](auto mode) {
+                    constexpr int MODE = decltype(mode)::value;
+                    for (int ey = r0; ey < LP_H; ey += 14) {
+                        const unsigned sy01 = (unsigned)ys0[ey];
+                        const unsigned* q0 = wd + __umul24((sy01 & 0xFFFFu) - wy0, ws);
+                        const unsigned* q1 = wd + __umul24((sy01 >> 16) - wy0, ws);
+                        const unsigned w00 = __builtin_amdgcn_alignbyte(q0[1], q0[0], shb);
+                        const unsigned w01 = __builtin_amdgcn_alignbyte(q0[2], q0[1], shb);
+                        const unsigned w10 = __builtin_amdgcn_alignbyte(q1[1], q1[0], shb);
+                        const unsigned w11 = __builtin_amdgcn_alignbyte(q1[2], q1[1], shb);
+                        const unsigned bv0 = (unsigned)yb[ey] & 0xFFFFu, bv1 = (unsigned)yb[ey] >> 16;
+                        unsigned d = 0;
 #pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const orb_u16x2 tk = __builtin_bit_cast(orb_u16x2, (unsigned)tap[k]);
-                        const unsigned S0 = __builtin_amdgcn_udot2(
-                            __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w01, w00, sel[k])), tk, 0u, false);
-                        const unsigned S1 = __builtin_amdgcn_udot2(
-                            __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w11, w10, sel[k])), tk, 0u, false);
-                        // <= 255: see resize4
-                        const unsigned v = (__umul24(S0, bv0) + __umul24(S1, bv1) + (1u << 21)) >> 22;
-                        d |= v << (8 * k);
+                        for (int k = 0; k < 4; k++) {
+                            const orb_u16x2 tk = __builtin_bit_cast(orb_u16x2, (unsigned)tap[k]);
+                            const unsigned S0 = __builtin_amdgcn_udot2(
+                                __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w01, w00, sel[k])), tk, 0u, false);
+                            const unsigned S1 = __builtin_amdgcn_udot2(
+                                __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w11, w10, sel[k])), tk, 0u, false);
+                            d |= vresize<MODE>(S0, S1, bv0, bv1, sse[k]) << (8 * k);
+                        }
+                        pxw[ey][c] = d;
+                        level_out(ey, c, d);
                     }
-                    pxw[ey][c] = d;
-                    level_out(ey, c, d);
-                }
+                };
+                if (rmode == 1) rows(ModeTag<1>{});
+                else if (rmode == 0) rows(ModeTag<0>{});
+                else rows(ModeTag<2>{});
             } else if (staged) {
                 const uint8_t* wb = (const uint8_t*)u.win;
                 const int ws = nwd * 4;
                 for (int ey = r0; ey < LP_H; ey += 14) {
                     const unsigned sy01 = (unsigned)ys0[ey];
                     const unsigned d = resize4(wb, __umul24((sy01 & 0xFFFFu) - wy0, ws),
-                                               __umul24((sy01 >> 16) - wy0, ws), sxk, nxk, tap, yb[ey]);
+                                               __umul24((sy01 >> 16) - wy0, ws), sxk, nxk, tap, yb[ey], sse);
                     pxw[ey][c] = d;
                     level_out(ey, c, d);
                 }
@@ -522,7 +553,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                     const unsigned sy01 = (unsigned)ys0[ey];
                     const unsigned d = resize4((const __attribute__((address_space(1))) uint8_t*)P.lvl,
                                                __umul24(sy01 & 0xFFFFu, sp), __umul24(sy01 >> 16, sp), sxk, nxk, tap,
-                                               yb[ey]);
+                                               yb[ey], sse);
                     pxw[ey][c] = d;
                     level_out(ey, c, d);
                 }
@@ -583,6 +614,13 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // rows r and r + 1 of the tile from 8 row-sum vectors
         static_assert((LT_H / 2) * (LT_W / 4) == 256, "one row pair and column group per thread");
         const int r = (tid >> 4) * 2, x0 = tx0 + 4 * gc;
+        // Column rounding: columns < bs_end as OpenCV's SSE2 column kernel
+        // (SymmColumnVec_32s8u: the taps as floats k / 2^16, the sum rounded
+        // half to even, which for these exact sums is (acc + 0x7FFF + bit 16
+        // of acc) >> 16), the others as its scalar one ((acc + 2^15) >> 16).
+        unsigned rb[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) rb[k] = x0 + k < L.bs_end ? 1u : 0u;
         uint4 q[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) q[i] = u.hs[r + i][gc];
@@ -597,7 +635,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                                      __umul24(q[hh + 2].comp + q[hh + 4].comp, k1) +                     \
                                      __umul24(q[hh + 1].comp + q[hh + 5].comp, k2) +                     \
                                      __umul24(q[hh].comp + q[hh + 6].comp, k3);                          \
-                const unsigned v = (acc + (1u << 15)) >> 16;                                            \
+                const unsigned v = (acc + (1u << 15) - rb[k] + ((acc >> 16) & rb[k])) >> 16;            \
                 out |= (v > 255u ? 255u : v) << (8 * (k));                                              \
             }
             ORB_VBLUR(x, 0) ORB_VBLUR(y, 1) ORB_VBLUR(z, 2) ORB_VBLUR(w, 3)
@@ -1626,8 +1664,13 @@ __global__ __launch_bounds__(256) void stereo_kernel(const StereoArgs* __restric
 {
     const int sub = threadIdx.x & (ST_G - 1);
     const int gbase = threadIdx.x & 63 & ~(ST_G - 1);
-    const StereoArgs& A = args[blockIdx.y];
-    const int iL = blockIdx.x * (256 / ST_G) + (int)(threadIdx.x / ST_G);
+    // XCD-aware order: each XCD takes a contiguous run of (pair, keypoint
+    // block), so the patch rows that a pair's neighbouring keypoints share
+    // are fetched into one L2 rather than into all of them.
+    const int lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int pair = lb / gridDim.x, kblk = lb - pair * gridDim.x;
+    const StereoArgs& A = args[pair];
+    const int iL = kblk * (256 / ST_G) + (int)(threadIdx.x / ST_G);
     const int nl = *A.nl, nr = *A.nr;
     if (iL >= nl || iL >= A.cap_l) return;             // the whole group leaves
     float outU = -1.0f, outD = -1.0f;

@@ -88,6 +88,16 @@ void orb_extractor_destroy(orb_extractor* ex)
     delete ex;
 }
 
+int orb_extractor_set_opencv_rounding(orb_extractor* ex, int mode)
+{
+    if (!ex || (mode != 0 && mode != 1)) return ORB_E_ARG;
+    if (ex->rounding != mode) {
+        ex->rounding = mode;
+        ex->plan.W = ex->plan.H = 0;   // the next call rebuilds the level tables
+    }
+    return ORB_OK;
+}
+
 int orb_extractor_levels(const orb_extractor* ex) { return ex ? ex->nlevels : ORB_E_ARG; }
 
 float orb_extractor_scale_factor(const orb_extractor* ex) { return ex ? (float)ex->scaleFactor : 0.f; }
@@ -574,8 +584,15 @@ struct orb_batch {
         drop_branches();
     }
 
+    // pinned staging of orb_batch_read_all
+    int* h_ctr = nullptr;
+    char* h_res = nullptr;
+    size_t h_res_cap = 0;
+
     ~orb_batch()
     {
+        if (h_ctr) (void)hipHostFree(h_ctr);
+        if (h_res) (void)hipHostFree(h_res);
         drop_graph();
         for (hipStream_t s2 : br_st) (void)hipStreamDestroy(s2);
         for (hipEvent_t e : br_ev) (void)hipEventDestroy(e);
@@ -657,6 +674,89 @@ int orb_batch_upload(orb_batch* b, int slot, const uint8_t* image, size_t stride
     int rc = e->upload_image(image, stride, hipMemcpyHostToDevice);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    return ORB_OK;
+}
+
+int orb_batch_upload_images(orb_batch* b, int n, const int* slots, const uint8_t* const* images,
+                            const size_t* strides)
+{
+    if (!b || n < 0 || (n && (!slots || !images || !strides))) return ORB_E_ARG;
+    for (int i = 0; i < n; i++)
+        if (slots[i] < 0 || slots[i] >= (int)b->ex.size() || !images[i] || strides[i] < (size_t)b->ex[slots[i]]->plan.W)
+            return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    for (int i = 0; i < n; i++) {
+        const int rc = b->ex[slots[i]]->upload_image(images[i], strides[i], hipMemcpyHostToDevice);
+        if (rc) return rc;
+    }
+    return ORB_OK;
+}
+
+int orb_batch_read_all(orb_batch* b, orb_keypoint* keypoints, uint8_t* descriptors, int capacity, int* counts,
+                       float* uRight, float* depth)
+{
+    if (!b || capacity < 0 || !counts || (capacity && (!keypoints || !descriptors))) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    const int n = (int)b->ex.size(), np = (int)b->pl.size();
+    if (!b->h_ctr && hipHostMalloc((void**)&b->h_ctr, (size_t)n * ORB_CNT_SLOTS * sizeof(int)) != hipSuccess) {
+        b->h_ctr = nullptr;
+        return ORB_E_OOM;
+    }
+    for (int s = 0; s < n; s++)
+        HIP_TRY(hipMemcpyAsync(b->h_ctr + (size_t)s * ORB_CNT_SLOTS, b->ex[s]->hjob.counters,
+                               ORB_CNT_SLOTS * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    size_t need = 0;
+    for (int s = 0; s < n; s++) {
+        const int* c = b->h_ctr + (size_t)s * ORB_CNT_SLOTS;
+        if (c[ORB_CNT_ERR]) return ORB_E_OVERFLOW;
+        counts[s] = c[ORB_CNT_NOUT];
+        if (counts[s] > capacity) return ORB_E_CAPACITY;
+        need += (size_t)counts[s] * (sizeof(orb_keypoint) + 32);
+    }
+    for (int p = 0; p < np; p++) need += (size_t)counts[b->pl[p]] * 8;
+    if (need > b->h_res_cap) {
+        if (b->h_res) (void)hipHostFree(b->h_res);
+        b->h_res = nullptr;
+        b->h_res_cap = 0;
+        if (hipHostMalloc((void**)&b->h_res, need) != hipSuccess) return ORB_E_OOM;
+        b->h_res_cap = need;
+    }
+    size_t o = 0;
+    std::vector<size_t> at(n + np);
+    for (int s = 0; s < n; s++) {
+        const size_t c = (size_t)counts[s];
+        at[s] = o;
+        if (c) {
+            HIP_TRY(hipMemcpyAsync(b->h_res + o, b->ex[s]->hjob.kps, c * sizeof(orb_keypoint), hipMemcpyDeviceToHost,
+                                   st));
+            HIP_TRY(hipMemcpyAsync(b->h_res + o + c * sizeof(orb_keypoint), b->ex[s]->hjob.desc, c * 32,
+                                   hipMemcpyDeviceToHost, st));
+        }
+        o += c * (sizeof(orb_keypoint) + 32);
+    }
+    for (int p = 0; p < np; p++) {
+        const orb_extractor* L = b->ex[b->pl[p]];
+        const size_t c = (size_t)counts[b->pl[p]];
+        at[n + p] = o;
+        if (c) {
+            HIP_TRY(hipMemcpyAsync(b->h_res + o, L->d_uR, c * 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(b->h_res + o + c * 4, L->d_depth, c * 4, hipMemcpyDeviceToHost, st));
+        }
+        o += c * 8;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int s = 0; s < n; s++) {
+        const size_t c = (size_t)counts[s];
+        std::memcpy(keypoints + (size_t)s * capacity, b->h_res + at[s], c * sizeof(orb_keypoint));
+        std::memcpy(descriptors + (size_t)s * capacity * 32, b->h_res + at[s] + c * sizeof(orb_keypoint), c * 32);
+    }
+    for (int p = 0; p < np; p++) {
+        const size_t c = (size_t)counts[b->pl[p]];
+        if (uRight) std::memcpy(uRight + (size_t)p * capacity, b->h_res + at[n + p], c * 4);
+        if (depth) std::memcpy(depth + (size_t)p * capacity, b->h_res + at[n + p] + c * 4, c * 4);
+    }
     return ORB_OK;
 }
 

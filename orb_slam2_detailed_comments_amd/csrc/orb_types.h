@@ -52,6 +52,8 @@ struct LevelJob {
     int key_cap;                     // candidate capacity (both thresholds)
     int ncells;                      // nCols * nRows
     int list_cap;                    // kept-keypoint capacity (<= ORB_LIST_CAP)
+    int rs_end;                      // level columns < rs_end: SSE2 resize rounding (else scalar)
+    int bs_end;                      // level columns < bs_end: SSE2 blur-column rounding (else scalar)
     uint8_t* lvl;                    // level image
     uint8_t* blur;                   // 7x7 sigma-2 blur of the level
     uint8_t* cellflag;               // per FAST cell: has an iniThFAST survivor
@@ -59,7 +61,7 @@ struct LevelJob {
     unsigned long long* keys2;       // candidates after the per-cell retry
     int* node;                       // per-candidate quadtree node (scratch)
     unsigned long long* ret;         // kept keypoints in output order
-    const int2* xtab;                // levels >= 1: per column {sx, a0 | a1 << 16} of the
+    const int2* xtab;                // levels >= 1: per column {sx | sse2 << 30, a0 | a1 << 16} of the
     const int2* ytab;                // resize from level-1; per row {sy, b0 | b1 << 16}
     const int2* xwin;                // levels >= 1: per tile column / row, the source
     const int2* ywin;                // columns / rows {first, last} the tile's taps read

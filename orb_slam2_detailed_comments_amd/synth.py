@@ -213,3 +213,58 @@ def vocabulary_queries(voc, n, seed, flip_ands=4):
     words = np.flatnonzero(voc["is_word"])
     pick = voc["desc"][rng.choice(words, size=n)]
     return pick ^ _sparse_bits(rng, pick.shape, flip_ands)
+
+
+PROJ_POINT_DTYPE = np.dtype([("proj_x", "<f4"), ("proj_y", "<f4"), ("proj_xr", "<f4"), ("view_cos", "<f4"),
+                             ("level", "<i4"), ("flags", "<i4")])
+LAST_POINT_DTYPE = np.dtype([("xc", "<f4"), ("yc", "<f4"), ("zc", "<f4"), ("angle", "<f4"), ("octave", "<i4"),
+                             ("flags", "<i4")])
+KEYPOINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"), ("response", "<f4"),
+                           ("octave", "<i4"), ("class_id", "<i4")])
+
+
+def projection_frame(seed, n_kp=2000, n_pts=1800, last=False, width=640, height=480):
+    """A tracking-like SearchByProjection input: keypoints over 8 levels
+    (half with mvuRight, 5 % already holding a map point) and points
+    projected a few px from a keypoint whose descriptor is a noisy copy of
+    theirs.  Returns (kp, desc, uright, occupied, bounds, points, point_desc,
+    camera) with points in the record of the overload (last=True: camera
+    coordinates for fx = fy = 512, cx = 320, cy = 240, mbf = 40)."""
+    rng = np.random.default_rng(seed)
+    scale = np.float32(1.2) ** np.arange(8, dtype=np.float32)
+    kp = np.zeros(n_kp, KEYPOINT_DTYPE)
+    kp["x"] = rng.uniform(0, width, n_kp)
+    kp["y"] = rng.uniform(0, height, n_kp)
+    kp["octave"] = rng.choice(8, n_kp, p=[0.3, 0.2, 0.15, 0.1, 0.09, 0.07, 0.05, 0.04])
+    kp["size"] = 31.0 * scale[kp["octave"]]
+    kp["angle"] = rng.uniform(0, 360, n_kp)
+    kp["class_id"] = -1
+    desc = rng.integers(0, 256, (n_kp, 32), dtype=np.uint8)
+    uright = np.where(rng.random(n_kp) < 0.5, kp["x"] - rng.uniform(0, 40, n_kp), -1.0).astype(np.float32)
+    occupied = (rng.random(n_kp) < 0.05).astype(np.uint8)
+    t = rng.integers(0, n_kp, n_pts)
+    flips = rng.integers(0, 60, n_pts)
+    pdesc = desc[t].copy()
+    for p in range(n_pts):
+        bits = rng.choice(256, flips[p], replace=False)
+        np.bitwise_xor.at(pdesc[p], bits // 8, (1 << (bits % 8)).astype(np.uint8))
+    x = kp["x"][t] + rng.normal(0, 3, n_pts)
+    y = kp["y"][t] + rng.normal(0, 3, n_pts)
+    cam = (512.0, 512.0, 320.0, 240.0, 40.0, 0.08, 0.5 * (seed % 3 - 1))
+    if last:
+        pts = np.zeros(n_pts, LAST_POINT_DTYPE)
+        z = rng.uniform(1.0, 20.0, n_pts)
+        pts["xc"] = (x - cam[2]) * z / cam[0]
+        pts["yc"] = (y - cam[3]) * z / cam[1]
+        pts["zc"] = z
+        pts["angle"] = (kp["angle"][t] + rng.normal(0, 8, n_pts)) % 360
+        pts["octave"] = kp["octave"][t]
+        pts["flags"] = np.where(rng.random(n_pts) < 0.95, 1 | 4, 0)
+    else:
+        pts = np.zeros(n_pts, PROJ_POINT_DTYPE)
+        pts["proj_x"], pts["proj_y"] = x, y
+        pts["proj_xr"] = x - rng.uniform(0, 40, n_pts)
+        pts["view_cos"] = rng.uniform(0.99, 1.0, n_pts)
+        pts["level"] = np.clip(kp["octave"][t] + rng.integers(-1, 2, n_pts), 0, 7)
+        pts["flags"] = np.where(rng.random(n_pts) < 0.95, 1 | 4, 0)
+    return kp, desc, uright, occupied, (0.0, float(width), 0.0, float(height)), pts, pdesc, cam

@@ -2,7 +2,9 @@
 
 The reference cannot be built here (it needs OpenCV), so these fixtures pin
 the oracle's own output on seeded inputs against drift; they are not
-reference vectors.  Run from the repository root:
+reference vectors.  They hold the oracle's default rounding of resize and the
+8U Gaussian (OpenCV's SSE2 kernels, as x86 builds run them).  Run from the
+repository root:
     python tests/golden/make_golden.py
 """
 import os

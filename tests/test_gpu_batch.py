@@ -177,3 +177,30 @@ def test_stereo_batch_32_pairs_graph_autotuned(orb, oracle):
         assert np.array_equal(gdl, odl) and np.array_equal(gdr, odr), (best, p)
         np.testing.assert_array_equal(uR.view(np.uint32), ouR.view(np.uint32), err_msg=f"pair {p}")
         np.testing.assert_array_equal(dep.view(np.uint32), odep.view(np.uint32), err_msg=f"pair {p}")
+
+
+def test_whole_batch_transfers_equal_per_slot_calls(orb, oracle):
+    """upload_all / read_all move the same bytes as per-slot upload / results."""
+    bf, fx = 386.1, 718.856
+    b = float(np.float32(bf) / np.float32(fx))
+    pairs = [synth.stereo_pair(1241, 376, 80 + i)[:2] for i in range(3)]
+    batch = orb.Batch([(1241, 376)] * 6, 2000)
+    batch.set_stereo([0, 2, 4], [1, 3, 5], bf, b)
+    batch.upload_all([im for p in pairs for im in p])
+    batch.run(1)
+    out = batch.read_all()
+    for s in range(6):
+        k, d = batch.results(s)
+        n = out["counts"][s]
+        assert n == len(k)
+        assert out["kps"][s, :n].tobytes() == k.tobytes() and np.array_equal(out["desc"][s, :n], d)
+    for p in range(3):
+        uR, dep = batch.stereo_results(p)
+        n = len(uR)
+        np.testing.assert_array_equal(out["uR"][p, :n].view(np.uint32), uR.view(np.uint32))
+        np.testing.assert_array_equal(out["depth"][p, :n].view(np.uint32), dep.view(np.uint32))
+    ol = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    okl, _, _, _, ouR, _ = oracle.stereo_frame(ol, orr, pairs[1][0], pairs[1][1], bf, b)
+    assert out["kps"][2, :len(okl)].tobytes() == okl.tobytes()
+    np.testing.assert_array_equal(out["uR"][1, :len(ouR)].view(np.uint32), ouR.view(np.uint32))

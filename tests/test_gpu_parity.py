@@ -199,3 +199,24 @@ def test_nfeatures_bound(orb):
     assert orb.ORBextractor(4703, 1.2, 8, 20, 7).capacity(1241, 376) > 0
     with pytest.raises(orb.OrbError):
         orb.ORBextractor(4704, 1.2, 8, 20, 7).capacity(1241, 376)
+
+
+@pytest.mark.parametrize("w,h,nf", [(640, 480, 1000), (1241, 376, 2000), (1280, 720, 2000)])
+def test_generic_scalar_opencv_rounding(orb, oracle, w, h, nf):
+    """opencv_sse2=False: resize and the Gaussian rounded as OpenCV's generic
+    scalar code, against the oracle in the same mode; the default (SSE2)
+    results differ from these."""
+    img = synth.shapes_frame(w, h, 61)
+    g = orb.ORBextractor(nf, 1.2, 8, 20, 7, opencv_sse2=False)
+    gk, gd = g(img)
+    with oracle.opencv_simd(0):
+        o = oracle.OracleExtractor(nf, 1.2, 8, 20, 7)
+        ok, od = o(img)
+    for level in range(8):
+        np.testing.assert_array_equal(g.pyramid_level(level), o.level_image(level), err_msg=f"level {level}")
+        np.testing.assert_array_equal(g.debug_image(level, 1), o.level_image(level, blurred=True),
+                                      err_msg=f"blur {level}")
+    _assert_same_keypoints(gk, gd, ok, od)
+    d = orb.ORBextractor(nf, 1.2, 8, 20, 7)
+    d(img)
+    assert not np.array_equal(d.pyramid_level(1), g.pyramid_level(1))

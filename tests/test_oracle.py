@@ -66,7 +66,7 @@ def test_umax_disc_is_symmetric(oracle):
 
 
 # --------------------------------------------------------------- resize
-def _resize_numpy(src, dw, dh):
+def _resize_numpy(src, dw, dh, sse2):
     """Independent numpy statement of OpenCV's 8U INTER_LINEAR fixed point."""
     sh, sw = src.shape
 
@@ -94,14 +94,46 @@ def _resize_numpy(src, dw, dh):
     h0 = s[r0][:, sx] * a0 + s[r0][:, nx] * a1
     h1 = s[r1][:, sx] * a0 + s[r1][:, nx] * a1
     v = (h0 * b0[:, None] + h1 * b1[:, None] + (1 << 21)) >> 22
+    if sse2:
+        # VResizeLinearVec_32s8u (OpenCV 2.4/3.x imgwarp.cpp, SSE2): (S >> 4)
+        # as int16, _mm_mulhi_epi16 by the int16 taps, saturating adds of the
+        # two and of 2, >> 2, packus.  It runs 16 columns at a time while
+        # x <= dw - 16, then 4 at a time while x < dw - 4; the rest is scalar.
+        end = dw // 16 * 16
+        if dw - 4 > end:
+            end += 4 * ((dw - 4 - end + 3) // 4)
+        x0, y0 = h0 >> 4, h1 >> 4
+        assert x0.max() <= 32767 and y0.max() <= 32767            # packs never saturates here
+        vs = (((x0 * b0[:, None]) >> 16) + ((y0 * b1[:, None]) >> 16) + 2) >> 2
+        v[:, :end] = vs[:, :end]
     return np.clip(v, 0, 255).astype(np.uint8)
 
 
-@pytest.mark.parametrize("sw,sh,dw,dh", [(640, 480, 533, 400), (1241, 376, 1034, 313), (100, 80, 83, 67),
-                                         (83, 67, 100, 80)])
+RESIZE_CASES = [(640, 480, 533, 400), (1241, 376, 1034, 313), (100, 80, 83, 67), (83, 67, 100, 80),
+                (64, 48, 36, 29), (64, 48, 37, 29), (30, 30, 19, 19)]
+
+
+@pytest.mark.parametrize("sw,sh,dw,dh", RESIZE_CASES)
 def test_resize_matches_independent_statement(oracle, sw, sh, dw, dh):
+    """Both roundings: the SSE2 vertical pass every x86 build of OpenCV
+    2.4/3.x runs (the oracle's default), and the generic scalar pass."""
     src = synth.shapes_frame(sw, sh, 3)
-    np.testing.assert_array_equal(oracle.resize_linear(src, dw, dh), _resize_numpy(src, dw, dh))
+    np.testing.assert_array_equal(oracle.resize_linear(src, dw, dh), _resize_numpy(src, dw, dh, sse2=True))
+    with oracle.opencv_simd(2):
+        np.testing.assert_array_equal(oracle.resize_linear(src, dw, dh), _resize_numpy(src, dw, dh, sse2=False))
+
+
+def test_resize_sse2_and_scalar_forms_differ_by_at_most_one(oracle):
+    """The SSE2 pass drops 4 bits, then floors each product's low 16 bits
+    before its final rounding: within one level of the scalar pass, and off
+    by one on a sizeable share of the pixels."""
+    src = synth.shapes_frame(1241, 376, 6)
+    a = oracle.resize_linear(src, 1034, 313).astype(int)
+    with oracle.opencv_simd(0):
+        b = oracle.resize_linear(src, 1034, 313).astype(int)
+    d = np.abs(a - b)
+    assert d.max() == 1 and 0.05 < d.mean() < 0.5
+    assert np.all(a[:, 1032:] == b[:, 1032:])          # the last columns run the scalar pass
 
 
 def test_resize_identity_and_float_bilinear(oracle):
@@ -132,16 +164,57 @@ def test_gaussian_taps(oracle):
     assert k.sum() == 257
 
 
+def _blur_numpy(img, k, sse2):
+    """Separable 7x7 blur, REFLECT_101, integer rows; columns either as the
+    scalar FixedPtCastEx ((sum + 2^15) >> 16) or as SymmColumnVec_32s8u's
+    SSE2 float statement (taps k / 2^16, s = S0*ky0 + 0, s += (S-t + S+t) *
+    kyt, round half to even) on the columns it covers (16 at a time while
+    x <= w - 16, then 4 at a time while x <= w - 4)."""
+    h, w = img.shape
+    p = np.pad(img.astype(np.int64), 3, mode="reflect")       # gfedcb|abcdefgh|gfedcba
+    rows = sum(k[i] * p[:, i:i + w] for i in range(7))
+    cols = sum(k[i] * rows[i:i + h, :] for i in range(7))
+    out = (cols + (1 << 15)) >> 16
+    if sse2:
+        f = np.float32
+        ky = [f(int(k[3 + t]) / 65536.0) for t in range(4)]
+        r = rows.astype(np.float32)
+        s = r[3:3 + h] * ky[0] + f(0)
+        for t in (1, 2, 3):
+            s = (s + (r[3 + t:3 + t + h] + r[3 - t:3 - t + h]) * ky[t]).astype(np.float32)
+        end = w // 4 * 4
+        out[:, :end] = np.rint(s)[:, :end]
+    return np.clip(out, 0, 255).astype(np.uint8), cols
+
+
 def test_blur_matches_separable_numpy(oracle):
     k = oracle.gaussian7_kernel().astype(np.int64)
-    for (w, h, seed) in [(64, 48, 1), (533, 400, 2), (7, 9, 3)]:
+    for (w, h, seed) in [(64, 48, 1), (533, 400, 2), (7, 9, 3), (30, 21, 4)]:
         img = synth.shapes_frame(max(w, 8), max(h, 8), seed)[:h, :w] if w >= 8 else \
             np.random.default_rng(seed).integers(0, 256, (h, w), dtype=np.uint8)
-        p = np.pad(img.astype(np.int64), 3, mode="reflect")   # gfedcb|abcdefgh|gfedcba
-        rows = sum(k[i] * p[:, i:i + w] for i in range(7))
-        cols = sum(k[i] * rows[i:i + h, :] for i in range(7))
-        ref = np.clip((cols + (1 << 15)) >> 16, 0, 255).astype(np.uint8)
-        np.testing.assert_array_equal(oracle.gaussian7(img), ref)
+        np.testing.assert_array_equal(oracle.gaussian7(img), _blur_numpy(img, k, True)[0])
+        with oracle.opencv_simd(1):
+            np.testing.assert_array_equal(oracle.gaussian7(img), _blur_numpy(img, k, False)[0])
+
+
+def test_blur_sse2_differs_exactly_at_even_ties(oracle):
+    """The float column pass equals round-half-even of sum / 2^16: it is one
+    below the scalar pass exactly where the low 17 bits of the sum are
+    0x08000 (a tie with an even quotient), on the columns it covers."""
+    k = oracle.gaussian7_kernel().astype(np.int64)
+    rng = np.random.default_rng(7)
+    ties = 0
+    for _ in range(6):
+        img = rng.integers(0, 256, (200, 301), dtype=np.uint8)
+        a = oracle.gaussian7(img).astype(int)
+        with oracle.opencv_simd(1):
+            b = oracle.gaussian7(img).astype(int)
+        _, cols = _blur_numpy(img, k, False)
+        want = ((cols & 0x1FFFF) == 0x8000) & (cols < (1 << 24))
+        want[:, 300:] = False                        # column 300 runs the scalar pass
+        np.testing.assert_array_equal(b - a, want.astype(int))
+        ties += int(want.sum())
+    assert ties > 0
 
 
 # ------------------------------------------------------------------ FAST
@@ -525,10 +598,42 @@ def test_pattern_table_integrity():
     if os.path.exists(ref):
         import importlib.util
         spec = importlib.util.spec_from_file_location(
-            "gen_pattern", os.path.join(os.path.dirname(path), "gen_pattern.py"))
+            "gen_pattern", os.path.join(os.path.dirname(__file__), "golden", "gen_pattern.py"))
         gp = importlib.util.module_from_spec(spec)
         spec.loader.exec_module(gp)
         assert gp.parse_pattern(open(ref, encoding="utf-8", errors="replace").read()) == vals
+
+
+def _extraction_diff(oracle, img, flags):
+    base = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    k0, d0 = base(img)
+    with oracle.opencv_simd(flags):
+        ex = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+        k1, d1 = ex(img)
+    px = sum(int((base.level_image(l) != ex.level_image(l)).sum()) for l in range(8))
+    key = lambda k: (float(k["x"]), float(k["y"]), int(k["octave"]))
+    s0 = {key(k): i for i, k in enumerate(k0)}
+    s1 = {key(k): i for i, k in enumerate(k1)}
+    common = set(s0) & set(s1)
+    desc = sum(1 for c in common if not np.array_equal(d0[s0[c]], d1[s1[c]]))
+    return px, len(set(s0) ^ set(s1)), desc, len(k0)
+
+
+def test_opencv_rounding_divergence(oracle):
+    """How far a reference linked against OpenCV's generic scalar code would
+    be from the default (SSE2) statement: the resize rounding alone moves
+    about a tenth of the keypoints and changes most descriptors; the
+    Gaussian column rounding alone changes a handful of pixels."""
+    tot = np.zeros((2, 4), np.int64)
+    for seed in (100, 101):
+        img = synth.shapes_frame(1241, 376, seed)
+        tot[0] += _extraction_diff(oracle, img, 2)     # scalar resize, SSE2 Gaussian
+        tot[1] += _extraction_diff(oracle, img, 1)     # SSE2 resize, scalar Gaussian
+    px, kp, desc, n = tot[0]
+    assert 0.2 < px / (2 * 977481) < 0.45            # of the level >= 1 pixels
+    assert 0.05 < kp / n < 0.3 and desc > n // 4
+    px, kp, desc, n = tot[1]
+    assert px == 0 and kp == 0 and desc <= 10
 
 
 def test_descriptor_straddles_are_rare(oracle):
