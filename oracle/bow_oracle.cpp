@@ -723,6 +723,14 @@ int oracle_search_by_projection_last(const oracle_keypoint* kp, const uint8_t* d
 {
     const int plant = oracle_planted(4);
     int nmatches = 0;
+    // Angles are cv::KeyPoint angles in [0, 360]; others would put a rotation
+    // bin outside the histogram (the reference asserts, :1836).
+    if (check_orientation) {
+        for (int i = 0; i < n; i++)
+            if (!(kp[i].angle >= 0.f && kp[i].angle <= 360.f)) return -1;
+        for (int p = 0; p < npts; p++)
+            if ((pts[p].flags & 1) && !(pts[p].angle >= 0.f && pts[p].angle <= 360.f)) return -1;
+    }
     std::vector<int> rotHist[kHisto];
     const float factor = kHisto / 360.0f;
     const float fx = cam[0], fy = cam[1], cx = cam[2], cy = cam[3], mbf = cam[4], mb = cam[5], tlc_z = cam[6];

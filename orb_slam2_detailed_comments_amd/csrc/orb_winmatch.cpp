@@ -177,6 +177,15 @@ int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp
     }
     for (int i = 0; i < N; i++)
         if (kp[i].octave < 0 || kp[i].octave >= ORB_MAX_LEVELS) return ORB_E_ARG;
+    // The rotation check bins (last angle - current angle) mod 360; keypoint
+    // angles lie in [0, 360] (the reference asserts a bin in range, :1836).
+    if (rule == 2 && check_orientation) {
+        for (int i = 0; i < N; i++)
+            if (!(kp[i].angle >= 0.f && kp[i].angle <= 360.f)) return ORB_E_ARG;
+        const orb_last_point* lp = (const orb_last_point*)pts;
+        for (int q = 0; q < Q; q++)
+            if ((lp[q].flags & 1) && !(lp[q].angle >= 0.f && lp[q].angle <= 360.f)) return ORB_E_ARG;
+    }
     HIP_TRY(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int F = n_frames;

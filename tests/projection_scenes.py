@@ -463,7 +463,7 @@ def random_frame(seed, n_kp=1500, n_pts=1200, last=False, w=640, h=480):
             if rng.random() < 0.02:
                 z = -z
             fl = PRESENT | (HAS_OBS if rng.random() < 0.9 else 0) if rng.random() < 0.95 else 0
-            s.camera_point(d, x, y, z, octave=int(lev[t]), angle=float(ang[t] + rng.normal(0, 8) % 360), flags=fl)
+            s.camera_point(d, x, y, z, octave=int(lev[t]), angle=float((ang[t] + rng.normal(0, 8)) % 360), flags=fl)
         else:
             fl = (IN_VIEW if rng.random() < 0.95 else 0) | (BAD if rng.random() < 0.02 else 0) | \
                  (HAS_OBS if rng.random() < 0.95 else 0)

@@ -209,13 +209,13 @@ def test_generic_scalar_opencv_rounding(orb, oracle, w, h, nf):
     img = synth.shapes_frame(w, h, 61)
     g = orb.ORBextractor(nf, 1.2, 8, 20, 7, opencv_sse2=False)
     gk, gd = g(img)
-    with oracle.opencv_simd(0):
+    with oracle.opencv_simd(0):      # the oracle blurs a level when asked for it: stay in the mode
         o = oracle.OracleExtractor(nf, 1.2, 8, 20, 7)
         ok, od = o(img)
-    for level in range(8):
-        np.testing.assert_array_equal(g.pyramid_level(level), o.level_image(level), err_msg=f"level {level}")
-        np.testing.assert_array_equal(g.debug_image(level, 1), o.level_image(level, blurred=True),
-                                      err_msg=f"blur {level}")
+        for level in range(8):
+            np.testing.assert_array_equal(g.pyramid_level(level), o.level_image(level), err_msg=f"level {level}")
+            np.testing.assert_array_equal(g.debug_image(level, 1), o.level_image(level, blurred=True),
+                                          err_msg=f"blur {level}")
     _assert_same_keypoints(gk, gd, ok, od)
     d = orb.ORBextractor(nf, 1.2, 8, 20, 7)
     d(img)

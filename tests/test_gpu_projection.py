@@ -98,3 +98,14 @@ def test_empty_inputs(orb):
     f, sc = _frame_tuple(s)
     n, pm, kpm = orb.search_by_projection([f], sc, 1.0, 0.8)[0]
     assert n == 0 and len(pm) == 0 and np.all(kpm == -1)
+
+
+def test_last_rejects_angles_outside_0_360(orb):
+    """The rotation check bins angle differences mod 360: an angle outside
+    [0, 360] would index past the histogram (the reference asserts)."""
+    s = ps.scene_last_rotation()
+    s.pts[0] = s.pts[0][:3] + (400.0,) + s.pts[0][4:]
+    f, sc = _frame_tuple(s)
+    with pytest.raises(orb.OrbError):
+        orb.search_by_projection_last([f], sc, 15.0, False, True)
+    assert orb.search_by_projection_last([f], sc, 15.0, False, False)[0][0] > 0   # no check: accepted
