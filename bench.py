@@ -247,8 +247,9 @@ def batch_extra(orb, ctx, steps):
     batch = orb.Batch([(w, h) for w, h, _ in slots], [nf for _, _, nf in slots], context=ctx)
     for i, im in enumerate(imgs):
         batch.upload(i, im)
+    th, _, _ = batch.autotune_tiles(steps=5)
     best, tms = batch.autotune(steps=5)
-    out = {"frames": 64, "graph_branches": best}
+    out = {"frames": 64, "graph_branches": best, "tile_heights": th}
     for name, mode in (("graph", orb.Batch.GRAPH), ("per_frame_launches", orb.Batch.PER_IMAGE)):
         for _ in range(3):
             batch.run(mode)
@@ -414,6 +415,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip every extra workload")
     ap.add_argument("--extras", default="all",
                     help="comma-separated extras to run: knn, batch, serving, bow, mappoint, winmatch, projection")
+    ap.add_argument("--tiles", default="auto",
+                    help="level kernel tile heights: auto (the tile autotuner), 32, 16, or 8 comma-separated values")
     ap.add_argument("--branches", type=int, default=0,
                     help="independent hipGraph branches (mode graph); 0: pick by the batch's autotuner")
     args = ap.parse_args()
@@ -434,6 +437,16 @@ def main():
         batch.upload(2 * i + 1, r)
     b = float(np.float32(BF) / np.float32(FX))
     batch.set_stereo(list(range(0, 2 * B, 2)), list(range(1, 2 * B, 2)), BF, b)
+    tiles = None
+    if args.tiles == "auto":
+        th, m32, m16 = batch.autotune_tiles(steps=10)
+        tiles = {"tile_heights": th, "ms_64x32": {str(k): round(v, 4) for k, v in m32.items()},
+                 "ms_64x16": {str(k): round(v, 4) for k, v in m16.items()}}
+        print(f"[bench] tile autotune: {tiles}", file=sys.stderr, flush=True)
+    else:
+        th = [int(v) for v in args.tiles.split(",")] if "," in args.tiles else [int(args.tiles)] * NLEV
+        batch.set_tile_heights(th)
+        tiles = {"tile_heights": th}
     tuned = None
     if args.branches > 0:
         batch.set_branches(args.branches)
@@ -506,6 +519,7 @@ def main():
            "config": {"workload": f"stereo extract+match, {W}x{H}, nFeatures {NF}, 8 levels, scale 1.2, FAST 20/7, "
                                   f"bf {BF}, fx {FX}", "frames_per_step": B, "images_per_step": 2 * B,
                       "run_mode": args.mode, "graph_branches": args.branches, "branch_autotune": tuned,
+                      "tile_autotune": tiles,
                       "mean_keypoints_per_image": round(float(np.mean(kps)), 1),
                       "mean_stereo_matches": round(float(np.mean(matched)), 1)},
            "roofline": roofline, "host_to_host": host_to_host}

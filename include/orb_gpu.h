@@ -211,6 +211,20 @@ int orb_batch_set_branches(orb_batch* b, int branches);
  * *best.  ms (8 floats, may be NULL) receives the mean replay time in ms at k
  * branches in ms[k-1], or -1 where k was not timed. */
 int orb_batch_autotune(orb_batch* b, int steps, int* best, float* ms);
+/* Tile shape of the level kernel, per pyramid level: 64 x 32 (the default)
+ * or 64 x 16 tiles (tile_h[l] = 32 or 16, levels 0 .. n-1).  No counterpart
+ * in the reference; results are identical either way.  Smaller tiles halve
+ * each workgroup's serial work, which shortens the small levels, whose few
+ * tiles leave the GPU partly idle; larger ones share their halo over more
+ * pixels.  On a batch it applies to every slot, whose images stay in place. */
+int orb_extractor_set_tile_heights(orb_extractor* ex, int n, const int* tile_h);
+int orb_batch_set_tile_heights(orb_batch* b, int n, const int* tile_h);
+/* Autotuner of the tile shapes: times the pyramid stages (mode 0, `steps`
+ * runs) with 64 x 32 and with 64 x 16 tiles at every level, keeps per level
+ * the faster one (64 x 16 only when more than 1 % faster) and leaves the
+ * batch with that choice.  tile_h receives it (one int per level), ms32 /
+ * ms16 (optional) the level times in ms. */
+int orb_batch_autotune_tiles(orb_batch* b, int steps, int* tile_h, float* ms32, float* ms16);
 /* Stage timing with HIP events on the batch's stream, averaged over `steps`
  * mode-0 runs.  ms receives orb_batch_stage_count() floats: pyramid levels
  * 0..L-1 (pyramid, blur, FAST with per-cell suppression), quadtree, describe,

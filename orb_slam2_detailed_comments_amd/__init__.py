@@ -43,6 +43,9 @@ _P, _I, _F, _S, _V = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_siz
 SIGNATURES = {
     "orb_context_create": (_I, [_I, _P]),
     "orb_batch_upload_images": (_I, [_P, _I, _P, _P, _P]),
+    "orb_extractor_set_tile_heights": (_I, [_P, _I, _P]),
+    "orb_batch_set_tile_heights": (_I, [_P, _I, _P]),
+    "orb_batch_autotune_tiles": (_I, [_P, _I, _P, _P, _P]),
     "orb_batch_read_all": (_I, [_P, _P, _P, _I, _P, _P, _P]),
     "orb_extractor_set_opencv_rounding": (_I, [_P, _I]),
     "orb_context_destroy": (_V, [_P]),
@@ -241,6 +244,11 @@ class ORBextractor:
         if rc < 0:
             raise OrbError("orb_extractor_capacity", rc)
         return rc
+
+    def set_tile_heights(self, tile_h):
+        """Level kernel tiles per level: 64 x 32 (32) or 64 x 16 (16)."""
+        th = np.ascontiguousarray(tile_h, np.int32)
+        _check("orb_extractor_set_tile_heights", lib().orb_extractor_set_tile_heights(self.h, len(th), _ptr(th)))
 
     # -- operator() (ORBextractor.h:111)
     def __call__(self, image, mask=None):
@@ -478,6 +486,21 @@ class Batch:
         ms = np.zeros(8, np.float32)
         _check("orb_batch_autotune", lib().orb_batch_autotune(self.h, int(steps), ctypes.byref(best), _ptr(ms)))
         return best.value, {k + 1: float(v) for k, v in enumerate(ms) if v >= 0}
+
+    def set_tile_heights(self, tile_h):
+        """Level kernel tiles per level (32 or 16) for every slot."""
+        th = np.ascontiguousarray(tile_h, np.int32)
+        _check("orb_batch_set_tile_heights", lib().orb_batch_set_tile_heights(self.h, len(th), _ptr(th)))
+
+    def autotune_tiles(self, steps=10):
+        """Per level, the faster of 64 x 32 and 64 x 16 tiles on the current
+        images; returns (tile heights, {level: ms at 32}, {level: ms at 16})."""
+        th = np.zeros(self.nlevels, np.int32)
+        m32 = np.zeros(self.nlevels, np.float32)
+        m16 = np.zeros(self.nlevels, np.float32)
+        _check("orb_batch_autotune_tiles", lib().orb_batch_autotune_tiles(self.h, int(steps), _ptr(th), _ptr(m32),
+                                                                          _ptr(m16)))
+        return th.tolist(), dict(enumerate(m32.tolist())), dict(enumerate(m16.tolist()))
 
     def run(self, mode=GRAPH):
         _check("orb_batch_run", lib().orb_batch_run(self.h, int(mode)))

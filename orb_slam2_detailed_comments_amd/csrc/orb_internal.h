@@ -97,6 +97,7 @@ inline int reflect101(int i, int n)
 struct LevelPlan {
     int w, h, pitch;
     size_t img_off;
+    int tile_h;                 // level kernel tile height (32 or 16)
     int tiles_x, tiles_y, ncells;
     int nCols, nRows, wCell, hCell, maxBorderX, maxBorderY;
     int nIni;
@@ -169,6 +170,8 @@ struct orb_extractor {
     // 1: resize and Gaussian 8U rounded as OpenCV's SSE2 kernels do (every
     // x86 build of OpenCV 2.4/3.x); 0: as its generic scalar code
     int rounding = 1;
+    // level kernel tile height per level (64 x 32 or 64 x 16 tiles)
+    int tile_h[ORB_MAX_LEVELS] = {32, 32, 32, 32, 32, 32, 32, 32, 32, 32, 32, 32, 32, 32, 32, 32};
     std::vector<float> scale, invScale, sigma2, invSigma2;
     std::vector<int> quota;
 
@@ -264,8 +267,9 @@ struct orb_extractor {
             L.pitch = (int)align_up((size_t)L.w, 64);
             L.img_off = img;
             img += align_up((size_t)L.pitch * L.h, 256);
+            L.tile_h = tile_h[l];
             L.tiles_x = (L.w + ORB_TILE_W - 1) / ORB_TILE_W;
-            L.tiles_y = (L.h + ORB_TILE_H - 1) / ORB_TILE_H;
+            L.tiles_y = (L.h + L.tile_h - 1) / L.tile_h;
             const int minBorderX = ORB_MIN_BORDER, minBorderY = ORB_MIN_BORDER;
             L.maxBorderX = L.w - ORB_EDGE + 3;
             L.maxBorderY = L.h - ORB_EDGE + 3;
@@ -332,7 +336,9 @@ struct orb_extractor {
         const size_t o_rows = take((size_t)nlevels * ((size_t)H + 2) * 8);
         size_t ntab = 0;
         for (int l = 1; l < nlevels; l++)
-            ntab += (size_t)p.lv[l].w + p.lv[l].h + p.lv[l].tiles_x + p.lv[l].tiles_y;
+            // tile-row windows sized for the smallest tile height, so that a
+            // change of tile heights never reallocates (the frames stay put)
+            ntab += (size_t)p.lv[l].w + p.lv[l].h + p.lv[l].tiles_x + (p.lv[l].h + 15) / 16;
         const size_t o_tab = take(ntab * 8);
         const size_t total = off;
         if (total > mem_bytes) {
@@ -453,8 +459,9 @@ struct orb_extractor {
                 J.lv[l].ywin = (const int2*)(d_mem + o_tab) + t;
                 for (int r = 0; r < p.lv[l].tiles_y; r++, t++) {
                     int lo = INT_MAX, hi = -1;
-                    for (int i = 0; i < ORB_TILE_H + 8; i++) {
-                        const int sy01 = tab[ty + reflect101(r * ORB_TILE_H - 4 + i, h)].x;
+                    const int th = p.lv[l].tile_h;
+                    for (int i = 0; i < th + 8; i++) {
+                        const int sy01 = tab[ty + reflect101(r * th - 4 + i, h)].x;
                         lo = std::min(lo, sy01 & 0xFFFF);
                         hi = std::max(hi, sy01 >> 16);
                     }
@@ -475,19 +482,31 @@ struct orb_extractor {
         if (!d_rt && hipMalloc(&d_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
         if (!h_rt && hipHostMalloc(&h_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
         const orb_extractor* one[1] = {this};
-        build_table(one, 1, *h_rt);
+        int rc_t = build_table(one, 1, *h_rt);
+        if (rc_t) return rc_t;
         HIP_TRY(hipMemcpy(d_job, &hjob, sizeof(ImageJob), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d_rt, h_rt, sizeof(RunTable), hipMemcpyHostToDevice));
         return ORB_OK;
     }
 
-    static void build_table(const orb_extractor* const* exs, int n, RunTable& rt)
+    // Launch table of a run over these jobs.  The level kernel's tile height
+    // is a template parameter, so every job of a run must use the same one at
+    // each level (ORB_E_ARG otherwise).
+    static int build_table(const orb_extractor* const* exs, int n, RunTable& rt)
     {
         std::memset(&rt, 0, sizeof(rt));
         rt.njobs = n;
         int maxl = 0;
         for (int j = 0; j < n; j++) maxl = std::max(maxl, exs[j]->nlevels);
         rt.max_levels = maxl;
+        for (int l = 0; l < maxl; l++) {
+            rt.tile_h[l] = 0;
+            for (int j = 0; j < n; j++) {
+                if (l >= exs[j]->nlevels) continue;
+                if (rt.tile_h[l] == 0) rt.tile_h[l] = exs[j]->plan.lv[l].tile_h;
+                else if (rt.tile_h[l] != exs[j]->plan.lv[l].tile_h) return ORB_E_ARG;
+            }
+        }
         for (int l = 0; l < maxl; l++) {
             int acc = 0;
             for (int j = 0; j < n; j++) {
@@ -512,6 +531,7 @@ struct orb_extractor {
         for (int j = 0; j < n; j++)
             eq = eq && rt.desc_start[(j + 1) * maxl] - rt.desc_start[j * maxl] == rt.desc_start[maxl];
         rt.desc_per_job = eq ? rt.desc_start[maxl] : 0;
+        return ORB_OK;
     }
 
     // Input frame into level 0 of the pyramid (hjob.src, src_pitch bytes a
@@ -571,7 +591,8 @@ struct orb_extractor {
         if (!d_pair_rt && hipMalloc(&d_pair_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
         if (!h_pair_rt && hipHostMalloc(&h_pair_rt, sizeof(RunTable)) != hipSuccess) return ORB_E_OOM;
         const orb_extractor* two[2] = {this, r};
-        build_table(two, 2, *h_pair_rt);
+        const int rc_t = build_table(two, 2, *h_pair_rt);
+        if (rc_t) return rc_t;
         ImageJob jobs[2] = {hjob, r->hjob};
         HIP_TRY(hipMemcpy(d_pair_jobs, jobs, sizeof(jobs), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d_pair_rt, h_pair_rt, sizeof(RunTable), hipMemcpyHostToDevice));

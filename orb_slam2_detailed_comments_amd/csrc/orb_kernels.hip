@@ -164,12 +164,9 @@ __device__ __forceinline__ int lanes_below(unsigned long long mask)
 //      kernel keeps minThFAST candidates only for cells without that flag,
 //      which is the reference's per-cell retry.
 #define LT_W ORB_TILE_W
-#define LT_H ORB_TILE_H
 #define LT_HALO 4
 #define LP_W (LT_W + 2 * LT_HALO)      // 72 bytes = 18 dwords
-#define LP_H (LT_H + 2 * LT_HALO)      // 40 rows
 #define LP_WD (LP_W / 4)
-#define LM_H (LT_H + 2)                // FAST strength rows: tile -1 .. 32
 #define LK_CAP 1536                  // per-tile candidates, both thresholds
 
 typedef unsigned short orb_us2 __attribute__((ext_vector_type(2)));
@@ -303,24 +300,26 @@ __device__ __forceinline__ unsigned resize4(BytePtr base, unsigned o0, unsigned 
 // tile's candidate list (phases 3-5).
 #define LW_MAX 2656                  // source window dwords: scale <= ~1.9
 #define LS_K 6                       // window dwords per thread read before the first wait
+template <int TH>
 union LevelScratch {
     unsigned win[LW_MAX];
-    uint4 hs[LT_H + 6][LT_W / 4];
+    uint4 hs[TH + 6][LT_W / 4];
     struct {
-        unsigned short queue[LM_H * (LT_W + 2)];
+        unsigned short queue[(TH + 2) * (LT_W + 2)];
         unsigned kloc[LK_CAP];       // tile-local candidate: ly | lx | score | retry
     } f;
 };
 
+template <int TH>
 __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__ jobs,
                                                     const RunTable* __restrict__ rt, int level)
 {
-    __shared__ __align__(16) unsigned pxw[LP_H][LP_WD];   // level pixels, tile + 4-px halo
-    __shared__ unsigned mmw[LM_H][LP_WD];         // FAST strength, tile + 1-px ring
-    __shared__ LevelScratch u;
-    __shared__ int xs0[LP_W], xa[LP_W], ys0[LP_H], yb[LP_H];
+    __shared__ __align__(16) unsigned pxw[(TH + 2 * LT_HALO)][LP_WD];   // level pixels, tile + 4-px halo
+    __shared__ unsigned mmw[(TH + 2)][LP_WD];         // FAST strength, tile + 1-px ring
+    __shared__ LevelScratch<TH> u;
+    __shared__ int xs0[LP_W], xa[LP_W], ys0[(TH + 2 * LT_HALO)], yb[(TH + 2 * LT_HALO)];
     __shared__ int s_qn, s_nq, s_kn, s_base;
-    __shared__ int s_col[LT_W], s_row[LT_H];      // FAST cell of each tile column / row, edge flags
+    __shared__ int s_col[LT_W], s_row[TH];      // FAST cell of each tile column / row, edge flags
     const uint8_t* px = (const uint8_t*)pxw;
     uint8_t* mm = (uint8_t*)mmw;
     const int tid = threadIdx.x;
@@ -340,10 +339,10 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const LevelJob& L = J.lv[level];
     const int t = blk - rt->tile_start[level][job];
     const int tx0 = (t % L.tiles_x) * LT_W;
-    const int ty0 = (t / L.tiles_x) * LT_H;
+    const int ty0 = (t / L.tiles_x) * TH;
     const int w = L.w, h = L.h, pitch = L.pitch;
     const bool interior = tx0 >= LT_HALO && tx0 + LT_W + LT_HALO <= w && ty0 >= LT_HALO &&
-                          ty0 + LT_H + LT_HALO <= h;
+                          ty0 + TH + LT_HALO <= h;
 
     // FAST detection area of the level.
     const int ax0 = ORB_EDGE, ay0 = ORB_EDGE;
@@ -354,7 +353,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     // likewise.  Built while the tile's first reads are in flight; read
     // after later barriers.
     auto cell_tables = [&]() {
-        if (tid < LT_W + LT_H) {
+        if (tid < LT_W + TH) {
             const bool isc = tid < LT_W;
             const int z = isc ? tx0 + tid : ty0 + (tid - LT_W);
             const int cs = isc ? L.wCell : L.hCell, a0 = isc ? ax0 : ay0, a1 = isc ? ax1 : ay1;
@@ -370,7 +369,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     // straight to the level image.
     auto level_out = [&](int ey, int c, unsigned d) {
         const int y = ty0 + ey - LT_HALO, x0 = tx0 + 4 * (c - 1);
-        if (ey >= LT_HALO && ey < LT_HALO + LT_H && c >= 1 && c <= LT_W / 4 && y < h) {
+        if (ey >= LT_HALO && ey < LT_HALO + TH && c >= 1 && c <= LT_W / 4 && y < h) {
             uint8_t* dst = L.lvl + (unsigned)(y * pitch + x0);
             if (x0 + 3 < w) {
                 gstore4(dst, d);
@@ -392,7 +391,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             unsigned d[3];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const int e = min(tid + 256 * k, LP_H * LP_WD - 1);
+                const int e = min(tid + 256 * k, (TH + 2 * LT_HALO) * LP_WD - 1);
                 const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
                 const unsigned off = (unsigned)((ty0 - LT_HALO + ey) * sp + (tx0 - LT_HALO + ex));
                 d[k] = (sp & 3) == 0 ? gload4(src + off) : gload4u(src + off);
@@ -401,11 +400,11 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 const int e = tid + 256 * k;
-                if (e < LP_H * LP_WD) pxw[e / LP_WD][e % LP_WD] = d[k];
+                if (e < (TH + 2 * LT_HALO) * LP_WD) pxw[e / LP_WD][e % LP_WD] = d[k];
             }
         } else {
             cell_tables();
-            for (int e = tid; e < LP_H * LP_WD; e += 256) {
+            for (int e = tid; e < (TH + 2 * LT_HALO) * LP_WD; e += 256) {
                 const int ey = e / LP_WD, ex = (e - ey * LP_WD) * 4;
                 const int y = reflect101(ty0 - LT_HALO + ey, h);
                 const uint8_t* r = src + (size_t)y * sp;
@@ -437,7 +436,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // 20-bit reciprocal, exact for e * nwd < 2^20).
         // (The row table follows the column table: ytab == xtab + w.)
         const int ti = tid < 128 ? reflect101(tx0 - LT_HALO + min(tid, LP_W - 1), w)
-                                 : w + reflect101(ty0 - LT_HALO + min(tid - 128, LP_H - 1), h);
+                                 : w + reflect101(ty0 - LT_HALO + min(tid - 128, (TH + 2 * LT_HALO) - 1), h);
         const uint2 tv = gload8((const uint8_t*)(L.xtab + ti));
         const int nst = staged ? nwd * nwr : 0;
         const unsigned rcp = ((1u << 20) + (unsigned)nwd - 1u) / (unsigned)nwd;
@@ -457,7 +456,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         if (tid < LP_W) {
             xs0[tid] = (int)tv.x;
             xa[tid] = (int)tv.y;
-        } else if (tid >= 128 && tid < 128 + LP_H) {
+        } else if (tid >= 128 && tid < 128 + (TH + 2 * LT_HALO)) {
             ys0[tid - 128] = (int)tv.x;
             yb[tid - 128] = (int)tv.y;
         }
@@ -512,7 +511,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                     sel[k] = 0x0C000C00u | ((unsigned)(nxk[k] - b0) << 16) | (unsigned)(sxk[k] - b0);
                 auto rows = [&](auto mode) {
                     constexpr int MODE = decltype(mode)::value;
-                    for (int ey = r0; ey < LP_H; ey += 14) {
+                    for (int ey = r0; ey < (TH + 2 * LT_HALO); ey += 14) {
                         const unsigned sy01 = (unsigned)ys0[ey];
                         const unsigned* q0 = wd + __umul24((sy01 & 0xFFFFu) - wy0, ws);
                         const unsigned* q1 = wd + __umul24((sy01 >> 16) - wy0, ws);
@@ -541,7 +540,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
             } else if (staged) {
                 const uint8_t* wb = (const uint8_t*)u.win;
                 const int ws = nwd * 4;
-                for (int ey = r0; ey < LP_H; ey += 14) {
+                for (int ey = r0; ey < (TH + 2 * LT_HALO); ey += 14) {
                     const unsigned sy01 = (unsigned)ys0[ey];
                     const unsigned d = resize4(wb, __umul24((sy01 & 0xFFFFu) - wy0, ws),
                                                __umul24((sy01 >> 16) - wy0, ws), sxk, nxk, tap, yb[ey], sse);
@@ -549,7 +548,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                     level_out(ey, c, d);
                 }
             } else {
-                for (int ey = r0; ey < LP_H; ey += 14) {
+                for (int ey = r0; ey < (TH + 2 * LT_HALO); ey += 14) {
                     const unsigned sy01 = (unsigned)ys0[ey];
                     const unsigned d = resize4((const __attribute__((address_space(1))) uint8_t*)P.lvl,
                                                __umul24(sy01 & 0xFFFFu, sp), __umul24(sy01 >> 16, sp), sxk, nxk, tap,
@@ -569,7 +568,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     // Level 0 copied out when the input is not level 0 itself (levels >= 1
     // are written by the resize).
     if (level == 0 && J.src != L.lvl) {
-        for (int r = tid >> 4; r < LT_H; r += 16) {
+        for (int r = tid >> 4; r < TH; r += 16) {
             const int y = ty0 + r, x0 = tx0 + 4 * gc;
             if (y >= h) continue;
             const unsigned d = pxw[r + LT_HALO][gc + 1];
@@ -585,7 +584,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const unsigned T1 = (unsigned)c_gauss[0] | (c_gauss[1] << 8) | (c_gauss[2] << 16) | ((unsigned)c_gauss[3] << 24);
     const unsigned T2 = (unsigned)c_gauss[4] | (c_gauss[5] << 8) | (c_gauss[6] << 16);
     // Two adjacent 4-column groups per task (they share two source dwords).
-    for (int e = tid; e < (LT_H + 6) * (LT_W / 8); e += 256) {
+    for (int e = tid; e < (TH + 6) * (LT_W / 8); e += 256) {
         const int rr = e >> 3, c = 2 * (e & 7);  // blur row rr <-> px row rr+1
         const unsigned* pr = &pxw[rr + 1][c];
         const unsigned D[4] = {pr[0], pr[1], pr[2], pr[3]};
@@ -611,9 +610,10 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         // Row sums are <= 255 * 257 and taps < 256: 24-bit multiplies.
         const unsigned k0 = (unsigned)c_gauss[3], k1 = (unsigned)c_gauss[4], k2 = (unsigned)c_gauss[5],
                        k3 = (unsigned)c_gauss[6];
-        // rows r and r + 1 of the tile from 8 row-sum vectors
-        static_assert((LT_H / 2) * (LT_W / 4) == 256, "one row pair and column group per thread");
-        const int r = (tid >> 4) * 2, x0 = tx0 + 4 * gc;
+        // rows r .. r + RP - 1 of the tile from 6 + RP row-sum vectors
+        constexpr int RP = TH / 16;
+        static_assert(RP * 16 == TH && 16 * (LT_W / 4) == 256, "RP rows and one column group per thread");
+        const int r = (tid >> 4) * RP, x0 = tx0 + 4 * gc;
         // Column rounding: columns < bs_end as OpenCV's SSE2 column kernel
         // (SymmColumnVec_32s8u: the taps as floats k / 2^16, the sum rounded
         // half to even, which for these exact sums is (acc + 0x7FFF + bit 16
@@ -621,11 +621,11 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
         unsigned rb[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) rb[k] = x0 + k < L.bs_end ? 1u : 0u;
-        uint4 q[8];
+        uint4 q[6 + RP];
 #pragma unroll
-        for (int i = 0; i < 8; i++) q[i] = u.hs[r + i][gc];
+        for (int i = 0; i < 6 + RP; i++) q[i] = u.hs[r + i][gc];
 #pragma unroll
-        for (int hh = 0; hh < 2; hh++) {
+        for (int hh = 0; hh < RP; hh++) {
             const int y = ty0 + r + hh;
             if (y >= h) continue;
             unsigned out = 0;
@@ -658,18 +658,18 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
     const int tmin = min(tA, tB);
     const unsigned t2 = (unsigned)tmin | ((unsigned)tmin << 16);
     // tile plus its 1-px ring wholly inside the detection area
-    const bool tile_in = tx0 - 1 >= ax0 && tx0 + LT_W < ax1 && ty0 - 1 >= ay0 && ty0 + LT_H < ay1;
+    const bool tile_in = tx0 - 1 >= ax0 && tx0 + LT_W < ax1 && ty0 - 1 >= ay0 && ty0 + TH < ay1;
     // Lanes: rows tile -1..32 and 12-column groups g = 0..5 (px dwords 3g ..
     // 3g + 2, tile columns 12g - 4 .. 12g + 7): 204 lanes, one pass.  Group 0
     // only contributes its pixels 3.. (tile columns -1..) and group 5 its
     // pixels ..8 (columns ..64): the 1-px ring.  Their missing neighbour
     // dwords (columns -8 and 68 onwards) feed only the bytes masked off.
-    static_assert(LM_H * 6 <= 256, "one pre-test pass");
+    static_assert((TH + 2) * 6 <= 256, "one pre-test pass");
     static_assert(LP_WD == 18, "6 lanes of 3 dwords cover a strength-map row");
     {
         unsigned cand = 0u;                       // bit k: pixel k of the group
         int my = 0, g = 0;
-        if (tid < LM_H * 6) {
+        if (tid < (TH + 2) * 6) {
             my = tid / 6;
             g = tid - my * 6;
             const int y = ty0 - 1 + my;
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                 const int my = e[h] >> 8, col = e[h] & 255;
                 const int sv = fast_strength(px + (my + 3) * LP_W + col, LP_W);
                 mm[my * LP_W + col] = (uint8_t)sv;
-                keep[h] = sv > tmin && my >= 1 && my <= LT_H && col >= 4 && col < 4 + LT_W;
+                keep[h] = sv > tmin && my >= 1 && my <= TH && col >= 4 && col < 4 + LT_W;
             }
         }
         __syncthreads();
@@ -2100,7 +2100,10 @@ extern "C" hipError_t orb_launch_extract(const ImageJob* d_jobs, const RunTable*
     zero_counters_kernel<<<h_rt->njobs, 256, 0, st>>>(d_jobs, h_rt->njobs);
     for (int l = 0; l < h_rt->max_levels; l++) {
         const int nb = h_rt->tile_start[l][h_rt->njobs];
-        if (nb > 0) level_kernel<<<nb, 256, 0, st>>>(d_jobs, d_rt, l);
+        if (nb > 0) {
+            if (h_rt->tile_h[l] == 16) level_kernel<16><<<nb, 256, 0, st>>>(d_jobs, d_rt, l);
+            else level_kernel<32><<<nb, 256, 0, st>>>(d_jobs, d_rt, l);
+        }
         if (ev) (void)hipEventRecord(ev[e++], st);
     }
     octree_kernel<<<h_rt->njobs * h_rt->max_levels, OCT_T, 0, st>>>(d_jobs, h_rt->max_levels);

@@ -98,6 +98,20 @@ int orb_extractor_set_opencv_rounding(orb_extractor* ex, int mode)
     return ORB_OK;
 }
 
+int orb_extractor_set_tile_heights(orb_extractor* ex, int n, const int* tile_h)
+{
+    if (!ex || n < 0 || n > ex->nlevels || (n && !tile_h)) return ORB_E_ARG;
+    for (int l = 0; l < n; l++)
+        if (tile_h[l] != 16 && tile_h[l] != 32) return ORB_E_ARG;
+    bool change = false;
+    for (int l = 0; l < n; l++) {
+        change |= ex->tile_h[l] != tile_h[l];
+        ex->tile_h[l] = tile_h[l];
+    }
+    if (change) ex->plan.W = ex->plan.H = 0;   // the next call re-plans the tiles
+    return ORB_OK;
+}
+
 int orb_extractor_levels(const orb_extractor* ex) { return ex ? ex->nlevels : ORB_E_ARG; }
 
 float orb_extractor_scale_factor(const orb_extractor* ex) { return ex ? (float)ex->scaleFactor : 0.f; }
@@ -516,7 +530,8 @@ struct orb_batch {
             B.s = c[g];
             B.n = c[g + 1] - c[g];
             B.h_rt.reset(new RunTable());
-            orb_extractor::build_table(ex.data() + B.s, B.n, *B.h_rt);
+            const int rc_t = orb_extractor::build_table(ex.data() + B.s, B.n, *B.h_rt);
+            if (rc_t) return rc_t;
             std::vector<StereoArgs> sub;
             for (size_t p = 0; p < pl.size(); p++)
                 if (pl[p] >= B.s && pl[p] < B.s + B.n) {
@@ -634,7 +649,10 @@ int orb_batch_create(orb_context* ctx, int n, const int* widths, const int* heig
         delete b;
         return ORB_E_OOM;
     }
-    orb_extractor::build_table(b->ex.data(), n, *b->h_rt);
+    if (orb_extractor::build_table(b->ex.data(), n, *b->h_rt) != ORB_OK) {   // slots share tile heights
+        delete b;
+        return ORB_E_ARG;
+    }
     if (hipMemcpy(b->d_jobs, jobs.data(), n * sizeof(ImageJob), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_rt, b->h_rt, sizeof(RunTable), hipMemcpyHostToDevice) != hipSuccess) {
         delete b;
@@ -857,6 +875,61 @@ int orb_batch_autotune(orb_batch* b, int steps, int* best, float* ms)
     if (!rc) rc = orb_batch_run(b, 1);              // build the chosen graph
     if (!rc) *best = pick;
     return rc;
+}
+
+// Level kernel tile heights of every slot (levels 0 .. n-1), then the jobs
+// and launch tables again.  The level tables are sized for either height, so
+// no buffer moves and the slots' images stay in place.
+int orb_batch_set_tile_heights(orb_batch* b, int n, const int* tile_h)
+{
+    if (!b || n < 0 || n > ORB_MAX_LEVELS || (n && !tile_h)) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    const int ns = (int)b->ex.size();
+    std::vector<ImageJob> jobs(ns);
+    for (int i = 0; i < ns; i++) {
+        orb_extractor* e = b->ex[i];
+        const int W = e->plan.W, H = e->plan.H;
+        char* before = e->d_mem;
+        int rc = orb_extractor_set_tile_heights(e, std::min(n, e->nlevels), tile_h);
+        if (!rc) rc = e->ensure(W, H);
+        if (rc) return rc;
+        if (e->d_mem != before) return ORB_E_DEVICE;   // cannot happen: tables are sized for both heights
+        jobs[i] = e->hjob;
+    }
+    b->drop_graph();
+    if (orb_extractor::build_table(b->ex.data(), ns, *b->h_rt) != ORB_OK) return ORB_E_ARG;
+    HIP_TRY(hipMemcpy(b->d_jobs, jobs.data(), ns * sizeof(ImageJob), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_rt, b->h_rt, sizeof(RunTable), hipMemcpyHostToDevice));
+    return ORB_OK;
+}
+
+// The tile-shape autotuner: per-level stage times (mode 0, `steps` runs)
+// with 64 x 32 tiles at every level, then with 64 x 16 tiles; each level
+// keeps the faster shape (64 x 16 only when it is more than 1 % faster),
+// and the batch is left with that choice.  tile_h (max levels ints) gets
+// it, ms32 / ms16 (optional, max levels floats) the level times.
+int orb_batch_autotune_tiles(orb_batch* b, int steps, int* tile_h, float* ms32, float* ms16)
+{
+    if (!b || steps <= 0 || !tile_h) return ORB_E_ARG;
+    const int L = b->h_rt->max_levels, ns = orb_batch_stage_count(b);
+    std::vector<int> th(ORB_MAX_LEVELS, 32);
+    std::vector<float> t32(ns), t16(ns);
+    int rc = orb_batch_set_tile_heights(b, ORB_MAX_LEVELS, th.data());
+    if (!rc) rc = orb_batch_stage_times(b, steps, t32.data());
+    std::fill(th.begin(), th.end(), 16);
+    if (!rc) rc = orb_batch_set_tile_heights(b, ORB_MAX_LEVELS, th.data());
+    if (!rc) rc = orb_batch_stage_times(b, steps, t16.data());
+    if (rc) return rc;
+    for (int l = 0; l < ORB_MAX_LEVELS; l++) th[l] = l < L && t16[l] < 0.99f * t32[l] ? 16 : 32;
+    rc = orb_batch_set_tile_heights(b, ORB_MAX_LEVELS, th.data());
+    if (rc) return rc;
+    for (int l = 0; l < L; l++) {
+        tile_h[l] = th[l];
+        if (ms32) ms32[l] = t32[l];
+        if (ms16) ms16[l] = t16[l];
+    }
+    return ORB_OK;
 }
 
 int orb_batch_run(orb_batch* b, int mode)

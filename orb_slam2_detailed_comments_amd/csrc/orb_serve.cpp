@@ -181,7 +181,7 @@ struct orb_server {
             m.save_cap = c.cap;
             waves += m.cap_q;
         }
-        orb_extractor::build_table(exs.data(), n, *rt);
+        (void)orb_extractor::build_table(exs.data(), n, *rt);   // the server's extractors share tile heights
         (void)hipMemcpyAsync(L.d_jobs, jobs, n * sizeof(ImageJob), hipMemcpyHostToDevice, L.st);
         (void)hipMemcpyAsync(L.d_rt, rt, sizeof(RunTable), hipMemcpyHostToDevice, L.st);
         (void)hipMemcpyAsync(L.d_margs, ma, n * sizeof(MatchArgs), hipMemcpyHostToDevice, L.st);

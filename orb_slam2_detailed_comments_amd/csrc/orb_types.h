@@ -14,7 +14,6 @@
 #define ORB_MAX_ROOTS 16
 #define ORB_LIST_CAP 1024      // quadtree list capacity per level (LDS; one node per octree thread)
 #define ORB_TILE_W 64          // level kernel output tile
-#define ORB_TILE_H 32
 #define ORB_EDGE 19            // EDGE_THRESHOLD, ORBextractor.cc:81
 #define ORB_MIN_BORDER 16      // EDGE_THRESHOLD - 3, ORBextractor.cc:1049
 
@@ -87,6 +86,7 @@ struct RunTable {
     int tile_start[ORB_MAX_LEVELS][ORB_MAX_JOBS + 1];   // per level: first tile block of job j
     int desc_start[(ORB_MAX_JOBS + 1) * ORB_MAX_LEVELS];  // describe kernel: first block of (job, level)
     int tiles_per_job[ORB_MAX_LEVELS];  // > 0: every job has this many tiles at the level
+    int tile_h[ORB_MAX_LEVELS];         // level kernel tile height at the level (32 or 16)
     int desc_per_job;                   // > 0: every job has this many describe blocks
 };
 
