@@ -152,6 +152,15 @@ int orb_knn2(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* map,
 /* Same on device-resident buffers; returns once enqueued on the context stream. */
 int orb_knn2_device(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm, int method,
                     float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept);
+/* Allocation-free form of orb_knn2_device.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-chunk partial results live in a fixed 4 MiB array per GPU (2^19 keys),
+ * so nq * ceil(nm / 8192) <= 524288 (ORB_E_CAPACITY otherwise: e.g. 2000
+ * queries against up to 2.1 M map rows).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other contexts or streams) race. */
+int orb_knn2_device_noalloc(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm,
+                            int method, float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept);
 /* Copies query and map rows into HBM, then returns the average device
  * milliseconds of the search and merge kernels over `steps` runs (HIP events
  * on the context stream). */

@@ -77,6 +77,7 @@ SIGNATURES = {
     "orb_hbm_copy_time": (_I, [_P, _S, _I, _P]),
     "orb_knn2": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
     "orb_knn2_device": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
+    "orb_knn2_device_noalloc": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _P, _P, _P]),
     "orb_knn2_time": (_I, [_P, _P, _I, _P, ctypes.c_int64, _I, _F, _I, _P, _P]),
     "orb_batch_create": (_I, [_P, _I, _P, _P, _P, _F, _I, _I, _I, _P]),
     "orb_batch_destroy": (_V, [_P]),

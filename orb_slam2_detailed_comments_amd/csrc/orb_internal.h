@@ -29,6 +29,8 @@ extern "C" hipError_t orb_launch_distinctive(const uint8_t* desc, const int* off
                                              uint8_t* out_desc, hipStream_t st);
 extern "C" hipError_t orb_launch_hbm_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 extern "C" int orb_knn_chunks(long long nm);
+extern "C" int orb_knn_scratch_setup(int device);
+extern "C" uint2* orb_knn_scratch(int device, size_t* entries);
 extern "C" hipError_t orb_launch_knn2(const uint8_t* q, int nq, const uint8_t* m, long long nm, int method,
                                       float ratio, uint2* part, int* idx2, int* dist2, uint8_t* accept,
                                       hipStream_t st, hipEvent_t* ev);

@@ -2409,6 +2409,39 @@ extern "C" hipError_t orb_launch_knn2(const uint8_t* q, int nq, const uint8_t* m
     return hipGetLastError();
 }
 
+// ================================================= allocation-free dense k=2
+// orb_knn2_device_noalloc keeps the per-chunk partial keys in this fixed
+// array instead of a context buffer, so that it makes no allocation.  Each
+// GPU holds its own copy until the process ends; host code uses the address
+// orb_knn_scratch_setup looked up for the device, never the symbol itself.
+#define ORB_KNN_SCRATCH_ENTRIES (1 << 19)   // uint2 keys: 4 MiB per GPU
+#define ORB_KNN_SCRATCH_DEVICES 64
+static __device__ __align__(256) uint2 scratch[ORB_KNN_SCRATCH_ENTRIES];
+static uint2* g_scratch_ptr[ORB_KNN_SCRATCH_DEVICES];
+
+// With `device` current: look up this GPU's copy of the array (once).
+// Returns 0, or a HIP error code.
+extern "C" int orb_knn_scratch_setup(int device)
+{
+    if (device < 0 || device >= ORB_KNN_SCRATCH_DEVICES) return (int)hipErrorInvalidDevice;
+    if (__atomic_load_n(&g_scratch_ptr[device], __ATOMIC_ACQUIRE)) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!ptr) return (int)hipErrorInvalidSymbol;
+    __atomic_store_n(&g_scratch_ptr[device], (uint2*)ptr, __ATOMIC_RELEASE);
+    return 0;
+}
+
+// The device address of `device`'s array (null before its setup) and the
+// number of uint2 keys it holds.
+extern "C" uint2* orb_knn_scratch(int device, size_t* entries)
+{
+    if (entries) *entries = ORB_KNN_SCRATCH_ENTRIES;
+    if (device < 0 || device >= ORB_KNN_SCRATCH_DEVICES) return nullptr;
+    return __atomic_load_n(&g_scratch_ptr[device], __ATOMIC_ACQUIRE);
+}
+
 extern "C" hipError_t orb_launch_match_pairs(const MatchArgs* d_args, int npairs, int nwaves, hipStream_t st)
 {
     if (npairs <= 0) return hipSuccess;

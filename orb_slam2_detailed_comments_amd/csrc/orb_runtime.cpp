@@ -27,7 +27,7 @@ int orb_context_create(int device, orb_context** out)
     int umax[16], gauss[7];
     disc_umax(umax);
     gaussian_taps(gauss);
-    if (orb_upload_constants(umax, gauss) != hipSuccess) {
+    if (orb_upload_constants(umax, gauss) != hipSuccess || orb_knn_scratch_setup(device) != 0) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return ORB_E_DEVICE;
@@ -1051,6 +1051,22 @@ int orb_knn2_device(orb_context* ctx, const uint8_t* d_query, int nq, const uint
     if (rc) return rc;
     HIP_TRY(orb_launch_knn2(d_query, nq, d_map, nm, method, ratio, ctx->d_part, d_idx2, d_dist2, d_accept,
                             ctx->stream, nullptr));
+    return ORB_OK;
+}
+
+int orb_knn2_device_noalloc(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm,
+                            int method, float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept)
+{
+    if (!ctx || nq < 0 || nm < 0 || nm >= (1ll << 31) || (method != 0 && method != 1)) return ORB_E_ARG;
+    if (nq == 0) return ORB_OK;
+    if (!d_query || !d_idx2 || !d_dist2 || (nm && !d_map)) return ORB_E_ARG;
+    size_t entries = 0;
+    uint2* part = orb_knn_scratch(ctx->device, &entries);
+    if (!part) return ORB_E_DEVICE;
+    if ((size_t)nq * (size_t)orb_knn_chunks(nm) > entries) return ORB_E_CAPACITY;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(orb_launch_knn2(d_query, nq, d_map, nm, method, ratio, part, d_idx2, d_dist2, d_accept, ctx->stream,
+                            nullptr));
     return ORB_OK;
 }
 
