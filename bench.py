@@ -325,7 +325,7 @@ def committed_counters(stage, stage_ms, hbm_frac):
     if os.path.exists(tfile):
         with open(tfile) as f:
             out["traffic"] = json.load(f).get("bytes_per_launch")
-        out["traffic_kind"] = f"L2-fabric bytes (FETCH_SIZE+WRITE_SIZE), committed profiles/traffic_{stage}.json"
+        out["traffic_kind"] = f"L2 egress bytes (FETCH_SIZE+WRITE_SIZE, calibrated; an upper bound on HBM bytes), profiles/traffic_{stage}.json"
     vfile = os.path.join(ROOT, "profiles", f"valu_{stage}.json")
     if os.path.exists(vfile):
         with open(vfile) as f:
