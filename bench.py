@@ -269,10 +269,11 @@ def serving_extra(orb, ctx):
     descriptors.  Also the plain per-frame GPU path on the same frames."""
     from orb_slam2_detailed_comments_amd import serving
     streams = serving.MIXED_16
-    frames = serving.make_stream_frames(streams, 4, seed=300)
+    frames = serving.make_stream_frames(streams, 8, seed=300)
     srv = orb.Server(streams, max_inflight=3, context=ctx)
     out = {"workload": "16 streams (4x 640x480, 4x 752x480, 4x 1241x376, 4x 1280x720), Poisson arrivals, "
-                       "extract + match vs the stream's previous frame (TH_LOW 50, ratio 0.6)"}
+                       "extract + match vs the stream's previous frame (TH_LOW 50, ratio 0.6); 8 frames per stream "
+                       "replayed back and forth, so each match pairs sequence neighbours"}
     recs, _ = serving.run(srv, frames, fps=30.0, duration=3.0, seed=1)
     out["at_30fps_per_stream"] = serving.summarize(recs)
     print(f"[bench] serving 30 fps/stream: {out['at_30fps_per_stream']}", file=sys.stderr, flush=True)
@@ -294,7 +295,7 @@ def serving_extra(orb, ctx):
     for i, f in enumerate(frames):       # warm-up: one frame per stream
         prev[i] = exs[i](f[0])[1]
     n, t0 = 0, time.perf_counter()
-    for k in range(1, 4):
+    for k in range(1, len(frames[0])):
         for i, f in enumerate(frames):
             _, d = exs[i](f[k])
             orb.match_bf(d, prev[i], 50, 0.6, context=ctx)
@@ -347,7 +348,7 @@ def parity_sample(pyoracle, frames, gpu):
     b = float(np.float32(BF) / np.float32(FX))
     L = pyoracle.OracleExtractor(NF, 1.2, NLEV, 20, 7)
     R = pyoracle.OracleExtractor(NF, 1.2, NLEV, 20, 7)
-    bad, nkp = [], 0
+    bad, nkp, ndesc, amax = [], 0, 0, 0.0
     for p, (kl, dl, kr, dr, uR, dep) in sorted(gpu.items()):
         okl, odl, okr, odr, ouR, odep = pyoracle.stereo_frame(L, R, frames[p][0], frames[p][1], BF, b)
         same = (len(kl) == len(okl) and len(kr) == len(okr) and
@@ -355,11 +356,19 @@ def parity_sample(pyoracle, frames, gpu):
                 np.array_equal(dl, odl) and np.array_equal(dr, odr) and
                 uR.tobytes() == ouR.tobytes() and dep.tobytes() == odep.tobytes())
         nkp += len(okl) + len(okr)
+        for k, ok, d, od in ((kl, okl, dl, odl), (kr, okr, dr, odr)):
+            if len(k) == len(ok):
+                # descriptors that differ (the straddle count: rows whose
+                # float angle falls on the other side of a pattern rounding
+                # boundary) and the largest angle difference
+                ndesc += int(np.count_nonzero(np.any(d != od, axis=1)))
+                if len(k):
+                    amax = max(amax, float(np.max(np.abs(k["angle"].astype(np.float64) - ok["angle"]))))
         if not same:
             bad.append(p)
     return {"pairs": sorted(gpu), "images": 2 * len(gpu), "keypoints": nkp,
             "fields": "keypoints (all 28 bytes), descriptors, uRight, depth", "bit_exact": not bad,
-            "mismatched_pairs": bad}
+            "mismatched_pairs": bad, "descriptor_straddles": ndesc, "angle_max_abs_diff_deg": amax}
 
 
 def cpu_baseline(frames, seconds, gpu_sample=None):
@@ -409,7 +418,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=32, help="stereo frames per step")
-    ap.add_argument("--mode", choices=["graph", "stages", "per-image"], default="graph")
+    ap.add_argument("--mode", choices=["graph", "stages", "per-image", "streams"], default="graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip every extra workload")
@@ -426,7 +435,8 @@ def main():
         sys.exit(2)
 
     import orb_slam2_detailed_comments_amd as orb
-    mode = {"graph": orb.Batch.GRAPH, "stages": orb.Batch.STAGE_LAUNCHES, "per-image": orb.Batch.PER_IMAGE}[args.mode]
+    mode = {"graph": orb.Batch.GRAPH, "stages": orb.Batch.STAGE_LAUNCHES, "per-image": orb.Batch.PER_IMAGE,
+            "streams": orb.Batch.STREAMS}[args.mode]
     B = args.batch
     frames = make_frames(B)
     print(f"[bench] {B} stereo frames {W}x{H} generated", file=sys.stderr, flush=True)

@@ -196,6 +196,9 @@ int orb_stereo_frame(orb_extractor* left, orb_extractor* right, const uint8_t* i
  *           graph branches, so one group's small, short launches overlap
  *           another's large ones
  *   mode 2  per-image launches of the same kernels (the unbatched baseline)
+ *   mode 3  the branches of mode 1 launched directly on their own streams,
+ *           without a graph (one stream per branch, joined into the
+ *           context's stream)
  * Stereo pairs declared with orb_batch_set_stereo are matched after
  * extraction (Frame::ComputeStereoMatches, src/Frame.cc:831-1082). */
 typedef struct orb_batch orb_batch;

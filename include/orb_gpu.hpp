@@ -74,9 +74,9 @@ public:
     void operator()(const GrayImage& image, const void* /*mask: ignored*/, std::vector<orb_keypoint>& keypoints,
                     std::vector<uint8_t>& descriptors)
     {
+        if (image.width == 0 || image.height == 0) return;   // outputs untouched, ORBextractor.cc:1537
         keypoints.clear();
         descriptors.clear();
-        if (image.width == 0 || image.height == 0) return;
         const int cap = orb_extractor_capacity(h_, image.width, image.height);
         check("orb_extractor_capacity", cap < 0 ? cap : ORB_OK);
         keypoints.resize(cap);

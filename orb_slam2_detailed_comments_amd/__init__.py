@@ -407,7 +407,7 @@ class Batch:
     run(mode): 0 = one launch per stage over all slots, 1 = the same captured
     in a hipGraph and replayed, 2 = per-image launches of the same kernels.
     """
-    STAGE_LAUNCHES, GRAPH, PER_IMAGE = 0, 1, 2
+    STAGE_LAUNCHES, GRAPH, PER_IMAGE, STREAMS = 0, 1, 2, 3
 
     def __init__(self, sizes, nfeatures, scaleFactor=1.2, nlevels=8, iniThFAST=20, minThFAST=7, context=None):
         self.ctx = context or Context.default()

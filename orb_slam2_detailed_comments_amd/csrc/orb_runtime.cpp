@@ -934,11 +934,24 @@ int orb_batch_autotune_tiles(orb_batch* b, int steps, int* tile_h, float* ms32, 
 
 int orb_batch_run(orb_batch* b, int mode)
 {
-    if (!b || mode < 0 || mode > 2) return ORB_E_ARG;
+    if (!b || mode < 0 || mode > 3) return ORB_E_ARG;
     HIP_TRY(hipSetDevice(b->ctx->device));
-    if (mode != 1) return b->enqueue(mode);
     hipStream_t st = b->ctx->stream;
+    if (mode == 3) {
+        // the branches launched directly on their streams, no graph
+        if (b->want_branches > 1 && b->br.empty()) {
+            const int rc = b->build_branches();
+            if (rc) {
+                b->drop_branches();
+                return rc;
+            }
+        }
+        return b->br.size() > 1 ? b->enqueue_branches() : b->enqueue(0);
+    }
+    if (mode != 1) return b->enqueue(mode);
     if (!b->exec) {
+        // branches that direct (mode 3) launches may still be reading
+        if (!b->br.empty()) HIP_TRY(hipStreamSynchronize(st));
         int rc = b->want_branches > 1 ? b->build_branches() : ORB_OK;
         if (rc) {
             b->drop_branches();

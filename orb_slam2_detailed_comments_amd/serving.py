@@ -1,7 +1,9 @@
 """Open-loop driver for the serving loop: camera streams with Poisson arrivals.
 
 Frames of each stream come from synth.stream_frames (each frame the previous
-one moved by a small rotation and shift, plus fresh noise) and are cycled.
+one moved by a small rotation and shift, plus fresh noise) and are replayed
+forwards then backwards (0, 1, .., n-1, n-2, .., 1, 0, 1, ..), so that every
+request's previous frame on its stream is a neighbour in the sequence.
 Arrival times are drawn per stream (exponential inter-arrival at `fps`) and
 submitted at their scheduled time; latency is completion time minus scheduled
 arrival on the library clock.
@@ -17,6 +19,14 @@ MIXED_16 = [(640, 480, 1000)] * 4 + [(752, 480, 1000)] * 4 + [(1241, 376, 2000)]
 
 def make_stream_frames(streams, n_frames, seed=100):
     return [synth.stream_frames(w, h, seed + i, n_frames) for i, (w, h, _) in enumerate(streams)]
+
+
+def sequence_index(k, n):
+    """Frame of request k on a stream of n frames, replayed back and forth."""
+    if n <= 1:
+        return 0
+    j = k % (2 * n - 2)
+    return j if j < n else 2 * n - 2 - j
 
 
 def poisson_schedule(n_streams, fps, duration, seed):
@@ -45,7 +55,7 @@ def run(server, frames, fps=30.0, duration=3.0, seed=7, keep=None):
         while i < len(events) and t0 + events[i][0] <= now:
             t_rel, s, k = events[i]
             try:
-                ticket = server.submit(s, frames[s][k % len(frames[s])], t0 + t_rel)
+                ticket = server.submit(s, frames[s][sequence_index(k, len(frames[s]))], t0 + t_rel)
                 by_ticket[ticket] = (s, k)
             except OrbError as e:
                 if e.rc != ORB_E_ADMISSION:

@@ -1,5 +1,6 @@
 """Batched extraction: every slot of a mixed-size batch equals the oracle, in
-all three run modes (stage launches, hipGraph replay, per-image launches)."""
+every run mode (stage launches, hipGraph replay, per-image launches, and
+the graph's branches launched directly on their streams)."""
 import numpy as np
 import pytest
 
@@ -24,7 +25,7 @@ def mixed(orb, oracle):
     return batch, ref
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_batch_modes_match_oracle(mixed, mode):
     batch, ref = mixed
     for _ in range(2):   # the graph is replayed, not only captured
@@ -70,17 +71,21 @@ def test_stage_times_cover_pipeline(orb):
 
 
 def test_batch_graph_branches_match_oracle(mixed):
+    """Branches as graph replays (mode 1) and as direct launches on their own
+    streams (mode 3), alternating on one batch."""
     batch, ref = mixed
-    for k in (3, 8, 1):
+    for k in (3, 8, 1, 4):
         batch.set_branches(k)
-        for _ in range(2):
-            batch.run(1)
-        batch.synchronize()
-        for s, (ok, od) in enumerate(ref):
-            gk, gd = batch.results(s)
-            assert len(gk) == len(ok), (k, s, len(gk), len(ok))
-            np.testing.assert_array_equal(gk.view(np.uint8), ok.view(np.uint8), err_msg=f"branches {k} slot {s}")
-            np.testing.assert_array_equal(gd, od, err_msg=f"branches {k} slot {s} descriptors")
+        for mode in (1, 3, 1):
+            for _ in range(2):
+                batch.run(mode)
+            batch.synchronize()
+            for s, (ok, od) in enumerate(ref):
+                gk, gd = batch.results(s)
+                assert len(gk) == len(ok), (k, mode, s, len(gk), len(ok))
+                np.testing.assert_array_equal(gk.view(np.uint8), ok.view(np.uint8),
+                                              err_msg=f"branches {k} mode {mode} slot {s}")
+                np.testing.assert_array_equal(gd, od, err_msg=f"branches {k} mode {mode} slot {s} descriptors")
 
 
 def test_batch_autotune_keeps_results(mixed):
@@ -114,13 +119,14 @@ def test_batch_stereo_branches(orb, oracle, left, right):
     batch.synchronize()
     want = [batch.stereo_results(i) for i in range(3)]
     batch.set_branches(2)
-    for _ in range(3):
-        batch.run(1)
-    batch.synchronize()
-    for i in range(3):
-        uR, dep = batch.stereo_results(i)
-        np.testing.assert_array_equal(uR.view(np.uint32), want[i][0].view(np.uint32))
-        np.testing.assert_array_equal(dep.view(np.uint32), want[i][1].view(np.uint32))
+    for mode in (1, 3):
+        for _ in range(3):
+            batch.run(mode)
+        batch.synchronize()
+        for i in range(3):
+            uR, dep = batch.stereo_results(i)
+            np.testing.assert_array_equal(uR.view(np.uint32), want[i][0].view(np.uint32))
+            np.testing.assert_array_equal(dep.view(np.uint32), want[i][1].view(np.uint32))
     l, r = pairs[0]
     ol = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
     orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)

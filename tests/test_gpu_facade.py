@@ -1,0 +1,65 @@
+"""GPU tier: the C++ facade (include/orb_gpu.hpp) on the device.
+
+tests/_bin/facade_gpu (built by build() from tests/facade_gpu.cpp) runs
+orbgpu::ORBextractor::operator() on a stereo pair, ComputeStereoMatches and
+the pyramid and scale getters, the way Frame's stereo constructor uses the
+reference's classes (Frame.cc:146-169).  Its outputs must equal the oracle's
+bit for bit."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from orb_slam2_detailed_comments_amd import KEYPOINT_DTYPE, synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "_bin", "facade_gpu")
+
+
+@pytest.mark.parametrize("w,h,nf,seed", [(1241, 376, 2000, 41), (640, 480, 1000, 42)])
+def test_facade_stereo_frame_equals_oracle(oracle, tmp_path, w, h, nf, seed):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/_bin/facade_gpu is missing: run __graft_entry__.build() first")
+    imL, imR = synth.stereo_pair(w, h, seed)[:2]
+    imL, imR = np.ascontiguousarray(imL, np.uint8), np.ascontiguousarray(imR, np.uint8)
+    imL.tofile(tmp_path / "l.raw")
+    imR.tofile(tmp_path / "r.raw")
+    bf, fx = np.float32(386.1), np.float32(718.856)
+    b = np.float32(bf / fx)
+    res = subprocess.run([EXE, str(tmp_path / "l.raw"), str(tmp_path / "r.raw"), str(w), str(h), str(nf),
+                          repr(float(bf)), repr(float(b)), str(tmp_path)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, (res.returncode, res.stdout, res.stderr)
+    assert "facade gpu ok" in res.stdout
+
+    def rd(name, dtype):
+        return np.fromfile(tmp_path / name, dtype)
+
+    kl, kr = rd("kl.bin", KEYPOINT_DTYPE), rd("kr.bin", KEYPOINT_DTYPE)
+    dl, dr = rd("dl.bin", np.uint8).reshape(-1, 32), rd("dr.bin", np.uint8).reshape(-1, 32)
+    ur, depth = rd("ur.bin", np.float32), rd("depth.bin", np.float32)
+
+    L = oracle.OracleExtractor(nf, 1.2, 8, 20, 7)
+    R = oracle.OracleExtractor(nf, 1.2, 8, 20, 7)
+    okl, odl, okr, odr, our, odep = oracle.stereo_frame(L, R, imL, imR, float(bf), float(b))
+    assert len(kl) == len(okl) > 0 and len(kr) == len(okr) > 0
+    assert kl.tobytes() == okl.tobytes() and kr.tobytes() == okr.tobytes()
+    np.testing.assert_array_equal(dl, odl)
+    np.testing.assert_array_equal(dr, odr)
+    assert ur.tobytes() == our.tobytes() and depth.tobytes() == odep.tobytes()
+    assert np.count_nonzero(ur >= 0) > len(kl) // 4
+
+    pyr, off = rd("pyr.bin", np.uint8), 0
+    for lvl in range(8):
+        want = L.level_image(lvl)
+        got = pyr[off:off + want.size].reshape(want.shape)
+        np.testing.assert_array_equal(got, want, err_msg=f"level {lvl}")
+        off += want.size
+    assert off == pyr.size
+    sc = rd("scales.bin", np.float32).reshape(4, 8)
+    tables = L.tables()
+    for i, k in enumerate(("scale", "inv_scale", "sigma2", "inv_sigma2")):
+        assert sc[i].tobytes() == tables[k].tobytes(), k
