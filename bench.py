@@ -428,6 +428,10 @@ def main():
                     help="level kernel tile heights: auto (the tile autotuner), 32, 16, or 8 comma-separated values")
     ap.add_argument("--branches", type=int, default=0,
                     help="independent hipGraph branches (mode graph); 0: pick by the batch's autotuner")
+    ap.add_argument("--pipeline", type=int, default=0,
+                    help="batches in flight: step i runs batch i mod P, each batch on its own stream, so one "
+                         "step's last launches overlap the next step's first ones; 0: pick P (1-3) together with "
+                         "the graph branch count by timing them")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU (--gpus 1): frames and streams shard across GPUs with no collective, "
@@ -438,15 +442,22 @@ def main():
     mode = {"graph": orb.Batch.GRAPH, "stages": orb.Batch.STAGE_LAUNCHES, "per-image": orb.Batch.PER_IMAGE,
             "streams": orb.Batch.STREAMS}[args.mode]
     B = args.batch
-    frames = make_frames(B)
-    print(f"[bench] {B} stereo frames {W}x{H} generated", file=sys.stderr, flush=True)
-    ctx = orb.Context(0)
-    batch = orb.Batch([(W, H)] * (2 * B), NF, 1.2, NLEV, 20, 7, context=ctx)
-    for i, (l, r) in enumerate(frames):
-        batch.upload(2 * i, l)
-        batch.upload(2 * i + 1, r)
+    auto_pipe = args.pipeline <= 0 and mode == orb.Batch.GRAPH
+    P = 3 if auto_pipe else max(1, args.pipeline)
+    frames = make_frames(B * P)
+    print(f"[bench] {B * P} stereo frames {W}x{H} generated", file=sys.stderr, flush=True)
     b = float(np.float32(BF) / np.float32(FX))
-    batch.set_stereo(list(range(0, 2 * B, 2)), list(range(1, 2 * B, 2)), BF, b)
+    ctxs, batches = [], []
+    for k in range(P):
+        # one context (and so one stream) per batch in flight
+        ctxs.append(orb.Context(0))
+        bk = orb.Batch([(W, H)] * (2 * B), NF, 1.2, NLEV, 20, 7, context=ctxs[-1])
+        for i, (l, r) in enumerate(frames[k * B:(k + 1) * B]):
+            bk.upload(2 * i, l)
+            bk.upload(2 * i + 1, r)
+        bk.set_stereo(list(range(0, 2 * B, 2)), list(range(1, 2 * B, 2)), BF, b)
+        batches.append(bk)
+    ctx, batch = ctxs[0], batches[0]
     tiles = None
     if args.tiles == "auto":
         th, m32, m16 = batch.autotune_tiles(steps=10)
@@ -458,23 +469,56 @@ def main():
         batch.set_tile_heights(th)
         tiles = {"tile_heights": th}
     tuned = None
-    if args.branches > 0:
-        batch.set_branches(args.branches)
+    for bk in batches[1:]:
+        bk.set_tile_heights(th)
+    if auto_pipe:
+        # Batches in flight and graph branches picked together: 20 steps of
+        # each (P, branches) after a warm-up, the fastest kept (the smaller
+        # P, then the fewer branches, on a tie within 1 %).
+        cand = [(p_, br) for p_ in (1, 2, 3) for br in ((1, 2, 3) if args.branches <= 0 else (args.branches,))]
+        ms = {}
+        for p_, br in cand:
+            for bk in batches[:p_]:
+                bk.set_branches(br)
+            for i in range(2 * p_):
+                batches[i % p_].run(mode)
+            for bk in batches[:p_]:
+                bk.synchronize()
+            t0 = time.perf_counter()
+            for i in range(20):
+                batches[i % p_].run(mode)
+            for bk in batches[:p_]:
+                bk.synchronize()
+            ms[(p_, br)] = (time.perf_counter() - t0) * 1e3 / 20
+        fastest = min(ms.values())
+        P, args.branches = min((k for k in cand if ms[k] <= fastest * 1.01), key=lambda k: (k[0], k[1]))
+        for bk in batches[:P]:
+            bk.set_branches(args.branches)
+        tuned = {"chosen": {"batches_in_flight": P, "branches": args.branches},
+                 "ms_per_step": {f"{p_}x{br}": round(v, 4) for (p_, br), v in ms.items()}}
+        print(f"[bench] autotuned pipeline: {tuned}", file=sys.stderr, flush=True)
+    elif args.branches > 0:
+        for bk in batches:
+            bk.set_branches(args.branches)
     elif mode == orb.Batch.GRAPH:
         best, tms = batch.autotune(steps=10)
         tuned = {"chosen": best, "ms_per_replay": {str(k): round(v, 4) for k, v in tms.items()}}
         args.branches = best
+        for bk in batches[1:]:
+            bk.set_branches(best)
         print(f"[bench] autotuned graph branches: {best} ({tuned['ms_per_replay']})", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        batch.run(mode)
-    batch.synchronize()
+    for i in range(args.warmup * P):
+        batches[i % P].run(mode)
+    for bk in batches:
+        bk.synchronize()
     print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run(mode)
-    batch.synchronize()
+    for i in range(args.steps):
+        batches[i % P].run(mode)
+    for bk in batches:
+        bk.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = B * args.steps / (t1 - t0)
@@ -487,11 +531,15 @@ def main():
     matched = [int((batch.stereo_results(p)[0] >= 0).sum()) for p in range(B)]
     sample = sorted({round(i * (B - 1) / 5) for i in range(6)})
     gpu_sample = {p: batch.results(2 * p) + batch.results(2 * p + 1) + batch.stereo_results(p) for p in sample}
+    for k in range(1, P):           # two pairs of every other batch in flight
+        for p in (1, B - 2):
+            bk = batches[k]
+            gpu_sample[k * B + p] = bk.results(2 * p) + bk.results(2 * p + 1) + bk.stereo_results(p)
 
     # Host to host: the same step with the 64 images copied in from host
     # memory and every result read back (one step at a time, no overlap of
     # copies with compute), outside the headline figure.
-    imgs = [im for pair in frames for im in pair]
+    imgs = [im for pair in frames[:B] for im in pair]
     res = None
     for _ in range(2):
         batch.upload_all(imgs)
@@ -528,7 +576,8 @@ def main():
            "data": "synthetic (seeded shapes+noise frames; right view = left through a 12-plane disparity map)",
            "config": {"workload": f"stereo extract+match, {W}x{H}, nFeatures {NF}, 8 levels, scale 1.2, FAST 20/7, "
                                   f"bf {BF}, fx {FX}", "frames_per_step": B, "images_per_step": 2 * B,
-                      "run_mode": args.mode, "graph_branches": args.branches, "branch_autotune": tuned,
+                      "run_mode": args.mode, "graph_branches": args.branches, "pipeline_autotune": tuned,
+                      "batches_in_flight": P,
                       "tile_autotune": tiles,
                       "mean_keypoints_per_image": round(float(np.mean(kps)), 1),
                       "mean_stereo_matches": round(float(np.mean(matched)), 1)},

@@ -274,6 +274,13 @@ __device__ __forceinline__ unsigned vresize(unsigned S0, unsigned S1, unsigned b
     return MODE == 0 ? vc : (MODE == 1 ? vs : (sse ? vs : vc));
 }
 
+// High 32 bits of a 24 x 24-bit product (v_mul_hi_u32_u24, full rate): the
+// operands are masked to 24 bits, which is what lets the compiler pick it.
+__device__ __forceinline__ unsigned mulhi_u24(unsigned a, unsigned b)
+{
+    return (unsigned)(((unsigned long long)(a & 0xFFFFFFu) * (unsigned long long)(b & 0xFFFFFFu)) >> 32);
+}
+
 template <int M>
 struct ModeTag {
     static constexpr int value = M;
@@ -519,16 +526,42 @@ __global__ __launch_bounds__(256) void level_kernel(const ImageJob* __restrict__
                         const unsigned w01 = __builtin_amdgcn_alignbyte(q0[2], q0[1], shb);
                         const unsigned w10 = __builtin_amdgcn_alignbyte(q1[1], q1[0], shb);
                         const unsigned w11 = __builtin_amdgcn_alignbyte(q1[2], q1[1], shb);
-                        const unsigned bv0 = (unsigned)yb[ey] & 0xFFFFu, bv1 = (unsigned)yb[ey] >> 16;
+                        const unsigned ybv = (unsigned)yb[ey];
                         unsigned d = 0;
+                        if constexpr (MODE == 1) {
+                            // SSE2 rounding with the horizontal taps scaled by
+                            // 16: S16 = 16 S, so S16 & 0xFFFF00 = (S >> 4) << 8
+                            // (S16 < 2^23), and the high word of its product
+                            // with b << 8 is ((S >> 4) * b) >> 16 in one
+                            // v_mul_hi_u32_u24.  t = h0 + h1 + 2 <= 1023 fits
+                            // a 16-bit lane: the >> 2 is done two pixels at a
+                            // time and the four bytes gathered by one v_perm.
+                            const unsigned bs0 = (ybv << 8) & 0xFFFF00u, bs1 = (ybv >> 8) & 0xFFFF00u;
+                            unsigned t[4];
 #pragma unroll
-                        for (int k = 0; k < 4; k++) {
-                            const orb_u16x2 tk = __builtin_bit_cast(orb_u16x2, (unsigned)tap[k]);
-                            const unsigned S0 = __builtin_amdgcn_udot2(
-                                __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w01, w00, sel[k])), tk, 0u, false);
-                            const unsigned S1 = __builtin_amdgcn_udot2(
-                                __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w11, w10, sel[k])), tk, 0u, false);
-                            d |= vresize<MODE>(S0, S1, bv0, bv1, sse[k]) << (8 * k);
+                            for (int k = 0; k < 4; k++) {
+                                const orb_u16x2 tk = __builtin_bit_cast(orb_u16x2, (unsigned)tap[k] << 4);
+                                const unsigned S0 = __builtin_amdgcn_udot2(
+                                    __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w01, w00, sel[k])), tk, 0u, false);
+                                const unsigned S1 = __builtin_amdgcn_udot2(
+                                    __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w11, w10, sel[k])), tk, 0u, false);
+                                t[k] = mulhi_u24(S0 & 0xFFFF00u, bs0) + mulhi_u24(S1 & 0xFFFF00u, bs1) + 2u;
+                            }
+                            const orb_u16x2 two = {2, 2};
+                            const unsigned u01 = __builtin_bit_cast(unsigned, __builtin_bit_cast(orb_u16x2, t[0] | (t[1] << 16)) >> two);
+                            const unsigned u23 = __builtin_bit_cast(unsigned, __builtin_bit_cast(orb_u16x2, t[2] | (t[3] << 16)) >> two);
+                            d = __builtin_amdgcn_perm(u23, u01, 0x06040200u);
+                        } else {
+                            const unsigned bv0 = ybv & 0xFFFFu, bv1 = ybv >> 16;
+#pragma unroll
+                            for (int k = 0; k < 4; k++) {
+                                const orb_u16x2 tk = __builtin_bit_cast(orb_u16x2, (unsigned)tap[k]);
+                                const unsigned S0 = __builtin_amdgcn_udot2(
+                                    __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w01, w00, sel[k])), tk, 0u, false);
+                                const unsigned S1 = __builtin_amdgcn_udot2(
+                                    __builtin_bit_cast(orb_u16x2, __builtin_amdgcn_perm(w11, w10, sel[k])), tk, 0u, false);
+                                d |= vresize<MODE>(S0, S1, bv0, bv1, sse[k]) << (8 * k);
+                            }
                         }
                         pxw[ey][c] = d;
                         level_out(ey, c, d);
