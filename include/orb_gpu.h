@@ -39,7 +39,10 @@ typedef struct orb_context orb_context;
 typedef struct orb_extractor orb_extractor;
 
 /* Device context: binds a HIP device and owns the stream all work runs on.
- * No reference counterpart (the reference is CPU-only). */
+ * No reference counterpart (the reference is CPU-only).  Every extractor,
+ * batch, server, vocabulary and rectifier made on a context holds a
+ * reference to it: orb_context_destroy releases the caller's, and the context
+ * is freed with the last of them, so handles may be destroyed in any order. */
 int orb_context_create(int device, orb_context** out);
 void orb_context_destroy(orb_context* ctx);
 int orb_context_synchronize(orb_context* ctx);

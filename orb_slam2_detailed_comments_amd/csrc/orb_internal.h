@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -122,6 +123,10 @@ using namespace orb_detail;
 struct orb_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Owners: the creator's handle plus every extractor, batch, server,
+    // vocabulary and rectifier made on it; the last release frees it, so
+    // destroying the handle before its users is safe.
+    std::atomic<int> refs{1};
     // brute-force matcher scratch
     uint8_t* d_q = nullptr;
     uint8_t* d_t = nullptr;
@@ -164,8 +169,35 @@ struct orb_context {
     }
 };
 
+// Releases one owner of a context; the last frees its buffers and stream.
+void orb_context_release_ref(orb_context* c);
+inline void orb_context_retain_ref(orb_context* c)
+{
+    if (c) c->refs.fetch_add(1, std::memory_order_relaxed);
+}
+
+// A context pointer that owns one reference while set.
+struct CtxRef {
+    orb_context* p = nullptr;
+    CtxRef() = default;
+    CtxRef(const CtxRef&) = delete;
+    CtxRef& operator=(const CtxRef&) = delete;
+    CtxRef& operator=(orb_context* c)
+    {
+        if (c != p) {
+            orb_context_retain_ref(c);
+            orb_context_release_ref(p);
+            p = c;
+        }
+        return *this;
+    }
+    ~CtxRef() { orb_context_release_ref(p); }
+    operator orb_context*() const { return p; }
+    orb_context* operator->() const { return p; }
+};
+
 struct orb_extractor {
-    orb_context* ctx = nullptr;
+    CtxRef ctx;
     int nfeatures = 0, nlevels = 0, iniTh = 0, minTh = 0;
     float scaleFactorF = 1.2f;
     double scaleFactor = 1.2;

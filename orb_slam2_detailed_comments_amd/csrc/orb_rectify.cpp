@@ -15,7 +15,7 @@ extern "C" hipError_t orb_launch_remap(const uint8_t* src, size_t sstride, int s
                                        const float* mapy, int w, int h, uint8_t* dst, size_t dstride, hipStream_t st);
 
 struct orb_rectify {
-    orb_context* ctx = nullptr;
+    CtxRef ctx;
     int w = 0, h = 0;
     float* d_mapx = nullptr;
     float* d_mapy = nullptr;

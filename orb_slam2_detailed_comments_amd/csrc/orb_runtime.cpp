@@ -36,9 +36,12 @@ int orb_context_create(int device, orb_context** out)
     return ORB_OK;
 }
 
-void orb_context_destroy(orb_context* ctx)
+void orb_context_destroy(orb_context* ctx) { orb_context_release_ref(ctx); }
+}  // extern "C"
+
+void orb_context_release_ref(orb_context* ctx)
 {
-    if (!ctx) return;
+    if (!ctx || ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_q) (void)hipFree(ctx->d_q);
@@ -54,6 +57,8 @@ void orb_context_destroy(orb_context* ctx)
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
+
+extern "C" {
 
 int orb_context_synchronize(orb_context* ctx)
 {
@@ -456,7 +461,7 @@ int orb_extractor_debug_candidates(orb_extractor* ex, int level, int* xyr, int c
 // (mode 1), or the per-image launch sequence for comparison (mode 2).
 // Optional stereo pairs among the slots are matched in one multi-pair launch.
 struct orb_batch {
-    orb_context* ctx = nullptr;
+    CtxRef ctx;
     std::vector<orb_extractor*> ex;
     ImageJob* d_jobs = nullptr;
     RunTable* h_rt = nullptr;

@@ -79,7 +79,7 @@ size_t result_bytes(int cap) { return 256 + (size_t)cap * (sizeof(OrbKeyPoint) +
 }  // namespace
 
 struct orb_server {
-    orb_context* ctx = nullptr;
+    CtxRef ctx;
     float scaleFactor = 1.2f;
     int nlevels = 8, iniTh = 20, minTh = 7, th_low = 50;
     float ratio = 0.6f;

@@ -40,7 +40,7 @@ int grow(T** p, size_t* cap, size_t n)
 }  // namespace
 
 struct orb_vocabulary {
-    orb_context* ctx = nullptr;
+    CtxRef ctx;
     int k = 0, L = 0, scoring = 0, weighting = 0, n_nodes = 0, n_words = 0, max_depth = 0;
     int2* d_meta = nullptr;
     uint4* d_desc = nullptr;
