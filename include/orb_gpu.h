@@ -337,6 +337,16 @@ int orb_vocabulary_bow(orb_vocabulary* v, int n_images, const uint8_t* desc, con
 int orb_vocabulary_bow_time(orb_vocabulary* v, int n_images, const uint8_t* desc, const int32_t* offsets,
                             int levelsup, int steps, float* ms);
 
+/* Windowed searches (SearchForInitialization, SearchByProjection) resolve
+ * the reference's in-order passes in parallel rounds, up to `rounds` (0..48,
+ * default 48) before running the in-order pass for what has not settled;
+ * results are identical either way.  No reference counterpart: a tuning and
+ * test aid.  orb_context_last_rounds returns the rounds the last windowed
+ * search on this context ran and sets *sequential (may be NULL) to 1 if it
+ * fell back to the in-order pass for some pair or frame. */
+int orb_context_set_round_limit(orb_context* ctx, int rounds);
+int orb_context_last_rounds(const orb_context* ctx, int* sequential);
+
 /* ---------------------------------------------------------------- windowed matching
  * ORBmatcher::SearchForInitialization (src/ORBmatcher.cc:573-717) over the
  * Frame keypoint grid (64 x 48 cells, src/Frame.cc:399-423, 590-700), for
@@ -347,7 +357,9 @@ int orb_vocabulary_bow_time(orb_vocabulary* v, int n_images, const uint8_t* desc
  * updated in place for the matched keypoints.  The initializer calls it with
  * window 100, ratio 0.9 and the orientation check (Tracking.cc:915-927).
  * matches12 receives vnMatches12 per F1 keypoint, nmatches the count per
- * pair. */
+ * pair.  The reference's in-order pass over the F1 keypoints runs as parallel
+ * rounds that converge to its result; pairs still changing after the
+ * context's round limit get the in-order pass itself. */
 int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypoint* kp1, const uint8_t* desc1,
                                   const int32_t* off1, const orb_keypoint* kp2, const uint8_t* desc2,
                                   const int32_t* off2, const float* bounds, float* prev_matched, int window_size,

@@ -44,6 +44,8 @@ SIGNATURES = {
     "orb_context_create": (_I, [_I, _P]),
     "orb_batch_upload_images": (_I, [_P, _I, _P, _P, _P]),
     "orb_extractor_set_tile_heights": (_I, [_P, _I, _P]),
+    "orb_context_set_round_limit": (_I, [_P, _I]),
+    "orb_context_last_rounds": (_I, [_P, _P]),
     "orb_batch_set_tile_heights": (_I, [_P, _I, _P]),
     "orb_batch_autotune_tiles": (_I, [_P, _I, _P, _P, _P]),
     "orb_batch_read_all": (_I, [_P, _P, _P, _I, _P, _P, _P]),
@@ -171,6 +173,18 @@ class Context:
 
     def synchronize(self):
         _check("orb_context_synchronize", lib().orb_context_synchronize(self.h))
+
+    def set_round_limit(self, rounds):
+        """Parallel rounds of the windowed searches before their in-order pass
+        (0..48; 0 runs the in-order pass alone)."""
+        _check("orb_context_set_round_limit", lib().orb_context_set_round_limit(self.h, int(rounds)))
+
+    def last_rounds(self):
+        """(rounds, fell back to the in-order pass) of the last windowed search."""
+        seq = ctypes.c_int()
+        r = lib().orb_context_last_rounds(self.h, ctypes.byref(seq))
+        _check("orb_context_last_rounds", min(r, 0))
+        return r, bool(seq.value)
 
     def close(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -127,6 +127,10 @@ struct orb_context {
     // vocabulary and rectifier made on it; the last release frees it, so
     // destroying the handle before its users is safe.
     std::atomic<int> refs{1};
+    // windowed searches: rounds before the sequential pass, and what the
+    // last search used
+    int round_limit = 48;
+    int last_rounds = 0, last_sequential = 0;
     // brute-force matcher scratch
     uint8_t* d_q = nullptr;
     uint8_t* d_t = nullptr;

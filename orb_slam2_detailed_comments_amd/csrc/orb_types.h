@@ -176,7 +176,16 @@ struct WinPair {
     int2* cand;             // (i2, distance) lists
     int* m12;               // n1: vnMatches12
     int* nmatch;            // nmatches
+    // SearchForInitialization's in-order pass as parallel rounds (unused by
+    // the grid kernels of SearchByProjection, which leave them null)
+    int* out[2];            // n1 per round parity: accepted target << 9 | distance, or -1
+    unsigned* claim[3];     // n2 * ORB_WIN_SLOTS per round mod 3: query << 9 | distance
+    int* ccnt[3];           // n2 per round mod 3: claims written (beyond ORB_WIN_SLOTS: overflow)
+    int* changed;           // [ORB_WIN_MAXIT]: bit 0 an outcome changed, bit 1 claims overflowed,
+                            // bit 2 the round has not run
 };
+#define ORB_WIN_MAXIT 48
+#define ORB_WIN_SLOTS 8
 
 // ORBmatcher::SearchByProjection: one query (a map point to place in the
 // frame) after the per-point arithmetic of either overload.
@@ -211,7 +220,7 @@ struct ProjFrame {
     unsigned* cand;             // i2 | distance << 12 | octave << 21, GetFeaturesInArea order
     int* out[2];                // outcome per query by round parity: keypoint or -1
     int* claim[3];              // per keypoint: first claiming query (with observations), by round mod 3
-    int* changed;               // [ORB_PROJ_MAXIT]: round k changed some outcome
+    int* changed;               // [ORB_PROJ_MAXIT]: bit 0 round k changed some outcome, bit 2 not run
     int rule;                   // 1: local map (best + second, level ratio); 2: last frame (best, rotation)
     float nnratio;
     int check_orientation;

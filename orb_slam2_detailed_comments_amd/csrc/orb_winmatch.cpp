@@ -12,7 +12,10 @@ using namespace orb_detail;
 
 extern "C" hipError_t orb_launch_win_count(const WinPair* d_pairs, int npairs, int max_n1, hipStream_t st);
 extern "C" hipError_t orb_launch_win_scan(const WinPair* d_pairs, int npairs, int* d_totals, hipStream_t st);
-extern "C" hipError_t orb_launch_win_match(const WinPair* d_pairs, int npairs, int max_n1, hipStream_t st);
+extern "C" hipError_t orb_launch_win_fill(const WinPair* d_pairs, int npairs, int max_n1, hipStream_t st);
+extern "C" hipError_t orb_launch_win_rounds(const WinPair* d_pairs, int npairs, int max_n1, int k0, int k1,
+                                            hipStream_t st);
+extern "C" hipError_t orb_launch_win_finish(const WinPair* d_pairs, int npairs, int sequential, hipStream_t st);
 
 namespace {
 
@@ -49,8 +52,11 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
     const size_t s_cells = up((size_t)n_pairs * (64 * 48 + 1) * 4), s_cnt = up((size_t)N1 * 4);
     const size_t s_off = up((size_t)(N1 + n_pairs) * 4), s_m12 = up((size_t)N1 * 4), s_nm = up((size_t)n_pairs * 4);
     const size_t s_pairs = up((size_t)n_pairs * sizeof(WinPair)), s_tot = up((size_t)n_pairs * 4);
+    // round state: outcomes by parity, claims and claim counts by round mod 3, change flags
+    const size_t s_out = up((size_t)N1 * 4), s_claim = up((size_t)N2 * ORB_WIN_SLOTS * 4), s_ccnt = up((size_t)N2 * 4);
+    const size_t s_chg = up((size_t)n_pairs * ORB_WIN_MAXIT * 4);
     const size_t total = s_kp1 + s_d1 + s_kp2 + s_d2 + s_prev + s_sorted + s_cells + s_cnt + s_off + s_m12 + s_nm +
-                         s_pairs + s_tot;
+                         s_pairs + s_tot + 2 * s_out + 3 * s_claim + 3 * s_ccnt + s_chg;
     int rc = ctx->grow(&ctx->d_win, &ctx->win_cap, total);     // kept by the context between calls
     if (rc) return rc;
     char* m = (char*)ctx->d_win;
@@ -73,6 +79,10 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
     int* d_nm = (int*)take(s_nm);
     WinPair* d_pairs = (WinPair*)take(s_pairs);
     int* d_tot = (int*)take(s_tot);
+    int* d_out[2] = {(int*)take(s_out), (int*)take(s_out)};
+    unsigned* d_claim[3] = {(unsigned*)take(s_claim), (unsigned*)take(s_claim), (unsigned*)take(s_claim)};
+    int* d_ccnt[3] = {(int*)take(s_ccnt), (int*)take(s_ccnt), (int*)take(s_ccnt)};
+    int* d_chg = (int*)take(s_chg);
     if (N1) {
         HIP_TRY(hipMemcpyAsync(d_kp1, kp1, (size_t)N1 * sizeof(OrbKeyPoint), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(d_d1, desc1, (size_t)N1 * 32, hipMemcpyHostToDevice, st));
@@ -106,6 +116,12 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
         w.off = d_off + off1[p] + p;
         w.m12 = d_m12 + off1[p];
         w.nmatch = d_nm + p;
+        for (int b = 0; b < 2; b++) w.out[b] = d_out[b] + off1[p];
+        for (int b = 0; b < 3; b++) {
+            w.claim[b] = d_claim[b] + (size_t)off2[p] * ORB_WIN_SLOTS;
+            w.ccnt[b] = d_ccnt[b] + off2[p];
+        }
+        w.changed = d_chg + (size_t)p * ORB_WIN_MAXIT;
     }
     HIP_TRY(hipMemcpyAsync(d_pairs, pairs.data(), n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st));
     HIP_TRY(orb_launch_win_count(d_pairs, n_pairs, max_n1, st));
@@ -123,7 +139,31 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
     if (rc) return rc;
     for (int p = 0; p < n_pairs; p++) pairs[p].cand = (int2*)ctx->d_wcand + base[p];
     HIP_TRY(hipMemcpyAsync(d_pairs, pairs.data(), n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st));
-    HIP_TRY(orb_launch_win_match(d_pairs, n_pairs, max_n1, st));
+    HIP_TRY(orb_launch_win_fill(d_pairs, n_pairs, max_n1, st));
+    // Rounds in chunks of 4 between reads of the change flags, until every
+    // pair has settled; pairs still changing after ORB_WIN_MAXIT rounds get
+    // the sequential pass.
+    std::vector<int> chg((size_t)n_pairs * ORB_WIN_MAXIT);
+    int k = 0;
+    bool settled = false;
+    const int limit = std::min(ctx->round_limit, ORB_WIN_MAXIT);
+    while (!settled && k < limit) {
+        const int k1 = std::min(limit, k + 4);
+        HIP_TRY(orb_launch_win_rounds(d_pairs, n_pairs, max_n1, k, k1, st));
+        k = k1;
+        HIP_TRY(hipMemcpyAsync(chg.data(), d_chg, chg.size() * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        settled = true;
+        for (int p = 0; p < n_pairs && settled; p++) {
+            const int* c = chg.data() + (size_t)p * ORB_WIN_MAXIT;
+            bool sp = false;
+            for (int r = 1; r < k && !sp; r++) sp = c[r] == 0 && !(c[r - 1] & 2);
+            settled = sp;
+        }
+    }
+    ctx->last_rounds = k;
+    ctx->last_sequential = settled ? 0 : 1;
+    HIP_TRY(orb_launch_win_finish(d_pairs, n_pairs, settled ? 0 : 1, st));
     if (N1) {
         HIP_TRY(hipMemcpyAsync(matches12, d_m12, (size_t)N1 * 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipMemcpyAsync(prev_matched, d_prev, (size_t)N1 * 8, hipMemcpyDeviceToHost, st));
@@ -319,8 +359,9 @@ int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp
     std::vector<int> chg((size_t)F * ORB_PROJ_MAXIT);
     int k = 0;
     bool settled = false;
-    while (!settled && k < ORB_PROJ_MAXIT) {
-        const int k1 = std::min(ORB_PROJ_MAXIT, k + 4);
+    const int limit = std::min(ctx->round_limit, ORB_PROJ_MAXIT);
+    while (!settled && k < limit) {
+        const int k1 = std::min(limit, k + 4);
         HIP_TRY(orb_launch_proj_rounds(d_frames, F, max_nq, k, k1, st));
         k = k1;
         HIP_TRY(hipMemcpyAsync(chg.data(), d_chg, chg.size() * 4, hipMemcpyDeviceToHost, st));
@@ -333,6 +374,8 @@ int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp
         }
     }
     if (rounds_used) *rounds_used = k;
+    ctx->last_rounds = k;
+    ctx->last_sequential = settled ? 0 : 1;
     HIP_TRY(orb_launch_proj_finish(d_frames, F, settled ? 0 : 1, st));
     if (Q) HIP_TRY(hipMemcpyAsync(pt_match, d_ptm, (size_t)Q * 4, hipMemcpyDeviceToHost, st));
     if (N) HIP_TRY(hipMemcpyAsync(key_point, d_keyp, (size_t)N * 4, hipMemcpyDeviceToHost, st));
@@ -344,6 +387,20 @@ int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp
 }  // namespace
 
 extern "C" {
+
+int orb_context_set_round_limit(orb_context* ctx, int rounds)
+{
+    if (!ctx || rounds < 0 || rounds > ORB_WIN_MAXIT) return ORB_E_ARG;
+    ctx->round_limit = rounds;
+    return ORB_OK;
+}
+
+int orb_context_last_rounds(const orb_context* ctx, int* sequential)
+{
+    if (!ctx) return ORB_E_ARG;
+    if (sequential) *sequential = ctx->last_sequential;
+    return ctx->last_rounds;
+}
 
 int orb_search_by_projection(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
                              const float* uright, const uint8_t* occupied, const int32_t* key_off,

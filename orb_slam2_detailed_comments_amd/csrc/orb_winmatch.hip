@@ -15,10 +15,15 @@
 //   win_count_kernel   per query (one wavefront): how many candidates
 //   win_scan_kernel    per pair: exclusive scan of the counts
 //   win_fill_kernel    per query: the ordered (i2, distance) candidate list
+//   win_round_kernel   one round of the in-order pass resolved in parallel
+//                      (below), 16 lanes per query
+//   win_finish_kernel  per pair: vnMatches12, nmatches, the rotation check and
+//                      vbPrevMatched from the settled round's outcomes
 //   win_greedy_kernel  per pair (one wavefront): the reference's sequential
 //                      pass over the queries, each query's list reduced
 //                      across the lanes, state (vMatchedDistance,
-//                      vnMatches21, vnMatches12, the rotation histogram) in LDS
+//                      vnMatches21, vnMatches12, the rotation histogram) in
+//                      LDS; for pairs whose rounds did not settle
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -115,6 +120,10 @@ __global__ __launch_bounds__(1024) void win_grid_kernel(const WinPair* __restric
             __syncthreads();
         }
     for (int i = threadIdx.x; i < n; i += 1024) P.sorted[i] = (int)(key[i] & 0xFFFFFu);
+    if (P.changed) {   // SearchForInitialization's rounds: rounds 0 and 1 write claim buffers 1 and 2
+        for (int i = threadIdx.x; i < n; i += 1024) P.ccnt[0][i] = P.ccnt[1][i] = 0;
+        for (int k = threadIdx.x; k < ORB_WIN_MAXIT; k += 1024) P.changed[k] = k == 0 ? 1 : 4;   // bit 2: not run
+    }
     for (int c = threadIdx.x; c <= WIN_CELLS; c += 1024) {
         int lo = 0, hi = n;
         while (lo < hi) {
@@ -199,7 +208,17 @@ __global__ __launch_bounds__(256) void win_fill_kernel(const WinPair* __restrict
 // reference's sequential best / second.  Lane 0 applies TH_LOW, the ratio
 // and the steal of a matched target; the rotation histogram counts every
 // match event, as rotHist's sizes do.
-__global__ __launch_bounds__(64) void win_greedy_kernel(const WinPair* __restrict__ pairs)
+// The first round k >= 1 that ran, changed no outcome and read complete
+// claims (round k - 1 did not overflow its slots): its outcomes are the
+// in-order pass's.  0 if there is none.
+__device__ __forceinline__ int win_settled_round(const int* chg)
+{
+    for (int k = 1; k < ORB_WIN_MAXIT; k++)
+        if (chg[k] == 0 && !(chg[k - 1] & 2)) return k;
+    return 0;
+}
+
+__global__ __launch_bounds__(64) void win_greedy_kernel(const WinPair* __restrict__ pairs, int skip_settled)
 {
     __shared__ int vmd[WIN_CAP];
     __shared__ int m21[WIN_CAP];
@@ -208,6 +227,7 @@ __global__ __launch_bounds__(64) void win_greedy_kernel(const WinPair* __restric
     __shared__ int hist[WIN_HISTO];
     __shared__ int s_n;
     const WinPair& P = pairs[blockIdx.x];
+    if (skip_settled && win_settled_round(P.changed) > 0) return;
     const int lane = threadIdx.x;
     const int n1 = P.n1, n2 = P.n2;
     for (int i = lane; i < n2; i += 64) {
@@ -343,11 +363,10 @@ extern "C" hipError_t orb_launch_win_scan(const WinPair* d_pairs, int npairs, in
     return hipGetLastError();
 }
 
-extern "C" hipError_t orb_launch_win_match(const WinPair* d_pairs, int npairs, int max_n1, hipStream_t st)
+extern "C" hipError_t orb_launch_win_fill(const WinPair* d_pairs, int npairs, int max_n1, hipStream_t st)
 {
-    if (npairs <= 0) return hipSuccess;
-    if (max_n1 > 0) win_fill_kernel<<<dim3((max_n1 + 3) / 4, npairs), 256, 0, st>>>(d_pairs);
-    win_greedy_kernel<<<npairs, 64, 0, st>>>(d_pairs);
+    if (npairs <= 0 || max_n1 <= 0) return hipSuccess;
+    win_fill_kernel<<<dim3((max_n1 + 3) / 4, npairs), 256, 0, st>>>(d_pairs);
     return hipGetLastError();
 }
 
@@ -441,7 +460,7 @@ __global__ __launch_bounds__(256) void proj_prep_local_kernel(ProjFrame* __restr
     ProjFrame& P = frames[blockIdx.y];
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x == 0)
-        for (int k = threadIdx.x; k < ORB_PROJ_MAXIT; k += 256) P.changed[k] = k == 0;
+        for (int k = threadIdx.x; k < ORB_PROJ_MAXIT; k += 256) P.changed[k] = k == 0 ? 1 : 4;   // bit 2: not run
     if (q >= P.nq) return;
     const float* src = pts + (size_t)(pt_off[blockIdx.y] + q) * 6;   // proj_x, proj_y, proj_xr, view_cos, level, flags
     const int level = __float_as_int(src[4]), fl = __float_as_int(src[5]);
@@ -472,7 +491,7 @@ __global__ __launch_bounds__(256) void proj_prep_last_kernel(ProjFrame* __restri
     ProjFrame& P = frames[blockIdx.y];
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x == 0)
-        for (int k = threadIdx.x; k < ORB_PROJ_MAXIT; k += 256) P.changed[k] = k == 0;
+        for (int k = threadIdx.x; k < ORB_PROJ_MAXIT; k += 256) P.changed[k] = k == 0 ? 1 : 4;   // bit 2: not run
     if (q >= P.nq) return;
     const float* c = cam + (size_t)blockIdx.y * 7;
     const float fx = c[0], fy = c[1], cx = c[2], cy = c[3], mbf = c[4], mb = c[5], tlc_z = c[6];
@@ -590,12 +609,15 @@ __device__ __forceinline__ int proj_decide(const ProjFrame& P, int q, int sub, B
 
 // Round k (grid (ceil(max nq / 16), frames), 16 lanes per query): outcomes
 // against the claims of round k - 1, claims of these outcomes, and the reset
-// of the buffer round k + 1 will fill.  A frame whose round k - 1 changed
-// nothing has settled and is left alone.
+// of the buffer round k + 1 will fill.  A frame settled at an earlier round
+// (one that ran and changed nothing) is left alone.  A round clears bit 2 of
+// its flag (set: not run).
 __global__ __launch_bounds__(256) void proj_round_kernel(ProjFrame* __restrict__ frames, int k)
 {
     ProjFrame& P = frames[blockIdx.y];
-    if (k > 0 && P.changed[k - 1] == 0) return;
+    for (int j = 1; j < k; j++)
+        if (P.changed[j] == 0) return;                 // settled at round j: later rounds are left alone
+    if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAnd(&P.changed[k], ~4);
     const int* __restrict__ claimR = P.claim[k % 3];
     int* claimW = P.claim[(k + 1) % 3];
     int* claimC = P.claim[(k + 2) % 3];
@@ -775,5 +797,175 @@ extern "C" hipError_t orb_launch_proj_finish(ProjFrame* d_frames, int nframes, i
     if (nframes <= 0) return hipSuccess;
     if (sequential) proj_seq_kernel<<<nframes, 64, 0, st>>>(d_frames);
     proj_finish_kernel<<<nframes, 1024, 0, st>>>(d_frames);
+    return hipGetLastError();
+}
+
+// ============================== SearchForInitialization in parallel rounds
+// The reference's pass (ORBmatcher.cc:599-665) visits the F1 keypoints in
+// order; query i1 skips a candidate i2 whose vMatchedDistance is <= the
+// candidate's distance, and an accepted query takes i2 (stealing it from its
+// holder).  A target's successive holders have strictly decreasing
+// distances, so vMatchedDistance[i2] before query i1 is the smallest distance
+// of the queries < i1 accepted with target i2.  Query i1's outcome is thus a
+// function of the outcomes of the queries before it, and the in-order pass
+// is the unique fixed point of "recompute every outcome against the claims
+// of the previous round's outcomes" (round k fixes the first k queries).
+// Each target keeps up to ORB_WIN_SLOTS claims (query, distance) per round;
+// a round that overflows them is not trusted as the fixed point.
+
+// Round k (grid (ceil(max n1 / 16), pairs), 16 lanes per query): outcomes
+// against the claims of round k - 1, claims of these outcomes, and the reset
+// of the counts round k + 1 will fill.  A pair settled at an earlier round is
+// left alone.  A round clears bit 2 of its flag (set: not run).
+__global__ __launch_bounds__(256) void win_round_kernel(const WinPair* __restrict__ pairs, int k)
+{
+    const WinPair& P = pairs[blockIdx.y];
+    const int ks = win_settled_round(P.changed);
+    if (ks > 0 && ks < k) return;
+    if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAnd(&P.changed[k], ~4);
+    const int* __restrict__ cntR = P.ccnt[k % 3];
+    const unsigned* __restrict__ claimR = P.claim[k % 3];
+    int* cntW = P.ccnt[(k + 1) % 3];
+    unsigned* claimW = P.claim[(k + 1) % 3];
+    int* cntC = P.ccnt[(k + 2) % 3];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < P.n2; i += gridDim.x * 256) cntC[i] = 0;
+    const int q = blockIdx.x * 16 + (threadIdx.x >> 4), sub = threadIdx.x & 15;
+    const bool active = q < P.n1;
+    int res = -1;
+    if (active && P.cnt[q] > 0 && P.kp1[q].octave == 0) {   // the 16 lanes of a query agree
+        const int L = P.cnt[q];
+        const int2* list = P.cand + P.off[q];
+        unsigned b1 = 0xFFFFFFFFu;
+        int b2 = 0x7FFFFFFF;
+        for (int p = sub; p < L; p += 16) {
+            const int2 c = list[p];
+            const int nc = min(cntR[c.x], ORB_WIN_SLOTS);
+            bool blocked = false;
+            for (int j = 0; j < nc; j++) {
+                const unsigned e = claimR[(size_t)c.x * ORB_WIN_SLOTS + j];
+                blocked |= (int)(e >> 9) < q && (int)(e & 511u) <= c.y;
+            }
+            if (blocked) continue;
+            const unsigned key = ((unsigned)c.y << 12) | (unsigned)p;
+            if (key < b1) {
+                if (b1 != 0xFFFFFFFFu) b2 = min(b2, (int)(b1 >> 12));
+                b1 = key;
+            } else if (c.y < b2) {
+                b2 = c.y;
+            }
+        }
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            const unsigned ob1 = __shfl_xor(b1, o, 16);
+            const int ob2 = __shfl_xor(b2, o, 16);
+            const unsigned hi = max(b1, ob1);
+            int nb2 = min(b2, ob2);
+            if (hi != 0xFFFFFFFFu) nb2 = min(nb2, (int)(hi >> 12));
+            b1 = min(b1, ob1);
+            b2 = nb2;
+        }
+        if (b1 != 0xFFFFFFFFu) {
+            const int bestDist = (int)(b1 >> 12);
+            if (bestDist <= 50 && (float)bestDist < (float)b2 * P.nnratio) res = (list[b1 & 0xFFFu].x << 9) | bestDist;
+        }
+    }
+    if (active && sub == 0) {
+        P.out[k & 1][q] = res;
+        if (k > 0 && P.out[(k - 1) & 1][q] != res) atomicOr(&P.changed[k], 1);
+        if (res >= 0) {
+            const int t = res >> 9;
+            const int slot = atomicAdd(&cntW[t], 1);
+            if (slot < ORB_WIN_SLOTS) claimW[(size_t)t * ORB_WIN_SLOTS + slot] = ((unsigned)q << 9) | (unsigned)(res & 511);
+            else atomicOr(&P.changed[k], 2);
+        }
+    }
+}
+
+// One workgroup per settled pair: every accepted query is a match event (a
+// rotHist entry); a target stays with its last holder (vnMatches12 of the
+// others was reset by the steal); the rotation check drops the matches whose
+// event lies outside the three fullest bins (:667-690); vbPrevMatched takes
+// the kept matches' F2 positions (:693-695).
+__global__ __launch_bounds__(1024) void win_finish_kernel(const WinPair* __restrict__ pairs)
+{
+    __shared__ int owner[WIN_CAP];
+    __shared__ signed char bin[WIN_CAP];
+    __shared__ int hist[WIN_HISTO];
+    __shared__ int s_ind[3];
+    __shared__ int s_n[2];
+    const WinPair& P = pairs[blockIdx.x];
+    const int ks = win_settled_round(P.changed);
+    if (ks == 0) return;                           // the sequential pass takes this pair
+    const int* out = P.out[ks & 1];
+    const int n1 = P.n1, t = threadIdx.x;
+    for (int i = t; i < P.n2; i += 1024) owner[i] = -1;
+    if (t < WIN_HISTO) hist[t] = 0;
+    if (t < 2) s_n[t] = 0;
+    __syncthreads();
+    for (int q = t; q < n1; q += 1024) {
+        const int r = out[q];
+        if (r >= 0) atomicMax(&owner[r >> 9], q);
+    }
+    __syncthreads();
+    const float factor = WIN_HISTO / 360.0f;
+    int kept = 0;
+    for (int q = t; q < n1; q += 1024) {
+        const int r = out[q];
+        int m = -1, b = -1;
+        if (r >= 0) {
+            const int tg = r >> 9;
+            if (owner[tg] == q) {
+                m = tg;
+                kept++;
+            }
+            if (P.check_orientation) {
+                float rot = P.kp1[q].angle - P.kp2[tg].angle;
+                if (rot < 0.0f) rot += 360.0f;
+                b = (int)roundf(rot * factor);
+                if (b == WIN_HISTO) b = 0;
+                atomicAdd(&hist[b], 1);
+            }
+        }
+        bin[q] = (signed char)b;
+        P.m12[q] = m;
+    }
+    atomicAdd(&s_n[0], kept);
+    __syncthreads();
+    if (P.check_orientation) {
+        if (t == 0) three_maxima(hist, s_ind);
+        __syncthreads();
+        int dropped = 0;
+        for (int q = t; q < n1; q += 1024) {
+            const int b = bin[q];
+            if (b < 0 || b == s_ind[0] || b == s_ind[1] || b == s_ind[2]) continue;
+            if (P.m12[q] >= 0) {
+                P.m12[q] = -1;
+                dropped++;
+            }
+        }
+        atomicAdd(&s_n[1], dropped);
+        __syncthreads();
+    }
+    for (int q = t; q < n1; q += 1024) {
+        const int m = P.m12[q];
+        if (m >= 0) P.prev[q] = make_float2(P.kp2[m].x, P.kp2[m].y);
+    }
+    if (t == 0) *P.nmatch = s_n[0] - s_n[1];
+}
+
+extern "C" hipError_t orb_launch_win_rounds(const WinPair* d_pairs, int npairs, int max_n1, int k0, int k1,
+                                            hipStream_t st)
+{
+    if (npairs <= 0) return hipSuccess;
+    for (int k = k0; k < k1; k++)
+        win_round_kernel<<<dim3((max(max_n1, 1) + 15) / 16, npairs), 256, 0, st>>>(d_pairs, k);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t orb_launch_win_finish(const WinPair* d_pairs, int npairs, int sequential, hipStream_t st)
+{
+    if (npairs <= 0) return hipSuccess;
+    win_finish_kernel<<<npairs, 1024, 0, st>>>(d_pairs);
+    if (sequential) win_greedy_kernel<<<npairs, 64, 0, st>>>(d_pairs, 1);
     return hipGetLastError();
 }

@@ -109,3 +109,25 @@ def test_last_rejects_angles_outside_0_360(orb):
     with pytest.raises(orb.OrbError):
         orb.search_by_projection_last([f], sc, 15.0, False, True)
     assert orb.search_by_projection_last([f], sc, 15.0, False, False)[0][0] > 0   # no check: accepted
+
+
+@pytest.mark.parametrize("limit", [0, 1, 3])
+def test_in_order_pass_and_round_limits(orb, oracle, limit):
+    """With the round limit at 0 the in-order pass runs alone; with a few
+    rounds, frames that have not settled fall back to it.  Both equal the
+    oracle, for both overloads."""
+    ctx = orb.Context(0)
+    ctx.set_round_limit(limit)
+    scenes = [ps.random_frame(400 + i, n_kp=1500, n_pts=1500) for i in range(3)]
+    fs = [_frame_tuple(s)[0] for s in scenes]
+    got = orb.search_by_projection(fs, ps.SCALE, 3.0, 0.8, context=ctx)
+    for i, (f, g) in enumerate(zip(fs, got)):
+        _assert_same(g, _oracle_local(oracle, f, ps.SCALE, 3.0), f"local {i}")
+    assert ctx.last_rounds()[0] == limit
+    if limit == 0:
+        assert ctx.last_rounds()[1]
+    scenes = [ps.random_frame(500 + i, n_kp=1500, n_pts=1500, last=True) for i in range(3)]
+    fs = [_frame_tuple(s)[0] for s in scenes]
+    got = orb.search_by_projection_last(fs, ps.SCALE, 15.0, False, True, context=ctx)
+    for i, (f, g) in enumerate(zip(fs, got)):
+        _assert_same(g, _oracle_last(oracle, f, ps.SCALE, 15.0, False), f"last {i}")

@@ -58,3 +58,50 @@ def test_degenerate_pairs(orb, oracle):
         assert n == on == 0
         np.testing.assert_array_equal(m, om)
         np.testing.assert_array_equal(pv, opv)
+
+
+def _check_pairs(orb, oracle, pairs, ctx, window=100, check=True):
+    got = orb.search_for_initialization(pairs, window, 0.9, check, context=ctx)
+    for i, (p, (n, m, pv)) in enumerate(zip(pairs, got)):
+        on, om, opv = oracle.search_for_initialization(*p, window, 0.9, check)
+        assert n == on, (i, n, on)
+        np.testing.assert_array_equal(m, om, err_msg=f"pair {i}")
+        np.testing.assert_array_equal(pv.view(np.uint32), opv.view(np.uint32), err_msg=f"pair {i}")
+
+
+def test_rounds_and_in_order_pass_agree(orb, oracle):
+    """The parallel rounds settle on the in-order pass's result; with the
+    round limit at 0 the in-order pass runs alone, and with 2 the pairs that
+    have not settled fall back to it.  All three equal the oracle."""
+    ctx = orb.Context(0)
+    pairs = [_frames(oracle, 640, 480, 11, 3.0, (6.0, 4.0)), _frames(oracle, 1241, 376, 12, 8.0, (15.0, -5.0)),
+             _frames(oracle, 752, 480, 13, 1.0, (2.0, 2.0), 1000)]
+    _check_pairs(orb, oracle, pairs, ctx)
+    rounds, seq = ctx.last_rounds()
+    assert not seq and 2 <= rounds <= 48
+    for limit in (0, 2):
+        ctx.set_round_limit(limit)
+        _check_pairs(orb, oracle, pairs, ctx)
+        assert ctx.last_rounds() == (limit, True) or (limit == 2 and not ctx.last_rounds()[1])
+    ctx.set_round_limit(48)
+
+
+def test_contested_targets(orb, oracle):
+    """Many F1 keypoints over few F2 keypoints: one target is the best of many
+    queries (more claims than its slots in the first rounds) and is stolen
+    along long chains; the result is still the in-order pass's."""
+    ctx = orb.Context(0)
+    k1, d1, k2, d2, b, prev = _frames(oracle, 640, 480, 14, 2.0, (3.0, 3.0))
+    rng = np.random.default_rng(3)
+    keep = rng.choice(np.flatnonzero(k2["octave"] == 0), size=12, replace=False)
+    k2s, d2s = k2[keep], d2[keep]
+    reps = 60
+    k1c = np.repeat(k2s, reps)
+    k1c["x"] += rng.uniform(-20, 20, size=len(k1c)).astype(np.float32)
+    k1c["y"] += rng.uniform(-20, 20, size=len(k1c)).astype(np.float32)
+    d1c = np.repeat(d2s, reps, axis=0)
+    flip = rng.integers(0, 256, size=(len(d1c), 32), dtype=np.uint8) & (rng.random((len(d1c), 32)) < 0.08).astype(np.uint8) * 0x11
+    d1c = d1c ^ flip
+    p = (k1c, d1c, k2s, d2s, b, np.stack([k1c["x"], k1c["y"]], 1))
+    _check_pairs(orb, oracle, [p], ctx, window=100, check=True)
+    _check_pairs(orb, oracle, [p], ctx, window=100, check=False)
