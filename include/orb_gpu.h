@@ -364,6 +364,15 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
                                   const int32_t* off1, const orb_keypoint* kp2, const uint8_t* desc2,
                                   const int32_t* off2, const float* bounds, float* prev_matched, int window_size,
                                   float nn_ratio, int check_orientation, int32_t* matches12, int32_t* nmatches);
+/* Measurement aid, no reference counterpart: the call above, then `steps`
+ * more runs of its device work with the inputs resident in HBM (host reads
+ * of the candidate counts and change flags included, vbPrevMatched restored
+ * before each), their mean wall time in *ms.  The outputs are the call's. */
+int orb_search_for_initialization_time(orb_context* ctx, int n_pairs, const orb_keypoint* kp1, const uint8_t* desc1,
+                                       const int32_t* off1, const orb_keypoint* kp2, const uint8_t* desc2,
+                                       const int32_t* off2, const float* bounds, float* prev_matched, int window_size,
+                                       float nn_ratio, int check_orientation, int32_t* matches12, int32_t* nmatches,
+                                       int steps, float* ms);
 
 /* ORBmatcher::SearchByProjection, both per-frame overloads, for n_frames
  * frames at once.  Frame f's keypoints (mvKeysUn, at most 4096),
@@ -420,6 +429,23 @@ int orb_search_by_projection_last(orb_context* ctx, int n_frames, const orb_keyp
                                   int mono, const orb_last_point* points, const uint8_t* point_desc,
                                   const int32_t* point_off, float th, int check_orientation, int32_t* point_match,
                                   int32_t* key_point, int32_t* nmatches);
+/* Measurement aids, no reference counterpart: the calls above, then `steps`
+ * more runs of their device work with the inputs resident in HBM (host reads
+ * of the candidate counts and change flags included), their mean wall time in
+ * *ms.  The outputs are the call's. */
+int orb_search_by_projection_time(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                                  const float* uright, const uint8_t* occupied, const int32_t* key_off,
+                                  const float* bounds, const float* scale_factors, int nlevels,
+                                  const orb_proj_point* points, const uint8_t* point_desc, const int32_t* point_off,
+                                  float th, float nn_ratio, int32_t* point_match, int32_t* key_point,
+                                  int32_t* nmatches, int steps, float* ms);
+int orb_search_by_projection_last_time(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                                       const float* uright, const uint8_t* occupied, const int32_t* key_off,
+                                       const float* bounds, const float* scale_factors, int nlevels,
+                                       const float* camera, int mono, const orb_last_point* points,
+                                       const uint8_t* point_desc, const int32_t* point_off, float th,
+                                       int check_orientation, int32_t* point_match, int32_t* key_point,
+                                       int32_t* nmatches, int steps, float* ms);
 
 /* ---------------------------------------------------------------- map points
  * MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450) for

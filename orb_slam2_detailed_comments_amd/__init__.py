@@ -116,6 +116,12 @@ SIGNATURES = {
     "orb_search_by_projection": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _F, _F, _P, _P, _P]),
     "orb_search_by_projection_last": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _F, _I, _P,
                                            _P, _P]),
+    "orb_search_for_initialization_time": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _F, _I, _P, _P, _I,
+                                                _P]),
+    "orb_search_by_projection_time": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _F, _F, _P, _P, _P,
+                                           _I, _P]),
+    "orb_search_by_projection_last_time": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _F, _I,
+                                                _P, _P, _P, _I, _P]),
     "orb_distinctive_descriptors_time": (_I, [_P, _I, _P, _P, _I, _P]),
     "orb_extractor_debug_image": (_I, [_P, _I, _I, _P, _S]),
     "orb_extractor_debug_candidates": (_I, [_P, _I, _P, _I, _P]),
@@ -704,10 +710,12 @@ class Rectifier:
         return kps[:n.value].copy(), desc[:n.value].copy()
 
 
-def search_for_initialization(pairs, window=100, nnratio=0.9, check_orientation=True, context=None):
+def search_for_initialization(pairs, window=100, nnratio=0.9, check_orientation=True, context=None,
+                              timing_steps=0):
     """ORBmatcher::SearchForInitialization for a list of frame pairs
     (kp1, desc1, kp2, desc2, bounds=(minX, maxX, minY, maxY), prev (N1 x 2)).
-    Returns per pair (nmatches, matches12, updated prev)."""
+    Returns per pair (nmatches, matches12, updated prev); with timing_steps
+    > 0, (that list, mean ms of the device work with the inputs in HBM)."""
     ctx = context or Context.default()
     k1 = [np.ascontiguousarray(p[0], KEYPOINT_DTYPE) for p in pairs]
     d1 = [np.ascontiguousarray(p[1], np.uint8).reshape(-1, 32) for p in pairs]
@@ -724,11 +732,16 @@ def search_for_initialization(pairs, window=100, nnratio=0.9, check_orientation=
     prev = cat([np.asarray(p[5], np.float32).reshape(-1, 2) for p in pairs], np.float32, (1, 2)).copy()
     m12 = np.zeros(max(int(o1[-1]), 1), np.int32)
     nm = np.zeros(len(pairs), np.int32)
-    _check("orb_search_for_initialization",
-           lib().orb_search_for_initialization(ctx.h, len(pairs), _ptr(K1), _ptr(D1), _ptr(o1), _ptr(K2), _ptr(D2),
-                                               _ptr(o2), _ptr(bounds), _ptr(prev), int(window), float(nnratio),
-                                               int(bool(check_orientation)), _ptr(m12), _ptr(nm)))
-    return [(int(nm[i]), m12[o1[i]:o1[i + 1]].copy(), prev[o1[i]:o1[i + 1]].copy()) for i in range(len(pairs))]
+    args = (ctx.h, len(pairs), _ptr(K1), _ptr(D1), _ptr(o1), _ptr(K2), _ptr(D2), _ptr(o2), _ptr(bounds), _ptr(prev),
+            int(window), float(nnratio), int(bool(check_orientation)), _ptr(m12), _ptr(nm))
+    ms = ctypes.c_float()
+    if timing_steps > 0:
+        _check("orb_search_for_initialization_time",
+               lib().orb_search_for_initialization_time(*args, int(timing_steps), ctypes.byref(ms)))
+    else:
+        _check("orb_search_for_initialization", lib().orb_search_for_initialization(*args))
+    res = [(int(nm[i]), m12[o1[i]:o1[i + 1]].copy(), prev[o1[i]:o1[i + 1]].copy()) for i in range(len(pairs))]
+    return (res, ms.value) if timing_steps > 0 else res
 
 
 PROJ_POINT_DTYPE = np.dtype([("proj_x", "<f4"), ("proj_y", "<f4"), ("proj_xr", "<f4"), ("view_cos", "<f4"),
@@ -767,37 +780,49 @@ def _projection_results(a, n):
             for i in range(n)]
 
 
-def search_by_projection(frames, scale_factors, th=3.0, nn_ratio=0.8, context=None):
+def search_by_projection(frames, scale_factors, th=3.0, nn_ratio=0.8, context=None, timing_steps=0):
     """ORBmatcher::SearchByProjection(F, vpMapPoints, th) for a list of frames
     (kp, desc, uright, occupied, bounds, points PROJ_POINT_DTYPE, point_desc).
-    Returns per frame (nmatches, point_match, key_point)."""
+    Returns per frame (nmatches, point_match, key_point); with timing_steps
+    > 0, (that list, mean ms of the device work with the inputs in HBM)."""
     ctx = context or Context.default()
     a = _projection_frames(frames, PROJ_POINT_DTYPE)
     sc = np.ascontiguousarray(scale_factors, np.float32)
-    _check("orb_search_by_projection",
-           lib().orb_search_by_projection(ctx.h, len(frames), _ptr(a["kp"]), _ptr(a["desc"]), _ptr(a["uright"]),
-                                          _ptr(a["occupied"]), _ptr(a["ko"]), _ptr(a["bounds"]), _ptr(sc), len(sc),
-                                          _ptr(a["pts"]), _ptr(a["pdesc"]), _ptr(a["po"]), float(th),
-                                          float(nn_ratio), _ptr(a["pm"]), _ptr(a["kpm"]), _ptr(a["nm"])))
-    return _projection_results(a, len(frames))
+    args = (ctx.h, len(frames), _ptr(a["kp"]), _ptr(a["desc"]), _ptr(a["uright"]), _ptr(a["occupied"]), _ptr(a["ko"]),
+            _ptr(a["bounds"]), _ptr(sc), len(sc), _ptr(a["pts"]), _ptr(a["pdesc"]), _ptr(a["po"]), float(th),
+            float(nn_ratio), _ptr(a["pm"]), _ptr(a["kpm"]), _ptr(a["nm"]))
+    ms = ctypes.c_float()
+    if timing_steps > 0:
+        _check("orb_search_by_projection_time",
+               lib().orb_search_by_projection_time(*args, int(timing_steps), ctypes.byref(ms)))
+    else:
+        _check("orb_search_by_projection", lib().orb_search_by_projection(*args))
+    res = _projection_results(a, len(frames))
+    return (res, ms.value) if timing_steps > 0 else res
 
 
-def search_by_projection_last(frames, scale_factors, th, mono=False, check_orientation=True, context=None):
+def search_by_projection_last(frames, scale_factors, th, mono=False, check_orientation=True, context=None,
+                              timing_steps=0):
     """ORBmatcher::SearchByProjection(Current, Last, th, bMono) for a list of
     frames (kp, desc, uright, occupied, bounds, points LAST_POINT_DTYPE,
     point_desc, camera = (fx, fy, cx, cy, mbf, mb, tlc_z)).  Returns per
-    frame (nmatches, point_match, key_point)."""
+    frame (nmatches, point_match, key_point); with timing_steps > 0, (that
+    list, mean ms of the device work with the inputs in HBM)."""
     ctx = context or Context.default()
     a = _projection_frames(frames, LAST_POINT_DTYPE)
     sc = np.ascontiguousarray(scale_factors, np.float32)
     cam = np.ascontiguousarray([f[7] for f in frames], np.float32).reshape(-1)
-    _check("orb_search_by_projection_last",
-           lib().orb_search_by_projection_last(ctx.h, len(frames), _ptr(a["kp"]), _ptr(a["desc"]), _ptr(a["uright"]),
-                                               _ptr(a["occupied"]), _ptr(a["ko"]), _ptr(a["bounds"]), _ptr(sc),
-                                               len(sc), _ptr(cam), int(bool(mono)), _ptr(a["pts"]), _ptr(a["pdesc"]),
-                                               _ptr(a["po"]), float(th), int(bool(check_orientation)),
-                                               _ptr(a["pm"]), _ptr(a["kpm"]), _ptr(a["nm"])))
-    return _projection_results(a, len(frames))
+    args = (ctx.h, len(frames), _ptr(a["kp"]), _ptr(a["desc"]), _ptr(a["uright"]), _ptr(a["occupied"]), _ptr(a["ko"]),
+            _ptr(a["bounds"]), _ptr(sc), len(sc), _ptr(cam), int(bool(mono)), _ptr(a["pts"]), _ptr(a["pdesc"]),
+            _ptr(a["po"]), float(th), int(bool(check_orientation)), _ptr(a["pm"]), _ptr(a["kpm"]), _ptr(a["nm"]))
+    ms = ctypes.c_float()
+    if timing_steps > 0:
+        _check("orb_search_by_projection_last_time",
+               lib().orb_search_by_projection_last_time(*args, int(timing_steps), ctypes.byref(ms)))
+    else:
+        _check("orb_search_by_projection_last", lib().orb_search_by_projection_last(*args))
+    res = _projection_results(a, len(frames))
+    return (res, ms.value) if timing_steps > 0 else res
 
 
 def distinctive_descriptors(points, context=None):

@@ -3,6 +3,7 @@
 // orb_winmatch.hip).  Two launches with one host read between them: the
 // candidate counts size the (i2, distance) lists exactly.  Device buffers
 // are the context's, grown on demand and kept between calls.
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -23,12 +24,15 @@ size_t up(size_t v) { return (v + 255) / 256 * 256; }
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypoint* kp1, const uint8_t* desc1,
-                                  const int32_t* off1, const orb_keypoint* kp2, const uint8_t* desc2,
-                                  const int32_t* off2, const float* bounds, float* prev_matched, int window_size,
-                                  float nn_ratio, int check_orientation, int32_t* matches12, int32_t* nmatches)
+// steps > 0: after the call's own run, `steps` more runs of the device work
+// with the inputs resident in HBM (vbPrevMatched restored before each), their
+// mean wall time in *ms; the results returned are the call's.
+int sfi_impl(orb_context* ctx, int n_pairs, const orb_keypoint* kp1, const uint8_t* desc1, const int32_t* off1,
+             const orb_keypoint* kp2, const uint8_t* desc2, const int32_t* off2, const float* bounds,
+             float* prev_matched, int window_size, float nn_ratio, int check_orientation, int32_t* matches12,
+             int32_t* nmatches, int steps, float* ms)
 {
     if (!ctx || n_pairs < 0 || (n_pairs && (!off1 || !off2 || !bounds || !matches12 || !nmatches))) return ORB_E_ARG;
     if (n_pairs == 0) return ORB_OK;
@@ -55,8 +59,8 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
     // round state: outcomes by parity, claims and claim counts by round mod 3, change flags
     const size_t s_out = up((size_t)N1 * 4), s_claim = up((size_t)N2 * ORB_WIN_SLOTS * 4), s_ccnt = up((size_t)N2 * 4);
     const size_t s_chg = up((size_t)n_pairs * ORB_WIN_MAXIT * 4);
-    const size_t total = s_kp1 + s_d1 + s_kp2 + s_d2 + s_prev + s_sorted + s_cells + s_cnt + s_off + s_m12 + s_nm +
-                         s_pairs + s_tot + 2 * s_out + 3 * s_claim + 3 * s_ccnt + s_chg;
+    const size_t total = s_kp1 + s_d1 + s_kp2 + s_d2 + 2 * s_prev + s_sorted + s_cells + s_cnt + s_off + s_m12 +
+                         s_nm + s_pairs + s_tot + 2 * s_out + 3 * s_claim + 3 * s_ccnt + s_chg;
     int rc = ctx->grow(&ctx->d_win, &ctx->win_cap, total);     // kept by the context between calls
     if (rc) return rc;
     char* m = (char*)ctx->d_win;
@@ -83,6 +87,7 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
     unsigned* d_claim[3] = {(unsigned*)take(s_claim), (unsigned*)take(s_claim), (unsigned*)take(s_claim)};
     int* d_ccnt[3] = {(int*)take(s_ccnt), (int*)take(s_ccnt), (int*)take(s_ccnt)};
     int* d_chg = (int*)take(s_chg);
+    float2* d_prev0 = (float2*)take(s_prev);       // vbPrevMatched as passed, for timed repeats
     if (N1) {
         HIP_TRY(hipMemcpyAsync(d_kp1, kp1, (size_t)N1 * sizeof(OrbKeyPoint), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(d_d1, desc1, (size_t)N1 * 32, hipMemcpyHostToDevice, st));
@@ -123,6 +128,8 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
         }
         w.changed = d_chg + (size_t)p * ORB_WIN_MAXIT;
     }
+    if (steps > 0 && N1) HIP_TRY(hipMemcpyAsync(d_prev0, d_prev, (size_t)N1 * 8, hipMemcpyDeviceToDevice, st));
+    auto run_device = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(d_pairs, pairs.data(), n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st));
     HIP_TRY(orb_launch_win_count(d_pairs, n_pairs, max_n1, st));
     HIP_TRY(orb_launch_win_scan(d_pairs, n_pairs, d_tot, st));
@@ -135,8 +142,8 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
         base[p] = ncand;
         ncand += (size_t)tot[p];
     }
-    rc = ctx->grow(&ctx->d_wcand, &ctx->wcand_cap, std::max<size_t>(ncand, 1) * sizeof(int2));
-    if (rc) return rc;
+    const int rg = ctx->grow(&ctx->d_wcand, &ctx->wcand_cap, std::max<size_t>(ncand, 1) * sizeof(int2));
+    if (rg) return rg;
     for (int p = 0; p < n_pairs; p++) pairs[p].cand = (int2*)ctx->d_wcand + base[p];
     HIP_TRY(hipMemcpyAsync(d_pairs, pairs.data(), n_pairs * sizeof(WinPair), hipMemcpyHostToDevice, st));
     HIP_TRY(orb_launch_win_fill(d_pairs, n_pairs, max_n1, st));
@@ -164,6 +171,21 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
     ctx->last_rounds = k;
     ctx->last_sequential = settled ? 0 : 1;
     HIP_TRY(orb_launch_win_finish(d_pairs, n_pairs, settled ? 0 : 1, st));
+    return ORB_OK;
+    };
+    rc = run_device();
+    if (rc) return rc;
+    if (steps > 0) {
+        HIP_TRY(hipStreamSynchronize(st));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < steps; i++) {
+            if (N1) HIP_TRY(hipMemcpyAsync(d_prev, d_prev0, (size_t)N1 * 8, hipMemcpyDeviceToDevice, st));
+            rc = run_device();
+            if (rc) return rc;
+        }
+        HIP_TRY(hipStreamSynchronize(st));
+        *ms = (float)(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps);
+    }
     if (N1) {
         HIP_TRY(hipMemcpyAsync(matches12, d_m12, (size_t)N1 * 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipMemcpyAsync(prev_matched, d_prev, (size_t)N1 * 8, hipMemcpyDeviceToHost, st));
@@ -171,6 +193,30 @@ int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypo
     HIP_TRY(hipMemcpyAsync(nmatches, d_nm, n_pairs * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return ORB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orb_search_for_initialization(orb_context* ctx, int n_pairs, const orb_keypoint* kp1, const uint8_t* desc1,
+                                  const int32_t* off1, const orb_keypoint* kp2, const uint8_t* desc2,
+                                  const int32_t* off2, const float* bounds, float* prev_matched, int window_size,
+                                  float nn_ratio, int check_orientation, int32_t* matches12, int32_t* nmatches)
+{
+    return sfi_impl(ctx, n_pairs, kp1, desc1, off1, kp2, desc2, off2, bounds, prev_matched, window_size, nn_ratio,
+                    check_orientation, matches12, nmatches, 0, nullptr);
+}
+
+int orb_search_for_initialization_time(orb_context* ctx, int n_pairs, const orb_keypoint* kp1, const uint8_t* desc1,
+                                       const int32_t* off1, const orb_keypoint* kp2, const uint8_t* desc2,
+                                       const int32_t* off2, const float* bounds, float* prev_matched, int window_size,
+                                       float nn_ratio, int check_orientation, int32_t* matches12, int32_t* nmatches,
+                                       int steps, float* ms)
+{
+    if (steps <= 0 || !ms) return ORB_E_ARG;
+    return sfi_impl(ctx, n_pairs, kp1, desc1, off1, kp2, desc2, off2, bounds, prev_matched, window_size, nn_ratio,
+                    check_orientation, matches12, nmatches, steps, ms);
 }
 
 }  // extern "C"
@@ -197,7 +243,7 @@ int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp
                 const float* uright, const uint8_t* occupied, const int32_t* key_off, const float* bounds,
                 const float* scale, int nlevels, const void* pts, const uint8_t* pdesc, const int32_t* pt_off,
                 const float* cam, int mono, float th, float nnratio, int check_orientation, int32_t* pt_match,
-                int32_t* key_point, int32_t* nmatches, int* rounds_used)
+                int32_t* key_point, int32_t* nmatches, int steps, float* ms)
 {
     static_assert(sizeof(orb_proj_point) == 24 && sizeof(orb_last_point) == 24, "point records");
     if (!ctx || n_frames < 0 || nlevels < 1 || nlevels > ORB_MAX_LEVELS ||
@@ -337,6 +383,7 @@ int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp
         P.key_point = d_keyp + k0;
         P.nmatch = d_nm + f;
     }
+    auto run_device = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(d_grid, grid.data(), (size_t)F * sizeof(WinPair), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(d_frames, frames.data(), (size_t)F * sizeof(ProjFrame), hipMemcpyHostToDevice, st));
     HIP_TRY(orb_launch_proj_prep(d_frames, F, max_nq, rule, d_pts, d_ptoff, d_scale, nlevels, d_cam, mono, th, st));
@@ -346,8 +393,8 @@ int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp
     HIP_TRY(hipStreamSynchronize(st));
     size_t ncand = 0;
     for (int f = 0; f < F; f++) ncand += (size_t)tot[f];
-    rc = ctx->grow(&ctx->d_wcand, &ctx->wcand_cap, std::max<size_t>(ncand, 1) * sizeof(unsigned));
-    if (rc) return rc;
+    const int rg = ctx->grow(&ctx->d_wcand, &ctx->wcand_cap, std::max<size_t>(ncand, 1) * sizeof(unsigned));
+    if (rg) return rg;
     size_t base = 0;
     for (int f = 0; f < F; f++) {
         frames[f].cand = (unsigned*)ctx->d_wcand + base;
@@ -373,10 +420,23 @@ int proj_search(orb_context* ctx, int rule, int n_frames, const orb_keypoint* kp
             settled = s;
         }
     }
-    if (rounds_used) *rounds_used = k;
     ctx->last_rounds = k;
     ctx->last_sequential = settled ? 0 : 1;
     HIP_TRY(orb_launch_proj_finish(d_frames, F, settled ? 0 : 1, st));
+    return ORB_OK;
+    };
+    rc = run_device();
+    if (rc) return rc;
+    if (steps > 0) {        // repeats with the inputs resident in HBM (they are not modified)
+        HIP_TRY(hipStreamSynchronize(st));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < steps; i++) {
+            rc = run_device();
+            if (rc) return rc;
+        }
+        HIP_TRY(hipStreamSynchronize(st));
+        *ms = (float)(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps);
+    }
     if (Q) HIP_TRY(hipMemcpyAsync(pt_match, d_ptm, (size_t)Q * 4, hipMemcpyDeviceToHost, st));
     if (N) HIP_TRY(hipMemcpyAsync(key_point, d_keyp, (size_t)N * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(nmatches, d_nm, (size_t)F * 4, hipMemcpyDeviceToHost, st));
@@ -409,7 +469,21 @@ int orb_search_by_projection(orb_context* ctx, int n_frames, const orb_keypoint*
                              float th, float nn_ratio, int32_t* point_match, int32_t* key_point, int32_t* nmatches)
 {
     return proj_search(ctx, 1, n_frames, kp, desc, uright, occupied, key_off, bounds, scale_factors, nlevels, points,
-                       point_desc, point_off, nullptr, 0, th, nn_ratio, 0, point_match, key_point, nmatches, nullptr);
+                       point_desc, point_off, nullptr, 0, th, nn_ratio, 0, point_match, key_point, nmatches, 0,
+                       nullptr);
+}
+
+int orb_search_by_projection_time(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                                  const float* uright, const uint8_t* occupied, const int32_t* key_off,
+                                  const float* bounds, const float* scale_factors, int nlevels,
+                                  const orb_proj_point* points, const uint8_t* point_desc, const int32_t* point_off,
+                                  float th, float nn_ratio, int32_t* point_match, int32_t* key_point,
+                                  int32_t* nmatches, int steps, float* ms)
+{
+    if (steps <= 0 || !ms) return ORB_E_ARG;
+    return proj_search(ctx, 1, n_frames, kp, desc, uright, occupied, key_off, bounds, scale_factors, nlevels, points,
+                       point_desc, point_off, nullptr, 0, th, nn_ratio, 0, point_match, key_point, nmatches, steps,
+                       ms);
 }
 
 int orb_search_by_projection_last(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
@@ -421,7 +495,21 @@ int orb_search_by_projection_last(orb_context* ctx, int n_frames, const orb_keyp
 {
     return proj_search(ctx, 2, n_frames, kp, desc, uright, occupied, key_off, bounds, scale_factors, nlevels, points,
                        point_desc, point_off, camera, mono, th, 0.f, check_orientation, point_match, key_point,
-                       nmatches, nullptr);
+                       nmatches, 0, nullptr);
+}
+
+int orb_search_by_projection_last_time(orb_context* ctx, int n_frames, const orb_keypoint* kp, const uint8_t* desc,
+                                       const float* uright, const uint8_t* occupied, const int32_t* key_off,
+                                       const float* bounds, const float* scale_factors, int nlevels,
+                                       const float* camera, int mono, const orb_last_point* points,
+                                       const uint8_t* point_desc, const int32_t* point_off, float th,
+                                       int check_orientation, int32_t* point_match, int32_t* key_point,
+                                       int32_t* nmatches, int steps, float* ms)
+{
+    if (steps <= 0 || !ms) return ORB_E_ARG;
+    return proj_search(ctx, 2, n_frames, kp, desc, uright, occupied, key_off, bounds, scale_factors, nlevels, points,
+                       point_desc, point_off, camera, mono, th, 0.f, check_orientation, point_match, key_point,
+                       nmatches, steps, ms);
 }
 
 }  // extern "C"

@@ -131,3 +131,18 @@ def test_in_order_pass_and_round_limits(orb, oracle, limit):
     got = orb.search_by_projection_last(fs, ps.SCALE, 15.0, False, True, context=ctx)
     for i, (f, g) in enumerate(zip(fs, got)):
         _assert_same(g, _oracle_last(oracle, f, ps.SCALE, 15.0, False), f"last {i}")
+
+
+def test_timed_forms_return_the_calls_results(orb, oracle):
+    scenes = [ps.random_frame(600 + i, n_kp=1200, n_pts=1000) for i in range(2)]
+    fs = [_frame_tuple(s)[0] for s in scenes]
+    got, ms = orb.search_by_projection(fs, ps.SCALE, 3.0, 0.8, timing_steps=3)
+    assert ms > 0
+    for f, g in zip(fs, got):
+        _assert_same(g, _oracle_local(oracle, f, ps.SCALE, 3.0), "timed local")
+    scenes = [ps.random_frame(700 + i, n_kp=1200, n_pts=1000, last=True) for i in range(2)]
+    fs = [_frame_tuple(s)[0] for s in scenes]
+    got, ms = orb.search_by_projection_last(fs, ps.SCALE, 15.0, False, True, timing_steps=3)
+    assert ms > 0
+    for f, g in zip(fs, got):
+        _assert_same(g, _oracle_last(oracle, f, ps.SCALE, 15.0, False), "timed last")

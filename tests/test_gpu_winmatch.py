@@ -105,3 +105,16 @@ def test_contested_targets(orb, oracle):
     p = (k1c, d1c, k2s, d2s, b, np.stack([k1c["x"], k1c["y"]], 1))
     _check_pairs(orb, oracle, [p], ctx, window=100, check=True)
     _check_pairs(orb, oracle, [p], ctx, window=100, check=False)
+
+
+def test_timed_form_returns_the_calls_results(orb, oracle):
+    """The measurement aid repeats the device work with vbPrevMatched restored
+    each time, so its outputs are exactly one call's."""
+    pairs = [_frames(oracle, 640, 480, 15, 3.0, (6.0, 4.0)), _frames(oracle, 640, 480, 16, 1.0, (2.0, 3.0))]
+    want = orb.search_for_initialization(pairs)
+    got, ms = orb.search_for_initialization(pairs, timing_steps=3)
+    assert ms > 0
+    for (n, m, pv), (wn, wm, wpv) in zip(got, want):
+        assert n == wn
+        np.testing.assert_array_equal(m, wm)
+        np.testing.assert_array_equal(pv.view(np.uint32), wpv.view(np.uint32))
