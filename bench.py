@@ -183,11 +183,22 @@ def winmatch_extra(orb, ctx, cpu_seconds):
     reps, t0 = 5, time.perf_counter()
     for _ in range(reps):
         res = orb.search_for_initialization(pairs, context=ctx)
-    ms = (time.perf_counter() - t0) * 1e3 / reps
+    py_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # the C ABI call on flat host arrays, as a C++ caller makes it (the
+    # arrays built once; vbPrevMatched restored between calls, untimed)
+    a = orb._sfi_arrays(pairs)
+    prev0, ms = a["prev"].copy(), 0.0
+    for _ in range(reps):
+        a["prev"][:] = prev0
+        t0 = time.perf_counter()
+        orb._sfi_call(ctx, a, 100, 0.9, True)
+        ms += (time.perf_counter() - t0) * 1e3 / reps
     _, dev_ms = orb.search_for_initialization(pairs, context=ctx, timing_steps=10)
     out = {"workload": "SearchForInitialization over the 64x48 grid, 32 pairs of 640x480 (2000 features each), "
-                       "window 100, ratio 0.9, orientation check; host arrays in and out",
+                       "window 100, ratio 0.9, orientation check; ms_per_call: the C ABI call on flat host arrays in "
+                       "and out; python_ms_per_call adds the wrapper's concatenation of 32 pairs",
            "ms_per_call": round(ms, 3), "pairs_per_s": round(32 / (ms * 1e-3), 1),
+           "python_ms_per_call": round(py_ms, 3),
            "device_resident": {"ms_per_call": round(dev_ms, 3), "pairs_per_s": round(32 / (dev_ms * 1e-3), 1),
                                "includes": "the call's device work with its inputs already in HBM: grid, candidate "
                                            "lists, the parallel rounds and their flag reads, the finish"},
@@ -211,7 +222,8 @@ def projection_extra(orb, ctx, cpu_seconds):
     from orb_slam2_detailed_comments_amd import synth
     scale = np.float32(1.2) ** np.arange(8, dtype=np.float32)
     out = {"workload": "SearchByProjection, 32 frames x (2000 keypoints, 1800 points) per call; local map (th 3, "
-                       "ratio 0.8) and last frame (th 15, rotation check); host arrays in and out"}
+                       "ratio 0.8) and last frame (th 15, rotation check); ms_per_call: the C ABI call on flat host "
+                       "arrays in and out; python_ms_per_call adds the wrapper's concatenation of 32 frames"}
     for name, last in (("local_map", False), ("last_frame", True)):
         frames = [synth.projection_frame(500 + i, last=last) for i in range(32)]
         fn = (lambda **kw: orb.search_by_projection_last(frames, scale, 15.0, False, True, context=ctx, **kw)) \
@@ -220,9 +232,24 @@ def projection_extra(orb, ctx, cpu_seconds):
         reps, t0 = 5, time.perf_counter()
         for _ in range(reps):
             res = fn()
+        py_ms = (time.perf_counter() - t0) * 1e3 / reps
+        # the C ABI call on flat host arrays, as a C++ caller makes it
+        a = orb._projection_frames(frames, orb.LAST_POINT_DTYPE if last else orb.PROJ_POINT_DTYPE)
+        cam = np.ascontiguousarray([f[7] for f in frames], np.float32).reshape(-1)
+        P = orb._ptr
+        common = (ctx.h, 32, P(a["kp"]), P(a["desc"]), P(a["uright"]), P(a["occupied"]), P(a["ko"]), P(a["bounds"]),
+                  P(scale), len(scale))
+        call = (lambda: orb.lib().orb_search_by_projection_last(
+                    *common, P(cam), 0, P(a["pts"]), P(a["pdesc"]), P(a["po"]), 15.0, 1, P(a["pm"]), P(a["kpm"]),
+                    P(a["nm"]))) if last else \
+               (lambda: orb.lib().orb_search_by_projection(
+                    *common, P(a["pts"]), P(a["pdesc"]), P(a["po"]), 3.0, 0.8, P(a["pm"]), P(a["kpm"]), P(a["nm"])))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert call() == 0
         ms = (time.perf_counter() - t0) * 1e3 / reps
         _, dev_ms = fn(timing_steps=10)
-        r = {"ms_per_call": round(ms, 3), "frames_per_s": round(32 / (ms * 1e-3), 1),
+        r = {"ms_per_call": round(ms, 3), "frames_per_s": round(32 / (ms * 1e-3), 1), "python_ms_per_call": round(py_ms, 3),
              "device_resident": {"ms_per_call": round(dev_ms, 3), "frames_per_s": round(32 / (dev_ms * 1e-3), 1)},
              "mean_matches": round(float(np.mean([x[0] for x in res])), 1)}
         if cpu_seconds > 0:

@@ -710,6 +710,41 @@ class Rectifier:
         return kps[:n.value].copy(), desc[:n.value].copy()
 
 
+def _sfi_arrays(pairs):
+    """Concatenates per-pair (kp1, desc1, kp2, desc2, bounds, prev) tuples
+    into the flat arrays of the C ABI."""
+    k1 = [np.ascontiguousarray(p[0], KEYPOINT_DTYPE) for p in pairs]
+    k2 = [np.ascontiguousarray(p[2], KEYPOINT_DTYPE) for p in pairs]
+    o1 = np.zeros(len(pairs) + 1, np.int32)
+    o1[1:] = np.cumsum([len(k) for k in k1])
+    o2 = np.zeros(len(pairs) + 1, np.int32)
+    o2[1:] = np.cumsum([len(k) for k in k2])
+    cat = lambda xs, dt, shape: np.concatenate(xs) if sum(len(x) for x in xs) else np.zeros(shape, dt)
+    return dict(K1=cat(k1, KEYPOINT_DTYPE, 1),
+                D1=cat([np.ascontiguousarray(p[1], np.uint8).reshape(-1, 32) for p in pairs], np.uint8, (1, 32)),
+                K2=cat(k2, KEYPOINT_DTYPE, 1),
+                D2=cat([np.ascontiguousarray(p[3], np.uint8).reshape(-1, 32) for p in pairs], np.uint8, (1, 32)),
+                o1=o1, o2=o2, bounds=np.ascontiguousarray([p[4] for p in pairs], np.float32).reshape(-1),
+                prev=cat([np.asarray(p[5], np.float32).reshape(-1, 2) for p in pairs], np.float32, (1, 2)).copy(),
+                m12=np.zeros(max(int(o1[-1]), 1), np.int32), nm=np.zeros(len(pairs), np.int32))
+
+
+def _sfi_call(ctx, a, window, nnratio, check_orientation, timing_steps=0):
+    """orb_search_for_initialization on flat arrays (a from _sfi_arrays;
+    a["prev"], a["m12"], a["nm"] receive the outputs).  Returns the timed
+    form's ms, or None."""
+    args = (ctx.h, len(a["nm"]), _ptr(a["K1"]), _ptr(a["D1"]), _ptr(a["o1"]), _ptr(a["K2"]), _ptr(a["D2"]),
+            _ptr(a["o2"]), _ptr(a["bounds"]), _ptr(a["prev"]), int(window), float(nnratio),
+            int(bool(check_orientation)), _ptr(a["m12"]), _ptr(a["nm"]))
+    if timing_steps > 0:
+        ms = ctypes.c_float()
+        _check("orb_search_for_initialization_time",
+               lib().orb_search_for_initialization_time(*args, int(timing_steps), ctypes.byref(ms)))
+        return ms.value
+    _check("orb_search_for_initialization", lib().orb_search_for_initialization(*args))
+    return None
+
+
 def search_for_initialization(pairs, window=100, nnratio=0.9, check_orientation=True, context=None,
                               timing_steps=0):
     """ORBmatcher::SearchForInitialization for a list of frame pairs
@@ -717,31 +752,11 @@ def search_for_initialization(pairs, window=100, nnratio=0.9, check_orientation=
     Returns per pair (nmatches, matches12, updated prev); with timing_steps
     > 0, (that list, mean ms of the device work with the inputs in HBM)."""
     ctx = context or Context.default()
-    k1 = [np.ascontiguousarray(p[0], KEYPOINT_DTYPE) for p in pairs]
-    d1 = [np.ascontiguousarray(p[1], np.uint8).reshape(-1, 32) for p in pairs]
-    k2 = [np.ascontiguousarray(p[2], KEYPOINT_DTYPE) for p in pairs]
-    d2 = [np.ascontiguousarray(p[3], np.uint8).reshape(-1, 32) for p in pairs]
-    o1 = np.zeros(len(pairs) + 1, np.int32)
-    o1[1:] = np.cumsum([len(k) for k in k1])
-    o2 = np.zeros(len(pairs) + 1, np.int32)
-    o2[1:] = np.cumsum([len(k) for k in k2])
-    cat = lambda xs, dt, shape: np.concatenate(xs) if sum(len(x) for x in xs) else np.zeros(shape, dt)
-    K1, D1 = cat(k1, KEYPOINT_DTYPE, 1), cat(d1, np.uint8, (1, 32))
-    K2, D2 = cat(k2, KEYPOINT_DTYPE, 1), cat(d2, np.uint8, (1, 32))
-    bounds = np.ascontiguousarray([p[4] for p in pairs], np.float32).reshape(-1)
-    prev = cat([np.asarray(p[5], np.float32).reshape(-1, 2) for p in pairs], np.float32, (1, 2)).copy()
-    m12 = np.zeros(max(int(o1[-1]), 1), np.int32)
-    nm = np.zeros(len(pairs), np.int32)
-    args = (ctx.h, len(pairs), _ptr(K1), _ptr(D1), _ptr(o1), _ptr(K2), _ptr(D2), _ptr(o2), _ptr(bounds), _ptr(prev),
-            int(window), float(nnratio), int(bool(check_orientation)), _ptr(m12), _ptr(nm))
-    ms = ctypes.c_float()
-    if timing_steps > 0:
-        _check("orb_search_for_initialization_time",
-               lib().orb_search_for_initialization_time(*args, int(timing_steps), ctypes.byref(ms)))
-    else:
-        _check("orb_search_for_initialization", lib().orb_search_for_initialization(*args))
+    a = _sfi_arrays(pairs)
+    ms = _sfi_call(ctx, a, window, nnratio, check_orientation, timing_steps)
+    o1, m12, prev, nm = a["o1"], a["m12"], a["prev"], a["nm"]
     res = [(int(nm[i]), m12[o1[i]:o1[i + 1]].copy(), prev[o1[i]:o1[i + 1]].copy()) for i in range(len(pairs))]
-    return (res, ms.value) if timing_steps > 0 else res
+    return (res, ms) if timing_steps > 0 else res
 
 
 PROJ_POINT_DTYPE = np.dtype([("proj_x", "<f4"), ("proj_y", "<f4"), ("proj_xr", "<f4"), ("view_cos", "<f4"),
