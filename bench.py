@@ -443,7 +443,20 @@ def cpu_baseline(frames, seconds, gpu_sample=None):
             "sample": f"{min(len(frames), 8)} synthetic 1241x376 stereo frames cycled for ~{seconds:.0f}s: "
                       f"{npipe} independent Frame pipelines (L/R extraction on 2 threads + stereo match) on "
                       f"{cores} host cores",
-            "reference_structure_frames_per_s": round(single, 2), "host_cores": cores}, parity
+            "reference_structure_frames_per_s": round(single, 2), "host_cores": cores,
+            "cpu_model": cpu_model(), "build": "oracle/Makefile: g++ -O3 -march=x86-64-v3 -ffp-contract=off"}, parity
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
