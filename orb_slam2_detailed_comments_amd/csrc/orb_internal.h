@@ -318,7 +318,10 @@ struct orb_extractor {
             if (L.nCols <= 0 || L.nRows <= 0 || L.nCols > 255 || L.nRows > 255) return ORB_E_ARG;
             L.wCell = (int)std::ceil(width / L.nCols);
             L.hCell = (int)std::ceil(height / L.nRows);
-            if (L.wCell > 58 || L.hCell > 58) return ORB_E_ARG;
+            // A keypoint's offset from its cell's origin is < cell + 3 and is
+            // packed in 6 bits; with nCols = (int)(width / 30) a cell is at
+            // most 59 px (an integer width or height in [30, 60), one cell).
+            if (L.wCell > 60 || L.hCell > 60) return ORB_E_ARG;
             L.ncells = L.nCols * L.nRows;
             L.nIni = (int)std::round((float)(L.maxBorderX - minBorderX) / (L.maxBorderY - minBorderY));
             if (L.nIni <= 0 || L.nIni > ORB_MAX_ROOTS) return ORB_E_ARG;
@@ -417,7 +420,7 @@ struct orb_extractor {
             L.nRows = P.nRows;
             L.wCell = P.wCell;
             L.hCell = P.hCell;
-            // exact for x * cell < 2^32 (x < 2^16, cell <= 58)
+            // exact for x * cell < 2^32 (x < 2^16, cell <= 60)
             L.wrcp = (unsigned)(((1ull << 32) + (unsigned)P.wCell - 1) / (unsigned)P.wCell);
             L.hrcp = (unsigned)(((1ull << 32) + (unsigned)P.hCell - 1) / (unsigned)P.hCell);
             L.maxBorderX = P.maxBorderX;
