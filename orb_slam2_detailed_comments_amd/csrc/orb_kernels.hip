@@ -2185,8 +2185,13 @@ extern "C" hipError_t orb_launch_match_bf(const uint8_t* q, const int* nq, const
 #define KNN_CHUNK 8192
 #define KNN_NONE 0xFFFFFFFFu
 
-// Median of three (the compiler emits v_med3_u32).
+// Median of three (the compiler emits v_med3_u32 / v_med3_i32).
 __device__ __forceinline__ unsigned med3u(unsigned a, unsigned b, unsigned c)
+{
+    return max(min(a, b), min(max(a, b), c));
+}
+
+__device__ __forceinline__ int med3i(int a, int b, int c)
 {
     return max(min(a, b), min(max(a, b), c));
 }
@@ -2234,37 +2239,87 @@ __global__ __launch_bounds__(256) void knn_popc_kernel(const uint8_t* __restrict
     if (qi < nq) part[(size_t)qi * nchunks + chunk] = make_uint2(k0, k1);
 }
 
-// Bit -> int8 +-1 expansion shared by both MFMA operands.  Flat dword F
-// (0..63) of a descriptor holds, in bytes 0..3, bits {i, i+8, i+16, i+24} of
-// 32-bit word u = F/8, with i = F%8: one shift, one mask and one v_perm.
-__device__ __forceinline__ unsigned pm1_dword(unsigned w, int i)
+// Bit -> int8 expansion of the MFMA operands.  Flat dword F (0..63) of a
+// descriptor holds, in bytes 0..3, bits {i, i+8, i+16, i+24} of 32-bit word
+// u = F/8, with i = F%8.  Map side: a set bit becomes byte 0x80 (-128), a
+// clear bit 0 -- one shift and one mask.  Query side: a set bit becomes 0x40
+// (+64), a clear bit 0xC0 (-64) -- the same with an XOR.
+__device__ __forceinline__ unsigned expand_map_dword(unsigned w, int i)
 {
-    const unsigned sel = (i >= 2 ? (w >> (i - 2)) : (w << (2 - i))) & 0x04040404u;
-    return __builtin_amdgcn_perm(0x01010101u, 0xFFFFFFFFu, sel);
+    return (w << (7 - i)) & 0x80808080u;
 }
 
-// The same expansion with the signs flipped (bit set -> -1), used for the
-// map operand so that the MFMA accumulates -dot directly.
-__device__ __forceinline__ unsigned pm1n_dword(unsigned w, int i)
+__device__ __forceinline__ unsigned expand_query_dword(unsigned w, int i)
 {
-    const unsigned sel = (i >= 2 ? (w >> (i - 2)) : (w << (2 - i))) & 0x04040404u;
-    return __builtin_amdgcn_perm(0xFFFFFFFFu, 0x01010101u, sel);
+    return expand_map_dword(w, i) ^ 0xC0C0C0C0u;
 }
 
 typedef int knn_v4i __attribute__((ext_vector_type(4)));
 
+// One 64-row tile into LDS: this thread's four map words (rows 16j + fl%16)
+// expanded to the operand dwords F0..F0+3 of lane fl, k-chunk fc.
+__device__ __forceinline__ void knn_fill_tile(uint4 (*A)[4][64], const unsigned* wn, int fc, int fl, int fi0)
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint4 v;
+        v.x = expand_map_dword(wn[j], fi0 + 0);
+        v.y = expand_map_dword(wn[j], fi0 + 1);
+        v.z = expand_map_dword(wn[j], fi0 + 2);
+        v.w = expand_map_dword(wn[j], fi0 + 3);
+        A[j][fc][fl] = v;
+    }
+}
+
+// The 16 MFMAs of subtile s (16 map rows x 64 queries): acc[gq] = C + sum
+// over the four 64-wide k-chunks, C = `rows`.
+__device__ __forceinline__ void knn_subtile(const uint4 (*A)[4][64], int s, int lane, const knn_v4i (*B)[4],
+                                            knn_v4i rows, knn_v4i* acc)
+{
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const uint4 av = A[s][c][lane];
+        const knn_v4i a = (knn_v4i){(int)av.x, (int)av.y, (int)av.z, (int)av.w};
+#pragma unroll
+        for (int gq = 0; gq < 4; gq++)
+            acc[gq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, B[gq][c], c == 0 ? rows : acc[gq], 0, 0, 0);
+    }
+}
+
 // Workgroup: 4 wavefronts x 64 queries, one map chunk.  Map rows are
 // expanded 64 at a time into LDS in operand order [subtile][k-chunk][lane];
 // each wavefront runs 4 (k-chunks) x 4 (query groups) MFMAs per 16 rows.
-__global__ __launch_bounds__(256) void knn_mfma_kernel(const uint8_t* __restrict__ q, int nq,
-                                                       const uint8_t* __restrict__ m, long long nm,
-                                                       uint2* __restrict__ part, int nchunks)
+//
+// The operands are scaled so that the accumulator is the search key itself.
+// With map bits m -> -128 m and query bits q -> 64 (2q - 1),
+//   sum -128 m 64 (2q - 1) = 8192 sum m (1 - 2q) = 8192 (Hamming - |q|)
+// since Hamming = |q| + sum m (1 - 2q); the MFMA's C input is the row's index
+// in the chunk.  |q| is constant per query, so
+//   key = 8192 (Hamming - |q|) + row          (|key| < 2^22)
+// orders (distance, row) exactly for rows under 8192 = KNN_CHUNK, and the
+// top-2 update is one v_med3_i32 and one v_min_i32 per map row and query.
+// 256 VGPRs would leave 3 wavefronts per SIMD and 768 of the 1024 workgroup
+// slots for the 984 workgroups of a 2000 x 1M search; the bound of 4 keeps
+// every workgroup resident.
+__global__ __launch_bounds__(256, 4) void knn_mfma_kernel(const uint8_t* __restrict__ q, int nq,
+                                                          const uint8_t* __restrict__ m, long long nm,
+                                                          uint2* __restrict__ part, int nchunks)
 {
-    __shared__ uint4 A[4][4][64];   // 16 KB: 64 rows of +-1 bytes
+    __shared__ uint4 A[4][4][64];   // 16 KB: 64 rows of expanded bytes
+    __shared__ int popq[256];       // |q| of the workgroup's queries
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int g = lane >> 4, i16 = lane & 15;
     const int chunk = blockIdx.x;
     const int qbase = blockIdx.y * 256 + wv * 64;
+    {
+        const int qi = blockIdx.y * 256 + tid;
+        int pc = 0;
+        if (qi < nq) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) pc += __popc(*(const unsigned*)(q + (size_t)qi * 32 + 4 * u));
+        }
+        popq[tid] = pc;   // read after the loop's first barrier
+    }
 
     // Query operands: group gq, k-chunk c -> this lane's 4 dwords F = 16c+4g+d.
     knn_v4i B[4][4];
@@ -2279,112 +2334,106 @@ __global__ __launch_bounds__(256) void knn_mfma_kernel(const uint8_t* __restrict
 #pragma unroll
             for (int d = 0; d < 4; d++) {
                 const int F = 16 * c + 4 * g + d;
-                B[gq][c][d] = (int)pm1_dword(w[F >> 3], F & 7);
+                B[gq][c][d] = (int)expand_query_dword(w[F >> 3], F & 7);
             }
         }
     }
     const long long c0 = (long long)chunk * KNN_CHUNK;
     const int len = (int)min((long long)KNN_CHUNK, nm - c0);
-    unsigned k0[4], k1[4];
+    int k0[4], k1[4];
 #pragma unroll
-    for (int gq = 0; gq < 4; gq++) k0[gq] = k1[gq] = KNN_NONE;
+    for (int gq = 0; gq < 4; gq++) k0[gq] = k1[gq] = INT_MAX;
 
-    // Fill role of this thread: row fr (0..63) of the tile, k-chunk fc.  The
-    // map operand is expanded with flipped signs, so the accumulator holds
-    // -dot and key = (256 - dot) << 19 | row = (acc << 19) + (256 << 19 | row)
-    // is one shift-add.  Full 64-row tiles run without bounds checks; the
-    // chunk's last, partial tile takes the checked form.
-    const int fr = tid >> 2, fc = tid & 3;
+    // Accumulator (lane, r) = (map row t0 + 16s + 4g + r, query 16gq + i16):
+    // `rows` is the C input of subtile s, advanced by 16 per subtile.
+    knn_v4i rows = (knn_v4i){4 * g, 4 * g + 1, 4 * g + 2, 4 * g + 3};
+    // Fill role of this thread: k-chunk fc = wavefront, operand lane fl, so
+    // that the wavefront's 16-byte LDS writes are consecutive (no bank
+    // conflicts): for subtile j, row 16j + (fl & 15), dwords F0..F0+3 with
+    // F0 = 16 fc + 4 (fl >> 4), i.e. bits 4 (fl >> 4 & 1) + 0..3 (+8k) of
+    // word 2 fc + (fl >> 5).  The next tile's words are loaded before the
+    // current tile's MFMAs run.  Full 64-row tiles run without bounds checks;
+    // the chunk's last, partial tile takes the checked form.
+    // Rows past the chunk's end read its last row instead (their keys are
+    // discarded in the checked form).
+    const int fc = wv, fl = lane;
+    const int fi0 = 4 * ((fl >> 4) & 1);
+    const uint8_t* mchunk = m + (size_t)c0 * 32;                 // uniform base
+    const unsigned fword = 4u * (unsigned)(2 * fc + (fl >> 5));  // + 32-bit lane offsets
     const int nfull = len & ~63;
-    for (int t0 = 0; t0 < len; t0 += 64) {
-        const bool full = t0 < nfull;
-        const int n = full ? 64 : len - t0;
-        __syncthreads();
-        {
-            uint2 wv2 = make_uint2(0, 0);
-            if (full || fr < n) wv2 = *(const uint2*)(m + (size_t)(c0 + t0 + fr) * 32 + 8 * fc);
-            const int s = fr >> 4, ii = fr & 15;
+    unsigned wn[4];
 #pragma unroll
-            for (int gg = 0; gg < 4; gg++) {
-                uint4 v;
-                const int F0 = 16 * fc + 4 * gg;   // dwords F0..F0+3 of word 2fc + gg/2
-                const unsigned w = ((F0 >> 3) & 1) ? wv2.y : wv2.x;
-                v.x = pm1n_dword(w, (F0 + 0) & 7);
-                v.y = pm1n_dword(w, (F0 + 1) & 7);
-                v.z = pm1n_dword(w, (F0 + 2) & 7);
-                v.w = pm1n_dword(w, (F0 + 3) & 7);
-                A[s][fc][ii + 16 * gg] = v;
-            }
+    for (int j = 0; j < 4; j++)
+        wn[j] = *(const unsigned*)(mchunk + ((unsigned)min(16 * j + (fl & 15), len - 1) * 32u + fword));
+    for (int t0 = 0; t0 < nfull; t0 += 64) {
+        __syncthreads();
+        knn_fill_tile(A, wn, fc, fl, fi0);
+        __syncthreads();
+        if (t0 + 64 < len) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                wn[j] = *(const unsigned*)(mchunk + ((unsigned)min(t0 + 64 + 16 * j + (fl & 15), len - 1) * 32u + fword));
         }
-        __syncthreads();
-        if (full) {
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                knn_v4i acc[4];
+        for (int s = 0; s < 4; s++) {
+            knn_v4i acc[4];
+            knn_subtile(A, s, lane, B, rows, acc);
+            rows += 16;
 #pragma unroll
-                for (int gq = 0; gq < 4; gq++) acc[gq] = (knn_v4i){0, 0, 0, 0};
+            for (int r = 0; r < 4; r++) {
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const uint4 av = A[s][c][lane];
-                    const knn_v4i a = (knn_v4i){(int)av.x, (int)av.y, (int)av.z, (int)av.w};
-#pragma unroll
-                    for (int gq = 0; gq < 4; gq++)
-                        acc[gq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, B[gq][c], acc[gq], 0, 0, 0);
-                }
-                // Accumulator (lane, r) = (map row 16s + 4g + r, query 16gq + i16).
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const unsigned cbase = (256u << 19) + (unsigned)(t0 + 16 * s + 4 * g + r);
-#pragma unroll
-                    for (int gq = 0; gq < 4; gq++) {
-                        const unsigned key = ((unsigned)acc[gq][r] << 19) + cbase;
-                        k1[gq] = med3u(k0[gq], k1[gq], key);
-                        k0[gq] = min(k0[gq], key);
-                    }
-                }
-            }
-        } else {
-            for (int s = 0; s < 4; s++) {
-                if (16 * s >= n) break;
-                knn_v4i acc[4];
-#pragma unroll
-                for (int gq = 0; gq < 4; gq++) acc[gq] = (knn_v4i){0, 0, 0, 0};
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const uint4 av = A[s][c][lane];
-                    const knn_v4i a = (knn_v4i){(int)av.x, (int)av.y, (int)av.z, (int)av.w};
-#pragma unroll
-                    for (int gq = 0; gq < 4; gq++)
-                        acc[gq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, B[gq][c], acc[gq], 0, 0, 0);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int row = t0 + 16 * s + 4 * g + r;
-                    const unsigned cbase = (256u << 19) + (unsigned)row;
-                    const bool ok = row < len;
-#pragma unroll
-                    for (int gq = 0; gq < 4; gq++) {
-                        const unsigned key = ok ? ((unsigned)acc[gq][r] << 19) + cbase : KNN_NONE;
-                        k1[gq] = med3u(k0[gq], k1[gq], key);
-                        k0[gq] = min(k0[gq], key);
-                    }
+                for (int gq = 0; gq < 4; gq++) {
+                    const int key = acc[gq][r];
+                    k1[gq] = med3i(k0[gq], k1[gq], key);
+                    k0[gq] = min(k0[gq], key);
                 }
             }
         }
     }
-    // Merge the four lanes that hold the same query (lanes i16 + 16g).
+    if (nfull < len) {
+        __syncthreads();
+        knn_fill_tile(A, wn, fc, fl, fi0);
+        __syncthreads();
+#pragma unroll 1
+        for (int s = 0; 16 * s < len - nfull; s++) {
+            knn_v4i acc[4];
+            knn_subtile(A, s, lane, B, rows, acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const bool ok = rows[r] < len;
+#pragma unroll
+                for (int gq = 0; gq < 4; gq++) {
+                    const int key = ok ? acc[gq][r] : INT_MAX;
+                    k1[gq] = med3i(k0[gq], k1[gq], key);
+                    k0[gq] = min(k0[gq], key);
+                }
+            }
+            rows += 16;
+        }
+    }
+    // Merge the four lanes that hold the same query (lanes i16 + 16g) and
+    // convert to the partial format distance << 20 | row.
 #pragma unroll
     for (int gq = 0; gq < 4; gq++) {
 #pragma unroll
         for (int o = 16; o <= 32; o <<= 1) {
-            const unsigned b0 = __shfl_xor(k0[gq], o, 64), b1 = __shfl_xor(k1[gq], o, 64);
-            const unsigned n0 = min(k0[gq], b0);
-            const unsigned n1 = min(max(k0[gq], b0), min(k1[gq], b1));
+            const int b0 = __shfl_xor(k0[gq], o, 64), b1 = __shfl_xor(k1[gq], o, 64);
+            const int n0 = min(k0[gq], b0);
+            const int n1 = min(max(k0[gq], b0), min(k1[gq], b1));
             k0[gq] = n0;
             k1[gq] = n1;
         }
         const int qi = qbase + 16 * gq + i16;
-        if (g == 0 && qi < nq) part[(size_t)qi * nchunks + chunk] = make_uint2(k0[gq], k1[gq]);
+        if (g == 0 && qi < nq) {
+            const int pc = popq[wv * 64 + 16 * gq + i16];
+            const int ks[2] = {k0[gq], k1[gq]};
+            unsigned out[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+                out[t] = ks[t] == INT_MAX ? KNN_NONE
+                                          : ((unsigned)((ks[t] >> 13) + pc) << 20) | (unsigned)(ks[t] & (KNN_CHUNK - 1));
+            part[(size_t)qi * nchunks + chunk] = make_uint2(out[0], out[1]);
+        }
     }
 }
 

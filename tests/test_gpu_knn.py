@@ -53,3 +53,40 @@ def test_knn2_full_size_methods_agree(orb):
     d0 = np.unpackbits(q ^ m[mi[:, 0]], axis=1).sum(1)
     np.testing.assert_array_equal(d0, md[:, 0])
     assert np.all(md[:, 0] <= md[:, 1])
+
+
+@pytest.mark.parametrize("method", ["popcount", "mfma"])
+def test_knn2_extreme_distances_and_ties(orb, oracle, method):
+    """Queries of popcount 0 and 256 and their complements (distances 0 and
+    256, the ends of the key range), rows all equal (every row ties, so the
+    two lowest indices must win, across chunk boundaries), and a chunk's last
+    partial tile."""
+    rng = np.random.default_rng(9)
+    nm = 8192 * 2 + 37
+    q = rng.integers(0, 256, (6, 32), dtype=np.uint8)
+    q[0] = 0
+    q[1] = 255
+    m = np.repeat(rng.integers(0, 256, (1, 32), dtype=np.uint8), nm, axis=0)
+    m[8192 + 5] = 255          # distance 0 for q[1], 256 for q[0]
+    m[8192 * 2 + 30] = 0       # distance 0 for q[0], in the partial tile
+    m[8192 * 2 + 36] = q[2]    # the last row
+    m[3] = q[3]
+    m[8191] = q[3]             # a tie at distance 0 on a chunk's last row
+    m[8192] = q[4]
+    m[8192 * 2] = q[4]         # a tie across chunks
+    q[5] = m[0]                # every row but the planted ones ties at 0
+    gi, gd, ga = orb.knn2(q, m, 0.7, method)
+    oi, od, oa = oracle.knn2(q, m, 0.7, nthreads=8)
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(ga, oa)
+    assert gd[0, 0] == 0 and gi[0, 0] == 8192 * 2 + 30
+    assert gd[1, 0] == 0 and gi[1, 0] == 8192 + 5
+    assert list(gi[3]) == [3, 8191] and list(gd[3]) == [0, 0]
+    assert list(gi[4]) == [8192, 8192 * 2]
+    assert list(gi[5]) == [0, 1] and list(gd[5]) == [0, 0]
+    # distance 256, the top of the key range, as a kept second neighbour
+    q2 = np.array([[0] * 32, [255] * 32], np.uint8)
+    m2 = np.array([[255] * 32, [255] * 32, [0] * 32], np.uint8)
+    gi, gd, ga = orb.knn2(q2, m2, 0.7, method)
+    assert gi.tolist() == [[2, 0], [0, 1]] and gd.tolist() == [[0, 256], [0, 0]]
