@@ -642,7 +642,7 @@ def main():
                                           "GB_per_s": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
     if "knn" in want:
         print("[bench] dense k=2 search", file=sys.stderr, flush=True)
-        out["knn2"] = knn_extra(orb, ctx, max(3, args.steps // 2))
+        out["knn2"] = knn_extra(orb, ctx, max(10, args.steps))
     if "batch" in want:
         print("[bench] mixed-size batch", file=sys.stderr, flush=True)
         out["mixed_batch"] = batch_extra(orb, ctx, max(5, args.steps // 2))

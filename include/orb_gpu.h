@@ -165,8 +165,8 @@ int orb_knn2_device(orb_context* ctx, const uint8_t* d_query, int nq, const uint
 int orb_knn2_device_noalloc(orb_context* ctx, const uint8_t* d_query, int nq, const uint8_t* d_map, int64_t nm,
                             int method, float ratio, int32_t* d_idx2, int32_t* d_dist2, uint8_t* d_accept);
 /* Copies query and map rows into HBM, then returns the average device
- * milliseconds of the search and merge kernels over `steps` runs (HIP events
- * on the context stream). */
+ * milliseconds of the search and merge kernels over `steps` runs after three
+ * untimed ones (HIP events on the context stream). */
 int orb_knn2_time(orb_context* ctx, const uint8_t* query, int nq, const uint8_t* map, int64_t nm, int method,
                   float ratio, int steps, float* ms_search, float* ms_merge);
 

@@ -1142,7 +1142,7 @@ int orb_knn2_time(orb_context* ctx, const uint8_t* query, int nq, const uint8_t*
                 hipMemcpyAsync(ctx->d_km, map, (size_t)nm * 32, hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
         rc = ORB_E_DEVICE;
     double a = 0, b = 0;
-    for (int s = -1; s < steps && !rc; s++) {   // s = -1: untimed warm-up
+    for (int s = -3; s < steps && !rc; s++) {   // s < 0: untimed warm-up (clocks ramp over the first runs)
         if (orb_launch_knn2(ctx->d_kq, nq, ctx->d_km, nm, method, ratio, ctx->d_part, d_out, d_out + 2 * (size_t)nq,
                             (uint8_t*)(d_out + 4 * (size_t)nq), ctx->stream, ev) != hipSuccess ||
             hipEventSynchronize(ev[2]) != hipSuccess) {
