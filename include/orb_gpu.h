@@ -261,6 +261,32 @@ int orb_batch_upload_images(orb_batch* b, int n, const int* slots, const uint8_t
                             const size_t* strides);
 int orb_batch_read_all(orb_batch* b, orb_keypoint* keypoints, uint8_t* descriptors, int capacity, int* counts,
                        float* uRight, float* depth);
+/* Pinned whole-batch transfers (no reference counterpart), for frames that a
+ * capture or decode step writes straight into pinned host memory and for
+ * pipelines that keep several batches in flight: nothing here waits.
+ *   orb_batch_pinned_images: one pinned host buffer per slot (rows of
+ *     pitches[s] >= width bytes), allocated by the first call and owned by
+ *     the batch; write frame s at images[s].
+ *   orb_batch_upload_pinned: enqueue one host-to-device copy of all of them
+ *     and one launch that moves each into its slot's image; the next
+ *     orb_batch_run is ordered after it.  The host buffers may be rewritten
+ *     once the batch's stream has passed it (orb_batch_synchronize, or a
+ *     later orb_batch_pinned_results).
+ *   orb_batch_read_async: enqueue one launch that gathers every slot's
+ *     counters, keypoints and descriptors (valid entries only) and every
+ *     pair's uRight / depth into one device buffer, and one device-to-host
+ *     copy of it into pinned memory the batch owns.
+ *   orb_batch_pinned_results / orb_batch_pinned_stereo: wait for the last
+ *     orb_batch_read_async, then point into its pinned copy (n entries; valid
+ *     until the next read, stereo, tile-height change or destroy).
+ *     ORB_E_ARG if no read was enqueued since the last such change,
+ *     ORB_E_OVERFLOW as orb_batch_results. */
+int orb_batch_pinned_images(orb_batch* b, uint8_t** images, size_t* pitches);
+int orb_batch_upload_pinned(orb_batch* b);
+int orb_batch_read_async(orb_batch* b);
+int orb_batch_pinned_results(orb_batch* b, int slot, const orb_keypoint** keypoints, const uint8_t** descriptors,
+                             int* n);
+int orb_batch_pinned_stereo(orb_batch* b, int pair, const float** uRight, const float** depth, int* n);
 
 /* ---------------------------------------------------------------- serving
  * Continuous-batching request loop for many camera streams.  No reference

@@ -49,6 +49,11 @@ SIGNATURES = {
     "orb_batch_set_tile_heights": (_I, [_P, _I, _P]),
     "orb_batch_autotune_tiles": (_I, [_P, _I, _P, _P, _P]),
     "orb_batch_read_all": (_I, [_P, _P, _P, _I, _P, _P, _P]),
+    "orb_batch_pinned_images": (_I, [_P, _P, _P]),
+    "orb_batch_upload_pinned": (_I, [_P]),
+    "orb_batch_read_async": (_I, [_P]),
+    "orb_batch_pinned_results": (_I, [_P, _I, _P, _P, _P]),
+    "orb_batch_pinned_stereo": (_I, [_P, _I, _P, _P, _P]),
     "orb_extractor_set_opencv_rounding": (_I, [_P, _I]),
     "orb_context_destroy": (_V, [_P]),
     "orb_context_synchronize": (_I, [_P]),
@@ -488,6 +493,54 @@ class Batch:
                                                               _ptr(out["counts"]), _ptr(out["uR"]),
                                                               _ptr(out["depth"])))
         return out
+
+    def pinned_images(self):
+        """The batch's pinned host image buffers, one (height, width) uint8
+        view per slot (rows at the device pitch).  Write frames into them,
+        then upload_pinned().  The views are valid while the batch lives."""
+        n = len(self.sizes)
+        ptrs = (ctypes.c_void_p * n)()
+        pitches = np.zeros(n, np.uint64)
+        _check("orb_batch_pinned_images", lib().orb_batch_pinned_images(self.h, ptrs, _ptr(pitches)))
+        out = []
+        for s in range(n):
+            w, h = self.sizes[s]
+            buf = (ctypes.c_uint8 * (int(pitches[s]) * h)).from_address(ptrs[s])
+            full = np.frombuffer(buf, np.uint8).reshape(h, int(pitches[s]))
+            out.append(full[:, :w])
+        return out
+
+    def upload_pinned(self):
+        """Enqueue the pinned images' transfer; the next run() follows it."""
+        _check("orb_batch_upload_pinned", lib().orb_batch_upload_pinned(self.h))
+
+    def read_async(self):
+        """Enqueue the transfer of every result into pinned host memory."""
+        _check("orb_batch_read_async", lib().orb_batch_read_async(self.h))
+
+    def pinned_results(self, slot):
+        """(keypoints, descriptors) of the last read_async(), after waiting for
+        it: views into pinned memory, valid until the next read_async()."""
+        kp, de, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int()
+        _check("orb_batch_pinned_results", lib().orb_batch_pinned_results(self.h, slot, ctypes.byref(kp),
+                                                                          ctypes.byref(de), ctypes.byref(n)))
+        k = n.value
+        if k == 0:
+            return np.zeros(0, KEYPOINT_DTYPE), np.zeros((0, 32), np.uint8)
+        kps = np.frombuffer((ctypes.c_uint8 * (k * KEYPOINT_DTYPE.itemsize)).from_address(kp.value), KEYPOINT_DTYPE)
+        desc = np.frombuffer((ctypes.c_uint8 * (k * 32)).from_address(de.value), np.uint8).reshape(k, 32)
+        return kps, desc
+
+    def pinned_stereo(self, pair):
+        """(uRight, depth) of pair `pair` from the last read_async(), as views."""
+        ur, dp, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int()
+        _check("orb_batch_pinned_stereo", lib().orb_batch_pinned_stereo(self.h, pair, ctypes.byref(ur),
+                                                                        ctypes.byref(dp), ctypes.byref(n)))
+        k = n.value
+        if k == 0:
+            return np.zeros(0, np.float32), np.zeros(0, np.float32)
+        return (np.frombuffer((ctypes.c_float * k).from_address(ur.value), np.float32),
+                np.frombuffer((ctypes.c_float * k).from_address(dp.value), np.float32))
 
     def set_stereo(self, left_slots, right_slots, bf, baseline):
         l = np.ascontiguousarray(left_slots, np.int32)

@@ -29,6 +29,7 @@ extern "C" hipError_t orb_launch_gray(const uint8_t* src, size_t sstride, int sc
 extern "C" hipError_t orb_launch_distinctive(const uint8_t* desc, const int* offsets, int n_points, int* best,
                                              uint8_t* out_desc, hipStream_t st);
 extern "C" hipError_t orb_launch_hbm_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
+extern "C" hipError_t orb_launch_copy_ops(const CopyOp* d_ops, int nops, int blocks_per_op, hipStream_t st);
 extern "C" int orb_knn_chunks(long long nm);
 extern "C" int orb_knn_scratch_setup(int device);
 extern "C" uint2* orb_knn_scratch(int device, size_t* entries);

@@ -2532,6 +2532,34 @@ extern "C" hipError_t orb_launch_match_pairs(const MatchArgs* d_args, int npairs
     return hipGetLastError();
 }
 
+// ================================================== batch staged transfers
+// One launch for a list of regions (blockIdx.y = region): a batch's pinned
+// input images, landed in one device staging buffer by one host-to-device
+// copy, go to their slots' images; the slots' outputs go into one buffer
+// for one device-to-host copy.  A region whose length is a count read on the
+// device copies only the valid entries.
+__global__ __launch_bounds__(256) void copy_ops_kernel(const CopyOp* __restrict__ ops)
+{
+    const CopyOp o = ops[blockIdx.y];
+    int n = o.words;
+    if (o.count) n = min(n, max(0, *o.count) * o.unit_words);
+    const int stride = gridDim.x * 256;
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if ((n & 3) == 0) {
+        const int n4 = n >> 2;
+        for (; i < n4; i += stride) ((uint4*)o.dst)[i] = ((const uint4*)o.src)[i];
+    } else {
+        for (; i < n; i += stride) o.dst[i] = o.src[i];
+    }
+}
+
+extern "C" hipError_t orb_launch_copy_ops(const CopyOp* d_ops, int nops, int blocks_per_op, hipStream_t st)
+{
+    if (nops <= 0) return hipSuccess;
+    copy_ops_kernel<<<dim3(blocks_per_op, nops), 256, 0, st>>>(d_ops);
+    return hipGetLastError();
+}
+
 // ======================================================= HBM copy microbench
 // Plain streaming copy (16-byte loads and stores, grid-stride): the measured
 // HBM rate of this kernel re-confirms the bandwidth peak, and its counter

@@ -609,10 +609,164 @@ struct orb_batch {
     char* h_res = nullptr;
     size_t h_res_cap = 0;
 
+    // Pinned transfers (orb_batch_pinned_images / _upload_pinned /
+    // _read_async).  Inputs: every slot's image at its device pitch in one
+    // pinned buffer, one host-to-device copy into a device staging buffer,
+    // and one copy-list launch into the slots' images.  Outputs: every slot's
+    // counters, keypoints and descriptors and every pair's uRight / depth at
+    // fixed offsets of one device buffer (one copy-list launch, valid entries
+    // only), then one device-to-host copy into its pinned twin.
+    uint8_t* h_in = nullptr;
+    uint8_t* d_in = nullptr;
+    std::vector<size_t> in_off;
+    CopyOp* d_in_ops = nullptr;
+    size_t in_bytes = 0;
+    char* h_out = nullptr;
+    char* d_out = nullptr;
+    CopyOp* d_out_ops = nullptr;
+    size_t out_bytes = 0;
+    int n_out_ops = 0, out_pairs = -1;      // pair count the output layout was built for
+    std::vector<size_t> out_ctr, out_kps, out_desc, out_ur, out_dep;
+    hipEvent_t out_ev = nullptr;            // recorded after the device-to-host copy
+    bool out_ready = false;                 // a read has been enqueued since the layout was built
+
+    static size_t up256(size_t v) { return (v + 255) / 256 * 256; }
+
+    int setup_pinned_in()
+    {
+        if (h_in) return ORB_OK;
+        const int n = (int)ex.size();
+        in_off.assign(n, 0);
+        size_t o = 0;
+        for (int s = 0; s < n; s++) {
+            in_off[s] = o;
+            o += up256((size_t)ex[s]->hjob.src_pitch * ex[s]->plan.H);
+        }
+        in_bytes = o;
+        if (hipHostMalloc((void**)&h_in, in_bytes) != hipSuccess) {
+            h_in = nullptr;
+            return ORB_E_OOM;
+        }
+        std::memset(h_in, 0, in_bytes);
+        if (hipMalloc((void**)&d_in, in_bytes) != hipSuccess || hipMalloc(&d_in_ops, n * sizeof(CopyOp)) != hipSuccess)
+            return ORB_E_OOM;
+        return write_in_ops();
+    }
+
+    // The staging-to-image copy list, from the slots' current image addresses.
+    int write_in_ops()
+    {
+        const int n = (int)ex.size();
+        std::vector<CopyOp> ops(n);
+        for (int s = 0; s < n; s++) {
+            const ImageJob& J = ex[s]->hjob;
+            ops[s] = CopyOp{(const unsigned*)(d_in + in_off[s]), (unsigned*)const_cast<uint8_t*>(J.src), nullptr,
+                            (int)((size_t)J.src_pitch * ex[s]->plan.H / 4), 0};
+        }
+        HIP_TRY(hipMemcpy(d_in_ops, ops.data(), n * sizeof(CopyOp), hipMemcpyHostToDevice));
+        return ORB_OK;
+    }
+
+    // After the slots' plans or the stereo pairs change (stream idle): the
+    // input copy list is rewritten and the output layout rebuilt on the next
+    // read.
+    int refresh_pinned()
+    {
+        drop_pinned_out();
+        return d_in_ops ? write_in_ops() : ORB_OK;
+    }
+
+    void drop_pinned_out()
+    {
+        if (h_out) (void)hipHostFree(h_out);
+        if (d_out) (void)hipFree(d_out);
+        if (d_out_ops) (void)hipFree(d_out_ops);
+        h_out = nullptr;
+        d_out = nullptr;
+        d_out_ops = nullptr;
+        out_bytes = 0;
+        n_out_ops = 0;
+        out_pairs = -1;
+        out_ready = false;
+    }
+
+    int setup_pinned_out()
+    {
+        if (h_out && out_pairs == (int)pl.size()) return ORB_OK;
+        drop_pinned_out();
+        const int n = (int)ex.size(), np = (int)pl.size();
+        out_ctr.assign(n, 0);
+        out_kps.assign(n, 0);
+        out_desc.assign(n, 0);
+        out_ur.assign(np, 0);
+        out_dep.assign(np, 0);
+        std::vector<CopyOp> ops;
+        size_t o = 0;
+        for (int s = 0; s < n; s++) {
+            const ImageJob& J = ex[s]->hjob;
+            const size_t cap = (size_t)J.out_cap;
+            out_ctr[s] = o;
+            o += up256(ORB_CNT_SLOTS * 4);
+            out_kps[s] = o;
+            o += up256(cap * sizeof(OrbKeyPoint));
+            out_desc[s] = o;
+            o += up256(cap * 32);
+        }
+        for (int p = 0; p < np; p++) {
+            const size_t cap = (size_t)ex[pl[p]]->hjob.out_cap;
+            out_ur[p] = o;
+            o += up256(cap * 4);
+            out_dep[p] = o;
+            o += up256(cap * 4);
+        }
+        out_bytes = o;
+        if (hipHostMalloc((void**)&h_out, out_bytes) != hipSuccess) {
+            h_out = nullptr;
+            return ORB_E_OOM;
+        }
+        if (hipMalloc((void**)&d_out, out_bytes) != hipSuccess) return ORB_E_OOM;
+        for (int s = 0; s < n; s++) {
+            const ImageJob& J = ex[s]->hjob;
+            const int* cnt = J.counters + ORB_CNT_NOUT;
+            ops.push_back(CopyOp{(const unsigned*)J.counters, (unsigned*)(d_out + out_ctr[s]), nullptr, ORB_CNT_SLOTS, 0});
+            ops.push_back(CopyOp{(const unsigned*)J.kps, (unsigned*)(d_out + out_kps[s]), cnt,
+                                 J.out_cap * (int)(sizeof(OrbKeyPoint) / 4), (int)(sizeof(OrbKeyPoint) / 4)});
+            ops.push_back(CopyOp{(const unsigned*)J.desc, (unsigned*)(d_out + out_desc[s]), cnt, J.out_cap * 8, 8});
+        }
+        for (int p = 0; p < np; p++) {
+            const orb_extractor* L = ex[pl[p]];
+            const int* cnt = L->hjob.counters + ORB_CNT_NOUT;
+            ops.push_back(CopyOp{(const unsigned*)L->d_uR, (unsigned*)(d_out + out_ur[p]), cnt, L->hjob.out_cap, 1});
+            ops.push_back(CopyOp{(const unsigned*)L->d_depth, (unsigned*)(d_out + out_dep[p]), cnt, L->hjob.out_cap, 1});
+        }
+        n_out_ops = (int)ops.size();
+        if (hipMalloc(&d_out_ops, ops.size() * sizeof(CopyOp)) != hipSuccess) return ORB_E_OOM;
+        HIP_TRY(hipMemcpy(d_out_ops, ops.data(), ops.size() * sizeof(CopyOp), hipMemcpyHostToDevice));
+        if (!out_ev) HIP_TRY(hipEventCreateWithFlags(&out_ev, hipEventDisableTiming));
+        out_pairs = np;
+        return ORB_OK;
+    }
+
+    // After the last read: wait for it, then check the slot's error bits.
+    int pinned_count(int slot, int* n)
+    {
+        if (!h_out || !out_ready) return ORB_E_ARG;
+        HIP_TRY(hipEventSynchronize(out_ev));
+        const int* c = (const int*)(h_out + out_ctr[slot]);
+        if (c[ORB_CNT_ERR]) return ORB_E_OVERFLOW;
+        *n = c[ORB_CNT_NOUT];
+        return ORB_OK;
+    }
+
     ~orb_batch()
     {
         if (h_ctr) (void)hipHostFree(h_ctr);
         if (h_res) (void)hipHostFree(h_res);
+        if (h_in) (void)hipHostFree(h_in);
+        if (d_in) (void)hipFree(d_in);
+        if (d_in_ops) (void)hipFree(d_in_ops);
+        drop_pinned_out();
+        if (out_ev) (void)hipEventDestroy(out_ev);
         drop_graph();
         for (hipStream_t s2 : br_st) (void)hipStreamDestroy(s2);
         for (hipEvent_t e : br_ev) (void)hipEventDestroy(e);
@@ -783,6 +937,66 @@ int orb_batch_read_all(orb_batch* b, orb_keypoint* keypoints, uint8_t* descripto
     return ORB_OK;
 }
 
+int orb_batch_pinned_images(orb_batch* b, uint8_t** images, size_t* pitches)
+{
+    if (!b || !images) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    const int rc = b->setup_pinned_in();
+    if (rc) return rc;
+    for (size_t s = 0; s < b->ex.size(); s++) {
+        images[s] = b->h_in + b->in_off[s];
+        if (pitches) pitches[s] = (size_t)b->ex[s]->hjob.src_pitch;
+    }
+    return ORB_OK;
+}
+
+int orb_batch_upload_pinned(orb_batch* b)
+{
+    if (!b || !b->h_in) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    hipStream_t st = b->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(b->d_in, b->h_in, b->in_bytes, hipMemcpyHostToDevice, st));
+    HIP_TRY(orb_launch_copy_ops(b->d_in_ops, (int)b->ex.size(), 64, st));
+    return ORB_OK;
+}
+
+int orb_batch_read_async(orb_batch* b)
+{
+    if (!b) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    const int rc = b->setup_pinned_out();
+    if (rc) return rc;
+    hipStream_t st = b->ctx->stream;
+    HIP_TRY(orb_launch_copy_ops(b->d_out_ops, b->n_out_ops, 8, st));
+    HIP_TRY(hipMemcpyAsync(b->h_out, b->d_out, b->out_bytes, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(b->out_ev, st));
+    b->out_ready = true;
+    return ORB_OK;
+}
+
+int orb_batch_pinned_results(orb_batch* b, int slot, const orb_keypoint** keypoints, const uint8_t** descriptors,
+                             int* n)
+{
+    if (!b || slot < 0 || slot >= (int)b->ex.size() || !n) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    const int rc = b->pinned_count(slot, n);
+    if (rc) return rc;
+    if (keypoints) *keypoints = (const orb_keypoint*)(b->h_out + b->out_kps[slot]);
+    if (descriptors) *descriptors = (const uint8_t*)(b->h_out + b->out_desc[slot]);
+    return ORB_OK;
+}
+
+int orb_batch_pinned_stereo(orb_batch* b, int pair, const float** uRight, const float** depth, int* n)
+{
+    if (!b || pair < 0 || pair >= (int)b->pl.size() || !n || b->out_pairs != (int)b->pl.size()) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    const int rc = b->pinned_count(b->pl[pair], n);
+    if (rc) return rc;
+    if (uRight) *uRight = (const float*)(b->h_out + b->out_ur[pair]);
+    if (depth) *depth = (const float*)(b->h_out + b->out_dep[pair]);
+    return ORB_OK;
+}
+
 int orb_batch_set_stereo(orb_batch* b, int npairs, const int* left_slots, const int* right_slots, float bf,
                          float baseline)
 {
@@ -811,6 +1025,7 @@ int orb_batch_set_stereo(orb_batch* b, int npairs, const int* left_slots, const 
     HIP_TRY(hipSetDevice(b->ctx->device));
     HIP_TRY(hipStreamSynchronize(b->ctx->stream));
     b->drop_graph();
+    b->drop_pinned_out();
     if (b->d_sargs) (void)hipFree(b->d_sargs);
     b->d_sargs = nullptr;
     b->pl.clear();
@@ -903,6 +1118,7 @@ int orb_batch_set_tile_heights(orb_batch* b, int n, const int* tile_h)
         jobs[i] = e->hjob;
     }
     b->drop_graph();
+    if (int rc = b->refresh_pinned()) return rc;
     if (orb_extractor::build_table(b->ex.data(), ns, *b->h_rt) != ORB_OK) return ORB_E_ARG;
     HIP_TRY(hipMemcpy(b->d_jobs, jobs.data(), ns * sizeof(ImageJob), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->d_rt, b->h_rt, sizeof(RunTable), hipMemcpyHostToDevice));

@@ -134,6 +134,19 @@ struct MatchArgs {
     int save_cap;
 };
 
+// One region of a batch's staged host transfers, copied in 4-byte words:
+// `words` of them, or min(words, *count * unit_words) when count is set (a
+// slot's keypoint count, read on the device).  Source and destination are
+// 16-byte aligned; a region that is a whole number of 16-byte words is
+// copied 16 bytes at a time.
+struct CopyOp {
+    const unsigned* src;
+    unsigned* dst;
+    const int* count;
+    int words;
+    int unit_words;
+};
+
 // A DBoW2 vocabulary tree on the device, nodes renumbered breadth-first so
 // that every node's children are contiguous (in their file order).
 struct VocDev {

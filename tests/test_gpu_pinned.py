@@ -1,0 +1,94 @@
+"""Pinned whole-batch transfers (orb_batch_pinned_images, _upload_pinned,
+_read_async, _pinned_results, _pinned_stereo): frames written into the
+batch's pinned buffers and results read back through one gathered copy equal
+the oracle's, also with two batches kept in flight on two contexts, and the
+calls refuse results that no read has produced."""
+import numpy as np
+import pytest
+
+from orb_slam2_detailed_comments_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+BF, FX = 386.1, 718.856
+
+
+def _baseline():
+    return float(np.float32(BF) / np.float32(FX))
+
+
+def _oracle_pair(oracle, l, r):
+    ol = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    orr = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    return oracle.stereo_frame(ol, orr, l, r, BF, _baseline())
+
+
+def _check_pair(batch, p, ref):
+    okl, odl, okr, odr, ouR, odep = ref
+    kl, dl = batch.pinned_results(2 * p)
+    kr, dr = batch.pinned_results(2 * p + 1)
+    assert kl.tobytes() == okl.tobytes() and kr.tobytes() == okr.tobytes(), p
+    np.testing.assert_array_equal(dl, odl)
+    np.testing.assert_array_equal(dr, odr)
+    uR, dep = batch.pinned_stereo(p)
+    assert uR.tobytes() == ouR.tobytes() and dep.tobytes() == odep.tobytes(), p
+
+
+def test_pinned_round_trip_equals_oracle(orb, oracle):
+    pairs = [synth.stereo_pair(1241, 376, 70 + i)[:2] for i in range(2)]
+    batch = orb.Batch([(1241, 376)] * 4, 2000)
+    batch.set_stereo([0, 2], [1, 3], BF, _baseline())
+    with pytest.raises(orb.OrbError, match="bad argument"):
+        batch.pinned_results(0)            # no read yet
+    imgs = batch.pinned_images()
+    assert [im.shape for im in imgs] == [(376, 1241)] * 4
+    for p, (l, r) in enumerate(pairs):
+        imgs[2 * p][:] = l
+        imgs[2 * p + 1][:] = r
+    for mode in (1, 0):
+        batch.upload_pinned()
+        batch.run(mode)
+        batch.read_async()
+        for p, (l, r) in enumerate(pairs):
+            _check_pair(batch, p, _oracle_pair(oracle, l, r))
+    # the same results through the per-slot calls
+    for s in range(4):
+        gk, gd = batch.results(s)
+        pk, pd = batch.pinned_results(s)
+        assert gk.tobytes() == pk.tobytes()
+        np.testing.assert_array_equal(gd, pd)
+    # a new pair list drops the layout until the next read
+    batch.set_stereo([0], [1], BF, _baseline())
+    with pytest.raises(orb.OrbError, match="bad argument"):
+        batch.pinned_stereo(0)
+    batch.read_async()
+    uR, _ = batch.pinned_stereo(0)
+    assert len(uR) == len(batch.pinned_results(0)[0])
+
+
+def test_pinned_two_batches_in_flight(orb, oracle):
+    """Steps alternate between two batches on two contexts, each step
+    enqueued (upload, run, read) before the other batch's previous step is
+    consumed; every step's results equal the oracle's for its frames."""
+    frames = [synth.stereo_pair(1241, 376, 90 + i)[:2] for i in range(4)]
+    refs = [_oracle_pair(oracle, l, r) for l, r in frames]
+    batches = [orb.Batch([(1241, 376)] * 2, 2000, context=orb.Context(0)) for _ in range(2)]
+    views = []
+    for b in batches:
+        b.set_stereo([0], [1], BF, _baseline())
+        views.append(b.pinned_images())
+    pending = {}
+    for step in range(6):
+        k = step % 2
+        b = batches[k]
+        if k in pending:                   # consume this batch's previous step
+            _check_pair(b, 0, refs[pending.pop(k)])
+        f = step % 4
+        views[k][0][:] = frames[f][0]
+        views[k][1][:] = frames[f][1]
+        b.upload_pinned()
+        b.run(1)
+        b.read_async()
+        pending[k] = f
+    for k, f in pending.items():
+        _check_pair(batches[k], 0, refs[f])
