@@ -312,7 +312,7 @@ def serving_extra(orb, ctx):
     out["at_30fps_per_stream"] = serving.summarize(recs)
     print(f"[bench] serving 30 fps/stream: {out['at_30fps_per_stream']}", file=sys.stderr, flush=True)
     sweep = []
-    for fps in (120.0, 240.0, 480.0, 960.0):
+    for fps in (120.0, 240.0, 480.0, 960.0, 1920.0, 3840.0):
         recs, _ = serving.run(srv, frames, fps=fps, duration=1.0, seed=2)
         r = serving.summarize(recs)
         r["offered_fps_per_stream"] = fps

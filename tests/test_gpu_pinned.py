@@ -45,7 +45,8 @@ def test_pinned_round_trip_equals_oracle(orb, oracle):
     for p, (l, r) in enumerate(pairs):
         imgs[2 * p][:] = l
         imgs[2 * p + 1][:] = r
-    for mode in (1, 0):
+    for mode, branches in ((1, 1), (0, 1), (1, 2), (3, 2)):
+        batch.set_branches(branches)      # 2: one branch per pair, forked after the upload
         batch.upload_pinned()
         batch.run(mode)
         batch.read_async()
