@@ -321,6 +321,17 @@ def serving_extra(orb, ctx):
     out["sweep"] = sweep
     ok = [r for r in sweep if r["p99_ms"] <= 10.0 and r["errors"] == 0 and r["refused"] == 0]
     out["max_frames_per_s_at_p99_le_10ms"] = max((r["frames_per_s"] for r in ok), default=None)
+    # The same schedule shape from native threads (one submitter per stream,
+    # orb_server_replay), without the Python driver's per-frame cost.
+    native = []
+    for fps in (960.0, 1440.0, 1920.0, 2880.0):
+        r = srv.replay(frames, fps, 1.0, seed=4)
+        r["offered_fps_per_stream"] = fps
+        native.append(r)
+        print(f"[bench] serving, native driver, {fps} fps/stream: {r}", file=sys.stderr, flush=True)
+    out["native_driver_sweep"] = native
+    okn = [r for r in native if r["p99_ms"] <= 10.0 and r["errors"] == 0 and r["refused"] == 0]
+    out["native_max_frames_per_s_at_p99_le_10ms"] = max((r["frames_per_s"] for r in okn), default=None)
     out["server_stats"] = srv.stats()
     srv.close()
     # plain per-frame GPU path: one ORBextractor call and one match per frame

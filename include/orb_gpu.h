@@ -324,6 +324,21 @@ int orb_server_fetch(orb_server* s, uint64_t ticket, orb_keypoint* keypoints, in
 int orb_server_release(orb_server* s, uint64_t ticket);
 int orb_server_drain(orb_server* s, double timeout_s);
 int orb_server_stats(orb_server* s, int64_t* batches, int64_t* requests, size_t* hbm_used, int* slots);
+/* Load generator (no reference counterpart): an open-loop replay from native
+ * threads, so that the server is measured without a scripting-language
+ * driver.  Streams 0 .. nstreams-1 each get a submitting thread; stream s
+ * replays frames[s] (nframes[s] images of its size, rows contiguous) forwards
+ * then backwards, at exponential inter-arrivals of mean 1/fps (generator
+ * seeded with seed + s) for `duration` seconds, and the calling thread polls
+ * and releases until every submitted frame has completed.  Latency is
+ * completion minus scheduled arrival; frames_per_s counts completions from
+ * the first arrival to the last completion. */
+typedef struct orb_replay_stats {
+    int64_t requests, refused, errors;
+    double frames_per_s, p50_ms, p99_ms, max_ms;
+} orb_replay_stats;
+int orb_server_replay(orb_server* s, int nstreams, const uint8_t* const* frames, const int* nframes, double fps,
+                      double duration, uint64_t seed, orb_replay_stats* out);
 
 /* ---------------------------------------------------------------- vocabulary
  * DBoW2 bag of words (Thirdparty/DBoW2/DBoW2/TemplatedVocabulary.h), as

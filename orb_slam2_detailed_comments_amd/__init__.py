@@ -109,6 +109,7 @@ SIGNATURES = {
     "orb_server_release": (_I, [_P, ctypes.c_uint64]),
     "orb_server_drain": (_I, [_P, ctypes.c_double]),
     "orb_server_stats": (_I, [_P, _P, _P, _P, _P]),
+    "orb_server_replay": (_I, [_P, _I, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _P]),
     "orb_vocabulary_load_text": (_I, [_P, ctypes.c_char_p, _P]),
     "orb_vocabulary_create": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "orb_vocabulary_destroy": (_V, [_P]),
@@ -615,6 +616,12 @@ class _ServeInfo(ctypes.Structure):
                 ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
 
 
+class _ReplayStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_int64), ("refused", ctypes.c_int64), ("errors", ctypes.c_int64),
+                ("frames_per_s", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double)]
+
+
 class Server:
     """Continuous-batching serving loop for camera streams (orb_server_* in the C ABI).
 
@@ -686,6 +693,28 @@ class Server:
 
     def drain(self, timeout=60.0):
         _check("orb_server_drain", lib().orb_server_drain(self.h, float(timeout)))
+
+    def replay(self, frames, fps, duration, seed=1):
+        """Open-loop replay from native threads (orb_server_replay): frames[s]
+        is stream s's sequence, replayed forwards then backwards at Poisson
+        arrivals of `fps` per stream for `duration` seconds.  Returns the
+        request count, refusals, errors, frames/s and p50/p99/max latency."""
+        n = len(frames)
+        stacks = []
+        for s, seq in enumerate(frames):
+            w, h, _ = self.streams[s]
+            a = np.ascontiguousarray(np.stack([np.asarray(f, np.uint8) for f in seq]))
+            if a.shape[1:] != (h, w):
+                raise ValueError(f"stream {s} expects {w}x{h}")
+            stacks.append(a)
+        ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in stacks])
+        nf = np.array([len(a) for a in stacks], np.int32)
+        st = _ReplayStats()
+        _check("orb_server_replay", lib().orb_server_replay(self.h, n, ptrs, _ptr(nf), float(fps), float(duration),
+                                                            ctypes.c_uint64(seed), ctypes.byref(st)))
+        return {"requests": st.requests, "refused": st.refused, "errors": st.errors,
+                "frames_per_s": round(st.frames_per_s, 1), "p50_ms": round(st.p50_ms, 3),
+                "p99_ms": round(st.p99_ms, 3), "max_ms": round(st.max_ms, 3)}
 
     def stats(self):
         b, r, u, sl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_size_t(), ctypes.c_int()

@@ -2545,7 +2545,7 @@ __global__ __launch_bounds__(256) void copy_ops_kernel(const CopyOp* __restrict_
     if (o.count) n = min(n, max(0, *o.count) * o.unit_words);
     const int stride = gridDim.x * 256;
     int i = blockIdx.x * 256 + threadIdx.x;
-    if ((n & 3) == 0) {
+    if ((n & 3) == 0 && (((uintptr_t)o.src | (uintptr_t)o.dst) & 15) == 0) {
         const int n4 = n >> 2;
         for (; i < n4; i += stride) ((uint4*)o.dst)[i] = ((const uint4*)o.src)[i];
     } else {

@@ -9,21 +9,27 @@
 //   * admission: pending requests, oldest first, take a free extractor slot
 //     of their size (allocating one while the HBM budget allows); a stream
 //     contributes at most one frame per batch, so its frames stay ordered;
-//   * launch: the admitted requests form one batch -- uploads, one launch per
-//     pipeline stage over all of them, one multi-pair match launch, the
-//     resident-descriptor update and the result copies -- on a free HIP
-//     stream, so several batches overlap on the device.
+//   * launch: the admitted requests form one batch on a free HIP stream, so
+//     several batches overlap on the device: one host-to-device copy of the
+//     batch's tables, one launch that pulls every frame from its pinned
+//     staging buffer into its slot's level 0, one launch per pipeline stage
+//     over all of them, one multi-pair match launch (which also updates the
+//     resident descriptors), and one launch that writes every request's
+//     results straight into its pinned result buffer.  The HIP calls per
+//     batch do not grow with its number of frames.
 // Requests arriving while batches run are admitted as soon as a slot and a
 // HIP stream free up: continuous batching rather than a fixed batch window.
 // Frames of one stream may be in flight in several lanes at once (one per
 // batch): their extraction overlaps, and only the match of a frame waits
 // (hipStreamWaitEvent) for the batch that holds the stream's previous frame.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <map>
 #include <mutex>
+#include <random>
 #include <thread>
 
 #include "orb_internal.h"
@@ -57,24 +63,35 @@ struct Request {
     uint64_t ticket;
     int stream;
     double t_arrival, t_admit = 0, t_done = 0;
-    uint8_t* h_img = nullptr;    // pinned staging copy of the frame
+    uint8_t* h_img = nullptr;    // pinned staging copy of the frame, rows at the level-0 pitch
     char* h_res = nullptr;       // pinned results: counters | kps | desc | idx | dist
     int slot = -1, cap = 0, status = ORB_OK, n_kps = 0, n_matches = 0;
     int dep_lane = -1;           // lane whose batch holds this stream's previous frame
 };
 
+// Per-batch tables, in one pinned buffer and its device twin: jobs | run
+// table | match args | input copies | output copies.
+constexpr size_t kStageJobs = 0;
+constexpr size_t kStageRt = kStageJobs + ORB_MAX_JOBS * sizeof(ImageJob);
+constexpr size_t kStageMargs = kStageRt + sizeof(RunTable);
+constexpr size_t kStageIn = (kStageMargs + ORB_MAX_JOBS * sizeof(MatchArgs) + 15) / 16 * 16;
+constexpr int kOutOps = 5;       // counters, keypoints, descriptors, match index, match distance
+constexpr size_t kStageOut = kStageIn + ORB_MAX_JOBS * sizeof(CopyOp);
+constexpr size_t kStageBytes = kStageOut + ORB_MAX_JOBS * kOutOps * sizeof(CopyOp);
+
 struct Lane {                    // one HIP stream with its staging
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
-    ImageJob* d_jobs = nullptr;
-    RunTable* d_rt = nullptr;
-    MatchArgs* d_margs = nullptr;
-    char* h_stage = nullptr;     // pinned: jobs | run table | match args
+    char* d_stage = nullptr;
+    char* h_stage = nullptr;
     std::vector<Request*> reqs;
     bool busy = false;
 };
 
 size_t result_bytes(int cap) { return 256 + (size_t)cap * (sizeof(OrbKeyPoint) + 32 + 8); }
+
+// Row pitch of a frame's pinned staging copy: that of the extractor's level 0.
+size_t img_pitch(int w) { return ((size_t)w + 63) / 64 * 64; }
 
 }  // namespace
 
@@ -152,22 +169,27 @@ struct orb_server {
     void launch(Lane& L, std::vector<Request*>& batch)
     {
         const int n = (int)batch.size();
-        ImageJob* jobs = (ImageJob*)L.h_stage;
-        RunTable* rt = (RunTable*)(L.h_stage + ORB_MAX_JOBS * sizeof(ImageJob));
-        MatchArgs* ma = (MatchArgs*)((char*)rt + sizeof(RunTable));
+        ImageJob* jobs = (ImageJob*)(L.h_stage + kStageJobs);
+        RunTable* rt = (RunTable*)(L.h_stage + kStageRt);
+        MatchArgs* ma = (MatchArgs*)(L.h_stage + kStageMargs);
+        CopyOp* in_ops = (CopyOp*)(L.h_stage + kStageIn);
+        CopyOp* out_ops = (CopyOp*)(L.h_stage + kStageOut);
         std::vector<const orb_extractor*> exs(n);
         int waves = 0;
         for (int i = 0; i < n; i++) {
             Request* r = batch[i];
             Slot& s = slots[r->slot];
             Camera& c = cams[r->stream];
+            const ImageJob& J = s.ex->hjob;
             exs[i] = s.ex;
-            jobs[i] = s.ex->hjob;
-            (void)s.ex->upload_image(r->h_img, c.w, hipMemcpyHostToDevice, L.st);
+            jobs[i] = J;
+            // the frame, read by the device from its pinned staging copy
+            in_ops[i] = CopyOp{(const unsigned*)r->h_img, (unsigned*)const_cast<uint8_t*>(J.src), nullptr,
+                               (int)(img_pitch(c.w) * c.h / 4), 0};
             MatchArgs& m = ma[i];
             std::memset(&m, 0, sizeof(m));
-            m.q = s.ex->hjob.desc;
-            m.nq = s.ex->hjob.counters + ORB_CNT_NOUT;
+            m.q = J.desc;
+            m.nq = J.counters + ORB_CNT_NOUT;
             m.t = c.d_prev;
             m.nt = c.d_prev_n;
             m.th_low = th_low;
@@ -180,12 +202,29 @@ struct orb_server {
             m.save_n = c.d_prev_n;
             m.save_cap = c.cap;
             waves += m.cap_q;
+            // the results, written by the device into the pinned result
+            // buffer: counters | keypoints | descriptors | index | distance
+            const int cap = s.ex->plan.out_cap;
+            const int* cnt = J.counters + ORB_CNT_NOUT;
+            char* p = r->h_res;
+            CopyOp* o = out_ops + (size_t)kOutOps * i;
+            o[0] = CopyOp{(const unsigned*)J.counters, (unsigned*)p, nullptr, ORB_CNT_SLOTS, 0};
+            p += 256;
+            o[1] = CopyOp{(const unsigned*)J.kps, (unsigned*)p, cnt, cap * (int)(sizeof(OrbKeyPoint) / 4),
+                          (int)(sizeof(OrbKeyPoint) / 4)};
+            p += (size_t)cap * sizeof(OrbKeyPoint);
+            o[2] = CopyOp{(const unsigned*)J.desc, (unsigned*)p, cnt, cap * 8, 8};
+            p += (size_t)cap * 32;
+            o[3] = CopyOp{(const unsigned*)s.d_midx, (unsigned*)p, cnt, cap, 1};
+            o[4] = CopyOp{(const unsigned*)s.d_mdist, (unsigned*)(p + (size_t)cap * 4), cnt, cap, 1};
+            r->cap = cap;
         }
         (void)orb_extractor::build_table(exs.data(), n, *rt);   // the server's extractors share tile heights
-        (void)hipMemcpyAsync(L.d_jobs, jobs, n * sizeof(ImageJob), hipMemcpyHostToDevice, L.st);
-        (void)hipMemcpyAsync(L.d_rt, rt, sizeof(RunTable), hipMemcpyHostToDevice, L.st);
-        (void)hipMemcpyAsync(L.d_margs, ma, n * sizeof(MatchArgs), hipMemcpyHostToDevice, L.st);
-        hipError_t e = orb_launch_extract(L.d_jobs, L.d_rt, rt, L.st, nullptr);
+        hipError_t e = hipMemcpyAsync(L.d_stage, L.h_stage, kStageBytes, hipMemcpyHostToDevice, L.st);
+        if (e == hipSuccess) e = orb_launch_copy_ops((const CopyOp*)(L.d_stage + kStageIn), n, 32, L.st);
+        if (e == hipSuccess)
+            e = orb_launch_extract((const ImageJob*)(L.d_stage + kStageJobs), (const RunTable*)(L.d_stage + kStageRt),
+                                   rt, L.st, nullptr);
         // Extraction overlaps the previous frames of the same streams; the
         // match needs their descriptors, so it waits for those batches.
         std::vector<char> waited(lanes.size(), 0);
@@ -194,22 +233,10 @@ struct orb_server {
                 waited[r->dep_lane] = 1;
                 if (e == hipSuccess) e = hipStreamWaitEvent(L.st, lanes[r->dep_lane].ev, 0);
             }
-        if (e == hipSuccess) e = orb_launch_match_pairs(L.d_margs, n, waves, L.st);
-        for (int i = 0; i < n; i++) {
-            Request* r = batch[i];
-            Slot& s = slots[r->slot];
-            const int cap = s.ex->plan.out_cap;
-            char* p = r->h_res;
-            (void)hipMemcpyAsync(p, s.ex->hjob.counters, ORB_CNT_SLOTS * 4, hipMemcpyDeviceToHost, L.st);
-            p += 256;
-            (void)hipMemcpyAsync(p, s.ex->hjob.kps, (size_t)cap * sizeof(OrbKeyPoint), hipMemcpyDeviceToHost, L.st);
-            p += (size_t)cap * sizeof(OrbKeyPoint);
-            (void)hipMemcpyAsync(p, s.ex->hjob.desc, (size_t)cap * 32, hipMemcpyDeviceToHost, L.st);
-            p += (size_t)cap * 32;
-            (void)hipMemcpyAsync(p, s.d_midx, (size_t)cap * 8, hipMemcpyDeviceToHost, L.st);
-            r->cap = cap;
-        }
-        (void)hipEventRecord(L.ev, L.st);
+        if (e == hipSuccess) e = orb_launch_match_pairs((const MatchArgs*)(L.d_stage + kStageMargs), n, waves, L.st);
+        if (e == hipSuccess)
+            e = orb_launch_copy_ops((const CopyOp*)(L.d_stage + kStageOut), kOutOps * n, 4, L.st);
+        if (e == hipSuccess) e = hipEventRecord(L.ev, L.st);
         if (e != hipSuccess) error = ORB_E_DEVICE;
         L.reqs = batch;
     }
@@ -326,9 +353,7 @@ struct orb_server {
             if (L.st) (void)hipStreamSynchronize(L.st);
             if (L.ev) (void)hipEventDestroy(L.ev);
             if (L.st) (void)hipStreamDestroy(L.st);
-            if (L.d_jobs) (void)hipFree(L.d_jobs);
-            if (L.d_rt) (void)hipFree(L.d_rt);
-            if (L.d_margs) (void)hipFree(L.d_margs);
+            if (L.d_stage) (void)hipFree(L.d_stage);
             if (L.h_stage) (void)hipHostFree(L.h_stage);
             for (Request* r : L.reqs) pending.push_back(r);
         }
@@ -401,7 +426,7 @@ int orb_server_create(orb_context* ctx, int nstreams, const int* widths, const i
             return ORB_E_DEVICE;
         }
         s->used += (size_t)cap * 32 + 256;
-        s->max_img = std::max(s->max_img, (size_t)c.w * c.h);
+        s->max_img = std::max(s->max_img, img_pitch(c.w) * c.h);
         s->max_cap = std::max(s->max_cap, cap);
         s->cams.push_back(c);
     }
@@ -410,14 +435,10 @@ int orb_server_create(orb_context* ctx, int nstreams, const int* widths, const i
         return ORB_E_ADMISSION;
     }
     s->lanes.resize(max_inflight);
-    const size_t stage = ORB_MAX_JOBS * sizeof(ImageJob) + sizeof(RunTable) + ORB_MAX_JOBS * sizeof(MatchArgs);
     for (Lane& L : s->lanes) {
         if (hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&L.ev, hipEventDisableTiming) != hipSuccess ||
-            hipMalloc(&L.d_jobs, ORB_MAX_JOBS * sizeof(ImageJob)) != hipSuccess ||
-            hipMalloc(&L.d_rt, sizeof(RunTable)) != hipSuccess ||
-            hipMalloc(&L.d_margs, ORB_MAX_JOBS * sizeof(MatchArgs)) != hipSuccess ||
-            hipHostMalloc(&L.h_stage, stage) != hipSuccess) {
+            hipEventCreateWithFlags(&L.ev, hipEventDisableTiming | hipEventReleaseToSystem) != hipSuccess ||
+            hipMalloc(&L.d_stage, kStageBytes) != hipSuccess || hipHostMalloc(&L.h_stage, kStageBytes) != hipSuccess) {
             delete s;
             return ORB_E_OOM;
         }
@@ -450,9 +471,13 @@ int orb_server_submit(orb_server* s, int stream, const uint8_t* image, size_t st
             s->res_pool.pop_back();
         }
     }
-    if (!img && hipHostMalloc(&img, s->max_img) != hipSuccess) return ORB_E_OOM;
-    if (!res && hipHostMalloc(&res, result_bytes(s->max_cap)) != hipSuccess) return ORB_E_OOM;
-    for (int y = 0; y < c.h; y++) std::memcpy(img + (size_t)y * c.w, image + (size_t)y * stride, c.w);
+    // Coherent (fine-grained) pinned memory: the device reads the frame and
+    // writes the results over PCIe with no copy engine in between, so neither
+    // side may keep them in a cache.
+    if (!img && hipHostMalloc(&img, s->max_img, hipHostMallocCoherent) != hipSuccess) return ORB_E_OOM;
+    if (!res && hipHostMalloc(&res, result_bytes(s->max_cap), hipHostMallocCoherent) != hipSuccess) return ORB_E_OOM;
+    const size_t pitch = img_pitch(c.w);
+    for (int y = 0; y < c.h; y++) std::memcpy(img + (size_t)y * pitch, image + (size_t)y * stride, c.w);
     Request* r = new Request();
     r->stream = stream;
     r->t_arrival = t_arrival;
@@ -472,10 +497,10 @@ int orb_server_poll(orb_server* s, orb_serve_info* out, int max, int* n)
 {
     if (!s || !n || (max > 0 && !out)) return ORB_E_ARG;
     std::lock_guard<std::mutex> g(s->mu);
+    const int k_max = std::min(max, (int)s->completed.size());
     int k = 0;
-    while (k < max && !s->completed.empty()) {
-        Request* r = s->completed.front();
-        s->completed.erase(s->completed.begin());
+    while (k < k_max) {
+        Request* r = s->completed[k];
         orb_serve_info& o = out[k++];
         o.ticket = r->ticket;
         o.stream = r->stream;
@@ -487,6 +512,7 @@ int orb_server_poll(orb_server* s, orb_serve_info* out, int max, int* n)
         o.t_done = r->t_done;
         s->done[r->ticket] = r;
     }
+    s->completed.erase(s->completed.begin(), s->completed.begin() + k);
     *n = k;
     return s->error.load();
 }
@@ -542,6 +568,98 @@ int orb_server_drain(orb_server* s, double timeout_s)
         s->cv.wait_for(lk, std::chrono::milliseconds(1));
     }
     return s->error.load();
+}
+
+// Open-loop replay from native threads: one submitter per stream, the
+// calling thread polling and releasing.  Arrival times are exponential
+// inter-arrivals at fps per stream (one generator per stream, seeded with
+// seed + stream), starting 50 ms after the call; frame k of a stream is
+// frames[s][j] with j running 0 .. n-1 .. 1 0 1 .., so each match pairs
+// sequence neighbours.  Latency = completion - scheduled arrival.
+int orb_server_replay(orb_server* s, int nstreams, const uint8_t* const* frames, const int* nframes, double fps,
+                      double duration, uint64_t seed, orb_replay_stats* out)
+{
+    if (!s || !out || nstreams <= 0 || nstreams > (int)s->cams.size() || !frames || !nframes || !(fps > 0) ||
+        !(duration > 0))
+        return ORB_E_ARG;
+    for (int i = 0; i < nstreams; i++)
+        if (!frames[i] || nframes[i] <= 0) return ORB_E_ARG;
+    std::memset(out, 0, sizeof(*out));
+    std::atomic<int64_t> submitted{0}, refused{0};
+    std::atomic<int> failed{ORB_OK}, running{nstreams};
+    const double t0 = clock_s() + 0.05;
+    std::vector<std::thread> subs;
+    for (int si = 0; si < nstreams; si++)
+        subs.emplace_back([&, si]() {
+            const Camera& c = s->cams[si];
+            const size_t bytes = (size_t)c.w * c.h;
+            const int n = nframes[si];
+            std::mt19937_64 rng(seed + (uint64_t)si);
+            std::exponential_distribution<double> gap(fps);
+            double t = gap(rng);
+            for (long k = 0; t < duration && failed.load() == ORB_OK; k++, t += gap(rng)) {
+                const double at = t0 + t;
+                for (double now = clock_s(); now < at; now = clock_s()) {
+                    if (at - now > 3e-4)
+                        std::this_thread::sleep_for(std::chrono::duration<double>(at - now - 2e-4));
+                    else
+                        std::this_thread::yield();
+                }
+                const long j = n <= 1 ? 0 : k % (2 * n - 2);
+                const long f = j < n ? j : 2 * n - 2 - j;
+                uint64_t ticket = 0;
+                const int rc = orb_server_submit(s, si, frames[si] + (size_t)f * bytes, (size_t)c.w, at, &ticket);
+                if (rc == ORB_E_ADMISSION) {
+                    refused++;
+                } else if (rc) {
+                    failed = rc;
+                } else {
+                    submitted++;
+                }
+            }
+            running--;
+        });
+    std::vector<double> lat;
+    std::vector<orb_serve_info> info(1024);
+    int64_t done = 0, errors = 0;
+    double first = 1e300, last = 0, t_progress = clock_s();
+    int rc = ORB_OK;
+    while (true) {
+        int k = 0;
+        rc = orb_server_poll(s, info.data(), (int)info.size(), &k);
+        if (rc) break;
+        for (int i = 0; i < k; i++) {
+            const orb_serve_info& o = info[i];
+            lat.push_back((o.t_done - o.t_arrival) * 1e3);
+            first = std::min(first, o.t_arrival);
+            last = std::max(last, o.t_done);
+            errors += o.status != ORB_OK;
+            (void)orb_server_release(s, o.ticket);
+        }
+        done += k;
+        if (k) t_progress = clock_s();
+        if (running.load() == 0 && done >= submitted.load()) break;
+        if (clock_s() - t_progress > 30.0) {
+            rc = ORB_E_DEVICE;
+            break;
+        }
+        if (!k) std::this_thread::sleep_for(std::chrono::microseconds(100));
+    }
+    if (rc) failed = rc;
+    for (std::thread& th : subs) th.join();
+    if (failed.load()) return failed.load();
+    out->requests = done;
+    out->refused = refused.load();
+    out->errors = errors;
+    if (!lat.empty()) {
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double q) { return lat[std::min(lat.size() - 1, (size_t)(q * (double)(lat.size() - 1) + 0.5))]; };
+        out->p50_ms = pct(0.50);
+        out->p99_ms = pct(0.99);
+        out->max_ms = lat.back();
+        out->frames_per_s = last > first ? (double)done / (last - first) : 0.0;
+    }
+    return ORB_OK;
 }
 
 int orb_server_stats(orb_server* s, int64_t* batches, int64_t* requests, size_t* hbm_used, int* slots)

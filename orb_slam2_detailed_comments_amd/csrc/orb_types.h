@@ -137,8 +137,9 @@ struct MatchArgs {
 // One region of a batch's staged host transfers, copied in 4-byte words:
 // `words` of them, or min(words, *count * unit_words) when count is set (a
 // slot's keypoint count, read on the device).  Source and destination are
-// 16-byte aligned; a region that is a whole number of 16-byte words is
-// copied 16 bytes at a time.
+// 4-byte aligned; a region that is a whole number of 16-byte words at
+// 16-byte aligned addresses is copied 16 bytes at a time.  Either side may
+// be pinned host memory, which the device reads and writes over PCIe.
 struct CopyOp {
     const unsigned* src;
     unsigned* dst;

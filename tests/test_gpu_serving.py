@@ -83,3 +83,20 @@ def test_poisson_run_16_streams(orb, frames):
     assert s["errors"] == 0 and s["requests"] == len(recs) > 300
     assert s["p99_ms"] < 100.0
     srv.close()
+
+
+def test_native_replay_16_streams(orb, frames):
+    """orb_server_replay: native submitters at 240 fps per stream for 0.5 s.
+    Every scheduled frame is served without error, the count matches the
+    schedule's, and the server's own request counter agrees."""
+    srv = orb.Server(STREAMS, max_inflight=3)
+    r = srv.replay(frames, fps=240.0, duration=0.5, seed=5)
+    assert r["errors"] == 0 and r["refused"] == 0
+    # 16 Poisson streams of mean 120 arrivals each: 1920 +- a few sqrt(1920)
+    assert 1700 < r["requests"] < 2150, r
+    assert 0 < r["p50_ms"] <= r["p99_ms"] <= r["max_ms"] < 1000.0
+    assert r["frames_per_s"] > 0
+    assert srv.stats()["requests"] == r["requests"]
+    with pytest.raises(orb.OrbError, match="bad argument"):
+        srv.replay(frames, fps=0.0, duration=0.5)
+    srv.close()
