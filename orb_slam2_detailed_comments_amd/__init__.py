@@ -167,6 +167,14 @@ def _ptr(a):
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
 
 
+def _view(owner, address, nbytes, dtype):
+    """A numpy view of `nbytes` at `address` (memory that `owner` holds); the
+    view keeps `owner` alive."""
+    buf = (ctypes.c_uint8 * nbytes).from_address(address)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype)
+
+
 class Context:
     """A HIP device and the stream the front end runs on."""
 
@@ -498,7 +506,7 @@ class Batch:
     def pinned_images(self):
         """The batch's pinned host image buffers, one (height, width) uint8
         view per slot (rows at the device pitch).  Write frames into them,
-        then upload_pinned().  The views are valid while the batch lives."""
+        then upload_pinned().  The views keep the batch alive."""
         n = len(self.sizes)
         ptrs = (ctypes.c_void_p * n)()
         pitches = np.zeros(n, np.uint64)
@@ -506,8 +514,7 @@ class Batch:
         out = []
         for s in range(n):
             w, h = self.sizes[s]
-            buf = (ctypes.c_uint8 * (int(pitches[s]) * h)).from_address(ptrs[s])
-            full = np.frombuffer(buf, np.uint8).reshape(h, int(pitches[s]))
+            full = _view(self, ptrs[s], int(pitches[s]) * h, np.uint8).reshape(h, int(pitches[s]))
             out.append(full[:, :w])
         return out
 
@@ -528,8 +535,8 @@ class Batch:
         k = n.value
         if k == 0:
             return np.zeros(0, KEYPOINT_DTYPE), np.zeros((0, 32), np.uint8)
-        kps = np.frombuffer((ctypes.c_uint8 * (k * KEYPOINT_DTYPE.itemsize)).from_address(kp.value), KEYPOINT_DTYPE)
-        desc = np.frombuffer((ctypes.c_uint8 * (k * 32)).from_address(de.value), np.uint8).reshape(k, 32)
+        kps = _view(self, kp.value, k * KEYPOINT_DTYPE.itemsize, KEYPOINT_DTYPE)
+        desc = _view(self, de.value, k * 32, np.uint8).reshape(k, 32)
         return kps, desc
 
     def pinned_stereo(self, pair):
@@ -540,8 +547,7 @@ class Batch:
         k = n.value
         if k == 0:
             return np.zeros(0, np.float32), np.zeros(0, np.float32)
-        return (np.frombuffer((ctypes.c_float * k).from_address(ur.value), np.float32),
-                np.frombuffer((ctypes.c_float * k).from_address(dp.value), np.float32))
+        return _view(self, ur.value, 4 * k, np.float32), _view(self, dp.value, 4 * k, np.float32)
 
     def set_stereo(self, left_slots, right_slots, bf, baseline):
         l = np.ascontiguousarray(left_slots, np.int32)
