@@ -488,6 +488,24 @@ int orb_search_by_projection_last_time(orb_context* ctx, int n_frames, const orb
                                        int check_orientation, int32_t* point_match, int32_t* key_point,
                                        int32_t* nmatches, int steps, float* ms);
 
+/* ---------------------------------------------------------------- Frame keypoint ops
+ * Frame::UndistortKeyPoints() (include/Frame.h, src/Frame.cc:724-776): the
+ * keypoints' cv::undistortPoints(pts, pts, mK, mDistCoef, Mat(), mK).
+ * K = fx, fy, cx, cy; D = nD (4 or 5) coefficients k1 k2 p1 p2 [k3], as
+ * mDistCoef holds them.  out receives the keypoints with x, y undistorted
+ * (every other field copied; out may be kps).  k1 == 0 copies, as the
+ * reference does (Frame.cc:727). */
+int orb_undistort_keypoints(orb_context* ctx, int n, const orb_keypoint* kps, const float* K, const float* D, int nD,
+                            orb_keypoint* out);
+/* Frame::ComputeStereoFromRGBD() (src/Frame.cc:1085-1123): per keypoint,
+ * d = depth at row (int)y, column (int)x of the keypoint as extracted (the
+ * float depth image of width x height, `pitch` bytes a row, as Tracking has
+ * scaled it); d > 0 gives depth_out = d and uRight = kps_un.x - bf / d, else
+ * both -1.  A keypoint outside the image (read out of bounds by the
+ * reference) gets -1. */
+int orb_rgbd_depth(orb_context* ctx, int n, const orb_keypoint* kps, const orb_keypoint* kps_un, const float* depth,
+                   int width, int height, size_t pitch, float bf, float* uRight, float* depth_out);
+
 /* ---------------------------------------------------------------- map points
  * MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450) for
  * n_points points at once.  Point p's observation descriptors are rows

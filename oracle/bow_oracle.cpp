@@ -464,6 +464,77 @@ int oracle_remap_linear(const uint8_t* src, int sw, int sh, int sstride, const f
 
 }  // extern "C"
 
+// ---------------------------------------------------------------- Frame keypoint ops
+// Frame::UndistortKeyPoints (src/Frame.cc:724-776) and
+// Frame::ComputeStereoFromRGBD (src/Frame.cc:1085-1123).  The first calls
+// cv::undistortPoints(pts, pts, K, D, Mat(), K) when k1 != 0, restated from
+// OpenCV 2.4/3.x's published cvUndistortPoints (absent here): in double,
+// x = (u - cx) * (1/fx), five fixed-point iterations
+//   icdist = 1 / (1 + ((k3 r2 + k2) r2 + k1) r2),
+//   x = (x0 - (2 p1 x y + p2 (r2 + 2 x^2))) icdist,
+//   y = (y0 - (p1 (r2 + 2 y^2) + 2 p2 x y)) icdist,
+// then the new camera matrix P = K (with R = I) maps back: u = fx x + cx, cast
+// to float.  (3.x's extra terms for k4-k6, s1-s4 and the tilt are exact zeros
+// and ones for a 4- or 5-coefficient model, so both versions give these bits.)
+extern "C" {
+
+// xy: n (x, y) pairs; K4 = fx, fy, cx, cy; D: nD = 4 or 5 coefficients
+// k1 k2 p1 p2 [k3].  k1 == 0 copies, as the reference does.
+int oracle_undistort_points(const float* xy, int n, const float* K4, const float* D, int nD, float* out)
+{
+    if (!(nD == 4 || nD == 5) || n < 0) return -1;
+    if (D[0] == 0.0f) {
+        for (int i = 0; i < 2 * n; i++) out[i] = xy[i];
+        return 0;
+    }
+    double k[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < nD; i++) k[i] = D[i];
+    const double fx = K4[0], fy = K4[1], cx = K4[2], cy = K4[3];
+    const double ifx = 1. / fx, ify = 1. / fy;
+    for (int i = 0; i < n; i++) {
+        double x = xy[2 * i], y = xy[2 * i + 1];
+        x = (x - cx) * ifx;
+        y = (y - cy) * ify;
+        const double x0 = x, y0 = y;
+        for (int j = 0; j < 5; j++) {
+            const double r2 = x * x + y * y;
+            const double icdist = 1. / (1 + ((k[4] * r2 + k[1]) * r2 + k[0]) * r2);
+            const double deltaX = 2 * k[2] * x * y + k[3] * (r2 + 2 * x * x);
+            const double deltaY = k[2] * (r2 + 2 * y * y) + 2 * k[3] * x * y;
+            x = (x0 - deltaX) * icdist;
+            y = (y0 - deltaY) * icdist;
+        }
+        out[2 * i] = (float)(fx * x + cx);
+        out[2 * i + 1] = (float)(fy * y + cy);
+    }
+    return 0;
+}
+
+// xy: the keypoints as extracted, xun their undistorted x; depth: h rows of
+// w floats, `stride` floats apart.  imDepth.at<float>(v, u) takes the float
+// coordinates as ints (truncation).  A keypoint outside the image (which the
+// reference would read out of bounds) gets -1.
+int oracle_rgbd_depth(const float* xy, const float* xun, int n, const float* depth, int w, int h, int stride,
+                      float bf, float* uR, float* dep)
+{
+    for (int i = 0; i < n; i++) {
+        uR[i] = -1;
+        dep[i] = -1;
+        const float v = xy[2 * i + 1], u = xy[2 * i];
+        if (!(v >= 0 && u >= 0)) continue;
+        const int r = (int)v, c = (int)u;
+        if (r >= h || c >= w) continue;
+        const float d = depth[(size_t)r * stride + c];
+        if (d > 0) {
+            dep[i] = d;
+            uR[i] = xun[i] - bf / d;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"
+
 // ---------------------------------------------------------------- windowed matching
 // ORBmatcher::SearchForInitialization (src/ORBmatcher.cc:573-717) with the
 // keypoint grid of Frame::AssignFeaturesToGrid / PosInGrid /

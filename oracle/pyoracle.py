@@ -66,6 +66,8 @@ def lib():
             "oracle_cvt_gray": (I, [P, I, I, I, I, I, P]),
             "oracle_init_rectify_map": (I, [P, P, I, P, P, I, I, P, P]),
             "oracle_remap_linear": (I, [P, I, I, I, P, P, I, I, P]),
+            "oracle_undistort_points": (I, [P, I, P, P, I, P]),
+            "oracle_rgbd_depth": (I, [P, P, I, P, I, I, I, F, P, P]),
             "oracle_search_for_initialization": (I, [P, P, I, P, P, I, P, P, I, F, I, P]),
             "oracle_search_by_projection": (I, [P, P, P, P, I, P, P, P, P, I, F, F, P, P]),
             "oracle_search_by_projection_last": (I, [P, P, P, P, I, P, P, P, I, P, P, I, F, I, P, P]),
@@ -485,3 +487,27 @@ def search_for_initialization(kp1, d1, kp2, d2, bounds, prev, window=100, nnrati
     n = lib().oracle_search_for_initialization(_p(kp1), _p(d1), len(kp1), _p(kp2), _p(d2), len(kp2), _p(b), _p(pv),
                                                int(window), float(nnratio), int(bool(check_orientation)), _p(m))
     return n, m[:len(kp1)], pv
+
+
+def undistort_points(xy, K, D):
+    """Frame::UndistortKeyPoints' cv::undistortPoints (R = I, P = K) on an
+    (n, 2) float32 array; K = (fx, fy, cx, cy), D = 4 or 5 coefficients."""
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    K = np.ascontiguousarray(K, np.float32)
+    D = np.ascontiguousarray(D, np.float32)
+    out = np.empty_like(xy)
+    if lib().oracle_undistort_points(_p(xy), len(xy), _p(K), _p(D), len(D), _p(out)) != 0:
+        raise ValueError("bad coefficients")
+    return out
+
+
+def rgbd_depth(xy, xun, depth, bf):
+    """Frame::ComputeStereoFromRGBD: (uRight, depth) per keypoint."""
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    xun = np.ascontiguousarray(xun, np.float32)
+    depth = np.ascontiguousarray(depth, np.float32)
+    uR = np.empty(len(xy), np.float32)
+    dep = np.empty(len(xy), np.float32)
+    lib().oracle_rgbd_depth(_p(xy), _p(xun), len(xy), _p(depth), depth.shape[1], depth.shape[0], depth.shape[1],
+                            float(bf), _p(uR), _p(dep))
+    return uR, dep

@@ -25,6 +25,7 @@ import os
 import numpy as np
 
 __all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "Vocabulary", "match_bf",
+           "undistort_keypoints", "rgbd_depth",
            "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "cvt_color_gray", "Rectifier", "search_for_initialization", "lib", "library_path",
            "OrbError"]
 
@@ -110,6 +111,8 @@ SIGNATURES = {
     "orb_server_drain": (_I, [_P, ctypes.c_double]),
     "orb_server_stats": (_I, [_P, _P, _P, _P, _P]),
     "orb_server_replay": (_I, [_P, _I, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _P]),
+    "orb_undistort_keypoints": (_I, [_P, _I, _P, _P, _P, _I, _P]),
+    "orb_rgbd_depth": (_I, [_P, _I, _P, _P, _P, _I, _I, _S, _F, _P, _P]),
     "orb_vocabulary_load_text": (_I, [_P, ctypes.c_char_p, _P]),
     "orb_vocabulary_create": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "orb_vocabulary_destroy": (_V, [_P]),
@@ -727,6 +730,38 @@ class Server:
         _check("orb_server_stats", lib().orb_server_stats(self.h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(u),
                                                           ctypes.byref(sl)))
         return dict(batches=b.value, requests=r.value, hbm_used=u.value, slots=sl.value)
+
+
+def undistort_keypoints(kps, K, D, context=None):
+    """Frame::UndistortKeyPoints: keypoints (KEYPOINT_DTYPE) with x, y put
+    through cv::undistortPoints(K, D, R = I, P = K); K = (fx, fy, cx, cy),
+    D = 4 or 5 coefficients k1 k2 p1 p2 [k3]."""
+    ctx = context or Context.default()
+    kps = np.ascontiguousarray(kps, KEYPOINT_DTYPE)
+    K = np.ascontiguousarray(K, np.float32)
+    D = np.ascontiguousarray(D, np.float32)
+    if K.shape != (4,):
+        raise ValueError("K is (fx, fy, cx, cy)")
+    out = np.empty_like(kps)
+    _check("orb_undistort_keypoints", lib().orb_undistort_keypoints(ctx.h, len(kps), _ptr(kps), _ptr(K), _ptr(D),
+                                                                    len(D), _ptr(out)))
+    return out
+
+
+def rgbd_depth(kps, kps_un, depth, bf, context=None):
+    """Frame::ComputeStereoFromRGBD: (uRight, depth) per keypoint from a
+    float32 depth image."""
+    ctx = context or Context.default()
+    kps = np.ascontiguousarray(kps, KEYPOINT_DTYPE)
+    kun = np.ascontiguousarray(kps_un, KEYPOINT_DTYPE)
+    if len(kun) != len(kps):
+        raise ValueError("kps and kps_un differ in length")
+    depth = np.ascontiguousarray(depth, np.float32)
+    uR = np.empty(len(kps), np.float32)
+    dep = np.empty(len(kps), np.float32)
+    _check("orb_rgbd_depth", lib().orb_rgbd_depth(ctx.h, len(kps), _ptr(kps), _ptr(kun), _ptr(depth), depth.shape[1],
+                                                  depth.shape[0], depth.strides[0], float(bf), _ptr(uR), _ptr(dep)))
+    return uR, dep
 
 
 def cvt_color_gray(image, rgb=True, context=None):

@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "liborb_gpu.so")
 SOURCES = ["orb_kernels.hip", "orb_bow.hip", "orb_mappoint.hip", "orb_color.hip", "orb_winmatch.hip",
-           "orb_runtime.cpp", "orb_serve.cpp", "orb_vocabulary.cpp", "orb_rectify.cpp", "orb_winmatch.cpp"]
+           "orb_frame.hip", "orb_runtime.cpp", "orb_serve.cpp", "orb_vocabulary.cpp", "orb_rectify.cpp", "orb_winmatch.cpp"]
 ARCH = os.environ.get("ORB_OFFLOAD_ARCH", "gfx950")
 
 

@@ -51,7 +51,7 @@ void orb_context_release_ref(orb_context* ctx)
     if (ctx->d_kq) (void)hipFree(ctx->d_kq);
     if (ctx->d_km) (void)hipFree(ctx->d_km);
     if (ctx->d_kout) (void)hipFree(ctx->d_kout);
-    void* mp[] = {ctx->d_mp, ctx->d_mpo, ctx->d_mpb, ctx->d_mpd, ctx->d_color, ctx->d_win, ctx->d_wcand};
+    void* mp[] = {ctx->d_mp, ctx->d_mpo, ctx->d_mpb, ctx->d_mpd, ctx->d_color, ctx->d_win, ctx->d_wcand, ctx->d_fr};
     for (void* q : mp)
         if (q) (void)hipFree(q);
     (void)hipStreamDestroy(ctx->stream);
@@ -1407,6 +1407,60 @@ static int distinctive_stage(orb_context* ctx, int n_points, const uint8_t* desc
 }
 
 extern "C" {
+
+int orb_undistort_keypoints(orb_context* ctx, int n, const orb_keypoint* kps, const float* K, const float* D, int nD,
+                            orb_keypoint* out)
+{
+    if (!ctx || n < 0 || !K || !D || (nD != 4 && nD != 5) || (n && (!kps || !out))) return ORB_E_ARG;
+    if (n == 0) return ORB_OK;
+    if (D[0] == 0.0f) {                       // Frame.cc:727: k1 == 0, no distortion
+        if (out != kps) std::memmove(out, kps, (size_t)n * sizeof(orb_keypoint));
+        return ORB_OK;
+    }
+    if (!(K[0] != 0.0f) || !(K[1] != 0.0f)) return ORB_E_ARG;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)n * sizeof(orb_keypoint);
+    int rc = ctx->grow(&ctx->d_fr, &ctx->fr_cap, 2 * bytes);
+    if (rc) return rc;
+    OrbKeyPoint* d_in = (OrbKeyPoint*)ctx->d_fr;
+    OrbKeyPoint* d_out = d_in + n;
+    const double K4[4] = {K[0], K[1], K[2], K[3]};
+    double k5[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < nD; i++) k5[i] = D[i];
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(d_in, kps, bytes, hipMemcpyHostToDevice, st));
+    HIP_TRY(orb_launch_undistort(d_in, n, K4, k5, d_out, st));
+    HIP_TRY(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return ORB_OK;
+}
+
+int orb_rgbd_depth(orb_context* ctx, int n, const orb_keypoint* kps, const orb_keypoint* kps_un, const float* depth,
+                   int width, int height, size_t pitch, float bf, float* uRight, float* depth_out)
+{
+    if (!ctx || n < 0 || width <= 0 || height <= 0 || pitch < (size_t)width * 4 || pitch % 4 || !depth ||
+        (n && (!kps || !kps_un || !uRight || !depth_out)))
+        return ORB_E_ARG;
+    if (n == 0) return ORB_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t kb = (size_t)n * sizeof(orb_keypoint), ib = (size_t)width * 4 * height;
+    const size_t o_un = kb, o_img = 2 * kb, o_out = 2 * kb + (ib + 255) / 256 * 256;
+    int rc = ctx->grow(&ctx->d_fr, &ctx->fr_cap, o_out + (size_t)n * 8);
+    if (rc) return rc;
+    char* m = (char*)ctx->d_fr;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(m, kps, kb, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m + o_un, kps_un, kb, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpy2DAsync(m + o_img, (size_t)width * 4, depth, pitch, (size_t)width * 4, height,
+                             hipMemcpyHostToDevice, st));
+    float* d_uR = (float*)(m + o_out);
+    HIP_TRY(orb_launch_rgbd_depth((const OrbKeyPoint*)m, (const OrbKeyPoint*)(m + o_un), n, (const float*)(m + o_img),
+                                  width, height, (size_t)width * 4, bf, d_uR, d_uR + n, st));
+    HIP_TRY(hipMemcpyAsync(uRight, d_uR, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(depth_out, d_uR + n, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return ORB_OK;
+}
 
 int orb_distinctive_descriptors(orb_context* ctx, int n_points, const uint8_t* desc, const int32_t* offsets,
                                 int32_t* best, uint8_t* out_desc)
