@@ -166,6 +166,45 @@ def mappoint_extra(orb, ctx, steps, cpu_seconds):
     return out
 
 
+def frame_ops_extra(orb, ctx, steps, cpu_seconds):
+    """The RGB-D Frame constructor's per-keypoint steps after the extraction:
+    UndistortKeyPoints (TUM fr1's 5-coefficient model) and
+    ComputeStereoFromRGBD on a 640x480 float depth map, 2000 keypoints,
+    host arrays in and out per call."""
+    from orb_slam2_detailed_comments_amd import KEYPOINT_DTYPE
+    rng = np.random.default_rng(5)
+    n, w, h = 2000, 640, 480
+    kps = np.zeros(n, KEYPOINT_DTYPE)
+    kps["x"] = (rng.random(n) * w).astype(np.float32)
+    kps["y"] = (rng.random(n) * h).astype(np.float32)
+    depth = (rng.random((h, w)) * 4 + 0.3).astype(np.float32)
+    K = (517.306408, 516.469215, 318.643040, 255.313989)
+    D = (0.262383, -0.953104, -0.005358, 0.002628, 1.163314)
+    for _ in range(3):
+        kun = orb.undistort_keypoints(kps, K, D, context=ctx)
+        orb.rgbd_depth(kps, kun, depth, 40.0, context=ctx)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kun = orb.undistort_keypoints(kps, K, D, context=ctx)
+        orb.rgbd_depth(kps, kun, depth, 40.0, context=ctx)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = {"workload": "UndistortKeyPoints + ComputeStereoFromRGBD, 2000 keypoints, 640x480 depth, host arrays",
+           "ms_per_frame": round(ms, 4), "frames_per_s": round(1e3 / ms, 1)}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        xy = np.stack([kps["x"], kps["y"]], 1)
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            un = pyoracle.undistort_points(xy, K, D)
+            pyoracle.rgbd_depth(xy, un[:, 0], depth, 40.0)
+            k += 1
+        cpu = k / (time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": round(cpu, 1), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} frames through the oracle in ~{cpu_seconds:.0f}s"}
+    return out
+
+
 def winmatch_extra(orb, ctx, cpu_seconds):
     """SearchForInitialization (window 100, ratio 0.9, orientation check) for
     32 frame pairs of 640x480 with the initializer's 2000 features, one call:
@@ -481,7 +520,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip every extra workload")
     ap.add_argument("--extras", default="all",
-                    help="comma-separated extras to run: knn, batch, serving, bow, mappoint, winmatch, projection")
+                    help="comma-separated extras to run: knn, batch, serving, bow, mappoint, frame, winmatch, "
+                         "projection")
     ap.add_argument("--tiles", default="auto",
                     help="level kernel tile heights: auto (the tile autotuner), 32, 16, or 8 comma-separated values")
     ap.add_argument("--branches", type=int, default=0,
@@ -688,7 +728,7 @@ def main():
                       "mean_stereo_matches": round(float(np.mean(matched)), 1)},
            "roofline": roofline, "host_to_host": host_to_host}
     want = set() if args.no_extras else (
-        {"knn", "batch", "serving", "bow", "mappoint", "winmatch", "projection", "hbm"} if args.extras == "all"
+        {"knn", "batch", "serving", "bow", "mappoint", "frame", "winmatch", "projection", "hbm"} if args.extras == "all"
         else set(args.extras.split(",")))
     cpu_s = 0.0 if args.no_cpu else min(args.cpu_seconds, 3.0)
     if "hbm" in want:
@@ -714,6 +754,9 @@ def main():
     if "mappoint" in want:
         print("[bench] map-point descriptors", file=sys.stderr, flush=True)
         out["distinctive_descriptors"] = mappoint_extra(orb, ctx, max(args.steps, 5), cpu_s)
+    if "frame" in want:
+        print("[bench] frame keypoint ops", file=sys.stderr, flush=True)
+        out["frame_ops"] = frame_ops_extra(orb, ctx, max(args.steps, 20), min(cpu_s, 3.0))
     if "winmatch" in want:
         print("[bench] windowed matching", file=sys.stderr, flush=True)
         out["search_for_initialization"] = winmatch_extra(orb, ctx, cpu_s)
