@@ -15,6 +15,10 @@
 //       include/ORBmatcher.h:65
 //   ComputeStereoMatches(left, right, ...)   Frame::ComputeStereoMatches,
 //       include/Frame.h:249
+//   UndistortKeyPoints(ctx, mvKeys, K, mDistCoef, mvKeysUn)
+//       Frame::UndistortKeyPoints, src/Frame.cc:724-776
+//   ComputeStereoFromRGBD(ctx, mvKeys, mvKeysUn, depth, mbf, ...)
+//       Frame::ComputeStereoFromRGBD, src/Frame.cc:1085-1123
 //
 // With OpenCV present, cv::Mat / cv::KeyPoint convert directly: orb_keypoint
 // has cv::KeyPoint's layout, and a CV_8UC1 Mat is (data, cols, rows, step).
@@ -140,6 +144,29 @@ inline void ComputeStereoMatches(ORBextractor& left, ORBextractor& right, const 
     check("orb_stereo_match", orb_stereo_match(left.get(), right.get(), kl.data(), dl.data(), (int)kl.size(),
                                                kr.data(), dr.data(), (int)kr.size(), bf, b, mvuRight.data(),
                                                mvDepth.data()));
+}
+
+// Frame::UndistortKeyPoints: K = fx, fy, cx, cy; D = mDistCoef's 4 or 5
+// coefficients (k1 k2 p1 p2 [k3]); k1 == 0 copies.
+inline void UndistortKeyPoints(Context& ctx, const std::vector<orb_keypoint>& mvKeys, const float K[4],
+                               const std::vector<float>& D, std::vector<orb_keypoint>& mvKeysUn)
+{
+    mvKeysUn.resize(mvKeys.size());
+    check("orb_undistort_keypoints", orb_undistort_keypoints(ctx.get(), (int)mvKeys.size(), mvKeys.data(), K,
+                                                             D.data(), (int)D.size(), mvKeysUn.data()));
+}
+
+// Frame::ComputeStereoFromRGBD on a float depth image (CV_32F, as Tracking
+// scales it): mvuRight / mvDepth get one entry per keypoint (-1 without depth).
+inline void ComputeStereoFromRGBD(Context& ctx, const std::vector<orb_keypoint>& mvKeys,
+                                  const std::vector<orb_keypoint>& mvKeysUn, const float* depth, int width,
+                                  int height, size_t stride, float mbf, std::vector<float>& mvuRight,
+                                  std::vector<float>& mvDepth)
+{
+    mvuRight.assign(mvKeys.size(), -1.0f);
+    mvDepth.assign(mvKeys.size(), -1.0f);
+    check("orb_rgbd_depth", orb_rgbd_depth(ctx.get(), (int)mvKeys.size(), mvKeys.data(), mvKeysUn.data(), depth, width,
+                                           height, stride, mbf, mvuRight.data(), mvDepth.data()));
 }
 
 }  // namespace orbgpu

@@ -756,7 +756,9 @@ def rgbd_depth(kps, kps_un, depth, bf, context=None):
     kun = np.ascontiguousarray(kps_un, KEYPOINT_DTYPE)
     if len(kun) != len(kps):
         raise ValueError("kps and kps_un differ in length")
-    depth = np.ascontiguousarray(depth, np.float32)
+    depth = np.asarray(depth, np.float32)
+    if depth.ndim != 2 or depth.strides[1] != 4 or depth.strides[0] < 4 * depth.shape[1]:
+        depth = np.ascontiguousarray(depth)       # rows may be strided, pixels not
     uR = np.empty(len(kps), np.float32)
     dep = np.empty(len(kps), np.float32)
     _check("orb_rgbd_depth", lib().orb_rgbd_depth(ctx.h, len(kps), _ptr(kps), _ptr(kun), _ptr(depth), depth.shape[1],

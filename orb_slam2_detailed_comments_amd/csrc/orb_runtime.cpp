@@ -1443,7 +1443,10 @@ int orb_rgbd_depth(orb_context* ctx, int n, const orb_keypoint* kps, const orb_k
         return ORB_E_ARG;
     if (n == 0) return ORB_OK;
     HIP_TRY(hipSetDevice(ctx->device));
-    const size_t kb = (size_t)n * sizeof(orb_keypoint), ib = (size_t)width * 4 * height;
+    // The image goes over as one linear copy of its rows at the caller's
+    // pitch, which the kernel then uses (a pitched copy from pageable memory
+    // would go row by row).
+    const size_t kb = (size_t)n * sizeof(orb_keypoint), ib = pitch * (height - 1) + (size_t)width * 4;
     const size_t o_un = kb, o_img = 2 * kb, o_out = 2 * kb + (ib + 255) / 256 * 256;
     int rc = ctx->grow(&ctx->d_fr, &ctx->fr_cap, o_out + (size_t)n * 8);
     if (rc) return rc;
@@ -1451,11 +1454,10 @@ int orb_rgbd_depth(orb_context* ctx, int n, const orb_keypoint* kps, const orb_k
     hipStream_t st = ctx->stream;
     HIP_TRY(hipMemcpyAsync(m, kps, kb, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(m + o_un, kps_un, kb, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpy2DAsync(m + o_img, (size_t)width * 4, depth, pitch, (size_t)width * 4, height,
-                             hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m + o_img, depth, ib, hipMemcpyHostToDevice, st));
     float* d_uR = (float*)(m + o_out);
     HIP_TRY(orb_launch_rgbd_depth((const OrbKeyPoint*)m, (const OrbKeyPoint*)(m + o_un), n, (const float*)(m + o_img),
-                                  width, height, (size_t)width * 4, bf, d_uR, d_uR + n, st));
+                                  width, height, pitch, bf, d_uR, d_uR + n, st));
     HIP_TRY(hipMemcpyAsync(uRight, d_uR, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(depth_out, d_uR + n, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));

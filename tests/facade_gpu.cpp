@@ -2,7 +2,8 @@
 // tests/test_gpu_facade.py: the C++ facade (orb_gpu.hpp) on the device, the
 // way Frame's stereo constructor uses ORB-SLAM2's classes (Frame.cc:146-169):
 // two extractors, operator() on the left and right images, then
-// ComputeStereoMatches; also the pyramid getter, the scale getters, the
+// ComputeStereoMatches, UndistortKeyPoints and ComputeStereoFromRGBD; also
+// the pyramid getter, the scale getters, the
 // empty-image convention and the error channel.
 //
 // usage: facade_gpu left.raw right.raw width height nfeatures bf b outdir
@@ -57,6 +58,19 @@ int main(int argc, char** argv)
     std::vector<float> ur, depth;
     orbgpu::ComputeStereoMatches(left, right, kl, dl, kr, dr, bf, b, ur, depth);
 
+    // The RGB-D constructor's steps on the left keypoints: TUM fr1's model,
+    // and a depth ramp with a hole (the test restates both).
+    const float K[4] = {517.306408f, 516.469215f, 318.643040f, 255.313989f};
+    const std::vector<float> D = {0.262383f, -0.953104f, -0.005358f, 0.002628f, 1.163314f};
+    std::vector<orb_keypoint> kun;
+    orbgpu::UndistortKeyPoints(ctx, kl, K, D, kun);
+    std::vector<float> dimg((size_t)w * h);
+    for (int r = 0; r < h; r++)
+        for (int c = 0; c < w; c++)
+            dimg[(size_t)r * w + c] = (r >= 100 && r < 160 && c >= 300 && c < 380) ? 0.0f : 1.0f + (float)(r + c) / 400.0f;
+    std::vector<float> rur, rdep;
+    orbgpu::ComputeStereoFromRGBD(ctx, kl, kun, dimg.data(), w, h, (size_t)w * 4, bf, rur, rdep);
+
     // mvImagePyramid of the left extractor, levels concatenated
     std::vector<uint8_t> pyr;
     for (int l = 0; l < left.GetLevels(); l++) {
@@ -77,6 +91,9 @@ int main(int argc, char** argv)
     write_file(out + "/depth.bin", depth.data(), depth.size() * sizeof(float));
     write_file(out + "/pyr.bin", pyr.data(), pyr.size());
     write_file(out + "/scales.bin", scales.data(), scales.size() * sizeof(float));
+    write_file(out + "/kun.bin", kun.data(), kun.size() * sizeof(orb_keypoint));
+    write_file(out + "/rgbd_ur.bin", rur.data(), rur.size() * sizeof(float));
+    write_file(out + "/rgbd_depth.bin", rdep.data(), rdep.size() * sizeof(float));
 
     // An empty image returns with the outputs untouched (ORBextractor.cc:1537).
     std::vector<orb_keypoint> k0 = kl;

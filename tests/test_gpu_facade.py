@@ -1,8 +1,9 @@
 """GPU tier: the C++ facade (include/orb_gpu.hpp) on the device.
 
 tests/_bin/facade_gpu (built by build() from tests/facade_gpu.cpp) runs
-orbgpu::ORBextractor::operator() on a stereo pair, ComputeStereoMatches and
-the pyramid and scale getters, the way Frame's stereo constructor uses the
+orbgpu::ORBextractor::operator() on a stereo pair, ComputeStereoMatches,
+UndistortKeyPoints and ComputeStereoFromRGBD, and the pyramid and scale
+getters, the way Frame's stereo constructor uses the
 reference's classes (Frame.cc:146-169).  Its outputs must equal the oracle's
 bit for bit."""
 import os
@@ -59,6 +60,19 @@ def test_facade_stereo_frame_equals_oracle(oracle, tmp_path, w, h, nf, seed):
         np.testing.assert_array_equal(got, want, err_msg=f"level {lvl}")
         off += want.size
     assert off == pyr.size
+    # UndistortKeyPoints and ComputeStereoFromRGBD through the facade
+    K = (517.306408, 516.469215, 318.643040, 255.313989)
+    D = (0.262383, -0.953104, -0.005358, 0.002628, 1.163314)
+    kun = rd("kun.bin", KEYPOINT_DTYPE)
+    oun = oracle.undistort_points(np.stack([okl["x"], okl["y"]], 1), K, D)
+    assert kun["x"].tobytes() == oun[:, 0].tobytes() and kun["y"].tobytes() == oun[:, 1].tobytes()
+    r, c = np.arange(h)[:, None], np.arange(w)[None, :]
+    dimg = (np.float32(1.0) + (r + c).astype(np.float32) / np.float32(400.0)).astype(np.float32)
+    dimg[100:160, 300:380] = 0.0
+    our_rgbd, odep_rgbd = oracle.rgbd_depth(np.stack([okl["x"], okl["y"]], 1), oun[:, 0], dimg, float(bf))
+    assert rd("rgbd_ur.bin", np.float32).tobytes() == our_rgbd.tobytes()
+    assert rd("rgbd_depth.bin", np.float32).tobytes() == odep_rgbd.tobytes()
+
     sc = rd("scales.bin", np.float32).reshape(4, 8)
     tables = L.tables()
     for i, k in enumerate(("scale", "inv_scale", "sigma2", "inv_sigma2")):
