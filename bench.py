@@ -190,6 +190,20 @@ def frame_ops_extra(orb, ctx, steps, cpu_seconds):
     ms = (time.perf_counter() - t0) * 1e3 / steps
     out = {"workload": "UndistortKeyPoints + ComputeStereoFromRGBD, 2000 keypoints, 640x480 depth, host arrays",
            "ms_per_frame": round(ms, 4), "frames_per_s": round(1e3 / ms, 1)}
+    # The whole RGB-D Frame hot path in one call (orb_rgbd_frame): a 640x480
+    # frame extracted with 1000 features, then the two steps on the device.
+    from orb_slam2_detailed_comments_amd import synth
+    img = synth.shapes_frame(640, 480, 8)
+    ex = orb.ORBextractor(1000, 1.2, 8, 20, 7, context=ctx)
+    for _ in range(3):
+        orb.rgbd_frame(ex, img, depth, K, D, 40.0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orb.rgbd_frame(ex, img, depth, K, D, 40.0)
+    ms_f = (time.perf_counter() - t0) * 1e3 / steps
+    out["rgbd_frame"] = {"workload": "orb_rgbd_frame: 640x480 extraction (1000 features) + UndistortKeyPoints + "
+                                     "ComputeStereoFromRGBD, host image and depth in, results out",
+                         "ms_per_frame": round(ms_f, 4), "frames_per_s": round(1e3 / ms_f, 1)}
     if cpu_seconds > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
@@ -202,6 +216,17 @@ def frame_ops_extra(orb, ctx, steps, cpu_seconds):
         cpu = k / (time.perf_counter() - t0)
         out["cpu_baseline"] = {"value": round(cpu, 1), "unit": "frames/s", "cores": 1, "kind": "port",
                                "sample": f"{k} frames through the oracle in ~{cpu_seconds:.0f}s"}
+        oex = pyoracle.OracleExtractor(1000, 1.2, 8, 20, 7)
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            okps, _ = oex(img)
+            oxy = np.stack([okps["x"], okps["y"]], 1)
+            oun = pyoracle.undistort_points(oxy, K, D)
+            pyoracle.rgbd_depth(oxy, oun[:, 0], depth, 40.0)
+            k += 1
+        cpu = k / (time.perf_counter() - t0)
+        out["rgbd_frame"]["cpu_baseline"] = {"value": round(cpu, 1), "unit": "frames/s", "cores": 1, "kind": "port",
+                                             "sample": f"{k} frames through the oracle in ~{cpu_seconds:.0f}s"}
     return out
 
 

@@ -505,6 +505,16 @@ int orb_undistort_keypoints(orb_context* ctx, int n, const orb_keypoint* kps, co
  * reference) gets -1. */
 int orb_rgbd_depth(orb_context* ctx, int n, const orb_keypoint* kps, const orb_keypoint* kps_un, const float* depth,
                    int width, int height, size_t pitch, float bf, float* uRight, float* depth_out);
+/* The RGB-D Frame constructor's hot path in one call (src/Frame.cc:221-263:
+ * ExtractORB, UndistortKeyPoints, ComputeStereoFromRGBD): the extraction as
+ * orb_extract, then the undistortion and the depth lookup on the keypoints
+ * still on the device, one wait at the end.  keypoints / descriptors as
+ * orb_extract; keypoints_un, uRight and depth_out (each optional) receive
+ * *n entries.  depth: width x height floats, depth_pitch bytes a row. */
+int orb_rgbd_frame(orb_extractor* ex, const uint8_t* image, int width, int height, size_t stride, const float* depth,
+                   size_t depth_pitch, const float* K, const float* D, int nD, float bf, orb_keypoint* keypoints,
+                   orb_keypoint* keypoints_un, int capacity, uint8_t* descriptors, float* uRight, float* depth_out,
+                   int* n);
 
 /* ---------------------------------------------------------------- map points
  * MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cc:365-450) for

@@ -25,7 +25,7 @@ import os
 import numpy as np
 
 __all__ = ["KEYPOINT_DTYPE", "Context", "ORBextractor", "ORBmatcher", "Batch", "Server", "Vocabulary", "match_bf",
-           "undistort_keypoints", "rgbd_depth",
+           "undistort_keypoints", "rgbd_depth", "rgbd_frame",
            "knn2", "compute_stereo_matches", "stereo_frame", "distinctive_descriptors", "cvt_color_gray", "Rectifier", "search_for_initialization", "lib", "library_path",
            "OrbError"]
 
@@ -113,6 +113,7 @@ SIGNATURES = {
     "orb_server_replay": (_I, [_P, _I, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _P]),
     "orb_undistort_keypoints": (_I, [_P, _I, _P, _P, _P, _I, _P]),
     "orb_rgbd_depth": (_I, [_P, _I, _P, _P, _P, _I, _I, _S, _F, _P, _P]),
+    "orb_rgbd_frame": (_I, [_P, _P, _I, _I, _S, _P, _S, _P, _P, _I, _F, _P, _P, _I, _P, _P, _P, _P]),
     "orb_vocabulary_load_text": (_I, [_P, ctypes.c_char_p, _P]),
     "orb_vocabulary_create": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "orb_vocabulary_destroy": (_V, [_P]),
@@ -436,6 +437,34 @@ def stereo_frame(left, right, im_left, im_right, bf, b):
                                                      _ptr(dep)))
     n = nl.value
     return kl[:n].copy(), dl[:n].copy(), kr[:nr.value].copy(), dr[:nr.value].copy(), uR[:n].copy(), dep[:n].copy()
+
+
+def rgbd_frame(extractor, image, depth, K, D, bf):
+    """The RGB-D Frame constructor's hot path in one device pass: extraction,
+    UndistortKeyPoints and ComputeStereoFromRGBD.
+
+    Returns (keypoints, descriptors, keypoints_un, mvuRight, mvDepth)."""
+    img = np.ascontiguousarray(image, np.uint8)
+    dep_img = np.asarray(depth, np.float32)
+    if dep_img.ndim != 2 or dep_img.strides[1] != 4 or dep_img.strides[0] < 4 * dep_img.shape[1]:
+        dep_img = np.ascontiguousarray(dep_img)
+    h, w = img.shape
+    if dep_img.shape != (h, w):
+        raise ValueError("depth must have the image's size")
+    K = np.ascontiguousarray(K, np.float32)
+    D = np.ascontiguousarray(D, np.float32)
+    cap = extractor.capacity(w, h)
+    kps = np.zeros(cap, KEYPOINT_DTYPE)
+    kun = np.zeros(cap, KEYPOINT_DTYPE)
+    desc = np.zeros((cap, 32), np.uint8)
+    uR = np.zeros(cap, np.float32)
+    dep = np.zeros(cap, np.float32)
+    n = ctypes.c_int()
+    _check("orb_rgbd_frame", lib().orb_rgbd_frame(extractor.h, _ptr(img), w, h, img.strides[0], _ptr(dep_img),
+                                                  dep_img.strides[0], _ptr(K), _ptr(D), len(D), float(bf), _ptr(kps),
+                                                  _ptr(kun), cap, _ptr(desc), _ptr(uR), _ptr(dep), ctypes.byref(n)))
+    k = n.value
+    return kps[:k].copy(), desc[:k].copy(), kun[:k].copy(), uR[:k].copy(), dep[:k].copy()
 
 
 class Batch:

@@ -11,12 +11,13 @@
 // iterations x = (x0 - deltaX) * icdist, then u = fx * x + cx (P = K, R = I),
 // cast to float.  Built with contraction off, so every product and sum rounds
 // as written.  k = k1 k2 p1 p2 k3 (k3 = 0 for a 4-coefficient model).
-__global__ __launch_bounds__(256) void undistort_kernel(const OrbKeyPoint* __restrict__ in, int n, double fx,
+__global__ __launch_bounds__(256) void undistort_kernel(const OrbKeyPoint* __restrict__ in, int n,
+                                                        const int* __restrict__ count, double fx,
                                                         double fy, double cx, double cy, double k0, double k1,
                                                         double k2, double k3, double k4, OrbKeyPoint* __restrict__ out)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (count ? min(n, *count) : n)) return;
     OrbKeyPoint kp = in[i];
     const double ifx = 1. / fx, ify = 1. / fy;
     double x = ((double)kp.x - cx) * ifx;
@@ -40,12 +41,13 @@ __global__ __launch_bounds__(256) void undistort_kernel(const OrbKeyPoint* __res
 // (float), else both -1.  Keypoints outside the image get -1.
 __global__ __launch_bounds__(256) void rgbd_depth_kernel(const OrbKeyPoint* __restrict__ kps,
                                                          const OrbKeyPoint* __restrict__ kun, int n,
+                                                         const int* __restrict__ count,
                                                          const float* __restrict__ depth, int w, int h,
                                                          size_t pitch, float bf, float* __restrict__ uR,
                                                          float* __restrict__ dep)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    if (i >= (count ? min(n, *count) : n)) return;
     float r_out = -1.f, d_out = -1.f;
     const float u = kps[i].x, v = kps[i].y;
     if (v >= 0.f && u >= 0.f) {
@@ -62,19 +64,22 @@ __global__ __launch_bounds__(256) void rgbd_depth_kernel(const OrbKeyPoint* __re
     dep[i] = d_out;
 }
 
-extern "C" hipError_t orb_launch_undistort(const OrbKeyPoint* in, int n, const double* K4, const double* k5,
-                                           OrbKeyPoint* out, hipStream_t st)
+// n: the keypoints, or (count set) the capacity of a list whose length the
+// device holds in *count.
+extern "C" hipError_t orb_launch_undistort(const OrbKeyPoint* in, int n, const int* count, const double* K4,
+                                           const double* k5, OrbKeyPoint* out, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    undistort_kernel<<<(n + 255) / 256, 256, 0, st>>>(in, n, K4[0], K4[1], K4[2], K4[3], k5[0], k5[1], k5[2], k5[3],
-                                                      k5[4], out);
+    undistort_kernel<<<(n + 255) / 256, 256, 0, st>>>(in, n, count, K4[0], K4[1], K4[2], K4[3], k5[0], k5[1], k5[2],
+                                                      k5[3], k5[4], out);
     return hipGetLastError();
 }
 
-extern "C" hipError_t orb_launch_rgbd_depth(const OrbKeyPoint* kps, const OrbKeyPoint* kun, int n, const float* depth,
-                                            int w, int h, size_t pitch, float bf, float* uR, float* dep, hipStream_t st)
+extern "C" hipError_t orb_launch_rgbd_depth(const OrbKeyPoint* kps, const OrbKeyPoint* kun, int n, const int* count,
+                                            const float* depth, int w, int h, size_t pitch, float bf, float* uR,
+                                            float* dep, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    rgbd_depth_kernel<<<(n + 255) / 256, 256, 0, st>>>(kps, kun, n, depth, w, h, pitch, bf, uR, dep);
+    rgbd_depth_kernel<<<(n + 255) / 256, 256, 0, st>>>(kps, kun, n, count, depth, w, h, pitch, bf, uR, dep);
     return hipGetLastError();
 }

@@ -30,11 +30,11 @@ extern "C" hipError_t orb_launch_distinctive(const uint8_t* desc, const int* off
                                              uint8_t* out_desc, hipStream_t st);
 extern "C" hipError_t orb_launch_hbm_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 extern "C" hipError_t orb_launch_copy_ops(const CopyOp* d_ops, int nops, int blocks_per_op, hipStream_t st);
-extern "C" hipError_t orb_launch_undistort(const OrbKeyPoint* in, int n, const double* K4, const double* k5,
-                                           OrbKeyPoint* out, hipStream_t st);
-extern "C" hipError_t orb_launch_rgbd_depth(const OrbKeyPoint* kps, const OrbKeyPoint* kun, int n, const float* depth,
-                                            int w, int h, size_t pitch, float bf, float* uR, float* dep,
-                                            hipStream_t st);
+extern "C" hipError_t orb_launch_undistort(const OrbKeyPoint* in, int n, const int* count, const double* K4,
+                                           const double* k5, OrbKeyPoint* out, hipStream_t st);
+extern "C" hipError_t orb_launch_rgbd_depth(const OrbKeyPoint* kps, const OrbKeyPoint* kun, int n, const int* count,
+                                            const float* depth, int w, int h, size_t pitch, float bf, float* uR,
+                                            float* dep, hipStream_t st);
 extern "C" int orb_knn_chunks(long long nm);
 extern "C" int orb_knn_scratch_setup(int device);
 extern "C" uint2* orb_knn_scratch(int device, size_t* entries);

@@ -1429,7 +1429,7 @@ int orb_undistort_keypoints(orb_context* ctx, int n, const orb_keypoint* kps, co
     for (int i = 0; i < nD; i++) k5[i] = D[i];
     hipStream_t st = ctx->stream;
     HIP_TRY(hipMemcpyAsync(d_in, kps, bytes, hipMemcpyHostToDevice, st));
-    HIP_TRY(orb_launch_undistort(d_in, n, K4, k5, d_out, st));
+    HIP_TRY(orb_launch_undistort(d_in, n, nullptr, K4, k5, d_out, st));
     HIP_TRY(hipMemcpyAsync(out, d_out, bytes, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return ORB_OK;
@@ -1456,11 +1456,61 @@ int orb_rgbd_depth(orb_context* ctx, int n, const orb_keypoint* kps, const orb_k
     HIP_TRY(hipMemcpyAsync(m + o_un, kps_un, kb, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(m + o_img, depth, ib, hipMemcpyHostToDevice, st));
     float* d_uR = (float*)(m + o_out);
-    HIP_TRY(orb_launch_rgbd_depth((const OrbKeyPoint*)m, (const OrbKeyPoint*)(m + o_un), n, (const float*)(m + o_img),
-                                  width, height, pitch, bf, d_uR, d_uR + n, st));
+    HIP_TRY(orb_launch_rgbd_depth((const OrbKeyPoint*)m, (const OrbKeyPoint*)(m + o_un), n, nullptr,
+                                  (const float*)(m + o_img), width, height, pitch, bf, d_uR, d_uR + n, st));
     HIP_TRY(hipMemcpyAsync(uRight, d_uR, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(depth_out, d_uR + n, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return ORB_OK;
+}
+
+int orb_rgbd_frame(orb_extractor* ex, const uint8_t* image, int width, int height, size_t stride, const float* depth,
+                   size_t depth_pitch, const float* K, const float* D, int nD, float bf, orb_keypoint* keypoints,
+                   orb_keypoint* keypoints_un, int capacity, uint8_t* descriptors, float* uRight, float* depth_out,
+                   int* n)
+{
+    if (!ex || !n || width < 0 || height < 0) return ORB_E_ARG;
+    *n = 0;
+    if (width == 0 || height == 0) return ORB_OK;
+    if (!image || stride < (size_t)width || !depth || depth_pitch < (size_t)width * 4 || depth_pitch % 4 || !K ||
+        !D || (nD != 4 && nD != 5) || !(K[0] != 0.0f) || !(K[1] != 0.0f))
+        return ORB_E_ARG;
+    orb_context* ctx = ex->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = ex->ensure(width, height);
+    if (rc) return rc;
+    const int cap = ex->plan.out_cap;
+    const size_t ib = depth_pitch * (height - 1) + (size_t)width * 4;
+    const size_t o_un = (ib + 255) / 256 * 256, o_out = o_un + ((size_t)cap * sizeof(OrbKeyPoint) + 255) / 256 * 256;
+    rc = ctx->grow(&ctx->d_fr, &ctx->fr_cap, o_out + (size_t)cap * 8);
+    if (rc) return rc;
+    rc = ex->upload_image(image, stride, hipMemcpyHostToDevice);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    char* m = (char*)ctx->d_fr;
+    HIP_TRY(hipMemcpyAsync(m, depth, ib, hipMemcpyHostToDevice, st));   // one linear copy at the caller's pitch
+    HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, st, nullptr));
+    const int* cnt = ex->hjob.counters + ORB_CNT_NOUT;
+    const bool distorted = D[0] != 0.0f;                              // Frame.cc:727
+    OrbKeyPoint* d_kun = distorted ? (OrbKeyPoint*)(m + o_un) : ex->hjob.kps;
+    if (distorted) {
+        const double K4[4] = {K[0], K[1], K[2], K[3]};
+        double k5[5] = {0, 0, 0, 0, 0};
+        for (int i = 0; i < nD; i++) k5[i] = D[i];
+        HIP_TRY(orb_launch_undistort(ex->hjob.kps, cap, cnt, K4, k5, d_kun, st));
+    }
+    float* d_uR = (float*)(m + o_out);
+    HIP_TRY(orb_launch_rgbd_depth(ex->hjob.kps, d_kun, cap, cnt, (const float*)m, width, height, depth_pitch, bf, d_uR,
+                                  d_uR + cap, st));
+    rc = ex->read_results(keypoints, capacity, descriptors, n);
+    if (rc) return rc;
+    if (*n > 0) {
+        if (keypoints_un)
+            HIP_TRY(hipMemcpyAsync(keypoints_un, d_kun, (size_t)*n * sizeof(OrbKeyPoint), hipMemcpyDeviceToHost, st));
+        if (uRight) HIP_TRY(hipMemcpyAsync(uRight, d_uR, (size_t)*n * 4, hipMemcpyDeviceToHost, st));
+        if (depth_out) HIP_TRY(hipMemcpyAsync(depth_out, d_uR + cap, (size_t)*n * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
     return ORB_OK;
 }
 

@@ -92,3 +92,22 @@ def test_rgbd_frame_chain_equals_oracle(orb, oracle):
     uR, dep = orb.rgbd_depth(kps, kun, depth, 40.0)
     ouR, odep = oracle.rgbd_depth(np.stack([okps["x"], okps["y"]], 1), oun[:, 0], depth, 40.0)
     assert uR.tobytes() == ouR.tobytes() and dep.tobytes() == odep.tobytes()
+
+
+@pytest.mark.parametrize("D", [TUM1_D, (0.0, 0.1, 0.0, 0.0)])
+def test_rgbd_frame_one_call_equals_oracle(orb, oracle, D):
+    """orb_rgbd_frame: extraction, undistortion and depth in one device pass
+    (k1 = 0 reuses the keypoints as the undistorted ones)."""
+    img = synth.shapes_frame(640, 480, 22)
+    depth = (0.8 + (np.arange(480)[:, None] * 2 + np.arange(640)[None, :]) / 900.0).astype(np.float32)
+    depth[::7, ::5] = 0.0
+    ex = orb.ORBextractor(1000, 1.2, 8, 20, 7)
+    kps, desc, kun, uR, dep = orb.rgbd_frame(ex, img, depth, TUM1_K, D, 40.0)
+    okps, odesc = oracle.OracleExtractor(1000, 1.2, 8, 20, 7)(img)
+    assert kps.tobytes() == okps.tobytes()
+    np.testing.assert_array_equal(desc, odesc)
+    oun = oracle.undistort_points(np.stack([okps["x"], okps["y"]], 1), TUM1_K, D)
+    assert kun["x"].tobytes() == oun[:, 0].tobytes() and kun["y"].tobytes() == oun[:, 1].tobytes()
+    ouR, odep = oracle.rgbd_depth(np.stack([okps["x"], okps["y"]], 1), oun[:, 0], depth, 40.0)
+    assert uR.tobytes() == ouR.tobytes() and dep.tobytes() == odep.tobytes()
+    assert (dep == -1).sum() > 0 and (dep > 0).sum() > len(kps) // 2
