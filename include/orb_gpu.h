@@ -510,7 +510,9 @@ int orb_rgbd_depth(orb_context* ctx, int n, const orb_keypoint* kps, const orb_k
  * orb_extract, then the undistortion and the depth lookup on the keypoints
  * still on the device, one wait at the end.  keypoints / descriptors as
  * orb_extract; keypoints_un, uRight and depth_out (each optional) receive
- * *n entries.  depth: width x height floats, depth_pitch bytes a row. */
+ * *n entries.  depth: width x height floats, depth_pitch bytes a row; NULL
+ * for the monocular constructor (src/Frame.cc:174-216), which stops after
+ * the undistortion (uRight and depth_out are then not written). */
 int orb_rgbd_frame(orb_extractor* ex, const uint8_t* image, int width, int height, size_t stride, const float* depth,
                    size_t depth_pitch, const float* K, const float* D, int nD, float bf, orb_keypoint* keypoints,
                    orb_keypoint* keypoints_un, int capacity, uint8_t* descriptors, float* uRight, float* depth_out,

@@ -441,27 +441,32 @@ def stereo_frame(left, right, im_left, im_right, bf, b):
 
 def rgbd_frame(extractor, image, depth, K, D, bf):
     """The RGB-D Frame constructor's hot path in one device pass: extraction,
-    UndistortKeyPoints and ComputeStereoFromRGBD.
+    UndistortKeyPoints and ComputeStereoFromRGBD (depth None: the monocular
+    constructor's, extraction and UndistortKeyPoints; mvuRight and mvDepth
+    are then all -1).
 
     Returns (keypoints, descriptors, keypoints_un, mvuRight, mvDepth)."""
     img = np.ascontiguousarray(image, np.uint8)
-    dep_img = np.asarray(depth, np.float32)
-    if dep_img.ndim != 2 or dep_img.strides[1] != 4 or dep_img.strides[0] < 4 * dep_img.shape[1]:
-        dep_img = np.ascontiguousarray(dep_img)
     h, w = img.shape
-    if dep_img.shape != (h, w):
-        raise ValueError("depth must have the image's size")
+    dep_img = None
+    if depth is not None:
+        dep_img = np.asarray(depth, np.float32)
+        if dep_img.ndim != 2 or dep_img.strides[1] != 4 or dep_img.strides[0] < 4 * dep_img.shape[1]:
+            dep_img = np.ascontiguousarray(dep_img)
+        if dep_img.shape != (h, w):
+            raise ValueError("depth must have the image's size")
     K = np.ascontiguousarray(K, np.float32)
     D = np.ascontiguousarray(D, np.float32)
     cap = extractor.capacity(w, h)
     kps = np.zeros(cap, KEYPOINT_DTYPE)
     kun = np.zeros(cap, KEYPOINT_DTYPE)
     desc = np.zeros((cap, 32), np.uint8)
-    uR = np.zeros(cap, np.float32)
-    dep = np.zeros(cap, np.float32)
+    uR = np.full(cap, -1, np.float32)
+    dep = np.full(cap, -1, np.float32)
     n = ctypes.c_int()
     _check("orb_rgbd_frame", lib().orb_rgbd_frame(extractor.h, _ptr(img), w, h, img.strides[0], _ptr(dep_img),
-                                                  dep_img.strides[0], _ptr(K), _ptr(D), len(D), float(bf), _ptr(kps),
+                                                  dep_img.strides[0] if dep_img is not None else 0, _ptr(K), _ptr(D),
+                                                  len(D), float(bf), _ptr(kps),
                                                   _ptr(kun), cap, _ptr(desc), _ptr(uR), _ptr(dep), ctypes.byref(n)))
     k = n.value
     return kps[:k].copy(), desc[:k].copy(), kun[:k].copy(), uR[:k].copy(), dep[:k].copy()

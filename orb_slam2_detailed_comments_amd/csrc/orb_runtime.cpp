@@ -1472,7 +1472,7 @@ int orb_rgbd_frame(orb_extractor* ex, const uint8_t* image, int width, int heigh
     if (!ex || !n || width < 0 || height < 0) return ORB_E_ARG;
     *n = 0;
     if (width == 0 || height == 0) return ORB_OK;
-    if (!image || stride < (size_t)width || !depth || depth_pitch < (size_t)width * 4 || depth_pitch % 4 || !K ||
+    if (!image || stride < (size_t)width || (depth && (depth_pitch < (size_t)width * 4 || depth_pitch % 4)) || !K ||
         !D || (nD != 4 && nD != 5) || !(K[0] != 0.0f) || !(K[1] != 0.0f))
         return ORB_E_ARG;
     orb_context* ctx = ex->ctx;
@@ -1480,7 +1480,7 @@ int orb_rgbd_frame(orb_extractor* ex, const uint8_t* image, int width, int heigh
     int rc = ex->ensure(width, height);
     if (rc) return rc;
     const int cap = ex->plan.out_cap;
-    const size_t ib = depth_pitch * (height - 1) + (size_t)width * 4;
+    const size_t ib = depth ? depth_pitch * (height - 1) + (size_t)width * 4 : 0;
     const size_t o_un = (ib + 255) / 256 * 256, o_out = o_un + ((size_t)cap * sizeof(OrbKeyPoint) + 255) / 256 * 256;
     rc = ctx->grow(&ctx->d_fr, &ctx->fr_cap, o_out + (size_t)cap * 8);
     if (rc) return rc;
@@ -1488,7 +1488,7 @@ int orb_rgbd_frame(orb_extractor* ex, const uint8_t* image, int width, int heigh
     if (rc) return rc;
     hipStream_t st = ctx->stream;
     char* m = (char*)ctx->d_fr;
-    HIP_TRY(hipMemcpyAsync(m, depth, ib, hipMemcpyHostToDevice, st));   // one linear copy at the caller's pitch
+    if (depth) HIP_TRY(hipMemcpyAsync(m, depth, ib, hipMemcpyHostToDevice, st));   // one linear copy at its pitch
     HIP_TRY(orb_launch_extract(ex->d_job, ex->d_rt, ex->h_rt, st, nullptr));
     const int* cnt = ex->hjob.counters + ORB_CNT_NOUT;
     const bool distorted = D[0] != 0.0f;                              // Frame.cc:727
@@ -1500,15 +1500,17 @@ int orb_rgbd_frame(orb_extractor* ex, const uint8_t* image, int width, int heigh
         HIP_TRY(orb_launch_undistort(ex->hjob.kps, cap, cnt, K4, k5, d_kun, st));
     }
     float* d_uR = (float*)(m + o_out);
-    HIP_TRY(orb_launch_rgbd_depth(ex->hjob.kps, d_kun, cap, cnt, (const float*)m, width, height, depth_pitch, bf, d_uR,
-                                  d_uR + cap, st));
+    if (depth)
+        HIP_TRY(orb_launch_rgbd_depth(ex->hjob.kps, d_kun, cap, cnt, (const float*)m, width, height, depth_pitch, bf,
+                                      d_uR, d_uR + cap, st));
     rc = ex->read_results(keypoints, capacity, descriptors, n);
     if (rc) return rc;
     if (*n > 0) {
         if (keypoints_un)
             HIP_TRY(hipMemcpyAsync(keypoints_un, d_kun, (size_t)*n * sizeof(OrbKeyPoint), hipMemcpyDeviceToHost, st));
-        if (uRight) HIP_TRY(hipMemcpyAsync(uRight, d_uR, (size_t)*n * 4, hipMemcpyDeviceToHost, st));
-        if (depth_out) HIP_TRY(hipMemcpyAsync(depth_out, d_uR + cap, (size_t)*n * 4, hipMemcpyDeviceToHost, st));
+        if (depth && uRight) HIP_TRY(hipMemcpyAsync(uRight, d_uR, (size_t)*n * 4, hipMemcpyDeviceToHost, st));
+        if (depth && depth_out)
+            HIP_TRY(hipMemcpyAsync(depth_out, d_uR + cap, (size_t)*n * 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
     }
     return ORB_OK;

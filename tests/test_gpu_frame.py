@@ -111,3 +111,17 @@ def test_rgbd_frame_one_call_equals_oracle(orb, oracle, D):
     ouR, odep = oracle.rgbd_depth(np.stack([okps["x"], okps["y"]], 1), oun[:, 0], depth, 40.0)
     assert uR.tobytes() == ouR.tobytes() and dep.tobytes() == odep.tobytes()
     assert (dep == -1).sum() > 0 and (dep > 0).sum() > len(kps) // 2
+
+
+def test_mono_frame_one_call_equals_oracle(orb, oracle):
+    """orb_rgbd_frame with no depth image: the monocular constructor's
+    extraction and UndistortKeyPoints."""
+    img = synth.shapes_frame(752, 480, 23)
+    ex = orb.ORBextractor(1000, 1.2, 8, 20, 7)
+    kps, desc, kun, uR, dep = orb.rgbd_frame(ex, img, None, EUROC_K, EUROC_D, 0.0)
+    okps, odesc = oracle.OracleExtractor(1000, 1.2, 8, 20, 7)(img)
+    assert kps.tobytes() == okps.tobytes()
+    np.testing.assert_array_equal(desc, odesc)
+    oun = oracle.undistort_points(np.stack([okps["x"], okps["y"]], 1), EUROC_K, EUROC_D)
+    assert kun["x"].tobytes() == oun[:, 0].tobytes() and kun["y"].tobytes() == oun[:, 1].tobytes()
+    assert (uR == -1).all() and (dep == -1).all()
