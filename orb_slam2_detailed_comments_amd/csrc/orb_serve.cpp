@@ -86,6 +86,7 @@ struct Lane {                    // one HIP stream with its staging
     char* h_stage = nullptr;
     std::vector<Request*> reqs;
     bool busy = false;
+    bool failed = false;         // the batch's launch failed: its results are not valid
 };
 
 size_t result_bytes(int cap) { return 256 + (size_t)cap * (sizeof(OrbKeyPoint) + 32 + 8); }
@@ -237,7 +238,11 @@ struct orb_server {
         if (e == hipSuccess)
             e = orb_launch_copy_ops((const CopyOp*)(L.d_stage + kStageOut), kOutOps * n, 4, L.st);
         if (e == hipSuccess) e = hipEventRecord(L.ev, L.st);
-        if (e != hipSuccess) error = ORB_E_DEVICE;
+        L.failed = e != hipSuccess;
+        if (L.failed) {
+            error = ORB_E_DEVICE;
+            (void)hipStreamSynchronize(L.st);   // whatever was enqueued has finished before finish() reuses it
+        }
         L.reqs = batch;
     }
 
@@ -248,8 +253,8 @@ struct orb_server {
         for (Request* r : L.reqs) {
             const int* ctr = (const int*)r->h_res;
             r->t_done = t;
-            r->status = ctr[ORB_CNT_ERR] ? ORB_E_OVERFLOW : ORB_OK;
-            r->n_kps = ctr[ORB_CNT_ERR] ? 0 : ctr[ORB_CNT_NOUT];
+            r->status = L.failed ? ORB_E_DEVICE : ctr[ORB_CNT_ERR] ? ORB_E_OVERFLOW : ORB_OK;
+            r->n_kps = r->status ? 0 : ctr[ORB_CNT_NOUT];
             const int* idx = (const int*)(r->h_res + 256 + (size_t)r->cap * (sizeof(OrbKeyPoint) + 32));
             int m = 0;
             for (int i = 0; i < r->n_kps; i++) m += idx[i] >= 0;
