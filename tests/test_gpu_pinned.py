@@ -93,3 +93,25 @@ def test_pinned_two_batches_in_flight(orb, oracle):
         pending[k] = f
     for k, f in pending.items():
         _check_pair(batches[k], 0, refs[f])
+
+
+def test_pinned_mixed_sizes_equal_oracle(orb, oracle):
+    """Slots of four sizes (each at its own device pitch in the pinned
+    buffer), no stereo pairs: every slot's pinned results equal the oracle's."""
+    sizes = [(640, 480, 1000), (752, 480, 1000), (1241, 376, 2000), (1280, 720, 2000), (753, 481, 800)]
+    imgs = [synth.shapes_frame(w, h, 300 + i) for i, (w, h, _) in enumerate(sizes)]
+    batch = orb.Batch([(w, h) for w, h, _ in sizes], [nf for _, _, nf in sizes])
+    views = batch.pinned_images()
+    for v, im in zip(views, imgs):
+        assert v.shape == im.shape
+        v[:] = im
+    batch.upload_pinned()
+    batch.run(1)
+    batch.read_async()
+    for s, ((w, h, nf), im) in enumerate(zip(sizes, imgs)):
+        ok, od = oracle.OracleExtractor(nf, 1.2, 8, 20, 7)(im)
+        gk, gd = batch.pinned_results(s)
+        assert gk.tobytes() == ok.tobytes(), s
+        np.testing.assert_array_equal(gd, od)
+    with pytest.raises(orb.OrbError, match="bad argument"):
+        batch.pinned_stereo(0)                # no pairs declared
