@@ -423,24 +423,30 @@ def serving_extra(orb, ctx):
 VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
 
 
-def committed_counters(stage, stage_ms, hbm_frac):
+def committed_counters(stage, stage_ms, hbm_frac, n_images):
     """Counter-derived fields for the dominant stage, from the summaries the
     rocprofv3 runs left in profiles/ (counters are not collected while
     timing): L2-to-fabric traffic (FETCH_SIZE + WRITE_SIZE, which also counts
     Infinity Cache hits, so it bounds HBM bytes from above), and the VALU
     issue fraction = SQ_INSTS_VALU per launch x 4 cycles over this run's
-    launch time on 1024 SIMDs at 2.4 GHz, beside the profiler's VALUBusy."""
+    launch time on 1024 SIMDs at 2.4 GHz, beside the profiler's VALUBusy.
+    The summaries were taken on launches of 64 images of this size; both
+    counts are per-tile sums, so they are scaled to this run's n_images."""
     out = {}
     tfile = os.path.join(ROOT, "profiles", f"traffic_{stage}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            out["traffic"] = json.load(f).get("bytes_per_launch")
-        out["traffic_kind"] = f"L2 egress bytes (FETCH_SIZE+WRITE_SIZE, calibrated; an upper bound on HBM bytes), profiles/traffic_{stage}.json"
+            t = json.load(f)
+        scale = n_images / t.get("images_per_launch", 64)
+        out["traffic"] = int(round(t.get("bytes_per_launch") * scale))
+        out["traffic_kind"] = (f"L2 egress bytes (FETCH_SIZE+WRITE_SIZE, calibrated; an upper bound on HBM bytes), "
+                               f"profiles/traffic_{stage}.json, its {t.get('images_per_launch', 64)}-image launch "
+                               f"scaled to {n_images} images")
     vfile = os.path.join(ROOT, "profiles", f"valu_{stage}.json")
     if os.path.exists(vfile):
         with open(vfile) as f:
             v = json.load(f)
-        insts = v["valu_insts_per_launch"]
+        insts = int(round(v["valu_insts_per_launch"] * n_images / v.get("images_per_launch", 64)))
         frac = insts / (stage_ms * 1e-3) / VALU_WAVE_INSTS_PER_S
         out["valu"] = {"issue_frac": round(frac, 4), "valu_insts_per_launch": insts,
                        "cycles_per_inst": 4, "profiled_valu_busy": v.get("valu_busy"),
@@ -539,7 +545,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=32, help="stereo frames per step")
+    ap.add_argument("--batch", type=int, default=64, help="stereo frames per step")
     ap.add_argument("--mode", choices=["graph", "stages", "per-image", "streams"], default="graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -730,16 +736,23 @@ def main():
     print(f"[bench] host to host, pinned: {host_to_host['pinned_pipelined']}", file=sys.stderr, flush=True)
 
     stages = batch.stage_times(args.steps)
-    dom = max((k for k in stages if k != "stereo_median"), key=lambda k: stages[k])
+    # The dominant kernel is the one with the largest share of the step: the
+    # level kernel's eight launches against the other stages' one each.  For
+    # the level kernel the roofline is that of its largest launch, level 0.
+    fam = {"pyramid": sum(v for k, v in stages.items() if k.startswith("pyramid_"))}
+    fam.update({k: v for k, v in stages.items() if not k.startswith("pyramid_") and k != "stereo_median"})
+    top = max(fam, key=fam.get)
+    dom = "pyramid_l0" if top == "pyramid" else top
     cands = None
     bytes_dom = stage_bytes(dom, 2 * B, float(np.mean(kps)), 10 * NF if cands is None else cands)
     achieved = bytes_dom / (stages[dom] * 1e-3) / 1e9 if bytes_dom else None
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2) if achieved else None,
+    roofline = {"bound": "hbm", "kernel": dom, "kernel_share_of_stage_time": round(fam[top] / sum(fam.values()), 3),
+                "achieved": round(achieved, 2) if achieved else None,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
                 "traffic": None, "avg_launch_ms": round(stages[dom], 4),
                 "algorithmic_bytes_per_launch": bytes_dom,
                 "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
-    roofline.update(committed_counters(dom, stages[dom], roofline["frac"]))
+    roofline.update(committed_counters(dom, stages[dom], roofline["frac"], 2 * B))
     out = {"metric": METRIC, "value": round(value, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "u8",
