@@ -230,6 +230,57 @@ def frame_ops_extra(orb, ctx, steps, cpu_seconds):
     return out
 
 
+def input_extra(orb, ctx, steps, cpu_seconds):
+    """Input-side image ops in front of the pyramid: cvtColor to gray of a
+    1241x376 RGB frame and EuRoC cam0 rectification (initUndistortRectifyMap
+    maps in HBM, remap INTER_LINEAR) of a 752x480 frame, each alone (host
+    image in, host image out) and fused in front of the extraction (against
+    the plain extraction of the same gray frame), per frame; the oracle on
+    one core beside the standalone ops."""
+    from orb_slam2_detailed_comments_amd import synth
+    gray = synth.shapes_frame(1241, 376, 9)
+    rgb = np.stack([gray, np.roll(gray, 3, 1), np.roll(gray, 5, 0)], 2)
+    raw = synth.shapes_frame(752, 480, 10)
+    K = [[458.654, 0, 367.215], [0, 457.296, 248.375], [0, 0, 1]]
+    D = [-0.28340811, 0.07395907, 0.00019359, 1.76187114e-05, 0.0]
+    R = [[0.999966347530033, -0.001422739138722922, 0.008079580483432283],
+         [0.001365741834644127, 0.9999741760894847, 0.007055629199258132],
+         [-0.008089410156878961, -0.007044357138517677, 0.9999424675829176]]
+    P = [[435.2046959714599, 0, 367.4517211914062, 0], [0, 435.2046959714599, 252.2008514404297, 0], [0, 0, 1, 0]]
+    rect = orb.Rectifier(K, D, R, P, 752, 480, context=ctx)
+    ex_k = orb.ORBextractor(2000, 1.2, 8, 20, 7, context=ctx)
+    ex_e = orb.ORBextractor(1000, 1.2, 8, 20, 7, context=ctx)
+
+    def per_frame(fn):
+        for _ in range(3):
+            fn()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return round((time.perf_counter() - t0) * 1e3 / steps, 4)
+
+    out = {"workload": "cvtColor RGB->gray 1241x376; EuRoC cam0 remap 752x480 (maps resident); host images",
+           "gray_ms": per_frame(lambda: orb.cvt_color_gray(rgb, True, context=ctx)),
+           "remap_ms": per_frame(lambda: rect.remap(raw)),
+           "extract_gray_ms": per_frame(lambda: ex_k(gray)),
+           "extract_color_ms": per_frame(lambda: ex_k.extract_color(rgb, True)),
+           "extract_752x480_ms": per_frame(lambda: ex_e(raw)),
+           "extract_rectified_ms": per_frame(lambda: rect.extract(ex_e, raw))}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        mx, my = rect.maps()
+        for name, fn in (("gray", lambda: pyoracle.cvt_gray(rgb, True)),
+                         ("remap", lambda: pyoracle.remap_linear(raw, mx, my))):
+            k, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < cpu_seconds / 2:
+                fn()
+                k += 1
+            out[f"{name}_cpu_baseline"] = {"value": round(k / (time.perf_counter() - t0), 1), "unit": "frames/s",
+                                           "cores": 1, "kind": "port"}
+    return out
+
+
 def winmatch_extra(orb, ctx, cpu_seconds):
     """SearchForInitialization (window 100, ratio 0.9, orientation check) for
     32 frame pairs of 640x480 with the initializer's 2000 features, one call:
@@ -551,8 +602,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip every extra workload")
     ap.add_argument("--extras", default="all",
-                    help="comma-separated extras to run: knn, batch, serving, bow, mappoint, frame, winmatch, "
-                         "projection")
+                    help="comma-separated extras to run: knn, batch, serving, bow, mappoint, input, frame, "
+                         "winmatch, projection")
     ap.add_argument("--tiles", default="auto",
                     help="level kernel tile heights: auto (the tile autotuner), 32, 16, or 8 comma-separated values")
     ap.add_argument("--branches", type=int, default=0,
@@ -766,7 +817,8 @@ def main():
                       "mean_stereo_matches": round(float(np.mean(matched)), 1)},
            "roofline": roofline, "host_to_host": host_to_host}
     want = set() if args.no_extras else (
-        {"knn", "batch", "serving", "bow", "mappoint", "frame", "winmatch", "projection", "hbm"} if args.extras == "all"
+        {"knn", "batch", "serving", "bow", "mappoint", "input", "frame", "winmatch", "projection", "hbm"}
+        if args.extras == "all"
         else set(args.extras.split(",")))
     cpu_s = 0.0 if args.no_cpu else min(args.cpu_seconds, 3.0)
     if "hbm" in want:
@@ -792,6 +844,9 @@ def main():
     if "mappoint" in want:
         print("[bench] map-point descriptors", file=sys.stderr, flush=True)
         out["distinctive_descriptors"] = mappoint_extra(orb, ctx, max(args.steps, 5), cpu_s)
+    if "input" in want:
+        print("[bench] input-side image ops", file=sys.stderr, flush=True)
+        out["input_ops"] = input_extra(orb, ctx, max(args.steps, 20), min(cpu_s, 3.0))
     if "frame" in want:
         print("[bench] frame keypoint ops", file=sys.stderr, flush=True)
         out["frame_ops"] = frame_ops_extra(orb, ctx, max(args.steps, 20), min(cpu_s, 3.0))

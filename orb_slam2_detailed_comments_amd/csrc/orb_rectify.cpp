@@ -176,7 +176,10 @@ int orb_remap(orb_rectify* r, const uint8_t* src, int src_width, int src_height,
     hipStream_t st = r->ctx->stream;
     HIP_TRY(orb_launch_remap(r->d_src, src_width, src_width, src_height, r->d_mapx, r->d_mapy, r->w, r->h, r->d_dst,
                              r->w, st));
-    HIP_TRY(hipMemcpy2DAsync(dst, dst_stride, r->d_dst, r->w, r->w, r->h, hipMemcpyDeviceToHost, st));
+    if (dst_stride == (size_t)r->w)   // one linear copy (a pitched one into pageable memory goes row by row)
+        HIP_TRY(hipMemcpyAsync(dst, r->d_dst, (size_t)r->w * r->h, hipMemcpyDeviceToHost, st));
+    else
+        HIP_TRY(hipMemcpy2DAsync(dst, dst_stride, r->d_dst, r->w, r->w, r->h, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return ORB_OK;
 }

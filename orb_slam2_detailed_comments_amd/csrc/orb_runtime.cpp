@@ -232,15 +232,19 @@ int orb_cvt_color_gray(orb_context* ctx, const uint8_t* image, int width, int he
     if (width == 0 || height == 0) return ORB_OK;
     if (!image || !gray || stride < (size_t)width * channels || gray_stride < (size_t)width) return ORB_E_ARG;
     HIP_TRY(hipSetDevice(ctx->device));
-    uint8_t* d_gray = nullptr;
-    if (hipMalloc(&d_gray, (size_t)width * height) != hipSuccess) return ORB_E_OOM;
-    int rc = upload_color(ctx, image, width, height, stride, channels, rgb, d_gray, (size_t)width);
-    if (!rc && hipMemcpy2DAsync(gray, gray_stride, d_gray, width, width, height, hipMemcpyDeviceToHost,
-                                ctx->stream) != hipSuccess)
-        rc = ORB_E_DEVICE;
-    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ORB_E_DEVICE;
-    (void)hipFree(d_gray);
-    return rc;
+    int rc = ctx->grow(&ctx->d_fr, &ctx->fr_cap, (size_t)width * height);   // the context's scratch
+    if (rc) return rc;
+    uint8_t* d_gray = (uint8_t*)ctx->d_fr;
+    rc = upload_color(ctx, image, width, height, stride, channels, rgb, d_gray, (size_t)width);
+    if (rc) return rc;
+    // rows at the host's own pitch go back in one linear copy (a pitched copy
+    // into pageable memory goes row by row)
+    if (gray_stride == (size_t)width)
+        HIP_TRY(hipMemcpyAsync(gray, d_gray, (size_t)width * height, hipMemcpyDeviceToHost, ctx->stream));
+    else
+        HIP_TRY(hipMemcpy2DAsync(gray, gray_stride, d_gray, width, width, height, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ORB_OK;
 }
 
 int orb_extractor_pyramid_level(orb_extractor* ex, int level, uint8_t* dst, size_t dst_stride, int* width,
