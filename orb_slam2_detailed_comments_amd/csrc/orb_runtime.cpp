@@ -416,9 +416,20 @@ int orb_extractor_debug_image(orb_extractor* ex, int level, int kind, uint8_t* d
     const LevelJob& L = ex->hjob.lv[level];
     const uint8_t* src = kind == 0 ? L.lvl : L.blur;
     if (dst_stride < (size_t)L.w) return ORB_E_ARG;
-    HIP_TRY(hipSetDevice(ex->ctx->device));
-    HIP_TRY(hipStreamSynchronize(ex->ctx->stream));
-    HIP_TRY(hipMemcpy2D(dst, dst_stride, src, L.pitch, L.w, L.h, hipMemcpyDeviceToHost));
+    orb_context* ctx = ex->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    if (dst_stride == (size_t)L.w) {
+        // rows packed on the device first, then one linear copy (a pitched
+        // copy into pageable memory goes row by row)
+        const int rc = ctx->grow(&ctx->d_fr, &ctx->fr_cap, (size_t)L.w * L.h);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy2DAsync(ctx->d_fr, L.w, src, L.pitch, L.w, L.h, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dst, ctx->d_fr, (size_t)L.w * L.h, hipMemcpyDeviceToHost, st));
+    } else {
+        HIP_TRY(hipMemcpy2DAsync(dst, dst_stride, src, L.pitch, L.w, L.h, hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
     return ORB_OK;
 }
 
