@@ -481,8 +481,9 @@ def committed_counters(stage, stage_ms, hbm_frac, n_images):
     Infinity Cache hits, so it bounds HBM bytes from above), and the VALU
     issue fraction = SQ_INSTS_VALU per launch x 4 cycles over this run's
     launch time on 1024 SIMDs at 2.4 GHz, beside the profiler's VALUBusy.
-    The summaries were taken on launches of 64 images of this size; both
-    counts are per-tile sums, so they are scaled to this run's n_images."""
+    Both counts are per-tile sums over launches of images_per_launch images
+    of this size (the file says how many), so they are scaled to this run's
+    n_images."""
     out = {}
     tfile = os.path.join(ROOT, "profiles", f"traffic_{stage}.json")
     if os.path.exists(tfile):
