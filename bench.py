@@ -491,9 +491,10 @@ def committed_counters(stage, stage_ms, hbm_frac, n_images):
             t = json.load(f)
         scale = n_images / t.get("images_per_launch", 64)
         out["traffic"] = int(round(t.get("bytes_per_launch") * scale))
+        n_prof = t.get("images_per_launch", 64)
         out["traffic_kind"] = (f"L2 egress bytes (FETCH_SIZE+WRITE_SIZE, calibrated; an upper bound on HBM bytes), "
-                               f"profiles/traffic_{stage}.json, its {t.get('images_per_launch', 64)}-image launch "
-                               f"scaled to {n_images} images")
+                               f"profiles/traffic_{stage}.json, its {n_prof}-image launch"
+                               + ("" if n_prof == n_images else f" scaled to {n_images} images"))
     vfile = os.path.join(ROOT, "profiles", f"valu_{stage}.json")
     if os.path.exists(vfile):
         with open(vfile) as f:
