@@ -220,3 +220,25 @@ def test_generic_scalar_opencv_rounding(orb, oracle, w, h, nf):
     d = orb.ORBextractor(nf, 1.2, 8, 20, 7)
     d(img)
     assert not np.array_equal(d.pyramid_level(1), g.pyramid_level(1))
+
+
+def test_pyramid_read_back_packed_and_strided(orb, oracle):
+    """mvImagePyramid read-back (ORBextractor.h:162) into a packed buffer
+    (rows packed on the device, one linear copy) and into rows wider than the
+    level (one pitched copy): the same pixels, the oracle's, and nothing
+    written past each row's width."""
+    img = synth.shapes_frame(1241, 376, 5)
+    g = orb.ORBextractor(2000, 1.2, 8, 20, 7)
+    o = oracle.OracleExtractor(2000, 1.2, 8, 20, 7)
+    g(img)
+    o(img)
+    for level in (0, 3, 7):
+        for kind in (0, 1):
+            packed = g.debug_image(level, kind)
+            np.testing.assert_array_equal(packed, o.level_image(level, blurred=bool(kind)))
+            h, w = packed.shape
+            buf = np.full((h, w + 45), 7, np.uint8)
+            assert orb.lib().orb_extractor_debug_image(g.h, level, kind, buf.ctypes.data, w + 45) == 0
+            np.testing.assert_array_equal(buf[:, :w], packed)
+            assert np.all(buf[:, w:] == 7)
+    assert orb.lib().orb_extractor_debug_image(g.h, 0, 0, buf.ctypes.data, 10) != 0   # stride below the width
